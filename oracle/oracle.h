@@ -1,0 +1,137 @@
+/*
+ * FETI oracle -- CPU restatement of the reference solver path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * product in espreso_amd/.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product never links, loads
+ * or calls it.
+ *
+ * What it restates (citations are files of the reference snapshot):
+ *   - structured HEXA8 grid: clusters x domains x elements
+ *     (src/input/meshgenerator/composition/gridgenerator.cpp:44-69)
+ *   - HEXA8 trilinear element, 2x2x2 Gauss (src/mesh/elements/volume/hexahedron8.cpp:25-52),
+ *     isotropic elasticity matrix (src/assembler/physics/structuralmechanics3d.cpp:191-240),
+ *     body force rho*a (structuralmechanics3d.cpp:503-505), normal pressure (:511-581)
+ *   - B1: Dirichlet rows, redundant gluing, K-diagonal scaling
+ *     (src/assembler/constraints/equalityconstraints.cpp:25-423)
+ *   - kernels R (orthonormal rigid body modes) and fix-point regularisation
+ *     RegMat = N (N^T N)^-1 N^T * max diag K (structuralmechanics3d.cpp:34-184,
+ *     src/assembler/physics/physics.cpp:365-394)
+ *   - G = -(B R)^T, coarse (G G^T)^-1 (src/solver/specific/cluster.cpp:2018-2227,
+ *     src/solver/specific/itersolver.cpp:4931-5121)
+ *   - projected PCG (itersolver.cpp:1406-1724), projector (:5541-5623),
+ *     F apply (src/solver/specific/cpu/itersolvercpu.cpp:9-114),
+ *     preconditioners (itersolvercpu.cpp:520-602, clustercpu.cpp:167-333),
+ *     primal recovery (itersolver.cpp:372-431)
+ *   - an undecomposed global direct solve (profile Cholesky) as ground truth.
+ *
+ * Plain C ABI (ctypes).  All arrays are caller-owned.
+ */
+#ifndef FETI_ORACLE_H
+#define FETI_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Structured box problem.  Identical layout to espreso_grid_problem in
+ * include/espreso_amd.h (kept separate on purpose: the oracle shares no code
+ * with the product). */
+typedef struct {
+    int32_t physics;            /* 0: 3D linear elasticity, 1: 3D heat        */
+    int32_t clusters[3];
+    int32_t domains[3];         /* subdomains per cluster                     */
+    int32_t elements[3];        /* elements per subdomain                     */
+    double  length[3];          /* box [0,Lx]x[0,Ly]x[0,Lz]                   */
+    double  young, poisson, density;
+    double  accel[3];           /* body force density = density * accel       */
+    double  conductivity;       /* heat                                       */
+    double  heat_source;        /* heat: volumetric source                    */
+    int32_t fixed[6];           /* faces x0,x1,y0,y1,z0,z1: bitmask of fixed components */
+    double  fixed_value[6];     /* prescribed value on the face               */
+    double  pressure[6];        /* normal pressure on the face (>0 pushes inward) */
+    uint64_t traction_seed;     /* !=0: nodal traction U(-1,1) on the x1 face */
+    double  traction_scale;
+    uint64_t young_seed;        /* !=0: E_e = young*(0.1+0.9 U(0,1)) per element */
+    uint64_t conductivity_seed; /* !=0: k_s = conductivity*10^U(-1,1) per subdomain */
+} oracle_grid_problem;
+
+typedef struct {
+    int32_t method;             /* 0 TOTAL_FETI (HYBRID_FETI not restated)     */
+    int32_t preconditioner;     /* 0 NONE, 1 LUMPED, 2 WEIGHT_FUNCTION, 3 DIRICHLET */
+    int32_t max_iterations;
+    int32_t scaling;            /* 1: K-diagonal B1 scaling, 0: 1/multiplicity */
+    int32_t kplus;              /* 0: fix-point regularised K (reference ANALYTIC)
+                                   1: boundary-kernel generalised inverse      */
+    int32_t threads;            /* OpenMP threads (0 = default)                */
+    double  precision;          /* relative dual residual                      */
+} oracle_options;
+
+typedef struct {
+    int32_t iterations;
+    int32_t n_lambda;
+    int32_t n_coarse;
+    int32_t n_subdomains;
+    double  final_norm;         /* ||P r|| at the stop                          */
+    double  tol;                /* precision * min(||P r0||, ||b||)             */
+    double  norm_b;
+    double  kkt_primal;         /* ||K u - f + B^T lambda|| / ||f - B^T lambda|| */
+    double  kkt_dual;           /* ||B u - c|| / ||c|| (or absolute if c = 0)   */
+    double  time_setup;
+    double  time_solve;
+} oracle_stats;
+
+/* ---- undecomposed ground truth ---------------------------------------- */
+/* nodal solution u[node*ndof + c], node = i + Nx*(j + Ny*k)               */
+int oracle_global_solve(const oracle_grid_problem *p, double *u_nodal);
+int oracle_global_sizes(const oracle_grid_problem *p, int32_t *n_nodes, int32_t *ndof);
+/* global rhs/K actions for residual checks: y = K x (free rows) ; f */
+int oracle_global_residual(const oracle_grid_problem *p, const double *u_nodal, double *rel_res);
+
+/* ---- decomposed TFETI --------------------------------------------------- */
+void *oracle_create(const oracle_grid_problem *p, const oracle_options *o);
+void  oracle_destroy(void *h);
+/* sizes[0]=n_sub, [1]=n_lambda, [2]=n_coarse, [3]=n_nodes, [4]=ndof, [5]=kernel dim */
+void  oracle_sizes(void *h, int32_t *sizes);
+/* info[0]=n_i, [1]=m_i (boundary DOFs), [2]=number of lambda entries */
+void  oracle_subdomain_info(void *h, int32_t i, int32_t *info);
+/* local boundary DOFs (ascending), m_i entries */
+void  oracle_subdomain_boundary(void *h, int32_t i, int32_t *bdofs);
+/* lambda entries of subdomain i: lambda id, local dof, sign, scale */
+void  oracle_subdomain_lambdas(void *h, int32_t i, int32_t *lam, int32_t *dof, double *sign, double *scale);
+/* dense matrices of subdomain i, column-major:
+ *   which 0: K (n x n)        1: K_reg (n x n, fix-point)
+ *         2: R (n x k)        3: S_D Dirichlet Schur on boundary (m x m)
+ *         4: F_i = (K^+)_bb on boundary (m x m) for the active kplus mode   */
+int   oracle_subdomain_dense(void *h, int32_t i, int32_t which, double *out);
+void  oracle_subdomain_rhs(void *h, int32_t i, double *f);
+/* local node -> global node map of subdomain i */
+void  oracle_subdomain_l2g(void *h, int32_t i, int32_t *l2g);
+void  oracle_vec_c(void *h, double *c);
+void  oracle_vec_b(void *h, double *b);           /* b = sum B K^+ f - c  */
+void  oracle_vec_e(void *h, double *e);           /* e = -R^T f           */
+void  oracle_apply_F(void *h, const double *x, double *y);
+void  oracle_apply_prec(void *h, const double *x, double *y);
+void  oracle_project(void *h, const double *x, double *y);  /* P x */
+void  oracle_apply_G(void *h, const double *x, double *y);  /* y = G x (n_coarse) */
+/* full solve: nodal u (average of subdomain copies), lambda, stats */
+int   oracle_solve(void *h, double *u_nodal, double *lambda, oracle_stats *st);
+/* per-subdomain primal of the last solve */
+void  oracle_subdomain_primal(void *h, int32_t i, double *u);
+
+/* ---- CPU baseline sample (timed on the host) ----------------------------- */
+/* times[0] = seconds per subdomain of setup (factorisation + Dirichlet S_D),
+ * times[1] = seconds per subdomain of one K^+ apply (implicit F apply share),
+ * times[2] = seconds per subdomain of one S_D apply,
+ * sample subdomains are the first `sample` subdomains of the problem.       */
+int   oracle_cpu_sample(const oracle_grid_problem *p, const oracle_options *o,
+                        int32_t sample, double *times);
+
+const char *oracle_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,279 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: the checker for the HIP product.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+class OracleProblem(C.Structure):
+    _fields_ = [
+        ("physics", C.c_int32),
+        ("clusters", C.c_int32 * 3),
+        ("domains", C.c_int32 * 3),
+        ("elements", C.c_int32 * 3),
+        ("length", C.c_double * 3),
+        ("young", C.c_double),
+        ("poisson", C.c_double),
+        ("density", C.c_double),
+        ("accel", C.c_double * 3),
+        ("conductivity", C.c_double),
+        ("heat_source", C.c_double),
+        ("fixed", C.c_int32 * 6),
+        ("fixed_value", C.c_double * 6),
+        ("pressure", C.c_double * 6),
+        ("traction_seed", C.c_uint64),
+        ("traction_scale", C.c_double),
+        ("young_seed", C.c_uint64),
+        ("conductivity_seed", C.c_uint64),
+    ]
+
+
+class OracleOptions(C.Structure):
+    _fields_ = [
+        ("method", C.c_int32),
+        ("preconditioner", C.c_int32),
+        ("max_iterations", C.c_int32),
+        ("scaling", C.c_int32),
+        ("kplus", C.c_int32),
+        ("threads", C.c_int32),
+        ("precision", C.c_double),
+    ]
+
+
+class OracleStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("n_lambda", C.c_int32),
+        ("n_coarse", C.c_int32),
+        ("n_subdomains", C.c_int32),
+        ("final_norm", C.c_double),
+        ("tol", C.c_double),
+        ("norm_b", C.c_double),
+        ("kkt_primal", C.c_double),
+        ("kkt_dual", C.c_double),
+        ("time_setup", C.c_double),
+        ("time_solve", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+        _lib = C.CDLL(LIB_PATH)
+        d = C.POINTER(C.c_double)
+        i32 = C.POINTER(C.c_int32)
+        _lib.oracle_create.restype = C.c_void_p
+        _lib.oracle_create.argtypes = [C.POINTER(OracleProblem), C.POINTER(OracleOptions)]
+        _lib.oracle_destroy.argtypes = [C.c_void_p]
+        _lib.oracle_sizes.argtypes = [C.c_void_p, i32]
+        _lib.oracle_subdomain_info.argtypes = [C.c_void_p, C.c_int32, i32]
+        _lib.oracle_subdomain_boundary.argtypes = [C.c_void_p, C.c_int32, i32]
+        _lib.oracle_subdomain_lambdas.argtypes = [C.c_void_p, C.c_int32, i32, i32, d, d]
+        _lib.oracle_subdomain_dense.argtypes = [C.c_void_p, C.c_int32, C.c_int32, d]
+        _lib.oracle_subdomain_rhs.argtypes = [C.c_void_p, C.c_int32, d]
+        _lib.oracle_subdomain_l2g.argtypes = [C.c_void_p, C.c_int32, i32]
+        _lib.oracle_subdomain_primal.argtypes = [C.c_void_p, C.c_int32, d]
+        for name in ("oracle_vec_c", "oracle_vec_b", "oracle_vec_e"):
+            getattr(_lib, name).argtypes = [C.c_void_p, d]
+        for name in ("oracle_apply_F", "oracle_apply_prec", "oracle_project", "oracle_apply_G"):
+            getattr(_lib, name).argtypes = [C.c_void_p, d, d]
+        _lib.oracle_solve.argtypes = [C.c_void_p, d, d, C.POINTER(OracleStats)]
+        _lib.oracle_global_solve.argtypes = [C.POINTER(OracleProblem), d]
+        _lib.oracle_global_sizes.argtypes = [C.POINTER(OracleProblem), i32, i32]
+        _lib.oracle_global_residual.argtypes = [C.POINTER(OracleProblem), d, d]
+        _lib.oracle_cpu_sample.argtypes = [C.POINTER(OracleProblem), C.POINTER(OracleOptions), C.c_int32, d]
+        _lib.oracle_last_error.restype = C.c_char_p
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def make_problem(spec):
+    """Build an OracleProblem from a dict (see espreso_amd.problems)."""
+    p = OracleProblem()
+    p.physics = spec.get("physics", 0)
+    for k in ("clusters", "domains", "elements"):
+        getattr(p, k)[:] = spec[k]
+    p.length[:] = spec.get("length", (1.0, 1.0, 1.0))
+    p.young = spec.get("young", 2.1e11)
+    p.poisson = spec.get("poisson", 0.3)
+    p.density = spec.get("density", 7850.0)
+    p.accel[:] = spec.get("accel", (0.0, 0.0, 0.0))
+    p.conductivity = spec.get("conductivity", 1.0)
+    p.heat_source = spec.get("heat_source", 0.0)
+    p.fixed[:] = spec.get("fixed", (0, 0, 0, 0, 0, 0))
+    p.fixed_value[:] = spec.get("fixed_value", (0.0,) * 6)
+    p.pressure[:] = spec.get("pressure", (0.0,) * 6)
+    p.traction_seed = spec.get("traction_seed", 0)
+    p.traction_scale = spec.get("traction_scale", 1.0)
+    p.young_seed = spec.get("young_seed", 0)
+    p.conductivity_seed = spec.get("conductivity_seed", 0)
+    return p
+
+
+def make_options(preconditioner=3, precision=1e-8, max_iterations=500, scaling=1, kplus=0, threads=0):
+    o = OracleOptions()
+    o.method = 0
+    o.preconditioner = preconditioner
+    o.max_iterations = max_iterations
+    o.scaling = scaling
+    o.kplus = kplus
+    o.threads = threads
+    o.precision = precision
+    return o
+
+
+def global_solve(spec):
+    p = make_problem(spec)
+    nn = C.c_int32()
+    nd = C.c_int32()
+    lib().oracle_global_sizes(C.byref(p), C.byref(nn), C.byref(nd))
+    u = np.zeros(nn.value * nd.value)
+    rc = lib().oracle_global_solve(C.byref(p), _dp(u))
+    if rc:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return u.reshape(nn.value, nd.value)
+
+
+def global_residual(spec, u):
+    p = make_problem(spec)
+    u = np.ascontiguousarray(u, dtype=np.float64).ravel()
+    r = C.c_double()
+    lib().oracle_global_residual(C.byref(p), _dp(u), C.byref(r))
+    return r.value
+
+
+class Oracle:
+    """Decomposed TFETI restatement (one handle = one assembled problem)."""
+
+    def __init__(self, spec, **opts):
+        self.spec = spec
+        self._p = make_problem(spec)
+        self._o = make_options(**opts)
+        self.h = lib().oracle_create(C.byref(self._p), C.byref(self._o))
+        if not self.h:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        s = np.zeros(6, dtype=np.int32)
+        lib().oracle_sizes(self.h, _ip(s))
+        self.n_sub, self.n_lambda, self.n_coarse, self.n_nodes, self.ndof, self.kdim = (int(x) for x in s)
+
+    def close(self):
+        if self.h:
+            lib().oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self, i):
+        v = np.zeros(3, dtype=np.int32)
+        lib().oracle_subdomain_info(self.h, i, _ip(v))
+        return dict(n=int(v[0]), m=int(v[1]), nlam=int(v[2]))
+
+    def boundary(self, i):
+        b = np.zeros(self.info(i)["m"], dtype=np.int32)
+        lib().oracle_subdomain_boundary(self.h, i, _ip(b))
+        return b
+
+    def lambdas(self, i):
+        k = self.info(i)["nlam"]
+        lam = np.zeros(k, dtype=np.int32)
+        dof = np.zeros(k, dtype=np.int32)
+        sign = np.zeros(k)
+        scale = np.zeros(k)
+        lib().oracle_subdomain_lambdas(self.h, i, _ip(lam), _ip(dof), _dp(sign), _dp(scale))
+        return lam, dof, sign, scale
+
+    def dense(self, i, which):
+        inf = self.info(i)
+        n, m = inf["n"], inf["m"]
+        shape = {0: (n, n), 1: (n, n), 2: (self.kdim, n), 3: (m, m), 4: (m, m)}[which]
+        out = np.zeros(shape[0] * shape[1])
+        lib().oracle_subdomain_dense(self.h, i, which, _dp(out))
+        return out.reshape(shape).T if which != 2 else out.reshape(shape).T
+
+    def rhs(self, i):
+        f = np.zeros(self.info(i)["n"])
+        lib().oracle_subdomain_rhs(self.h, i, _dp(f))
+        return f
+
+    def l2g(self, i):
+        g = np.zeros(self.info(i)["n"] // self.ndof, dtype=np.int32)
+        lib().oracle_subdomain_l2g(self.h, i, _ip(g))
+        return g
+
+    def _vec(self, name, n):
+        v = np.zeros(n)
+        getattr(lib(), name)(self.h, _dp(v))
+        return v
+
+    def vec_c(self):
+        return self._vec("oracle_vec_c", self.n_lambda)
+
+    def vec_b(self):
+        return self._vec("oracle_vec_b", self.n_lambda)
+
+    def vec_e(self):
+        return self._vec("oracle_vec_e", self.n_coarse)
+
+    def _op(self, name, x, n_out):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(n_out)
+        getattr(lib(), name)(self.h, _dp(x), _dp(y))
+        return y
+
+    def apply_F(self, x):
+        return self._op("oracle_apply_F", x, self.n_lambda)
+
+    def apply_prec(self, x):
+        return self._op("oracle_apply_prec", x, self.n_lambda)
+
+    def project(self, x):
+        return self._op("oracle_project", x, self.n_lambda)
+
+    def apply_G(self, x):
+        return self._op("oracle_apply_G", x, self.n_coarse)
+
+    def solve(self):
+        u = np.zeros(self.n_nodes * self.ndof)
+        lam = np.zeros(self.n_lambda)
+        st = OracleStats()
+        rc = lib().oracle_solve(self.h, _dp(u), _dp(lam), C.byref(st))
+        if rc:
+            raise RuntimeError(lib().oracle_last_error().decode())
+        stats = {k: getattr(st, k) for k, _ in OracleStats._fields_}
+        return u.reshape(self.n_nodes, self.ndof), lam, stats
+
+    def primal(self, i):
+        u = np.zeros(self.info(i)["n"])
+        lib().oracle_subdomain_primal(self.h, i, _dp(u))
+        return u
+
+
+def cpu_sample(spec, sample, threads=0, preconditioner=3):
+    p = make_problem(spec)
+    o = make_options(preconditioner=preconditioner, threads=threads)
+    t = np.zeros(3)
+    rc = lib().oracle_cpu_sample(C.byref(p), C.byref(o), sample, _dp(t))
+    if rc:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return dict(setup_per_sub=t[0], kplus_per_sub=t[1], sd_apply_per_sub=t[2])

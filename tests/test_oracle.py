@@ -1,0 +1,229 @@
+"""Pins the CPU oracle (test infrastructure) before it is trusted as the checker.
+
+Each oracle function is pinned by something other than a retyped formula:
+  * global direct solve  -> closed-form patch tests (elasticity pressure patch,
+    linear heat) and the reference's golden .emr values (gravity), plus a
+    dense numpy solve on a tiny mesh
+  * TFETI solve          -> the undecomposed global solve (<= 1e-8 relative)
+  * B1 construction      -> multiplier counts (closed form), B u = c for the
+    global solution, sign/scale rules
+  * kernels / RegMat     -> K R = 0, R^T R = I, K_reg SPD
+  * F, P, G, M operators -> brute force with numpy dense inverses, P^2 = P,
+    G P = 0, symmetry
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from espreso_amd import problems as PB
+
+
+def _z1_magnitude(u, spec):
+    nn = [spec["clusters"][d] * spec["domains"][d] * spec["elements"][d] + 1 for d in range(3)]
+    mag = np.linalg.norm(u, axis=1).reshape(nn[2], nn[1], nn[0])
+    return mag[-1]
+
+
+def test_pressure_patch_global():
+    u = O.global_solve(PB.PRESSURE_PATCH)
+    z1 = _z1_magnitude(u, PB.PRESSURE_PATCH)
+    # golden HEXA8.TOTAL_FETI.emr: min = max = 1.000000e+00; closed form u_z = -0.01 z
+    assert np.allclose(z1, 1.0, rtol=0, atol=1e-12)
+    nn = (11, 21, 11)
+    z = np.repeat(np.linspace(0, 100, nn[2]), nn[0] * nn[1])
+    assert np.allclose(u[:, 2], -0.01 * z, atol=1e-12)
+    assert np.allclose(u[:, :2], 0.0, atol=1e-12)
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
+def test_pressure_patch_feti(prec):
+    o = O.Oracle(PB.PRESSURE_PATCH, preconditioner=prec, precision=1e-8)
+    u, lam, st = o.solve()
+    z1 = _z1_magnitude(u, PB.PRESSURE_PATCH)
+    assert np.allclose(z1, 1.0, rtol=0, atol=1e-7)
+    assert st["kkt_primal"] < 1e-10
+
+
+def test_gravity_emr_golden():
+    u = O.global_solve(PB.GRAVITY)
+    mag = np.linalg.norm(u, axis=1)
+    g = PB.GRAVITY_EMR
+    # reference harness: fail if |d| > 1e-4 and |d/v| > 1e-3
+    assert abs(mag.min() - g["min"]) < 1e-4
+    assert abs(mag.max() - g["max"]) / g["max"] < 1e-3
+    assert abs(mag.mean() - g["avg"]) / g["avg"] < 1e-3
+
+
+def test_heat_simple_golden():
+    u = O.global_solve(PB.HEAT_SIMPLE)[:, 0]
+    assert abs(u.min() - 100) < 1e-9 and abs(u.max() - 300) < 1e-9
+    assert abs(u.mean() - 200) < 1e-9
+    o = O.Oracle(PB.HEAT_SIMPLE, preconditioner=3, precision=1e-8)
+    uf, _, st = o.solve()
+    assert np.linalg.norm(uf[:, 0] - u) / np.linalg.norm(u) < 1e-8
+
+
+def _dense_global(spec):
+    """Brute-force dense assembly/solve from the oracle's subdomain matrices."""
+    o = O.Oracle(spec, preconditioner=0)
+    nd = o.ndof
+    n = o.n_nodes * nd
+    K = np.zeros((n, n))
+    f = np.zeros(n)
+    for s in range(o.n_sub):
+        g = np.repeat(o.l2g(s) * nd, nd) + np.tile(np.arange(nd), len(o.l2g(s)))
+        K[np.ix_(g, g)] += o.dense(s, 0)
+        f[g] += o.rhs(s)
+    return o, K, f
+
+
+def test_global_solve_vs_numpy_dense():
+    spec = PB.scaled(PB.GRAVITY, clusters=(1, 1, 1), domains=(2, 1, 2), elements=(2, 3, 2))
+    o, K, f = _dense_global(spec)
+    n = K.shape[0]
+    fixed = np.zeros(n, bool)
+    nn = [2 * 2 + 1, 3 + 1, 2 * 2 + 1]
+    for k in range(nn[2]):
+        if k == 0:
+            for j in range(nn[1]):
+                for i in range(nn[0]):
+                    fixed[3 * (i + nn[0] * (j + nn[1] * k)) + np.arange(3)] = True
+    free = ~fixed
+    u = np.zeros(n)
+    u[free] = np.linalg.solve(K[np.ix_(free, free)], f[free])
+    ug = O.global_solve(spec).ravel()
+    assert np.linalg.norm(ug - u) / np.linalg.norm(u) < 1e-12
+
+
+@pytest.mark.parametrize("spec,prec,scaling,kplus", [
+    (PB.GRAVITY, 3, 1, 0), (PB.GRAVITY, 1, 1, 0), (PB.GRAVITY, 3, 0, 1), (PB.GRAVITY, 2, 1, 1),
+    (PB.HEAT_SIMPLE, 1, 1, 0), (PB.HEAT_SIMPLE, 3, 0, 1),
+    (PB.scaled(PB.GRAVITY, traction_seed=7, traction_scale=1e6), 3, 1, 1),
+])
+def test_feti_matches_global(spec, prec, scaling, kplus):
+    ug = O.global_solve(spec)
+    o = O.Oracle(spec, preconditioner=prec, precision=1e-10, scaling=scaling, kplus=kplus)
+    u, lam, st = o.solve()
+    assert np.linalg.norm(u - ug) / np.linalg.norm(ug) < 1e-8
+    assert st["kkt_primal"] < 1e-9
+
+
+def test_multiplier_counts_closed_form():
+    # 4^3 subdomains of 5^3: glued DOFs on interior faces, z=0 Dirichlet copies
+    o = O.Oracle(PB.TFETI_64x5, preconditioner=0)
+    N = 21
+    # nodes shared by exactly 2, 4, 8 subdomains (off the z=0 plane), and the
+    # Dirichlet copies on z=0
+    planes = [5, 10, 15]
+    def count(axis_nodes):
+        return axis_nodes
+    shared = 0
+    for k in range(1, N):
+        for j in range(N):
+            for i in range(N):
+                m = (1 + (i in planes)) * (1 + (j in planes)) * (1 + (k in planes))
+                shared += m * (m - 1) // 2
+    dirichlet = sum((1 + (i in planes)) * (1 + (j in planes)) for i in range(N) for j in range(N))
+    assert o.n_lambda == 3 * (shared + dirichlet) == 21006
+    assert o.n_coarse == 6 * 64
+
+
+def test_b1_rules_and_bu_equals_c():
+    spec = PB.GRAVITY
+    o = O.Oracle(spec, preconditioner=3)
+    ug = O.global_solve(spec)
+    c = o.vec_c()
+    Bu = np.zeros(o.n_lambda)
+    touched = np.zeros(o.n_lambda, int)
+    for s in range(o.n_sub):
+        lam, dof, sign, scale = o.lambdas(s)
+        us = ug[o.l2g(s)].ravel()
+        np.add.at(Bu, lam, sign * us[dof])
+        np.add.at(touched, lam, 1)
+        assert np.all(np.diff(lam) > 0)
+        assert np.all((scale > 0) & (scale <= 1))
+    assert set(np.unique(touched)) <= {1, 2}
+    assert np.allclose(Bu, c, atol=1e-12)
+
+
+def test_kernels_and_regularisation():
+    o = O.Oracle(PB.scaled(PB.GRAVITY, elements=(3, 2, 4)), preconditioner=3)
+    for s in range(o.n_sub):
+        K = o.dense(s, 0)
+        R = o.dense(s, 2)
+        assert np.linalg.norm(K @ R) <= 1e-9 * np.linalg.norm(K)
+        assert np.allclose(R.T @ R, np.eye(R.shape[1]), atol=1e-12)
+        Kreg = o.dense(s, 1)
+        assert np.all(np.linalg.eigvalsh(Kreg) > 0)
+
+
+def test_subdomain_dual_blocks_bruteforce():
+    spec = PB.scaled(PB.GRAVITY, elements=(3, 2, 2))
+    for kplus in (0, 1):
+        o = O.Oracle(spec, preconditioner=3, kplus=kplus)
+        for s in (0, o.n_sub - 1):
+            K = o.dense(s, 0)
+            b = o.boundary(s)
+            i = np.setdiff1d(np.arange(K.shape[0]), b)
+            # Dirichlet preconditioner block = Schur complement of K on the boundary
+            S = K[np.ix_(b, b)] - K[np.ix_(b, i)] @ np.linalg.solve(K[np.ix_(i, i)], K[np.ix_(i, b)])
+            assert np.allclose(o.dense(s, 3), S, rtol=0, atol=1e-9 * np.abs(S).max())
+            Fb = o.dense(s, 4)
+            if kplus == 0:
+                ref = np.linalg.inv(o.dense(s, 1))[np.ix_(b, b)]
+            else:
+                Rb = o.dense(s, 2)[b]
+                Q, _ = np.linalg.qr(Rb)
+                ref = np.linalg.inv(S + S.diagonal().max() * Q @ Q.T)
+            assert np.allclose(Fb, ref, rtol=0, atol=1e-8 * np.abs(ref).max())
+            # generalised inverse on the boundary: S Fb S = S
+            assert np.allclose(S @ Fb @ S, S, rtol=0, atol=1e-7 * np.abs(S).max())
+
+
+def test_dual_operators_bruteforce():
+    spec = PB.scaled(PB.GRAVITY, elements=(2, 2, 2))
+    o = O.Oracle(spec, preconditioner=3, kplus=0)
+    nl = o.n_lambda
+    F = np.zeros((nl, nl))
+    Mlump = np.zeros((nl, nl))
+    Gm = np.zeros((o.n_coarse, nl))
+    off = 0
+    for s in range(o.n_sub):
+        lam, dof, sign, scale = o.lambdas(s)
+        n = o.info(s)["n"]
+        B = np.zeros((nl, n))
+        B[lam, dof] = sign
+        Bs = np.zeros((nl, n))
+        Bs[lam, dof] = sign * scale
+        F += B @ np.linalg.inv(o.dense(s, 1)) @ B.T
+        Mlump += Bs @ o.dense(s, 0) @ Bs.T
+        R = o.dense(s, 2)
+        Gm[off:off + R.shape[1]] = -(B @ R).T
+        off += R.shape[1]
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(nl)
+    assert np.allclose(o.apply_F(x), F @ x, rtol=0, atol=1e-10 * np.abs(F @ x).max())
+    assert np.allclose(o.apply_G(x), Gm @ x, atol=1e-12 * np.abs(Gm @ x).max())
+    P = np.eye(nl) - Gm.T @ np.linalg.solve(Gm @ Gm.T, Gm)
+    px = o.project(x)
+    assert np.allclose(px, P @ x, atol=1e-10)
+    assert np.allclose(o.project(px), px, atol=1e-10)
+    assert np.abs(o.apply_G(px)).max() < 1e-9 * np.abs(x).max()
+    o1 = O.Oracle(spec, preconditioner=1)
+    assert np.allclose(o1.apply_prec(x), Mlump @ x, rtol=0, atol=1e-9 * np.abs(Mlump @ x).max())
+    # projected F agrees between the two generalised inverses
+    o2 = O.Oracle(spec, preconditioner=3, kplus=1)
+    a = o.project(o.apply_F(px))
+    b = o2.project(o2.apply_F(px))
+    assert np.allclose(a, b, rtol=0, atol=1e-8 * np.abs(a).max())
+
+
+@pytest.mark.slow
+def test_tfeti_64x5_vs_global():
+    spec = PB.TFETI_64x5
+    ug = O.global_solve(spec)
+    for prec in (1, 3):
+        o = O.Oracle(spec, preconditioner=prec, precision=1e-8)
+        u, _, st = o.solve()
+        assert np.linalg.norm(u - ug) / np.linalg.norm(ug) < 1e-8
+        assert st["iterations"] < 100
