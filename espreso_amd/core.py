@@ -1,0 +1,373 @@
+"""ctypes mirror of the reference's FETISolver interface over the C ABI.
+
+Reference interface: espreso::LinearSolver {update, solve, finalize, precision}
+(src/linearsolver/linearsolver.h:9-23) implemented by FETISolver
+(src/solver/generic/FETISolver.h:22-76).  ``FETISolver`` below keeps that
+shape: construct with options, ``update`` with subdomain data (K_i, f_i,
+R_i, B1 rows, scaling, c), ``solve`` returns the per-subdomain primal.
+``GridFETI`` drives the same core from the structured-grid builder.
+
+No CPU fallback exists: a missing library raises ``LibraryNotBuilt``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libespreso_amd.so")
+
+PREC = {"none": 0, "lumped": 1, "weight": 2, "dirichlet": 3}
+SC = {"fp64": 0, "fp32": 1, "bf16": 2}
+
+
+class LibraryNotBuilt(RuntimeError):
+    pass
+
+
+class GridProblem(C.Structure):
+    _fields_ = [
+        ("physics", C.c_int32),
+        ("clusters", C.c_int32 * 3),
+        ("domains", C.c_int32 * 3),
+        ("elements", C.c_int32 * 3),
+        ("length", C.c_double * 3),
+        ("young", C.c_double),
+        ("poisson", C.c_double),
+        ("density", C.c_double),
+        ("accel", C.c_double * 3),
+        ("conductivity", C.c_double),
+        ("heat_source", C.c_double),
+        ("fixed", C.c_int32 * 6),
+        ("fixed_value", C.c_double * 6),
+        ("pressure", C.c_double * 6),
+        ("traction_seed", C.c_uint64),
+        ("traction_scale", C.c_double),
+        ("young_seed", C.c_uint64),
+        ("conductivity_seed", C.c_uint64),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("method", C.c_int32),
+        ("preconditioner", C.c_int32),
+        ("max_iterations", C.c_int32),
+        ("scaling", C.c_int32),
+        ("sc_precision", C.c_int32),
+        ("device", C.c_int32),
+        ("precision", C.c_double),
+        ("memory_budget_gb", C.c_double),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("n_lambda", C.c_int32),
+        ("n_coarse", C.c_int32),
+        ("n_subdomains", C.c_int32),
+        ("final_norm", C.c_double),
+        ("tol", C.c_double),
+        ("norm_b", C.c_double),
+        ("time_assembly", C.c_double),
+        ("time_coarse", C.c_double),
+        ("time_pcg", C.c_double),
+        ("time_primal", C.c_double),
+        ("time_total", C.c_double),
+        ("sc_bytes", C.c_double),
+        ("sd_bytes", C.c_double),
+        ("assembly_flops", C.c_double),
+        ("refinement_steps", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+# every entry point of include/espreso_amd.h with its ctypes signature
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int32)
+_H = C.c_void_p
+SIGNATURES = {
+    "espreso_last_error": (C.c_char_p, []),
+    "espreso_version": (C.c_int, []),
+    "espreso_grid_sizes": (C.c_int, [C.POINTER(GridProblem), _I, _I, _I]),
+    "espreso_grid_load": (C.c_int, [_H, C.POINTER(GridProblem)]),
+    "espreso_grid_l2g": (C.c_int, [_H, C.c_int32, _I]),
+    "espreso_grid_export": (C.c_int, [C.POINTER(GridProblem), C.c_int32, C.c_int32, _I, _I, _I, _D, _D, _D, _I, _I,
+                                      _D, _D, _D]),
+    "espreso_grid_nodal_solution": (C.c_int, [_H, _D]),
+    "espreso_feti_create": (C.c_int, [C.POINTER(_H), C.c_int32, C.POINTER(Options)]),
+    "espreso_feti_destroy": (C.c_int, [_H]),
+    "espreso_feti_sizes": (C.c_int, [_H, _I]),
+    "espreso_feti_set_subdomain": (C.c_int, [_H, C.c_int32, C.c_int32, _I, _I, _D, _D, C.c_int32, _D,
+                                             C.c_int32, _I, _I, _D, _D]),
+    "espreso_feti_set_dual": (C.c_int, [_H, C.c_int32, _D]),
+    "espreso_feti_setup": (C.c_int, [_H]),
+    "espreso_feti_solve": (C.c_int, [_H, C.POINTER(Stats)]),
+    "espreso_feti_primal": (C.c_int, [_H, C.c_int32, _D]),
+    "espreso_feti_dual": (C.c_int, [_H, _D]),
+    "espreso_feti_apply_F": (C.c_int, [_H, _D, _D]),
+    "espreso_feti_apply_prec": (C.c_int, [_H, _D, _D]),
+    "espreso_feti_project": (C.c_int, [_H, _D, _D]),
+    "espreso_feti_vec_b": (C.c_int, [_H, _D]),
+    "espreso_feti_subdomain_dense": (C.c_int, [_H, C.c_int32, C.c_int32, _D]),
+    "espreso_feti_subdomain_boundary": (C.c_int, [_H, C.c_int32, _I, _I]),
+    "espreso_feti_time_apply": (C.c_int, [_H, C.c_int32, C.c_int32, _D, _D]),
+    "espreso_debug_mfma": (C.c_int, [_D, _D, _D]),
+    "espreso_debug_tile_gemm": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _D, C.c_int32, _D,
+                                          C.c_int32, _D, C.c_int32, C.c_double, C.c_double]),
+}
+
+
+def debug_mfma(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.zeros(256)
+    _check(lib().espreso_debug_mfma(_dp(a), _dp(b), _dp(out)))
+    return out.reshape(64, 4)
+
+
+def debug_tile_gemm(A, B, C, ta=False, tb=False, alpha=1.0, beta=0.0):
+    """C = alpha op(A) op(B) + beta C on one 64x64 MFMA tile (column-major)."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    Cm = np.asfortranarray(C, dtype=np.float64).copy()
+    M, N = Cm.shape
+    K = A.shape[0] if ta else A.shape[1]
+    a = A.ravel(order="F")
+    b = B.ravel(order="F")
+    c = Cm.ravel(order="F")
+    _check(lib().espreso_debug_tile_gemm(int(ta), int(tb), M, N, K, _dp(a), A.shape[0], _dp(b), B.shape[0], _dp(c),
+                                         M, alpha, beta))
+    return c.reshape(N, M).T
+
+_lib = None
+
+
+def lib():
+    """Load libespreso_amd.so (raises LibraryNotBuilt when absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryNotBuilt(f"{LIB_PATH} is missing: run __graft_entry__.build() (make -C espreso_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(f"espreso_amd error {rc}: {lib().espreso_last_error().decode()}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I)
+
+
+def grid_problem(spec):
+    p = GridProblem()
+    p.physics = spec.get("physics", 0)
+    for k in ("clusters", "domains", "elements"):
+        getattr(p, k)[:] = spec[k]
+    p.length[:] = spec.get("length", (1.0, 1.0, 1.0))
+    p.young = spec.get("young", 2.1e11)
+    p.poisson = spec.get("poisson", 0.3)
+    p.density = spec.get("density", 7850.0)
+    p.accel[:] = spec.get("accel", (0.0, 0.0, 0.0))
+    p.conductivity = spec.get("conductivity", 1.0)
+    p.heat_source = spec.get("heat_source", 0.0)
+    p.fixed[:] = spec.get("fixed", (0,) * 6)
+    p.fixed_value[:] = spec.get("fixed_value", (0.0,) * 6)
+    p.pressure[:] = spec.get("pressure", (0.0,) * 6)
+    p.traction_seed = spec.get("traction_seed", 0)
+    p.traction_scale = spec.get("traction_scale", 1.0)
+    p.young_seed = spec.get("young_seed", 0)
+    p.conductivity_seed = spec.get("conductivity_seed", 0)
+    return p
+
+
+def make_options(preconditioner="dirichlet", precision=1e-8, max_iterations=500, scaling=1, sc_precision="fp64",
+                 device=0, memory_budget_gb=0.0):
+    o = Options()
+    o.method = 0
+    o.preconditioner = PREC[preconditioner] if isinstance(preconditioner, str) else int(preconditioner)
+    o.max_iterations = max_iterations
+    o.scaling = scaling
+    o.sc_precision = SC[sc_precision] if isinstance(sc_precision, str) else int(sc_precision)
+    o.device = device
+    o.precision = precision
+    o.memory_budget_gb = memory_budget_gb
+    return o
+
+
+def grid_export(spec, i, scaling=1):
+    """Host-only export of subdomain i of the structured-grid builder."""
+    p = grid_problem(spec)
+    sz = np.zeros(6, dtype=np.int32)
+    nul = None
+    _check(lib().espreso_grid_export(C.byref(p), scaling, i, _ip(sz), nul, nul, nul, nul, nul, nul, nul, nul, nul,
+                                     nul))
+    n, nnz, k, nl, ntot = (int(v) for v in sz[:5])
+    rp = np.zeros(n + 1, np.int32)
+    ci = np.zeros(nnz, np.int32)
+    kv = np.zeros(nnz)
+    f = np.zeros(n)
+    R = np.zeros(n * k)
+    lam = np.zeros(nl, np.int32)
+    dof = np.zeros(nl, np.int32)
+    sign = np.zeros(nl)
+    scale = np.zeros(nl)
+    c = np.zeros(ntot)
+    _check(lib().espreso_grid_export(C.byref(p), scaling, i, _ip(sz), _ip(rp), _ip(ci), _dp(kv), _dp(f), _dp(R),
+                                     _ip(lam), _ip(dof), _dp(sign), _dp(scale), _dp(c)))
+    return dict(n=n, rowptr=rp, colidx=ci, vals=kv, f=f, R=R.reshape(k, n).T, lam=lam, dof=dof, sign=sign,
+                scale=scale, c=c, n_sub=int(sz[5]))
+
+
+class FETISolver:
+    """One cluster of subdomains on one GPU (reference: FETISolver)."""
+
+    def __init__(self, n_subdomains, **opts):
+        self.n_sub = n_subdomains
+        self.opts = make_options(**opts)
+        h = C.c_void_p()
+        _check(lib().espreso_feti_create(C.byref(h), n_subdomains, C.byref(self.opts)))
+        self.h = h
+        self.n_lambda = None
+        self.stats = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().espreso_feti_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- LinearSolver::update --
+    def set_subdomain(self, i, K_csr, f, R, lam, dof, sign, scale):
+        rp, ci, kv = (np.ascontiguousarray(a) for a in K_csr)
+        rp = rp.astype(np.int32)
+        ci = ci.astype(np.int32)
+        kv = kv.astype(np.float64)
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        R = np.asfortranarray(R, dtype=np.float64)
+        lam = np.ascontiguousarray(lam, dtype=np.int32)
+        dof = np.ascontiguousarray(dof, dtype=np.int32)
+        sign = np.ascontiguousarray(sign, dtype=np.float64)
+        scale = np.ascontiguousarray(scale, dtype=np.float64)
+        n = len(f)
+        Rf = np.ascontiguousarray(R.T).ravel()  # column-major n x k
+        _check(lib().espreso_feti_set_subdomain(self.h, i, n, _ip(rp), _ip(ci), _dp(kv), _dp(f), R.shape[1], _dp(Rf),
+                                                len(lam), _ip(lam), _ip(dof), _dp(sign), _dp(scale)))
+
+    def set_dual(self, c):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        self.n_lambda = len(c)
+        _check(lib().espreso_feti_set_dual(self.h, len(c), _dp(c)))
+
+    def setup(self):
+        _check(lib().espreso_feti_setup(self.h))
+
+    # -- LinearSolver::solve --
+    def solve(self):
+        st = Stats()
+        _check(lib().espreso_feti_solve(self.h, C.byref(st)))
+        self.stats = st.as_dict()
+        return self.stats
+
+    def primal(self, i, n):
+        u = np.zeros(n)
+        _check(lib().espreso_feti_primal(self.h, i, _dp(u)))
+        return u
+
+    def dual(self):
+        lam = np.zeros(self.n_lambda)
+        _check(lib().espreso_feti_dual(self.h, _dp(lam)))
+        return lam
+
+    def _op(self, name, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n_lambda)
+        _check(getattr(lib(), name)(self.h, _dp(x), _dp(y)))
+        return y
+
+    def apply_F(self, x):
+        return self._op("espreso_feti_apply_F", x)
+
+    def apply_prec(self, x):
+        return self._op("espreso_feti_apply_prec", x)
+
+    def project(self, x):
+        return self._op("espreso_feti_project", x)
+
+    def vec_b(self):
+        b = np.zeros(self.n_lambda)
+        _check(lib().espreso_feti_vec_b(self.h, _dp(b)))
+        return b
+
+    def boundary(self, i):
+        m = C.c_int32()
+        _check(lib().espreso_feti_subdomain_boundary(self.h, i, C.byref(m), None))
+        b = np.zeros(m.value, dtype=np.int32)
+        _check(lib().espreso_feti_subdomain_boundary(self.h, i, C.byref(m), _ip(b)))
+        return b
+
+    def dense(self, i, which):
+        m = len(self.boundary(i))
+        out = np.zeros(m * m)
+        _check(lib().espreso_feti_subdomain_dense(self.h, i, which, _dp(out)))
+        return out.reshape(m, m).T.copy()
+
+    def sizes(self):
+        s = np.zeros(3, dtype=np.int32)
+        _check(lib().espreso_feti_sizes(self.h, _ip(s)))
+        return [int(v) for v in s]
+
+    def time_apply(self, op=0, reps=10):
+        a = C.c_double()
+        s = C.c_double()
+        _check(lib().espreso_feti_time_apply(self.h, op, reps, C.byref(a), C.byref(s)))
+        return a.value, s.value
+
+
+class GridFETI(FETISolver):
+    """FETISolver fed by the structured-grid builder."""
+
+    def __init__(self, spec, **opts):
+        self.spec = spec
+        self.problem = grid_problem(spec)
+        ns, nn, nd = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().espreso_grid_sizes(C.byref(self.problem), C.byref(ns), C.byref(nn), C.byref(nd)))
+        self.n_nodes, self.ndof = nn.value, nd.value
+        super().__init__(ns.value, **opts)
+        _check(lib().espreso_grid_load(self.h, C.byref(self.problem)))
+        self.n_lambda = self.sizes()[1]
+
+    def nodal_solution(self):
+        u = np.zeros(self.n_nodes * self.ndof)
+        _check(lib().espreso_grid_nodal_solution(self.h, _dp(u)))
+        return u.reshape(self.n_nodes, self.ndof)
+
+    def l2g(self, i):
+        n = (self.spec["elements"][0] + 1) * (self.spec["elements"][1] + 1) * (self.spec["elements"][2] + 1)
+        g = np.zeros(n, dtype=np.int32)
+        _check(lib().espreso_grid_l2g(self.h, i, _ip(g)))
+        return g

@@ -1,0 +1,431 @@
+// Dual-space kernels of the FETI iteration: the dual operator
+// F x = sum_i B_i F_i B_i^T x (itersolvercpu.cpp:9-114), the preconditioners
+// (itersolvercpu.cpp:520-602), the projector P = I - G^T (G G^T)^-1 G
+// (itersolver.cpp:5541-5623), PCG vector updates and dot products
+// (itersolver.cpp:1537-1684, 6427-6467), and the primal recovery
+// (itersolver.cpp:372-431).
+//
+// F_i and S_D live in HBM in the packed-strip layout (see internal.h): one
+// pass over the upper triangle applies both triangles.  A workgroup owns a
+// contiguous range of strips of one subdomain; B_i^T x is gathered straight
+// from lambda space into LDS, row and column contributions accumulate in LDS,
+// and each workgroup writes one partial vector that a second pass sums and
+// scatters back to lambda space in a fixed order (deterministic, no global
+// atomics).
+#include <hip/hip_runtime.h>
+
+#include "dual.h"
+
+namespace esp {
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Acc { typedef float type; };
+template <>
+struct Acc<double> { typedef double type; };
+
+template <typename T, typename A>
+__device__ __forceinline__ A load_cvt(T v) { return (A)v; }
+template <>
+__device__ __forceinline__ float load_cvt<uint16_t, float>(uint16_t v) { return __uint_as_float(((uint32_t)v) << 16); }
+
+// transpose-reduce R per-lane accumulators over the 64 lanes of a wave; on
+// return lane l holds the total of row (l / (64/R)) in v[0]
+template <int R, typename A>
+__device__ __forceinline__ A wave_rows_reduce(A *v, int lane) {
+    int n = R;
+#pragma unroll
+    for (int o = 32; o >= 64 / R; o >>= 1) {
+        const bool up = lane & o;
+        const int h = n >> 1;
+#pragma unroll
+        for (int i = 0; i < h; i++) {
+            A send = up ? v[i] : v[i + h];
+            A keep = up ? v[i + h] : v[i];
+            v[i] = keep + __shfl_xor(send, o, 64);
+        }
+        n = h;
+    }
+    A s = v[0];
+#pragma unroll
+    for (int o = 64 / R / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    return s;
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(256) k_symv(const SymvItem *__restrict__ items, const int *__restrict__ M,
+                                              const int64_t *__restrict__ boff, const T *__restrict__ pool,
+                                              const int64_t *__restrict__ poff, const float *__restrict__ fscale,
+                                              const int64_t *__restrict__ bl_ptr, const int *__restrict__ bl_lam,
+                                              const double *__restrict__ coef, const double *__restrict__ x,
+                                              int gather, double *__restrict__ ypart) {
+    typedef typename Acc<T>::type A;
+    extern __shared__ unsigned char smem[];
+    const SymvItem it = items[blockIdx.x];
+    const int s = it.sub, m = M[s];
+    const int j0 = it.t0 * R, len = m - j0;
+    A *xs = (A *)smem;
+    A *ys = xs + len;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t b0 = boff[s] + j0;
+    for (int j = tid; j < len; j += 256) {
+        double v;
+        if (gather) {
+            v = 0.0;
+            for (int64_t q = bl_ptr[b0 + j]; q < bl_ptr[b0 + j + 1]; q++) v += coef[q] * x[bl_lam[q]];
+        } else {
+            v = x[b0 + j];
+        }
+        xs[j] = (A)v;
+        ys[j] = (A)0;
+    }
+    __syncthreads();
+    const T *base = pool + poff[s];
+    for (int t = it.t0 + wave; t < it.t1; t += 4) {
+        const int r0 = t * R;
+        const int ncol = m - r0;
+        const T *S = base + strip_offset(m, R, t);
+        A xr[R], racc[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            xr[r] = (r0 + r < m) ? xs[r0 + r - j0] : (A)0;
+            racc[r] = (A)0;
+        }
+        for (int c = lane; c < ncol; c += 64) {
+            const u32x4 *src = (const u32x4 *)(S + (int64_t)c * R);
+            constexpr int NV = R * (int)sizeof(T) / 16;
+            u32x4 raw[NV];
+#pragma unroll
+            for (int q = 0; q < NV; q++) raw[q] = __builtin_nontemporal_load(src + q);
+            const T *a = (const T *)raw;
+            const A xc = xs[r0 + c - j0];
+            A col = (A)0;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const A av = load_cvt<T, A>(a[r]);
+                col = fma(av, xr[r], col);
+                racc[r] = fma(av, xc, racc[r]);
+            }
+            atomicAdd(&ys[r0 + c - j0], col);
+        }
+        const A rs = wave_rows_reduce<R, A>(racc, lane);
+        const int row = lane / (64 / R);
+        if ((lane & (64 / R - 1)) == 0 && r0 + row < m) atomicAdd(&ys[r0 + row - j0], rs);
+    }
+    __syncthreads();
+    const double sc = fscale ? (double)fscale[s] : 1.0;
+    for (int j = tid; j < len; j += 256) ypart[it.part_off + j] = (double)ys[j] * sc;
+}
+
+// yb[boff+j] = sum of the partials of the items of subdomain s covering j
+__global__ void k_part_reduce(const SymvItem *items, const int *item0, const int *nitems, const int *M,
+                              const int64_t *boff, int R, const double *ypart, double *yb) {
+    const int s = blockIdx.x;
+    const int m = M[s];
+    const int i0 = item0[s], ni = nitems[s];
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+        double v = 0.0;
+        for (int q = 0; q < ni; q++) {
+            const SymvItem it = items[i0 + q];
+            const int j0 = it.t0 * R;
+            if (j >= j0) v += ypart[it.part_off + j - j0];
+        }
+        yb[boff[s] + j] = v;
+    }
+}
+
+// lambda-space y = sum_e coef_e * yb[b_e]
+__global__ void k_lambda_gather(int nlam, const int *cnt, const int64_t *lb, const double *coef, const double *yb,
+                                double *y) {
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < nlam; l += gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int e = 0; e < cnt[l]; e++) v += coef[2 * l + e] * yb[lb[2 * l + e]];
+        y[l] = v;
+    }
+}
+
+// boundary gather xb = B^T x with coefficients
+__global__ void k_boundary_gather(int64_t nb, const int64_t *bl_ptr, const int *bl_lam, const double *coef,
+                                  const double *x, double *xb) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nb; j += (int64_t)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int64_t q = bl_ptr[j]; q < bl_ptr[j + 1]; q++) v += coef[q] * x[bl_lam[q]];
+        xb[j] = v;
+    }
+}
+
+// boundary SpMV with K_bb (lumped preconditioner): yb = K_bb xb
+__global__ void k_kbb(int64_t nb, const int64_t *ptr, const int64_t *col, const double *val, const double *xb,
+                      double *yb) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nb; j += (int64_t)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int64_t q = ptr[j]; q < ptr[j + 1]; q++) v += val[q] * xb[col[q]];
+        yb[j] = v;
+    }
+}
+
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double r = 0.0;
+    const int nw = blockDim.x >> 6;
+    for (int w = 0; w < nw; w++) r += red[w];
+    return r;
+}
+
+// d = G x, G = -(B R)^T: one workgroup per subdomain
+__global__ void __launch_bounds__(256) k_apply_G(const int *M, const int64_t *boff, const int *kdim, const int *coff,
+                                                 const int64_t *rboff, const double *Rb, const int64_t *bl_ptr,
+                                                 const int *bl_lam, const double *sign, const double *x, double *d) {
+    __shared__ double red[4];
+    extern __shared__ double xbs[];
+    const int s = blockIdx.x, m = M[s], k = kdim[s];
+    const int64_t b0 = boff[s];
+    for (int j = threadIdx.x; j < m; j += 256) {
+        double v = 0.0;
+        for (int64_t q = bl_ptr[b0 + j]; q < bl_ptr[b0 + j + 1]; q++) v += sign[q] * x[bl_lam[q]];
+        xbs[j] = v;
+    }
+    __syncthreads();
+    const double *R = Rb + rboff[s];
+    for (int a = 0; a < k; a++) {
+        double v = 0.0;
+        for (int j = threadIdx.x; j < m; j += 256) v += R[j + (int64_t)a * m] * xbs[j];
+        v = block_sum(v, red);
+        if (threadIdx.x == 0) d[coff[s] + a] = -v;
+    }
+}
+
+// e = GGinv d (symmetric, column access)
+__global__ void __launch_bounds__(256) k_coarse_gemv(int nc, const double *Ginv, const double *d, double *e) {
+    __shared__ double red[4];
+    const int i = blockIdx.x;
+    const double *col = Ginv + (int64_t)i * nc;
+    double v = 0.0;
+    for (int j = threadIdx.x; j < nc; j += 256) v += col[j] * d[j];
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) e[i] = v;
+}
+
+// mode 0: y = x - G^T e ; mode 1: y = G^T e
+__global__ void k_project_finish(int nlam, const int *cnt, const int64_t *lb, const int *lsub, const double *sign,
+                                 const int64_t *boff, const int *M, const int *kdim, const int *coff,
+                                 const int64_t *rboff, const double *Rb, const double *e, const double *x, double *y,
+                                 int mode) {
+    for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < nlam; l += gridDim.x * blockDim.x) {
+        double g = 0.0;
+        for (int q = 0; q < cnt[l]; q++) {
+            const int s = lsub[2 * l + q];
+            const int64_t j = lb[2 * l + q] - boff[s];
+            const int m = M[s];
+            const double *R = Rb + rboff[s];
+            const double *es = e + coff[s];
+            double acc = 0.0;
+            for (int a = 0; a < kdim[s]; a++) acc += R[j + (int64_t)a * m] * es[a];
+            g -= sign[2 * l + q] * acc;
+        }
+        y[l] = mode == 0 ? x[l] - g : g;
+    }
+}
+
+// ---- vector kernels ----
+__global__ void __launch_bounds__(256) k_dot_partial(const double *a, const double *b, int64_t n, double *partial) {
+    __shared__ double red[4];
+    double v = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v += a[i] * b[i];
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+__global__ void __launch_bounds__(256) k_dot_final(const double *partial, int np, double *out) {
+    __shared__ double red[4];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) v += partial[i];
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) *out = v;
+}
+
+// p = y + beta p with beta = num/den (first: p = y)
+__global__ void k_p_update(double *p, const double *y, int64_t n, const double *num, const double *den, int first) {
+    const double beta = first ? 0.0 : (*num) / (*den);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = first ? y[i] : y[i] + beta * p[i];
+}
+// alpha = num/den; x += alpha p; r -= alpha Ap
+__global__ void k_xr_update(double *x, double *r, const double *p, const double *Ap, int64_t n, const double *num,
+                            const double *den) {
+    const double alpha = (*num) / (*den);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        r[i] -= alpha * Ap[i];
+    }
+}
+__global__ void k_axpby(double *y, const double *a, const double *b, int64_t n, double alpha, double beta) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = alpha * a[i] + beta * b[i];
+}
+__global__ void k_copy_scalar(const double *src, double *dst) { *dst = *src; }
+
+__global__ void k_symmetrize(double *A, int n) {
+    const int64_t nn = (int64_t)n * n;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nn; e += (int64_t)gridDim.x * blockDim.x) {
+        int i = (int)(e % n), j = (int)(e / n);
+        if (i < j) A[e] = A[j + (int64_t)i * n];
+    }
+}
+
+// ---- primal recovery: u_int = K_ii^-1 (f_int - K_ib u_b) via the banded factor ----
+// one workgroup per subdomain; work holds ni values per subdomain
+__global__ void __launch_bounds__(256) k_primal_interior(const PrimalDev P, const double *ub, double *u) {
+    __shared__ double red[4];
+    __shared__ double piv;
+    const int s = blockIdx.x;
+    const int ni = P.ni[s], m = P.m[s], n = P.n[s], bw = P.bw[s];
+    const int w = bw + 1;
+    const double *L = P.band + P.band_off[s];
+    const double *yf = P.yf + P.yf_off[s];
+    const int *perm = P.perm + P.perm_off[s];
+    const int *iperm = P.iperm + P.perm_off[s];
+    const int *rp = P.rowptr + P.rp_off[s];
+    const int *ci = P.colidx + P.nz_off[s];
+    const double *kv = P.vals + P.nz_off[s];
+    const double *ubs = ub + P.boff[s];
+    double *z = P.work + P.work_off[s];
+    double *us = u + P.perm_off[s];
+    // z = K_ib u_b (interior rows)
+    for (int r = threadIdx.x; r < ni; r += 256) {
+        const int lr = perm[r];
+        double v = 0.0;
+        for (int q = rp[lr]; q < rp[lr + 1]; q++) {
+            const int pc = iperm[ci[q]];
+            if (pc >= ni) v += kv[q] * ubs[pc - ni];
+        }
+        z[r] = v;
+    }
+    __syncthreads();
+    // z <- L^-1 z (column sweep)
+    for (int c = 0; c < ni; c++) {
+        if (threadIdx.x == 0) {
+            z[c] /= L[(int64_t)c * w];
+            piv = z[c];
+        }
+        __syncthreads();
+        const double zc = piv;
+        for (int d = 1 + threadIdx.x; d <= bw && c + d < ni; d += 256) z[c + d] -= L[(int64_t)c * w + d] * zc;
+        __syncthreads();
+    }
+    // z <- y_f - z
+    for (int r = threadIdx.x; r < ni; r += 256) z[r] = yf[r] - z[r];
+    __syncthreads();
+    // z <- L^-T z (row sweep from the bottom)
+    for (int c = ni - 1; c >= 0; c--) {
+        double v = 0.0;
+        for (int d = 1 + threadIdx.x; d <= bw && c + d < ni; d += 256) v += L[(int64_t)c * w + d] * z[c + d];
+        v = block_sum(v, red);
+        if (threadIdx.x == 0) z[c] = (z[c] - v) / L[(int64_t)c * w];
+        __syncthreads();
+    }
+    // scatter into local DOF order and add R alpha
+    const double *R = P.R + P.r_off[s];
+    const double *al = P.alpha + P.coff[s];
+    const int k = P.kdim[s];
+    for (int r = threadIdx.x; r < n; r += 256) {
+        const int lr = perm[r];
+        double v = r < ni ? z[r] : ubs[r - ni];
+        for (int a = 0; a < k; a++) v += R[lr + (int64_t)a * n] * al[a];
+        us[lr] = v;
+    }
+}
+
+// ------------------------------------------------------------ launchers ----
+static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+static inline unsigned vgrid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, 256), 4096)); }
+
+void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gather, bool scaled, hipStream_t st) {
+    if (sd) prec = 0;
+    const bool lp = prec != 0;
+    const SymvItem *items = lp ? D.items_lp : D.items_d;
+    const int n_items = lp ? D.n_items_lp : D.n_items_d;
+    const double *coef = scaled ? D.bl_sscale : D.bl_sign;
+    const size_t shm = 2 * (size_t)D.max_m * (lp ? 4 : 8);
+    if (prec == 0) {
+        const double *pool = sd ? D.SD : D.Fd;
+        const int64_t *off = sd ? D.sdoff : D.foff;
+        hipLaunchKernelGGL((k_symv<double, STRIP_F64>), dim3(n_items), dim3(256), shm, st, items, D.m, D.boff, pool,
+                           off, (const float *)nullptr, D.bl_ptr, D.bl_lam, coef, x, gather, D.ypart);
+    } else if (prec == 1) {
+        hipLaunchKernelGGL((k_symv<float, STRIP_F32>), dim3(n_items), dim3(256), shm, st, items, D.m, D.boff, D.Ff,
+                           D.foff_lp, (const float *)nullptr, D.bl_ptr, D.bl_lam, coef, x, gather, D.ypart);
+    } else {
+        hipLaunchKernelGGL((k_symv<uint16_t, STRIP_B16>), dim3(n_items), dim3(256), shm, st, items, D.m, D.boff, D.Fh,
+                           D.foff_lp, (const float *)D.fscale, D.bl_ptr, D.bl_lam, coef, x, gather, D.ypart);
+    }
+    const int R = strip_rows(prec);
+    hipLaunchKernelGGL(k_part_reduce, dim3(D.nsub), dim3(256), 0, st, items, lp ? D.item0_lp : D.item0_d,
+                       lp ? D.nitems_lp : D.nitems_d, D.m, D.boff, R, D.ypart, D.yb);
+}
+
+void launch_lambda_gather(const DualDev &D, const double *yb, bool scaled, double *y, hipStream_t st) {
+    hipLaunchKernelGGL(k_lambda_gather, dim3(vgrid(D.nlam)), dim3(256), 0, st, D.nlam, D.le_cnt, D.le_b,
+                       scaled ? D.le_sscale : D.le_sign, yb, y);
+}
+
+void launch_boundary_gather(const DualDev &D, const double *x, bool scaled, double *xb, hipStream_t st) {
+    hipLaunchKernelGGL(k_boundary_gather, dim3(vgrid(D.nbnd)), dim3(256), 0, st, D.nbnd, D.bl_ptr, D.bl_lam,
+                       scaled ? D.bl_sscale : D.bl_sign, x, xb);
+}
+
+void launch_kbb(const DualDev &D, const double *xb, double *yb, hipStream_t st) {
+    hipLaunchKernelGGL(k_kbb, dim3(vgrid(D.nbnd)), dim3(256), 0, st, D.nbnd, D.kbb_ptr, D.kbb_col, D.kbb_val, xb, yb);
+}
+
+void launch_project(const DualDev &D, const double *x, double *y, int mode, hipStream_t st) {
+    // mode 0: y = P x;  mode 1: y = G^T GGinv x (x coarse);  mode 2: y = GGinv G x (coarse out)
+    if (mode != 1)
+        hipLaunchKernelGGL(k_apply_G, dim3(D.nsub), dim3(256), (size_t)D.max_m * 8, st, D.m, D.boff, D.kdim, D.coff,
+                           D.rboff, D.Rb, D.bl_ptr, D.bl_lam, D.bl_sign, x, D.cd);
+    const double *din = mode == 1 ? x : D.cd;
+    double *eout = mode == 2 ? y : D.ce;
+    hipLaunchKernelGGL(k_coarse_gemv, dim3(D.ncoarse), dim3(256), 0, st, D.ncoarse, D.GGinv, din, eout);
+    if (mode == 2) return;
+    hipLaunchKernelGGL(k_project_finish, dim3(vgrid(D.nlam)), dim3(256), 0, st, D.nlam, D.le_cnt, D.le_b, D.le_sub,
+                       D.le_sign, D.boff, D.m, D.kdim, D.coff, D.rboff, D.Rb, D.ce, x, y, mode);
+}
+
+void launch_dot(const double *a, const double *b, int64_t n, double *partial, double *out, hipStream_t st) {
+    const int np = 512;
+    hipLaunchKernelGGL(k_dot_partial, dim3(np), dim3(256), 0, st, a, b, n, partial);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, st, partial, np, out);
+}
+
+void launch_p_update(double *p, const double *y, int64_t n, const double *num, const double *den, int first,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_p_update, dim3(vgrid(n)), dim3(256), 0, st, p, y, n, num, den, first);
+}
+
+void launch_xr_update(double *x, double *r, const double *p, const double *Ap, int64_t n, const double *num,
+                      const double *den, hipStream_t st) {
+    hipLaunchKernelGGL(k_xr_update, dim3(vgrid(n)), dim3(256), 0, st, x, r, p, Ap, n, num, den);
+}
+
+void launch_axpby(double *y, const double *a, const double *b, int64_t n, double alpha, double beta, hipStream_t st) {
+    hipLaunchKernelGGL(k_axpby, dim3(vgrid(n)), dim3(256), 0, st, y, a, b, n, alpha, beta);
+}
+
+void launch_copy_scalar(const double *src, double *dst, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy_scalar, dim3(1), dim3(1), 0, st, src, dst);
+}
+
+void launch_symmetrize(double *A, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_symmetrize, dim3(vgrid((int64_t)n * n)), dim3(256), 0, st, A, n);
+}
+
+void launch_primal(const PrimalDev &P, int nsub, const double *ub, double *u, hipStream_t st) {
+    hipLaunchKernelGGL(k_primal_interior, dim3(nsub), dim3(256), 0, st, P, ub, u);
+}
+
+}  // namespace esp

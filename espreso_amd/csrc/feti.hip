@@ -1,0 +1,1160 @@
+// Host driver of the MI355X FETI core and its C ABI (include/espreso_amd.h).
+//
+// Mirrors the reference's setup order (FETISolver::init, FETISolver.cpp:668-735):
+// subdomain data -> lambda maps (cluster.cpp:33-220) -> coarse problem
+// (itersolver.cpp:4931-5121) -> Dirichlet preconditioner and local dual
+// operators (clustercpu.cpp:11-40, 167-333) -> projected PCG
+// (itersolver.cpp:1406-1724) -> primal recovery (itersolver.cpp:372-431).
+// Everything after the host preprocessing runs on one GPU; the host only
+// issues launches and reads back the PCG residual norm.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+
+#include "dual.h"
+#include "grid.h"
+#include "internal.h"
+
+using namespace esp;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+double wall() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct SubHost {
+    int n = 0, ni = 0, m = 0, kdim = 0;
+    std::vector<int> rowptr, colidx;
+    std::vector<double> vals, f, R;
+    std::vector<int> lam, dof;
+    std::vector<double> sign, scale;
+    // derived
+    std::vector<int> perm, iperm, bdofs, colstart;
+    int band_tiles = 0, bw = 0;
+    bool set = false;
+};
+
+// orthonormalise the k columns of V (n x k); false when they are dependent
+bool mgs(int n, int k, double *V) {
+    for (int pass = 0; pass < 2; pass++)
+        for (int j = 0; j < k; j++) {
+            double *v = V + (size_t)j * n;
+            for (int i = 0; i < j; i++) {
+                const double *u = V + (size_t)i * n;
+                double d = 0;
+                for (int r = 0; r < n; r++) d += u[r] * v[r];
+                for (int r = 0; r < n; r++) v[r] -= d * u[r];
+            }
+            double nr = 0;
+            for (int r = 0; r < n; r++) nr += v[r] * v[r];
+            if (!(nr > 1e-24)) return false;
+            nr = 1.0 / std::sqrt(nr);
+            for (int r = 0; r < n; r++) v[r] *= nr;
+        }
+    return true;
+}
+
+}  // namespace
+
+struct espreso_feti_s {
+    espreso_feti_options opt{};
+    int nsub = 0, nlam = -1, ncoarse = 0;
+    std::vector<SubHost> sub;
+    std::vector<double> c;
+    hipStream_t st = nullptr;
+    std::vector<void *> allocs;
+    bool ready = false, solved = false;
+    // grid bookkeeping
+    int ndof = 0, nnodes = 0;
+    std::vector<std::vector<int>> l2g;
+    // device
+    DualDev D;
+    PrimalDev P;
+    double *d_hf = nullptr, *d_ubf = nullptr, *d_fb = nullptr;  // boundary arrays
+    double *d_x = nullptr, *d_r = nullptr, *d_w = nullptr, *d_y = nullptr, *d_z = nullptr, *d_p = nullptr,
+           *d_Ap = nullptr, *d_b = nullptr, *d_tmp = nullptr, *d_r2 = nullptr, *d_partial = nullptr,
+           *d_scal = nullptr;
+    double *d_u = nullptr;    // per-subdomain primal (local order), sum n
+    double *d_e = nullptr;    // -R^T f
+    std::vector<int64_t> uoff;
+    std::vector<double> lambda_h;
+    std::vector<double> u_h;
+    espreso_feti_stats last{};
+    double fsc_bytes = 0, sd_bytes = 0, asm_flops = 0;
+
+    ~espreso_feti_s() {
+        for (void *p : allocs) (void)hipFree(p);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    template <class T>
+    T *alloc(size_t n) {
+        void *p = nullptr;
+        if (n == 0) n = 1;
+        ESP_HIP(hipMalloc(&p, n * sizeof(T)));
+        allocs.push_back(p);
+        return (T *)p;
+    }
+    template <class T>
+    T *upload(const std::vector<T> &v) {
+        T *p = alloc<T>(v.size());
+        if (!v.empty()) ESP_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        return p;
+    }
+    void release(void *p) {
+        auto it = std::find(allocs.begin(), allocs.end(), p);
+        if (it != allocs.end()) {
+            (void)hipFree(p);
+            allocs.erase(it);
+        }
+    }
+    void sync() { ESP_HIP(hipStreamSynchronize(st)); }
+
+    void setup();
+    void assemble(const double *d_f, const double *d_Q, const std::vector<int64_t> &foff,
+                  const std::vector<int64_t> &lpoff, const std::vector<int64_t> &boff);
+    void coarse(const std::vector<double> &GGt);
+    void build_items(int prec, std::vector<SymvItem> &items, std::vector<int> &item0, std::vector<int> &nit,
+                     int64_t &plen);
+    // dual operators on device vectors
+    void apply_F(const double *x, double *y, int prec);
+    void apply_prec(const double *x, double *y);
+    void solve(espreso_feti_stats *st_out);
+    void pcg(const double *b, double *x, double *r, int prec, int &iters, double &norm, double &tol, bool quiet_tol,
+             double tol_abs);
+};
+
+// ----------------------------------------------------------------- setup ----
+void espreso_feti_s::setup() {
+    ESP_CHECK(nlam >= 0, ESPRESO_ERR_STATE, "espreso_feti_set_dual not called");
+    for (int s = 0; s < nsub; s++) ESP_CHECK(sub[s].set, ESPRESO_ERR_STATE, "subdomain " + std::to_string(s) + " not set");
+    double t0 = wall();
+    const int prec = opt.sc_precision;
+    ESP_CHECK(prec >= 0 && prec <= 2, ESPRESO_ERR_ARG, "sc_precision must be 0, 1 or 2");
+
+    // ---- host preprocessing: boundary sets, permutations, bands ----
+    std::vector<int64_t> boff(nsub + 1, 0), coffv(nsub + 1, 0);
+#pragma omp parallel for schedule(dynamic)
+    for (int s = 0; s < nsub; s++) {
+        SubHost &h = sub[s];
+        std::vector<int> b(h.dof);
+        std::sort(b.begin(), b.end());
+        b.erase(std::unique(b.begin(), b.end()), b.end());
+        h.bdofs = b;
+        h.m = (int)b.size();
+        h.ni = h.n - h.m;
+        h.iperm.assign(h.n, -1);
+        for (int j = 0; j < h.m; j++) h.iperm[b[j]] = h.ni + j;
+        h.perm.assign(h.n, 0);
+        int q = 0;
+        for (int i = 0; i < h.n; i++)
+            if (h.iperm[i] < 0) { h.iperm[i] = q; h.perm[q++] = i; }
+        for (int j = 0; j < h.m; j++) h.perm[h.ni + j] = b[j];
+        // band of K_ii and first coupled interior row of each K_ib column
+        int bw = 0, bt = 0;
+        std::vector<int> first_int(h.m + 1, h.ni);
+        first_int[h.m] = 0;  // the f column is dense
+        for (int r = 0; r < h.ni; r++) {
+            const int lr = h.perm[r];
+            int fr = r;
+            for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) {
+                const int pc = h.iperm[h.colidx[k]];
+                if (pc < h.ni) fr = std::min(fr, pc);
+                else first_int[pc - h.ni] = std::min(first_int[pc - h.ni], r);
+            }
+            bw = std::max(bw, r - fr);
+            bt = std::max(bt, r / TILE - fr / TILE);
+        }
+        h.bw = bw;
+        h.band_tiles = std::max(bt, 1);
+        const int ntc = (h.m + 1 + TILE - 1) / TILE;
+        h.colstart.assign(ntc, 0);
+        for (int J = 0; J < ntc; J++) {
+            int mn = h.ni;
+            for (int j = J * TILE; j < std::min(h.m + 1, (J + 1) * TILE); j++) mn = std::min(mn, first_int[j]);
+            h.colstart[J] = mn / TILE;
+        }
+    }
+    int max_m = 0;
+    for (int s = 0; s < nsub; s++) {
+        boff[s + 1] = boff[s] + sub[s].m;
+        coffv[s + 1] = coffv[s] + sub[s].kdim;
+        max_m = std::max(max_m, sub[s].m);
+    }
+    ncoarse = (int)coffv[nsub];
+    const int64_t nb = boff[nsub];
+
+    // boundary -> lambda CSR and lambda -> entries
+    std::vector<int64_t> bl_ptr(nb + 1, 0);
+    for (int s = 0; s < nsub; s++)
+        for (size_t q = 0; q < sub[s].lam.size(); q++) bl_ptr[boff[s] + sub[s].iperm[sub[s].dof[q]] - sub[s].ni + 1]++;
+    for (int64_t j = 0; j < nb; j++) bl_ptr[j + 1] += bl_ptr[j];
+    std::vector<int> bl_lam(bl_ptr[nb]);
+    std::vector<double> bl_sign(bl_ptr[nb]), bl_ss(bl_ptr[nb]);
+    std::vector<int64_t> fill(bl_ptr.begin(), bl_ptr.end() - 1);
+    std::vector<int> le_cnt(nlam, 0), le_sub(2 * (size_t)nlam, 0);
+    std::vector<int64_t> le_b(2 * (size_t)nlam, 0);
+    std::vector<double> le_sign(2 * (size_t)nlam, 0.0), le_ss(2 * (size_t)nlam, 0.0);
+    for (int s = 0; s < nsub; s++) {
+        const SubHost &h = sub[s];
+        for (size_t q = 0; q < h.lam.size(); q++) {
+            const int l = h.lam[q];
+            ESP_CHECK(l >= 0 && l < nlam, ESPRESO_ERR_ARG, "lambda id out of range");
+            const int64_t bj = boff[s] + h.iperm[h.dof[q]] - h.ni;
+            const int64_t k = fill[bj]++;
+            bl_lam[k] = l;
+            bl_sign[k] = h.sign[q];
+            bl_ss[k] = h.sign[q] * h.scale[q];
+            ESP_CHECK(le_cnt[l] < 2, ESPRESO_ERR_ARG, "a lambda row may touch at most two subdomain DOFs");
+            const int e = le_cnt[l]++;
+            le_b[2 * (size_t)l + e] = bj;
+            le_sub[2 * (size_t)l + e] = s;
+            le_sign[2 * (size_t)l + e] = h.sign[q];
+            le_ss[2 * (size_t)l + e] = h.sign[q] * h.scale[q];
+        }
+    }
+    for (int l = 0; l < nlam; l++) ESP_CHECK(le_cnt[l] > 0, ESPRESO_ERR_ARG, "lambda row without entries");
+
+    // R_b, Q = orth(R_b), fb, K_bb (boundary CSR), e = -R^T f, GG^T
+    std::vector<int64_t> rboff(nsub + 1, 0);
+    for (int s = 0; s < nsub; s++) rboff[s + 1] = rboff[s] + (int64_t)sub[s].m * sub[s].kdim;
+    std::vector<double> Rb(rboff[nsub]), Q(rboff[nsub]), fb(nb), e(ncoarse);
+    std::vector<int64_t> kbb_cnt(nsub);
+    std::vector<char> rank_ok(nsub, 1);
+#pragma omp parallel for schedule(dynamic)
+    for (int s = 0; s < nsub; s++) {
+        const SubHost &h = sub[s];
+        for (int a = 0; a < h.kdim; a++)
+            for (int j = 0; j < h.m; j++) Rb[rboff[s] + j + (int64_t)a * h.m] = h.R[h.bdofs[j] + (size_t)a * h.n];
+        std::copy(Rb.begin() + rboff[s], Rb.begin() + rboff[s + 1], Q.begin() + rboff[s]);
+        rank_ok[s] = mgs(h.m, h.kdim, Q.data() + rboff[s]);
+        for (int j = 0; j < h.m; j++) fb[boff[s] + j] = h.f[h.bdofs[j]];
+        for (int a = 0; a < h.kdim; a++) {
+            double acc = 0;
+            for (int i = 0; i < h.n; i++) acc += h.R[i + (size_t)a * h.n] * h.f[i];
+            e[coffv[s] + a] = -acc;
+        }
+        int64_t cnt = 0;
+        for (int j = 0; j < h.m; j++) {
+            const int lr = h.bdofs[j];
+            for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) cnt += h.iperm[h.colidx[k]] >= h.ni;
+        }
+        kbb_cnt[s] = cnt;
+    }
+    for (int s = 0; s < nsub; s++)
+        ESP_CHECK(rank_ok[s], ESPRESO_ERR_NUMERIC,
+                  "kernel restricted to the boundary of subdomain " + std::to_string(s) + " is rank deficient");
+    std::vector<int64_t> kbb_ptr(nb + 1, 0), kbb_col;
+    std::vector<double> kbb_val;
+    {
+        int64_t tot = 0;
+        for (int s = 0; s < nsub; s++) tot += kbb_cnt[s];
+        kbb_col.reserve(tot);
+        kbb_val.reserve(tot);
+        for (int s = 0; s < nsub; s++) {
+            const SubHost &h = sub[s];
+            for (int j = 0; j < h.m; j++) {
+                const int lr = h.bdofs[j];
+                for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) {
+                    const int pc = h.iperm[h.colidx[k]];
+                    if (pc >= h.ni) {
+                        kbb_col.push_back(boff[s] + pc - h.ni);
+                        kbb_val.push_back(h.vals[k]);
+                    }
+                }
+                kbb_ptr[boff[s] + j + 1] = (int64_t)kbb_col.size();
+            }
+        }
+    }
+    // GG^T (dense, ncoarse^2): sum over lambda of G-column outer products
+    std::vector<double> GGt((size_t)ncoarse * ncoarse, 0.0);
+    for (int l = 0; l < nlam; l++) {
+        for (int a = 0; a < le_cnt[l]; a++)
+            for (int b2 = 0; b2 < le_cnt[l]; b2++) {
+                const int s1 = le_sub[2 * l + a], s2 = le_sub[2 * l + b2];
+                const int64_t j1 = le_b[2 * l + a] - boff[s1], j2 = le_b[2 * l + b2] - boff[s2];
+                const double g1 = le_sign[2 * l + a], g2 = le_sign[2 * l + b2];
+                const int m1 = sub[s1].m, m2 = sub[s2].m;
+                for (int i = 0; i < sub[s1].kdim; i++)
+                    for (int j = 0; j < sub[s2].kdim; j++)
+                        GGt[(coffv[s1] + i) + (size_t)(coffv[s2] + j) * ncoarse] +=
+                            g1 * Rb[rboff[s1] + j1 + (int64_t)i * m1] * g2 * Rb[rboff[s2] + j2 + (int64_t)j * m2];
+            }
+    }
+
+    // ---- device: problem arrays ----
+    D.nsub = nsub;
+    D.nlam = nlam;
+    D.ncoarse = ncoarse;
+    D.max_m = max_m;
+    D.nbnd = nb;
+    std::vector<int> mv(nsub), kd(nsub), cv(nsub);
+    for (int s = 0; s < nsub; s++) { mv[s] = sub[s].m; kd[s] = sub[s].kdim; cv[s] = (int)coffv[s]; }
+    D.m = upload(mv);
+    D.kdim = upload(kd);
+    D.coff = upload(cv);
+    D.boff = upload(boff);
+    D.rboff = upload(rboff);
+    D.bl_ptr = upload(bl_ptr);
+    D.bl_lam = upload(bl_lam);
+    D.bl_sign = upload(bl_sign);
+    D.bl_sscale = upload(bl_ss);
+    D.le_cnt = upload(le_cnt);
+    D.le_b = upload(le_b);
+    D.le_sub = upload(le_sub);
+    D.le_sign = upload(le_sign);
+    D.le_sscale = upload(le_ss);
+    D.kbb_ptr = upload(kbb_ptr);
+    D.kbb_col = upload(kbb_col);
+    D.kbb_val = upload(kbb_val);
+    D.Rb = upload(Rb);
+    D.yb = alloc<double>(nb);
+    D.xb = alloc<double>(nb);
+    D.cd = alloc<double>(ncoarse);
+    D.ce = alloc<double>(ncoarse);
+    d_fb = upload(fb);
+    d_hf = alloc<double>(nb);
+    d_ubf = alloc<double>(nb);
+    d_e = upload(e);
+    double *d_Q = upload(Q);
+
+    // packed pools
+    std::vector<int64_t> foff(nsub + 1, 0), lpoff(nsub + 1, 0);
+    const int Rlp = strip_rows(prec);
+    for (int s = 0; s < nsub; s++) {
+        foff[s + 1] = foff[s] + strip_size(sub[s].m, STRIP_F64);
+        lpoff[s + 1] = lpoff[s] + (prec ? strip_size(sub[s].m, Rlp) : 0);
+    }
+    D.Fd = alloc<double>(foff[nsub]);
+    D.SD = alloc<double>(foff[nsub]);
+    D.foff = upload(foff);
+    D.sdoff = D.foff;
+    fsc_bytes = (double)foff[nsub] * 8;
+    sd_bytes = (double)foff[nsub] * 8;
+    if (prec == 1) { D.Ff = alloc<float>(lpoff[nsub]); fsc_bytes = (double)lpoff[nsub] * 4; }
+    if (prec == 2) { D.Fh = alloc<uint16_t>(lpoff[nsub]); fsc_bytes = (double)lpoff[nsub] * 2; }
+    D.foff_lp = upload(lpoff);
+    D.fscale = alloc<float>(nsub);
+    {
+        std::vector<float> ones(nsub, 1.0f);
+        ESP_HIP(hipMemcpyAsync(D.fscale, ones.data(), nsub * sizeof(float), hipMemcpyHostToDevice, st));
+    }
+
+    // primal support
+    std::vector<int64_t> nz_off(nsub + 1, 0), rp_off(nsub + 1, 0), n_off(nsub + 1, 0), band_off(nsub + 1, 0),
+        yf_off(nsub + 1, 0), r_off(nsub + 1, 0);
+    for (int s = 0; s < nsub; s++) {
+        const SubHost &h = sub[s];
+        nz_off[s + 1] = nz_off[s] + h.rowptr[h.n];
+        rp_off[s + 1] = rp_off[s] + h.n + 1;
+        n_off[s + 1] = n_off[s] + h.n;
+        band_off[s + 1] = band_off[s] + (int64_t)h.ni * (h.bw + 1);
+        yf_off[s + 1] = yf_off[s] + h.ni;
+        r_off[s + 1] = r_off[s] + (int64_t)h.n * h.kdim;
+    }
+    {
+        std::vector<int> rp(rp_off[nsub]), ci(nz_off[nsub]), pm(n_off[nsub]), ip(n_off[nsub]);
+        std::vector<double> kv(nz_off[nsub]), ff(n_off[nsub]), RR(r_off[nsub]);
+        std::vector<int> niv(nsub), nv(nsub), bwv(nsub);
+#pragma omp parallel for schedule(dynamic)
+        for (int s = 0; s < nsub; s++) {
+            const SubHost &h = sub[s];
+            std::copy(h.rowptr.begin(), h.rowptr.end(), rp.begin() + rp_off[s]);
+            std::copy(h.colidx.begin(), h.colidx.end(), ci.begin() + nz_off[s]);
+            std::copy(h.vals.begin(), h.vals.end(), kv.begin() + nz_off[s]);
+            std::copy(h.perm.begin(), h.perm.end(), pm.begin() + n_off[s]);
+            std::copy(h.iperm.begin(), h.iperm.end(), ip.begin() + n_off[s]);
+            std::copy(h.f.begin(), h.f.end(), ff.begin() + n_off[s]);
+            std::copy(h.R.begin(), h.R.end(), RR.begin() + r_off[s]);
+            niv[s] = h.ni;
+            nv[s] = h.n;
+            bwv[s] = h.bw;
+        }
+        P.rowptr = upload(rp);
+        P.colidx = upload(ci);
+        P.vals = upload(kv);
+        P.perm = upload(pm);
+        P.iperm = upload(ip);
+        P.R = upload(RR);
+        P.ni = upload(niv);
+        P.n = upload(nv);
+        P.bw = upload(bwv);
+        P.m = D.m;
+        P.kdim = D.kdim;
+        P.coff = D.coff;
+        P.boff = D.boff;
+        P.nz_off = upload(nz_off);
+        P.rp_off = upload(rp_off);
+        P.perm_off = upload(n_off);
+        P.band_off = upload(band_off);
+        P.yf_off = upload(yf_off);
+        P.work_off = P.yf_off;
+        P.r_off = upload(r_off);
+        P.band = alloc<double>(band_off[nsub]);
+        P.yf = alloc<double>(yf_off[nsub]);
+        P.work = alloc<double>(yf_off[nsub]);
+        P.alpha = alloc<double>(ncoarse);
+        d_u = alloc<double>(n_off[nsub]);
+        uoff = n_off;
+        double *d_f = upload(ff);
+        sync();
+        // ---- assembly of S_D and F_i on the GPU, in waves ----
+        last.time_coarse = 0;
+        double ta = wall();
+        assemble(d_f, d_Q, foff, lpoff, boff);
+        last.time_assembly = wall() - ta;
+        release(d_f);
+        release(d_Q);
+    }
+
+    // ---- coarse problem (GG^T)^-1 on the GPU ----
+    double tc = wall();
+    coarse(GGt);
+    last.time_coarse = wall() - tc;
+
+    // ---- SYMV work lists ----
+    {
+        std::vector<SymvItem> it;
+        std::vector<int> i0, ni;
+        int64_t plen_d = 0, plen_lp = 0;
+        build_items(0, it, i0, ni, plen_d);
+        D.items_d = upload(it);
+        D.n_items_d = (int)it.size();
+        D.item0_d = upload(i0);
+        D.nitems_d = upload(ni);
+        if (prec) {
+            build_items(prec, it, i0, ni, plen_lp);
+            D.items_lp = upload(it);
+            D.n_items_lp = (int)it.size();
+            D.item0_lp = upload(i0);
+            D.nitems_lp = upload(ni);
+        }
+        D.ypart = alloc<double>(std::max(plen_d, plen_lp));
+    }
+    // K^+ f on the boundary: ubf = F_i hf (FP64)
+    launch_symv(D, false, 0, d_hf, 0, false, st);
+    ESP_HIP(hipMemcpyAsync(d_ubf, D.yb, nb * sizeof(double), hipMemcpyDeviceToDevice, st));
+
+    // PCG vectors
+    d_x = alloc<double>(nlam); d_r = alloc<double>(nlam); d_w = alloc<double>(nlam); d_y = alloc<double>(nlam);
+    d_z = alloc<double>(nlam); d_p = alloc<double>(nlam); d_Ap = alloc<double>(nlam); d_b = alloc<double>(nlam);
+    d_tmp = alloc<double>(nlam);
+    d_r2 = alloc<double>(nlam);
+    d_partial = alloc<double>(1024);
+    d_scal = alloc<double>(16);
+    // b = B K^+ f - c
+    double *d_c = upload(c);
+    launch_lambda_gather(D, d_ubf, false, d_b, st);
+    launch_axpby(d_b, d_b, d_c, nlam, 1.0, -1.0, st);
+    sync();
+    release(d_c);
+    ready = true;
+    (void)t0;
+}
+
+void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::vector<int64_t> &foff,
+                              const std::vector<int64_t> &lpoff, const std::vector<int64_t> &boff) {
+    const int prec = opt.sc_precision;
+    // per-subdomain workspace (doubles)
+    auto ws = [&](const SubHost &h) {
+        const int64_t nt = std::max((h.ni + TILE - 1) / TILE, (h.m + TILE - 1) / TILE);
+        return (int64_t)h.ni * h.ni + (int64_t)h.ni * (h.m + 1) + 2 * (int64_t)h.m * h.m + nt * TILE * TILE;
+    };
+    size_t freeb = 0, totalb = 0;
+    ESP_HIP(hipMemGetInfo(&freeb, &totalb));
+    double budget = opt.memory_budget_gb > 0 ? opt.memory_budget_gb * 1e9 : 0.85 * (double)freeb;
+    budget = std::min(budget, 0.85 * (double)freeb);
+    int s0 = 0;
+    asm_flops = 0;
+    while (s0 < nsub) {
+        int s1 = s0;
+        double bytes = 0;
+        while (s1 < nsub && (s1 == s0 || bytes + 8.0 * ws(sub[s1]) <= budget)) bytes += 8.0 * ws(sub[s1++]);
+        ESP_CHECK(bytes <= budget, ESPRESO_ERR_MEMORY, "one subdomain's assembly workspace exceeds the device budget");
+        const int nbt = s1 - s0;
+        double *pool = alloc<double>((size_t)(bytes / 8));
+        launch_fill(pool, (int64_t)(bytes / 8), 0.0, st);
+        std::vector<DenseDesc> desc(nbt);
+        std::vector<int> cs_all;
+        std::vector<int64_t> cs_off(nbt);
+        int max_ntii = 0, max_ntm = 0, max_ntc = 0, max_band = 0, max_n = 0, max_m = 0, max_ni = 0;
+        double *cur = pool;
+        for (int b = 0; b < nbt; b++) {
+            const SubHost &h = sub[s0 + b];
+            DenseDesc &d = desc[b];
+            d.ni = h.ni;
+            d.m = h.m;
+            d.ntii = (h.ni + TILE - 1) / TILE;
+            d.ntm = (h.m + TILE - 1) / TILE;
+            d.ntc = (h.m + 1 + TILE - 1) / TILE;
+            d.band_ii = h.band_tiles;
+            d.Kii = cur; cur += (int64_t)h.ni * h.ni;
+            d.Kib = cur; cur += (int64_t)h.ni * (h.m + 1);
+            d.S = cur; cur += (int64_t)h.m * h.m;
+            d.W = cur; cur += (int64_t)h.m * h.m;
+            d.Dinv = cur; cur += (int64_t)std::max(d.ntii, d.ntm) * TILE * TILE;
+            cs_off[b] = (int64_t)cs_all.size();
+            cs_all.insert(cs_all.end(), h.colstart.begin(), h.colstart.end());
+            max_ntii = std::max(max_ntii, d.ntii);
+            max_ntm = std::max(max_ntm, d.ntm);
+            max_ntc = std::max(max_ntc, d.ntc);
+            max_band = std::max(max_band, d.band_ii);
+            max_n = std::max(max_n, h.n);
+            max_m = std::max(max_m, h.m);
+            max_ni = std::max(max_ni, h.ni);
+            // executed flops (banded/profile aware estimate for reporting)
+            asm_flops += (double)h.ni * std::pow((double)std::min(h.ni, (h.band_tiles + 1) * TILE), 2) +
+                         2.0 * h.ni * std::min(h.ni, (h.band_tiles + 1) * TILE) * (h.m + 1) + (double)h.ni * h.m * h.m +
+                         (double)h.m * h.m * h.m;
+        }
+        int *d_cs = upload(cs_all);
+        int *d_info = alloc<int>(nbt);
+        ESP_HIP(hipMemsetAsync(d_info, 0, nbt * sizeof(int), st));
+        for (int b = 0; b < nbt; b++) {
+            desc[b].colstart = d_cs + cs_off[b];
+            desc[b].info = d_info + b;
+        }
+        DenseDesc *d_desc = upload(desc);
+        // wave-local offset tables
+        std::vector<int64_t> rp_off(nbt), nz_off(nbt), n_off(nbt), b_off(nbt), q_off(nbt), f_off(nbt),
+            lp_off(nbt), band_off(nbt), yf_off(nbt);
+        std::vector<int> nloc(nbt), kd(nbt), bw(nbt);
+        {
+            // global offsets of this wave's subdomains in the persistent pools
+            int64_t rp = 0, nz = 0, nn = 0, bo = 0, qo = 0;
+            for (int s = 0; s < s0; s++) {
+                rp += sub[s].n + 1;
+                nz += sub[s].rowptr[sub[s].n];
+                nn += sub[s].n;
+                qo += (int64_t)sub[s].m * sub[s].kdim;
+            }
+            int64_t band = 0, yf = 0;
+            for (int s = 0; s < s0; s++) { band += (int64_t)sub[s].ni * (sub[s].bw + 1); yf += sub[s].ni; }
+            for (int b = 0; b < nbt; b++) {
+                const SubHost &h = sub[s0 + b];
+                rp_off[b] = rp; nz_off[b] = nz; n_off[b] = nn; b_off[b] = boff[s0 + b]; q_off[b] = qo;
+                f_off[b] = foff[s0 + b]; lp_off[b] = lpoff[s0 + b];
+                band_off[b] = band; yf_off[b] = yf;
+                rp += h.n + 1; nz += h.rowptr[h.n]; nn += h.n; qo += (int64_t)h.m * h.kdim;
+                band += (int64_t)h.ni * (h.bw + 1); yf += h.ni;
+                nloc[b] = h.n; kd[b] = h.kdim; bw[b] = h.bw;
+                (void)bo;
+            }
+        }
+        int64_t *d_rp_off = upload(rp_off), *d_nz_off = upload(nz_off), *d_n_off = upload(n_off),
+                *d_b_off = upload(b_off), *d_q_off = upload(q_off), *d_f_off = upload(f_off),
+                *d_lp_off = upload(lp_off), *d_band_off = upload(band_off), *d_yf_off = upload(yf_off);
+        int *d_nloc = upload(nloc), *d_kd = upload(kd), *d_bw = upload(bw);
+        double *d_rho = alloc<double>(nbt);
+
+        launch_scatter(d_desc, nbt, P.rowptr, d_rp_off, P.colidx, P.vals, d_nz_off, P.iperm, d_n_off, d_f, d_nloc,
+                       max_n, st);
+        for (int k = 0; k < max_ntii; k++) launch_potrf_step(d_desc, nbt, 0, k, max_band, st);
+        for (int k = 0; k < max_ntii; k++) launch_trsm_step(d_desc, nbt, 0, k, max_band, max_ntc, st);
+        launch_syrk(d_desc, nbt, 0, max_ntm, st);
+        launch_hf(d_desc, nbt, d_fb, d_hf, d_b_off, max_m, st);
+        launch_band_extract(d_desc, nbt, P.band, d_band_off, d_bw, P.yf, d_yf_off, max_ni, st);
+        launch_pack(d_desc, nbt, 0, D.SD, d_f_off, nullptr, max_m, st);
+        launch_kernel_reg(d_desc, nbt, d_Q, d_q_off, d_kd, d_rho, max_m, st);
+        for (int k = 0; k < max_ntm; k++) launch_potrf_step(d_desc, nbt, 1, k, max_ntm, st);
+        launch_identity(d_desc, nbt, max_m, st);
+        for (int k = 0; k < max_ntm; k++) launch_trsm_step(d_desc, nbt, 1, k, max_ntm, max_ntm, st);
+        launch_syrk(d_desc, nbt, 1, max_ntm, st);
+        launch_pack(d_desc, nbt, 0, D.Fd, d_f_off, nullptr, max_m, st);
+        if (prec == 1) launch_pack(d_desc, nbt, 1, D.Ff, d_lp_off, nullptr, max_m, st);
+        if (prec == 2) launch_pack(d_desc, nbt, 2, D.Fh, d_lp_off, D.fscale + s0, max_m, st);
+        std::vector<int> info(nbt);
+        ESP_HIP(hipMemcpyAsync(info.data(), d_info, nbt * sizeof(int), hipMemcpyDeviceToHost, st));
+        sync();
+        for (int b = 0; b < nbt; b++)
+            ESP_CHECK(info[b] == 0, ESPRESO_ERR_NUMERIC,
+                      "dense factorisation breakdown in subdomain " + std::to_string(s0 + b) + " at column " +
+                          std::to_string(info[b]));
+        for (void *p : {(void *)pool, (void *)d_cs, (void *)d_info, (void *)d_desc, (void *)d_rp_off, (void *)d_nz_off,
+                        (void *)d_n_off, (void *)d_b_off, (void *)d_q_off, (void *)d_f_off, (void *)d_lp_off,
+                        (void *)d_band_off, (void *)d_yf_off, (void *)d_nloc, (void *)d_kd, (void *)d_bw,
+                        (void *)d_rho})
+            release(p);
+        s0 = s1;
+    }
+}
+
+void espreso_feti_s::coarse(const std::vector<double> &GGt) {
+    const int nc = ncoarse;
+    const int nt = (nc + TILE - 1) / TILE;
+    double *S = upload(GGt);
+    double *W = alloc<double>((size_t)nc * nc);
+    double *Dinv = alloc<double>((size_t)nt * TILE * TILE);
+    int *info = alloc<int>(1);
+    ESP_HIP(hipMemsetAsync(info, 0, sizeof(int), st));
+    DenseDesc d{};
+    d.S = S;
+    d.W = W;
+    d.Dinv = Dinv;
+    d.info = info;
+    d.ni = 0;
+    d.m = nc;
+    d.ntm = nt;
+    d.ntc = nt;
+    DenseDesc *dd = upload(std::vector<DenseDesc>{d});
+    for (int k = 0; k < nt; k++) launch_potrf_step(dd, 1, 1, k, nt, st);
+    launch_identity(dd, 1, nc, st);
+    for (int k = 0; k < nt; k++) launch_trsm_step(dd, 1, 1, k, nt, nt, st);
+    launch_syrk(dd, 1, 1, nt, st);
+    launch_symmetrize(S, nc, st);
+    int hinfo = 0;
+    ESP_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st));
+    sync();
+    ESP_CHECK(hinfo == 0, ESPRESO_ERR_NUMERIC, "GG^T factorisation breakdown");
+    D.GGinv = S;
+    release(W);
+    release(Dinv);
+    release(info);
+    release(dd);
+}
+
+void espreso_feti_s::build_items(int prec, std::vector<SymvItem> &items, std::vector<int> &item0,
+                                 std::vector<int> &nit, int64_t &plen) {
+    const int R = strip_rows(prec);
+    const int64_t target = 160 * 1024;  // matrix elements per workgroup
+    items.clear();
+    item0.assign(nsub, 0);
+    nit.assign(nsub, 0);
+    plen = 0;
+    for (int s = 0; s < nsub; s++) {
+        const int m = sub[s].m;
+        const int nst = (int)strip_count(m, R);
+        const int64_t tot = strip_size(m, R);
+        int want = (int)std::max<int64_t>(1, std::min<int64_t>(nst, (tot + target - 1) / target));
+        item0[s] = (int)items.size();
+        int t = 0;
+        for (int k = 0; k < want && t < nst; k++) {
+            const int64_t goal = tot * (k + 1) / want;
+            int t1 = t + 1;
+            while (t1 < nst && strip_offset(m, R, t1) < goal) t1++;
+            if (k == want - 1) t1 = nst;
+            SymvItem it{};
+            it.sub = s;
+            it.t0 = t;
+            it.t1 = t1;
+            it.part_off = plen;
+            plen += m - t * R;
+            items.push_back(it);
+            t = t1;
+        }
+        nit[s] = (int)items.size() - item0[s];
+    }
+}
+
+// ------------------------------------------------------ dual operators ----
+void espreso_feti_s::apply_F(const double *x, double *y, int prec) {
+    launch_symv(D, false, prec, x, 1, false, st);
+    launch_lambda_gather(D, D.yb, false, y, st);
+}
+
+void espreso_feti_s::apply_prec(const double *x, double *y) {
+    switch (opt.preconditioner) {
+    case ESPRESO_PREC_NONE:
+        ESP_HIP(hipMemcpyAsync(y, x, nlam * sizeof(double), hipMemcpyDeviceToDevice, st));
+        break;
+    case ESPRESO_PREC_LUMPED:
+        launch_boundary_gather(D, x, true, D.xb, st);
+        launch_kbb(D, D.xb, D.yb, st);
+        launch_lambda_gather(D, D.yb, true, y, st);
+        break;
+    case ESPRESO_PREC_WEIGHT:
+        launch_boundary_gather(D, x, true, D.xb, st);
+        launch_lambda_gather(D, D.xb, true, y, st);
+        break;
+    case ESPRESO_PREC_DIRICHLET:
+        launch_symv(D, true, 0, x, 1, true, st);
+        launch_lambda_gather(D, D.yb, true, y, st);
+        break;
+    default:
+        throw Error(ESPRESO_ERR_ARG, "unknown preconditioner");
+    }
+}
+
+// Projected PCG (itersolver.cpp:1537-1684) on device vectors.  r holds the
+// initial residual on entry.  Scalars stay on the device; the host reads
+// ||w|| once per iteration for the stop test.
+void espreso_feti_s::pcg(const double *b, double *x, double *r, int prec, int &iters, double &norm, double &tol,
+                         bool use_abs, double tol_abs) {
+    double *s_yw = d_scal + 0, *s_yw_prev = d_scal + 1, *s_pAp = d_scal + 2, *s_ww = d_scal + 3, *s_tmp = d_scal + 4;
+    // tol = precision * min(||P r0||, ||b||)
+    launch_project(D, r, d_w, 0, st);
+    launch_dot(d_w, d_w, nlam, d_partial, s_tmp, st);
+    launch_dot(b, b, nlam, d_partial, s_tmp + 1, st);
+    double hv[2];
+    ESP_HIP(hipMemcpyAsync(hv, s_tmp, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    sync();
+    tol = use_abs ? tol_abs : opt.precision * std::min(std::sqrt(hv[0]), std::sqrt(hv[1]));
+    last.norm_b = std::sqrt(hv[1]);
+    iters = 0;
+    norm = 0;
+    for (int it = 0; it < opt.max_iterations; it++) {
+        launch_project(D, r, d_w, 0, st);
+        if (opt.preconditioner == ESPRESO_PREC_NONE) {
+            ESP_HIP(hipMemcpyAsync(d_y, d_w, nlam * sizeof(double), hipMemcpyDeviceToDevice, st));
+        } else {
+            apply_prec(d_w, d_z);
+            launch_project(D, d_z, d_y, 0, st);
+        }
+        if (it > 0) launch_copy_scalar(s_yw, s_yw_prev, st);
+        launch_dot(d_y, d_w, nlam, d_partial, s_yw, st);
+        launch_p_update(d_p, d_y, nlam, s_yw, s_yw_prev, it == 0, st);
+        apply_F(d_p, d_Ap, prec);
+        launch_dot(d_p, d_Ap, nlam, d_partial, s_pAp, st);
+        launch_xr_update(x, r, d_p, d_Ap, nlam, s_yw, s_pAp, st);
+        launch_dot(d_w, d_w, nlam, d_partial, s_ww, st);
+        double ww = 0;
+        ESP_HIP(hipMemcpyAsync(&ww, s_ww, sizeof(double), hipMemcpyDeviceToHost, st));
+        sync();
+        ESP_CHECK(std::isfinite(ww), ESPRESO_ERR_NUMERIC, "PCG produced a non-finite residual");
+        norm = std::sqrt(ww);
+        iters = it + 1;
+        if (norm < tol) break;
+    }
+}
+
+void espreso_feti_s::solve(espreso_feti_stats *out) {
+    ESP_CHECK(ready, ESPRESO_ERR_STATE, "espreso_feti_setup not called");
+    double t0 = wall();
+    // lambda0 = G^T (GG^T)^-1 e ; r = b - F lambda0
+    launch_project(D, d_e, d_x, 1, st);
+    apply_F(d_x, d_Ap, 0);
+    launch_axpby(d_r, d_b, d_Ap, nlam, 1.0, -1.0, st);
+    int iters = 0;
+    double norm = 0, tol = 0;
+    const int prec = opt.sc_precision;
+    int refine = 0;
+    if (prec == 0) {
+        pcg(d_b, d_x, d_r, 0, iters, norm, tol, false, 0);
+    } else {
+        // mixed precision: inner PCG on the low-precision F_i, outer FP64
+        // residual r = b - F lambda with the FP64 F_i (iterative refinement)
+        launch_project(D, d_r, d_w, 0, st);
+        launch_dot(d_w, d_w, nlam, d_partial, d_scal + 4, st);
+        launch_dot(d_b, d_b, nlam, d_partial, d_scal + 5, st);
+        double hv[2];
+        ESP_HIP(hipMemcpyAsync(hv, d_scal + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+        sync();
+        const double tol64 = opt.precision * std::min(std::sqrt(hv[0]), std::sqrt(hv[1]));
+        last.norm_b = std::sqrt(hv[1]);
+        double *dx = d_tmp;
+        int total = 0;
+        for (refine = 0; refine < 20; refine++) {
+            // correction: solve P F P dx = P r with the low-precision operator
+            ESP_HIP(hipMemsetAsync(dx, 0, nlam * sizeof(double), st));
+            double *rr = d_r2;  // inner residual
+            ESP_HIP(hipMemcpyAsync(rr, d_r, nlam * sizeof(double), hipMemcpyDeviceToDevice, st));
+            int it = 0;
+            double nn = 0, tt = 0;
+            // inner tolerance: relative 1e-5 of the current projected residual, floor tol64
+            launch_project(D, d_r, d_w, 0, st);
+            launch_dot(d_w, d_w, nlam, d_partial, d_scal + 6, st);
+            double cur = 0;
+            ESP_HIP(hipMemcpyAsync(&cur, d_scal + 6, sizeof(double), hipMemcpyDeviceToHost, st));
+            sync();
+            cur = std::sqrt(cur);
+            if (cur < tol64) break;
+            pcg(d_r, dx, rr, prec, it, nn, tt, true, std::max(tol64 * 0.5, cur * 1e-5));
+            total += it;
+            launch_axpby(d_x, d_x, dx, nlam, 1.0, 1.0, st);
+            apply_F(d_x, d_Ap, 0);
+            launch_axpby(d_r, d_b, d_Ap, nlam, 1.0, -1.0, st);
+        }
+        launch_project(D, d_r, d_w, 0, st);
+        launch_dot(d_w, d_w, nlam, d_partial, d_scal + 6, st);
+        double fin = 0;
+        ESP_HIP(hipMemcpyAsync(&fin, d_scal + 6, sizeof(double), hipMemcpyDeviceToHost, st));
+        sync();
+        iters = total;
+        norm = std::sqrt(fin);
+        tol = tol64;
+    }
+    double tp = wall();
+    last.time_pcg = tp - t0;
+    // amplitudes alpha = (GG^T)^-1 G r ; primal
+    launch_project(D, d_r, P.alpha, 2, st);
+    // u_b = K^+ f|_b - F_i B^T lambda
+    launch_symv(D, false, 0, d_x, 1, false, st);
+    launch_axpby(D.xb, d_ubf, D.yb, D.nbnd, 1.0, -1.0, st);
+    launch_primal(P, nsub, D.xb, d_u, st);
+    sync();
+    last.time_primal = wall() - tp;
+    last.iterations = iters;
+    last.final_norm = norm;
+    last.tol = tol;
+    last.n_lambda = nlam;
+    last.n_coarse = ncoarse;
+    last.n_subdomains = nsub;
+    last.refinement_steps = refine;
+    last.sc_bytes = fsc_bytes;
+    last.sd_bytes = sd_bytes;
+    last.assembly_flops = asm_flops;
+    last.time_total = last.time_assembly + last.time_coarse + last.time_pcg + last.time_primal;
+    solved = true;
+    if (out) *out = last;
+}
+
+// ================================================================ C ABI ====
+#define ESP_API_BEGIN try {
+#define ESP_API_END                      \
+    }                                    \
+    catch (const Error &e) {             \
+        g_last_error = e.what();         \
+        return e.code;                   \
+    }                                    \
+    catch (const std::exception &e) {    \
+        g_last_error = e.what();         \
+        return ESPRESO_ERR_ARG;          \
+    }                                    \
+    return ESPRESO_OK;
+
+extern "C" {
+
+const char *espreso_last_error(void) { return g_last_error.c_str(); }
+int espreso_version(void) { return 1; }
+
+int espreso_feti_create(espreso_feti **h, int32_t n_subdomains, const espreso_feti_options *o) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && o && n_subdomains > 0, ESPRESO_ERR_ARG, "invalid arguments to espreso_feti_create");
+    *h = nullptr;
+    int ndev = 0;
+    ESP_HIP(hipGetDeviceCount(&ndev));
+    ESP_CHECK(ndev > 0, ESPRESO_ERR_HIP, "no HIP device");
+    ESP_CHECK(o->device >= 0 && o->device < ndev, ESPRESO_ERR_ARG, "device ordinal out of range");
+    ESP_HIP(hipSetDevice(o->device));
+    auto f = std::make_unique<espreso_feti_s>();
+    f->opt = *o;
+    if (f->opt.max_iterations <= 0) f->opt.max_iterations = 200;
+    if (!(f->opt.precision > 0)) f->opt.precision = 1e-5;
+    f->nsub = n_subdomains;
+    f->sub.resize(n_subdomains);
+    ESP_HIP(hipStreamCreate(&f->st));
+    *h = f.release();
+    ESP_API_END
+}
+
+int espreso_feti_destroy(espreso_feti *h) {
+    ESP_API_BEGIN
+    delete h;
+    ESP_API_END
+}
+
+int espreso_feti_sizes(espreso_feti *h, int32_t *sizes) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && sizes, ESPRESO_ERR_ARG, "null argument");
+    sizes[0] = h->nsub;
+    sizes[1] = h->nlam;
+    sizes[2] = h->ncoarse;
+    ESP_API_END
+}
+
+int espreso_feti_set_subdomain(espreso_feti *h, int32_t i, int32_t n, const int32_t *rowptr, const int32_t *colidx,
+                               const double *values, const double *f, int32_t kdim, const double *R, int32_t nlam,
+                               const int32_t *lambda, const int32_t *dof, const double *sign, const double *scale) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && i >= 0 && i < h->nsub && n > 0, ESPRESO_ERR_ARG, "invalid subdomain");
+    ESP_CHECK(kdim >= 1 && kdim <= 6, ESPRESO_ERR_ARG, "kernel dimension must be 1..6");
+    ESP_CHECK(nlam > 0, ESPRESO_ERR_ARG, "subdomain without multipliers (not floating-glued)");
+    ESP_CHECK(!h->ready, ESPRESO_ERR_STATE, "setup already done");
+    SubHost &s = h->sub[i];
+    s.n = n;
+    s.rowptr.assign(rowptr, rowptr + n + 1);
+    ESP_CHECK(s.rowptr[0] == 0, ESPRESO_ERR_ARG, "CSR must be 0-based");
+    const int nnz = s.rowptr[n];
+    s.colidx.assign(colidx, colidx + nnz);
+    s.vals.assign(values, values + nnz);
+    for (int q = 0; q < nnz; q++) {
+        ESP_CHECK(s.colidx[q] >= 0 && s.colidx[q] < n, ESPRESO_ERR_ARG, "CSR column out of range");
+        ESP_CHECK(std::isfinite(s.vals[q]), ESPRESO_ERR_NUMERIC, "non-finite value in K");
+    }
+    s.f.assign(f, f + n);
+    for (double v : s.f) ESP_CHECK(std::isfinite(v), ESPRESO_ERR_NUMERIC, "non-finite value in f");
+    s.kdim = kdim;
+    s.R.assign(R, R + (size_t)n * kdim);
+    s.lam.assign(lambda, lambda + nlam);
+    s.dof.assign(dof, dof + nlam);
+    s.sign.assign(sign, sign + nlam);
+    s.scale.assign(scale, scale + nlam);
+    for (int q = 0; q < nlam; q++) {
+        ESP_CHECK(s.dof[q] >= 0 && s.dof[q] < n, ESPRESO_ERR_ARG, "multiplier DOF out of range");
+        ESP_CHECK(q == 0 || s.lam[q] > s.lam[q - 1], ESPRESO_ERR_ARG, "multipliers must be ascending");
+    }
+    s.set = true;
+    ESP_API_END
+}
+
+int espreso_feti_set_dual(espreso_feti *h, int32_t n_lambda, const double *c) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && n_lambda > 0 && c, ESPRESO_ERR_ARG, "invalid dual size");
+    h->nlam = n_lambda;
+    h->c.assign(c, c + n_lambda);
+    ESP_API_END
+}
+
+int espreso_feti_setup(espreso_feti *h) {
+    ESP_API_BEGIN
+    ESP_CHECK(h, ESPRESO_ERR_ARG, "null handle");
+    ESP_HIP(hipSetDevice(h->opt.device));
+    h->setup();
+    ESP_API_END
+}
+
+int espreso_feti_solve(espreso_feti *h, espreso_feti_stats *st) {
+    ESP_API_BEGIN
+    ESP_CHECK(h, ESPRESO_ERR_ARG, "null handle");
+    ESP_HIP(hipSetDevice(h->opt.device));
+    h->solve(st);
+    ESP_API_END
+}
+
+int espreso_feti_primal(espreso_feti *h, int32_t i, double *u) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && h->solved && i >= 0 && i < h->nsub, ESPRESO_ERR_STATE, "no solution");
+    ESP_HIP(hipMemcpy(u, h->d_u + h->uoff[i], h->sub[i].n * sizeof(double), hipMemcpyDeviceToHost));
+    ESP_API_END
+}
+
+int espreso_feti_dual(espreso_feti *h, double *lambda) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && h->solved, ESPRESO_ERR_STATE, "no solution");
+    ESP_HIP(hipMemcpy(lambda, h->d_x, h->nlam * sizeof(double), hipMemcpyDeviceToHost));
+    ESP_API_END
+}
+
+static int dual_op(espreso_feti *h, const double *x, double *y, int which) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && h->ready, ESPRESO_ERR_STATE, "setup not done");
+    const size_t bytes = h->nlam * sizeof(double);
+    ESP_HIP(hipMemcpyAsync(h->d_z, x, bytes, hipMemcpyHostToDevice, h->st));
+    if (which == 0) h->apply_F(h->d_z, h->d_tmp, h->opt.sc_precision);
+    else if (which == 1) h->apply_prec(h->d_z, h->d_tmp);
+    else launch_project(h->D, h->d_z, h->d_tmp, 0, h->st);
+    ESP_HIP(hipMemcpyAsync(y, h->d_tmp, bytes, hipMemcpyDeviceToHost, h->st));
+    h->sync();
+    ESP_API_END
+}
+
+int espreso_feti_apply_F(espreso_feti *h, const double *x, double *y) { return dual_op(h, x, y, 0); }
+int espreso_feti_apply_prec(espreso_feti *h, const double *x, double *y) { return dual_op(h, x, y, 1); }
+int espreso_feti_project(espreso_feti *h, const double *x, double *y) { return dual_op(h, x, y, 2); }
+
+int espreso_feti_vec_b(espreso_feti *h, double *b) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && h->ready, ESPRESO_ERR_STATE, "setup not done");
+    ESP_HIP(hipMemcpy(b, h->d_b, h->nlam * sizeof(double), hipMemcpyDeviceToHost));
+    ESP_API_END
+}
+
+int espreso_feti_subdomain_boundary(espreso_feti *h, int32_t i, int32_t *m, int32_t *bdofs) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && h->ready && i >= 0 && i < h->nsub, ESPRESO_ERR_STATE, "setup not done");
+    *m = h->sub[i].m;
+    if (bdofs) std::copy(h->sub[i].bdofs.begin(), h->sub[i].bdofs.end(), bdofs);
+    ESP_API_END
+}
+
+int espreso_feti_subdomain_dense(espreso_feti *h, int32_t i, int32_t which, double *out) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && h->ready && i >= 0 && i < h->nsub, ESPRESO_ERR_STATE, "setup not done");
+    const int m = h->sub[i].m;
+    double *tmp = h->alloc<double>((size_t)m * m);
+    std::vector<int64_t> off(2);
+    ESP_HIP(hipMemcpy(off.data(), h->D.foff + i, 2 * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (which == 0) launch_unpack(h->D.Fd, 0, off[0], m, 1.0f, tmp, h->st);
+    else if (which == 1) launch_unpack(h->D.SD, 0, off[0], m, 1.0f, tmp, h->st);
+    else {
+        const int prec = h->opt.sc_precision;
+        std::vector<int64_t> lo(1);
+        float sc = 1.0f;
+        if (prec) {
+            ESP_HIP(hipMemcpy(lo.data(), h->D.foff_lp + i, sizeof(int64_t), hipMemcpyDeviceToHost));
+            ESP_HIP(hipMemcpy(&sc, h->D.fscale + i, sizeof(float), hipMemcpyDeviceToHost));
+        }
+        if (prec == 0) launch_unpack(h->D.Fd, 0, off[0], m, 1.0f, tmp, h->st);
+        else if (prec == 1) launch_unpack(h->D.Ff, 1, lo[0], m, 1.0f, tmp, h->st);
+        else launch_unpack(h->D.Fh, 2, lo[0], m, sc, tmp, h->st);
+    }
+    ESP_HIP(hipMemcpyAsync(out, tmp, (size_t)m * m * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    h->sync();
+    h->release(tmp);
+    ESP_API_END
+}
+
+int espreso_feti_time_apply(espreso_feti *h, int32_t op, int32_t reps, double *ms_apply, double *ms_symv) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && h->ready && reps > 0, ESPRESO_ERR_STATE, "setup not done");
+    hipEvent_t e0, e1, s0, s1;
+    ESP_HIP(hipEventCreate(&e0));
+    ESP_HIP(hipEventCreate(&e1));
+    ESP_HIP(hipEventCreate(&s0));
+    ESP_HIP(hipEventCreate(&s1));
+    launch_axpby(h->d_z, h->d_b, h->d_b, h->nlam, 1.0, 0.0, h->st);
+    // warm-up
+    if (op == 0) h->apply_F(h->d_z, h->d_tmp, h->opt.sc_precision);
+    else h->apply_prec(h->d_z, h->d_tmp);
+    double symv_ms = 0;
+    ESP_HIP(hipEventRecord(e0, h->st));
+    for (int r = 0; r < reps; r++) {
+        if (op == 0) h->apply_F(h->d_z, h->d_tmp, h->opt.sc_precision);
+        else h->apply_prec(h->d_z, h->d_tmp);
+    }
+    ESP_HIP(hipEventRecord(e1, h->st));
+    ESP_HIP(hipEventSynchronize(e1));
+    float tot = 0;
+    ESP_HIP(hipEventElapsedTime(&tot, e0, e1));
+    for (int r = 0; r < reps; r++) {
+        ESP_HIP(hipEventRecord(s0, h->st));
+        launch_symv(h->D, op != 0, op == 0 ? h->opt.sc_precision : 0, h->d_z, 1, op != 0, h->st);
+        ESP_HIP(hipEventRecord(s1, h->st));
+        ESP_HIP(hipEventSynchronize(s1));
+        float t = 0;
+        ESP_HIP(hipEventElapsedTime(&t, s0, s1));
+        symv_ms += t;
+    }
+    *ms_apply = tot / reps;
+    *ms_symv = symv_ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(s0);
+    (void)hipEventDestroy(s1);
+    ESP_API_END
+}
+
+// ---- diagnostics of the MFMA tile routine ----
+int espreso_debug_mfma(const double *a, const double *b, double *out) {
+    ESP_API_BEGIN
+    double *d = nullptr;
+    ESP_HIP(hipMalloc(&d, (64 + 64 + 256) * sizeof(double)));
+    hipError_t e = hipMemcpy(d, a, 64 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + 64, b, 64 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_probe_mfma(d, d + 64, d + 128, nullptr);
+        e = hipMemcpy(out, d + 128, 256 * sizeof(double), hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(d);
+    ESP_HIP(e);
+    ESP_API_END
+}
+
+int espreso_debug_tile_gemm(int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const double *A, int32_t lda,
+                            const double *B, int32_t ldb, double *C, int32_t ldc, double alpha, double beta) {
+    ESP_API_BEGIN
+    ESP_CHECK(M >= 0 && M <= 64 && N >= 0 && N <= 64 && K >= 0, ESPRESO_ERR_ARG, "tile GEMM shape out of range");
+    const size_t na = (size_t)lda * (ta ? M : K), nbb = (size_t)ldb * (tb ? K : N), nc = (size_t)ldc * N;
+    ESP_CHECK(lda >= (ta ? K : M) && ldb >= (tb ? N : K) && ldc >= M, ESPRESO_ERR_ARG, "leading dimension too small");
+    double *d = nullptr;
+    ESP_HIP(hipMalloc(&d, (na + nbb + nc + 1) * sizeof(double)));
+    hipError_t e = hipMemcpy(d, A, na * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + na, B, nbb * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + na + nbb, C, nc * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_probe_gemm(ta, tb, M, N, K, d, lda, d + na, ldb, d + na + nbb, ldc, alpha, beta, nullptr);
+        e = hipMemcpy(C, d + na + nbb, nc * sizeof(double), hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(d);
+    ESP_HIP(e);
+    ESP_API_END
+}
+
+// ---- structured grid front end ----
+
+int espreso_grid_sizes(const espreso_grid_problem *p, int32_t *n_subdomains, int32_t *n_nodes, int32_t *ndof) {
+    ESP_API_BEGIN
+    ESP_CHECK(p, ESPRESO_ERR_ARG, "null problem");
+    *n_subdomains = grid_subdomain_count(*p);
+    *n_nodes = grid_node_count(*p);
+    *ndof = p->physics == 0 ? 3 : 1;
+    ESP_API_END
+}
+
+int espreso_grid_load(espreso_feti *h, const espreso_grid_problem *p) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && p, ESPRESO_ERR_ARG, "null argument");
+    ESP_CHECK(grid_subdomain_count(*p) == h->nsub, ESPRESO_ERR_ARG, "subdomain count differs from the instance");
+    GridProblem g;
+    grid_build(*p, h->opt.scaling, g);
+    h->ndof = g.ndof;
+    h->nnodes = g.nnodes;
+    h->l2g.resize(h->nsub);
+    for (int s = 0; s < h->nsub; s++) {
+        GridSubdomain &sd = g.sub[s];
+        int rc = espreso_feti_set_subdomain(h, s, sd.n, sd.rowptr.data(), sd.colidx.data(), sd.vals.data(),
+                                            sd.f.data(), sd.kdim, sd.R.data(), (int)sd.lam.size(), sd.lam.data(),
+                                            sd.dof.data(), sd.sign.data(), sd.scale.data());
+        if (rc) return rc;
+        h->l2g[s].swap(sd.l2g);
+        GridSubdomain().rowptr.swap(sd.rowptr);
+        std::vector<int>().swap(sd.colidx);
+        std::vector<double>().swap(sd.vals);
+    }
+    return espreso_feti_set_dual(h, g.nlam, g.c.data());
+    ESP_API_END
+}
+
+int espreso_grid_export(const espreso_grid_problem *p, int32_t scaling, int32_t i, int32_t *sizes, int32_t *rowptr,
+                        int32_t *colidx, double *values, double *f, double *R, int32_t *lambda, int32_t *dof,
+                        double *sign, double *scale, double *c) {
+    ESP_API_BEGIN
+    ESP_CHECK(p && sizes, ESPRESO_ERR_ARG, "null argument");
+    ESP_CHECK(i >= 0 && i < grid_subdomain_count(*p), ESPRESO_ERR_ARG, "subdomain out of range");
+    GridProblem g;
+    grid_build(*p, scaling, g);
+    const GridSubdomain &sd = g.sub[i];
+    sizes[0] = sd.n;
+    sizes[1] = sd.rowptr[sd.n];
+    sizes[2] = sd.kdim;
+    sizes[3] = (int)sd.lam.size();
+    sizes[4] = g.nlam;
+    sizes[5] = (int)g.sub.size();
+    if (rowptr) std::copy(sd.rowptr.begin(), sd.rowptr.end(), rowptr);
+    if (colidx) std::copy(sd.colidx.begin(), sd.colidx.end(), colidx);
+    if (values) std::copy(sd.vals.begin(), sd.vals.end(), values);
+    if (f) std::copy(sd.f.begin(), sd.f.end(), f);
+    if (R) std::copy(sd.R.begin(), sd.R.end(), R);
+    if (lambda) std::copy(sd.lam.begin(), sd.lam.end(), lambda);
+    if (dof) std::copy(sd.dof.begin(), sd.dof.end(), dof);
+    if (sign) std::copy(sd.sign.begin(), sd.sign.end(), sign);
+    if (scale) std::copy(sd.scale.begin(), sd.scale.end(), scale);
+    if (c) std::copy(g.c.begin(), g.c.end(), c);
+    ESP_API_END
+}
+
+int espreso_grid_l2g(espreso_feti *h, int32_t i, int32_t *l2g) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && i >= 0 && i < (int)h->l2g.size(), ESPRESO_ERR_STATE, "grid not loaded");
+    std::copy(h->l2g[i].begin(), h->l2g[i].end(), l2g);
+    ESP_API_END
+}
+
+int espreso_grid_nodal_solution(espreso_feti *h, double *u_nodal) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && h->solved && !h->l2g.empty(), ESPRESO_ERR_STATE, "no grid solution");
+    const int nd = h->ndof;
+    std::vector<double> u(h->uoff[h->nsub]);
+    ESP_HIP(hipMemcpy(u.data(), h->d_u, u.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<int> cnt(h->nnodes, 0);
+    std::fill(u_nodal, u_nodal + (size_t)h->nnodes * nd, 0.0);
+    for (int s = 0; s < h->nsub; s++)
+        for (size_t ln = 0; ln < h->l2g[s].size(); ln++) {
+            const int g = h->l2g[s][ln];
+            cnt[g]++;
+            for (int d = 0; d < nd; d++) u_nodal[(size_t)g * nd + d] += u[h->uoff[s] + ln * nd + d];
+        }
+    for (int g = 0; g < h->nnodes; g++)
+        for (int d = 0; d < nd; d++) u_nodal[(size_t)g * nd + d] /= std::max(cnt[g], 1);
+    ESP_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+// Internal declarations shared by the host driver (feti.hip, grid.cpp) and the
+// assembly kernels (dense.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/espreso_amd.h"
+
+namespace esp {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define ESP_HIP(expr)                                                                                \
+    do {                                                                                             \
+        hipError_t _e = (expr);                                                                      \
+        if (_e != hipSuccess)                                                                        \
+            throw ::esp::Error(ESPRESO_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+#define ESP_CHECK(cond, code, msg)                      \
+    do {                                                \
+        if (!(cond)) throw ::esp::Error((code), (msg)); \
+    } while (0)
+
+constexpr int TILE = 64;        // dense tile edge of the assembly kernels
+constexpr int STRIP_F64 = 16;   // rows per packed strip, FP64 storage
+constexpr int STRIP_F32 = 32;   // FP32 storage
+constexpr int STRIP_B16 = 64;   // BF16 storage
+
+__host__ __device__ inline int strip_rows(int prec) { return prec == 0 ? STRIP_F64 : (prec == 1 ? STRIP_F32 : STRIP_B16); }
+
+// Packed-strip layout of a symmetric m x m matrix with R rows per strip:
+// strip t holds rows [tR, tR+R) x columns [tR, m), column-major inside the
+// strip (R consecutive values per column), zero below the diagonal, diagonal
+// stored halved so that one pass applies both triangles.
+__host__ __device__ inline int64_t strip_offset(int64_t m, int64_t R, int64_t t) { return R * (t * m - R * t * (t - 1) / 2); }
+__host__ __device__ inline int64_t strip_count(int64_t m, int64_t R) { return (m + R - 1) / R; }
+__host__ __device__ inline int64_t strip_size(int64_t m, int64_t R) { return strip_offset(m, R, strip_count(m, R)); }
+
+// Per-subdomain descriptor of the dense assembly workspace (device memory).
+// Interior DOFs first (ascending local DOF), boundary DOFs after them.
+struct DenseDesc {
+    double *Kii;          // ni x ni, ld ni (lower used); becomes L (K_ii = L L^T)
+    double *Kib;          // ni x (m+1), ld ni: [K_ib | f_i]  ->  [Y | y_f] = L^-1 [K_ib | f_i]
+    double *S;            // m x m, ld m (lower): K_bb - Y^T Y -> + rho Q Q^T -> L_s -> F
+    double *W;            // m x m, ld m: identity -> L_s^-1
+    double *Dinv;         // max(ntii, ntm) x 64 x 64: inverses of diagonal blocks
+    const int *colstart;  // ntc: first tile row of a nonzero in each column tile of K_ib
+    int *info;            // factorisation status (0 = ok, else 1-based failing column)
+    int ni, m;
+    int band_ii;          // block bandwidth of K_ii in tiles
+    int ntii, ntm, ntc;   // tile counts: ceil(ni/64), ceil(m/64), ceil((m+1)/64)
+};
+
+// Launchers (dense.hip).  `desc` is a device array of nb descriptors.
+// which = 0: operate on K_ii / [K_ib|f];  which = 1: on S / W.
+void launch_fill(double *p, int64_t n, double v, hipStream_t st);
+void launch_potrf_step(const DenseDesc *desc, int nb, int which, int step, int max_band, hipStream_t st);
+void launch_trsm_step(const DenseDesc *desc, int nb, int which, int step, int max_band, int max_ntc,
+                      hipStream_t st);
+void launch_syrk(const DenseDesc *desc, int nb, int which, int max_ntm, hipStream_t st);
+void launch_identity(const DenseDesc *desc, int nb, int max_m, hipStream_t st);
+// S += rho Q Q^T with rho = max diag S; Q (m x k, column-major) at q[q_off[b]]
+void launch_kernel_reg(const DenseDesc *desc, int nb, const double *q, const int64_t *q_off,
+                       const int *kdim, double *rho, int max_m, hipStream_t st);
+// hf = fb - Y^T y_f  (fb, hf: m values at off[b])
+void launch_hf(const DenseDesc *desc, int nb, const double *fb, double *hf, const int64_t *off, int max_m,
+               hipStream_t st);
+// pack the lower triangle of S into strips: prec 0 FP64, 1 FP32, 2 BF16 (scaled by 1/max|S|)
+void launch_pack(const DenseDesc *desc, int nb, int prec, void *pool, const int64_t *off, float *scale,
+                 int max_m, hipStream_t st);
+// copy band of L (ni x (bw+1), column c holds L[c..c+bw][c]) and y_f out of the workspace
+void launch_band_extract(const DenseDesc *desc, int nb, double *band, const int64_t *band_off, const int *bw,
+                         double *yf, const int64_t *yf_off, int max_ni, hipStream_t st);
+// unpack one strip-packed matrix into dense m x m (column-major, full)
+void launch_unpack(const void *pool, int prec, int64_t off, int m, float scale, double *out, hipStream_t st);
+// scatter CSR K_i (local DOFs) into K_ii, [K_ib|f], K_bb; iperm maps local -> internal index
+void launch_scatter(const DenseDesc *desc, int nb, const int *rowptr, const int64_t *rp_off, const int *colidx,
+                    const double *vals, const int64_t *nz_off, const int *iperm, const int64_t *perm_off,
+                    const double *f, const int *n, int max_n, hipStream_t st);
+
+// diagnostics (dense.hip)
+void launch_probe_mfma(const double *a, const double *b, double *out, hipStream_t st);
+void launch_probe_gemm(int ta, int tb, int M, int N, int K, const double *A, int lda, const double *B, int ldb,
+                       double *C, int ldc, double alpha, double beta, hipStream_t st);
+
+}  // namespace esp

@@ -1,0 +1,186 @@
+/*
+ * espreso_amd -- MI355X-native Total FETI dual-loop core, C ABI.
+ *
+ * The reference's C++ solver layer (src/solver, src/linearsolver) reaches the
+ * accelerator through these entry points.  Each replaces one step of the
+ * reference's FETISolver / IterSolver / Cluster call chain:
+ *
+ *   espreso_feti_create          FETISolver::update -> new SuperCluster/IterSolver
+ *                                (src/solver/generic/FETISolver.cpp:59-76)
+ *   espreso_feti_set_subdomain   Domain::SetDomain inputs: K_i, f_i, R_i (kernel),
+ *                                B1 rows, B1_scale_vec (src/solver/generic/Domain.cpp:67-215,
+ *                                src/solver/generic/FETISolver.cpp:99-103)
+ *   espreso_feti_set_dual        B1c (vec_c) (src/solver/specific/cluster.cpp:2732-2769)
+ *   espreso_feti_setup           FETISolver::init: G/GG^T (itersolver.cpp:4931-5121),
+ *                                Dirichlet preconditioner (clustercpu.cpp:167-333),
+ *                                local Schur complements (clustercpu.cpp:11-40)
+ *   espreso_feti_solve           FETISolver::solve -> Solve_RegCG + primal
+ *                                (FETISolver.cpp:119-149, itersolver.cpp:1406-1724, :372-431)
+ *   espreso_feti_apply_F         IterSolverCPU::apply_A_l_comp_dom_B (itersolvercpu.cpp:9-114)
+ *   espreso_feti_apply_prec      IterSolverCPU::Apply_Prec (itersolvercpu.cpp:520-602)
+ *   espreso_feti_project         IterSolverBase::Projector_Inv mode 0 (itersolver.cpp:5541-5623)
+ *
+ * plus a structured-grid problem builder standing in for the reference's
+ * GRID generator + assembler (gridgenerator.cpp:44-69, physics.cpp:147-228,
+ * equalityconstraints.cpp:25-423, structuralmechanics3d.cpp:34-184).
+ *
+ * Conventions: every function returns 0 on success or a negative status; the
+ * message of the last failure on the calling thread is espreso_last_error().
+ * All pointers are caller-owned host memory unless the name says "device".
+ * The library never calls exit().
+ */
+#ifndef ESPRESO_AMD_H
+#define ESPRESO_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ESPRESO_OK = 0,
+    ESPRESO_ERR_ARG = -1,        /* invalid argument                     */
+    ESPRESO_ERR_HIP = -2,        /* HIP runtime failure / no device      */
+    ESPRESO_ERR_NUMERIC = -3,    /* factorisation breakdown, NaN/Inf     */
+    ESPRESO_ERR_STATE = -4,      /* call out of order                    */
+    ESPRESO_ERR_MEMORY = -5      /* device memory budget exceeded        */
+};
+
+/* Structured box problem (same field meaning as the oracle's). */
+typedef struct {
+    int32_t physics;            /* 0: 3D linear elasticity, 1: 3D heat        */
+    int32_t clusters[3];
+    int32_t domains[3];         /* subdomains per cluster                     */
+    int32_t elements[3];        /* elements per subdomain                     */
+    double  length[3];
+    double  young, poisson, density;
+    double  accel[3];
+    double  conductivity;
+    double  heat_source;
+    int32_t fixed[6];           /* faces x0,x1,y0,y1,z0,z1: fixed component bitmask */
+    double  fixed_value[6];
+    double  pressure[6];        /* normal pressure (>0 pushes inward)          */
+    uint64_t traction_seed;
+    double  traction_scale;
+    uint64_t young_seed;
+    uint64_t conductivity_seed;
+} espreso_grid_problem;
+
+/* Values follow the reference enums (src/config/ecf/solver/feti.h). */
+enum { ESPRESO_PREC_NONE = 0, ESPRESO_PREC_LUMPED = 1, ESPRESO_PREC_WEIGHT = 2, ESPRESO_PREC_DIRICHLET = 3 };
+enum { ESPRESO_SC_FP64 = 0, ESPRESO_SC_FP32 = 1, ESPRESO_SC_BF16 = 2 };
+
+typedef struct {
+    int32_t method;             /* 0 TOTAL_FETI                                */
+    int32_t preconditioner;     /* ESPRESO_PREC_*                              */
+    int32_t max_iterations;
+    int32_t scaling;            /* 1: K-diagonal B1 scaling; 0: 1/multiplicity */
+    int32_t sc_precision;       /* storage of F_i: ESPRESO_SC_*                */
+    int32_t device;             /* HIP device ordinal                          */
+    double  precision;          /* stop: ||P r|| < precision*min(||P r0||,||b||) */
+    double  memory_budget_gb;   /* 0 = 85% of free device memory              */
+} espreso_feti_options;
+
+typedef struct {
+    int32_t iterations;
+    int32_t n_lambda;
+    int32_t n_coarse;
+    int32_t n_subdomains;
+    double  final_norm;
+    double  tol;
+    double  norm_b;
+    double  time_assembly;      /* S_i, S_D assembly on the GPU (s)            */
+    double  time_coarse;        /* G, GG^T setup (s)                           */
+    double  time_pcg;           /* PCG loop (s)                                */
+    double  time_primal;        /* primal recovery (s)                         */
+    double  time_total;         /* setup + solve (s)                           */
+    double  sc_bytes;           /* bytes of packed F_i (storage precision)     */
+    double  sd_bytes;           /* bytes of packed S_D                         */
+    double  assembly_flops;     /* flops executed by the assembly kernels      */
+    int32_t refinement_steps;   /* outer FP64 refinement steps (mixed precision) */
+    int32_t pad;
+} espreso_feti_stats;
+
+typedef struct espreso_feti_s espreso_feti;
+
+const char *espreso_last_error(void);
+int espreso_version(void);
+
+/* ---- problem builder (host) ----------------------------------------------
+ * Builds the decomposed problem of a structured box and feeds it to a FETI
+ * instance: per-subdomain K_i (CSR), f_i, orthonormal kernel R_i, B1 rows
+ * (Dirichlet + redundant gluing) with scaling, and c.  Lambda order: all
+ * Dirichlet rows (by global DOF, then subdomain), then gluing rows (by global
+ * DOF, then subdomain pair).                                                  */
+int espreso_grid_sizes(const espreso_grid_problem *p, int32_t *n_subdomains,
+                       int32_t *n_nodes, int32_t *ndof);
+int espreso_grid_load(espreso_feti *h, const espreso_grid_problem *p);
+/* Host-only export of subdomain i of the builder (no device needed).
+ * sizes: [0] n, [1] nnz, [2] kdim, [3] lambda rows of i, [4] total lambda,
+ * [5] subdomains.  Pass NULL arrays to query sizes only.  c (total lambda)
+ * may be NULL.                                                               */
+int espreso_grid_export(const espreso_grid_problem *p, int32_t scaling, int32_t i, int32_t *sizes,
+                        int32_t *rowptr, int32_t *colidx, double *values, double *f, double *R,
+                        int32_t *lambda, int32_t *dof, double *sign, double *scale, double *c);
+/* global node id of every local node of subdomain i (after espreso_grid_load) */
+int espreso_grid_l2g(espreso_feti *h, int32_t i, int32_t *l2g);
+/* merge per-subdomain primal of the last solve into nodal u (average of copies) */
+int espreso_grid_nodal_solution(espreso_feti *h, double *u_nodal);
+
+/* ---- FETI core --------------------------------------------------------- */
+int espreso_feti_create(espreso_feti **h, int32_t n_subdomains, const espreso_feti_options *o);
+int espreso_feti_destroy(espreso_feti *h);
+/* sizes[0] = subdomains, [1] = lambda (after set_dual, else -1),
+ * [2] = coarse dimension (after setup, else 0)                               */
+int espreso_feti_sizes(espreso_feti *h, int32_t *sizes);
+
+/* K: full symmetric CSR (0-based) of n x n; f: n; R: n x kdim column-major,
+ * orthonormal columns spanning ker K; lambda rows touching the subdomain
+ * (ascending lambda id) with local DOF, sign (+1/-1) and scaling weight.     */
+int espreso_feti_set_subdomain(espreso_feti *h, int32_t i, int32_t n,
+                               const int32_t *rowptr, const int32_t *colidx, const double *values,
+                               const double *f, int32_t kdim, const double *R,
+                               int32_t nlam, const int32_t *lambda, const int32_t *dof,
+                               const double *sign, const double *scale);
+int espreso_feti_set_dual(espreso_feti *h, int32_t n_lambda, const double *c);
+
+/* GPU setup: boundary Schur complements, F_i = (K^+)_bb, S_D, coarse problem */
+int espreso_feti_setup(espreso_feti *h);
+
+/* Projected PCG to the stop criterion, then primal recovery. */
+int espreso_feti_solve(espreso_feti *h, espreso_feti_stats *st);
+/* per-subdomain primal solution of the last solve (n_i values) */
+int espreso_feti_primal(espreso_feti *h, int32_t i, double *u);
+/* dual solution lambda of the last solve (n_lambda values) */
+int espreso_feti_dual(espreso_feti *h, double *lambda);
+
+/* Dual operators on host vectors of length n_lambda (parity / inspection). */
+int espreso_feti_apply_F(espreso_feti *h, const double *x, double *y);
+int espreso_feti_apply_prec(espreso_feti *h, const double *x, double *y);
+int espreso_feti_project(espreso_feti *h, const double *x, double *y);
+int espreso_feti_vec_b(espreso_feti *h, double *b);
+/* dense m_i x m_i (column-major, boundary DOFs in ascending local order):
+ * which 0: F_i = (K^+)_bb in FP64, 1: S_D, 2: F_i as stored (after rounding) */
+int espreso_feti_subdomain_dense(espreso_feti *h, int32_t i, int32_t which, double *out);
+int espreso_feti_subdomain_boundary(espreso_feti *h, int32_t i, int32_t *m, int32_t *bdofs);
+
+/* Timed dual-operator applies on device-resident vectors (benchmark use):
+ * runs `reps` applies of F (op 0) or of the preconditioner (op 1) and
+ * returns the mean time of one apply and of its SYMV kernel in ms.          */
+int espreso_feti_time_apply(espreso_feti *h, int32_t op, int32_t reps,
+                            double *ms_apply, double *ms_symv);
+
+/* Diagnostics of the FP64 MFMA tile routine used by the assembly.
+ * espreso_debug_mfma: one v_mfma_f64_16x16x4f64 with lane l holding a[l], b[l];
+ * out[4*l + v] = accumulator value v of lane l.
+ * espreso_debug_tile_gemm: C = alpha op(A) op(B) + beta C for one tile
+ * (M, N <= 64), column-major host arrays.                                     */
+int espreso_debug_mfma(const double *a, const double *b, double *out);
+int espreso_debug_tile_gemm(int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const double *A, int32_t lda,
+                            const double *B, int32_t ldb, double *C, int32_t ldc, double alpha, double beta);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,175 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bars: assembled matrices and dual operators (same generalised inverse as the
+oracle's kplus=1 mode) within 1e-10 relative (FP64 reassociation only);
+primal solutions within 1e-8 relative of the undecomposed direct solve
+(FP64, stop at precision 1e-10) and of the golden values.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from espreso_amd import core
+from espreso_amd import problems as PB
+
+pytestmark = pytest.mark.gpu
+
+SMALL = PB.scaled(PB.GRAVITY, traction_seed=9, traction_scale=1e5)          # 16 sub of 5^3
+MULTI = PB.scaled(PB.GRAVITY, clusters=(1, 1, 1), domains=(2, 2, 2), elements=(8, 8, 8),
+                  length=(1.0, 1.0, 1.0), young=2.1e11, traction_seed=3, traction_scale=1e3)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def small_pair():
+    say("case: SMALL dirichlet fp64")
+    g = core.GridFETI(SMALL, preconditioner="dirichlet", precision=1e-10)
+    g.setup()
+    o = O.Oracle(SMALL, preconditioner=3, precision=1e-10, kplus=1)
+    yield g, o
+    g.close()
+    o.close()
+
+
+@pytest.mark.parametrize("spec_name", ["SMALL", "MULTI"])
+def test_assembled_blocks(spec_name, small_pair):
+    spec = SMALL if spec_name == "SMALL" else MULTI
+    say("case: assembled blocks", spec_name)
+    if spec_name == "SMALL":
+        g, o = small_pair
+    else:
+        g = core.GridFETI(spec, preconditioner="dirichlet")
+        g.setup()
+        o = O.Oracle(spec, preconditioner=3, kplus=1)
+    for s in range(o.n_sub):
+        assert np.array_equal(g.boundary(s), o.boundary(s))
+        SD = g.dense(s, 1)
+        assert rel(SD, o.dense(s, 3)) < 1e-10, s
+        F = g.dense(s, 0)
+        Fo = o.dense(s, 4)
+        assert rel(F, Fo) < 1e-9, s
+    if spec_name != "SMALL":
+        g.close()
+
+
+def test_dual_operators(small_pair):
+    g, o = small_pair
+    say("case: dual operators")
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(o.n_lambda)
+    assert g.n_lambda == o.n_lambda
+    assert rel(g.apply_F(x), o.apply_F(x)) < 1e-10
+    assert rel(g.apply_prec(x), o.apply_prec(x)) < 1e-10
+    assert rel(g.project(x), o.project(x)) < 1e-10
+    assert rel(g.vec_b(), o.vec_b()) < 1e-10
+
+
+@pytest.mark.parametrize("prec,oprec", [("lumped", 1), ("weight", 2), ("none", 0)])
+def test_other_preconditioners(prec, oprec):
+    say("case: preconditioner", prec)
+    g = core.GridFETI(SMALL, preconditioner=prec)
+    g.setup()
+    o = O.Oracle(SMALL, preconditioner=oprec, kplus=1)
+    x = np.random.default_rng(2).standard_normal(o.n_lambda)
+    assert rel(g.apply_prec(x), o.apply_prec(x)) < 1e-12
+    g.close()
+
+
+def test_solve_small_vs_global(small_pair):
+    g, o = small_pair
+    say("case: solve SMALL")
+    st = g.solve()
+    u = g.nodal_solution()
+    ug = O.global_solve(SMALL)
+    assert rel(u, ug) < 1e-8
+    uo, _, sto = o.solve()
+    assert abs(st["iterations"] - sto["iterations"]) <= 1
+    for s in range(o.n_sub):
+        assert rel(g.primal(s, o.info(s)["n"]), o.primal(s)) < 1e-8
+
+
+def test_pressure_patch_golden():
+    say("case: pressure patch")
+    g = core.GridFETI(PB.PRESSURE_PATCH, preconditioner="lumped", precision=1e-8)
+    g.setup()
+    g.solve()
+    u = g.nodal_solution()
+    nn = (11, 21, 11)
+    z1 = np.linalg.norm(u, axis=1).reshape(nn[2], nn[1], nn[0])[-1]
+    assert np.allclose(z1, 1.0, rtol=0, atol=1e-7)
+    g.close()
+
+
+def test_gravity_emr_golden():
+    say("case: gravity")
+    g = core.GridFETI(PB.GRAVITY, preconditioner="dirichlet", precision=1e-8)
+    g.setup()
+    g.solve()
+    mag = np.linalg.norm(g.nodal_solution(), axis=1)
+    gold = PB.GRAVITY_EMR
+    assert abs(mag.max() - gold["max"]) / gold["max"] < 1e-3
+    assert abs(mag.mean() - gold["avg"]) / gold["avg"] < 1e-3
+    g.close()
+
+
+def test_heat_golden():
+    say("case: heat")
+    g = core.GridFETI(PB.HEAT_SIMPLE, preconditioner="dirichlet", precision=1e-10)
+    g.setup()
+    g.solve()
+    u = g.nodal_solution()[:, 0]
+    assert abs(u.min() - 100) < 1e-6 and abs(u.max() - 300) < 1e-6 and abs(u.mean() - 200) < 1e-6
+    g.close()
+
+
+@pytest.mark.parametrize("prec", ["dirichlet", "lumped"])
+def test_tfeti_64x5_vs_global(prec):
+    say("case: 64 subdomains of 5^3", prec)
+    spec = PB.TFETI_64x5
+    g = core.GridFETI(spec, preconditioner=prec, precision=1e-10)
+    g.setup()
+    st = g.solve()
+    ug = O.global_solve(spec)
+    assert rel(g.nodal_solution(), ug) < 1e-8
+    assert st["iterations"] < 200
+    g.close()
+
+
+@pytest.mark.parametrize("sc", ["fp32", "bf16"])
+def test_mixed_precision_refinement(sc):
+    say("case: mixed precision", sc)
+    g = core.GridFETI(MULTI, preconditioner="dirichlet", precision=1e-10, sc_precision=sc)
+    g.setup()
+    st = g.solve()
+    ug = O.global_solve(MULTI)
+    err = rel(g.nodal_solution(), ug)
+    say("mixed", sc, "err", err, "iters", st["iterations"], "refine", st["refinement_steps"])
+    assert err < 1e-8
+    # stored F_i is the FP64 F_i rounded to the storage precision
+    F64 = g.dense(0, 0)
+    Flp = g.dense(0, 2)
+    tol = 2 ** -23 if sc == "fp32" else 2 ** -8
+    assert np.abs(Flp - F64).max() <= 2 * tol * np.abs(F64).max()
+    g.close()
+
+
+def test_invalid_input_is_reported():
+    say("case: invalid input")
+    s = core.FETISolver(1, preconditioner="dirichlet")
+    with pytest.raises(RuntimeError):
+        s.setup()  # no subdomain data
+    rp = np.array([0, 1], np.int32)
+    with pytest.raises(RuntimeError):
+        s.set_subdomain(0, (rp, np.array([0], np.int32), np.array([np.nan])), np.zeros(1), np.ones((1, 1)),
+                        [0], [0], [1.0], [1.0])
+    s.close()
