@@ -80,6 +80,9 @@ class Stats(C.Structure):
         ("assembly_flops", C.c_double),
         ("refinement_steps", C.c_int32),
         ("pad", C.c_int32),
+        ("time_load", C.c_double),
+        ("symv_bytes", C.c_double),
+        ("symv_bytes_sd", C.c_double),
     ]
 
     def as_dict(self):
@@ -117,6 +120,7 @@ SIGNATURES = {
     "espreso_feti_subdomain_boundary": (C.c_int, [_H, C.c_int32, _I, _I]),
     "espreso_feti_time_apply": (C.c_int, [_H, C.c_int32, C.c_int32, _D, _D]),
     "espreso_debug_mfma": (C.c_int, [_D, _D, _D]),
+    "espreso_debug_copy": (C.c_int, [C.c_int64, C.c_int32, _D]),
     "espreso_debug_tile_gemm": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _D, C.c_int32, _D,
                                           C.c_int32, _D, C.c_int32, C.c_double, C.c_double]),
 }
@@ -128,6 +132,12 @@ def debug_mfma(a, b):
     out = np.zeros(256)
     _check(lib().espreso_debug_mfma(_dp(a), _dp(b), _dp(out)))
     return out.reshape(64, 4)
+
+
+def debug_copy(nbytes, reps=5):
+    ms = C.c_double()
+    _check(lib().espreso_debug_copy(int(nbytes), reps, C.byref(ms)))
+    return ms.value
 
 
 def debug_tile_gemm(A, B, C, ta=False, tb=False, alpha=1.0, beta=0.0):

@@ -68,7 +68,9 @@ struct PrimalDev {
 // y_b (D.yb) = A x_b for every subdomain; A = S_D (sd) or F_i in storage
 // precision `prec` (0 FP64, 1 FP32, 2 BF16).  x is lambda-space and gathered
 // with sign (or sign*scale when `scaled`) if `gather`, else boundary-space.
-void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gather, bool scaled, hipStream_t st);
+// ev0/ev1 (optional) are recorded around the SYMV kernel alone.
+void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gather, bool scaled, hipStream_t st,
+                 hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_lambda_gather(const DualDev &D, const double *yb, bool scaled, double *y, hipStream_t st);
 void launch_boundary_gather(const DualDev &D, const double *x, bool scaled, double *xb, hipStream_t st);
 void launch_kbb(const DualDev &D, const double *xb, double *yb, hipStream_t st);
@@ -83,5 +85,7 @@ void launch_axpby(double *y, const double *a, const double *b, int64_t n, double
 void launch_copy_scalar(const double *src, double *dst, hipStream_t st);
 void launch_symmetrize(double *A, int n, hipStream_t st);
 void launch_primal(const PrimalDev &P, int nsub, const double *ub, double *u, hipStream_t st);
+// plain copy of n doubles (n multiple of 4): calibration of HBM byte counters
+void launch_copy(const double *src, double *dst, int64_t n, hipStream_t st);
 
 }  // namespace esp

@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(256) k_primal_interior(const PrimalDev P, cons
     __shared__ double red[4];
     __shared__ double piv;
     const int s = blockIdx.x;
-    const int ni = P.ni[s], m = P.m[s], n = P.n[s], bw = P.bw[s];
+    const int ni = P.ni[s], n = P.n[s], bw = P.bw[s];
     const int w = bw + 1;
     const double *L = P.band + P.band_off[s];
     const double *yf = P.yf + P.yf_off[s];
@@ -345,13 +345,15 @@ __global__ void __launch_bounds__(256) k_primal_interior(const PrimalDev P, cons
 static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 static inline unsigned vgrid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, 256), 4096)); }
 
-void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gather, bool scaled, hipStream_t st) {
+void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gather, bool scaled, hipStream_t st,
+                 hipEvent_t ev0, hipEvent_t ev1) {
     if (sd) prec = 0;
     const bool lp = prec != 0;
     const SymvItem *items = lp ? D.items_lp : D.items_d;
     const int n_items = lp ? D.n_items_lp : D.n_items_d;
     const double *coef = scaled ? D.bl_sscale : D.bl_sign;
     const size_t shm = 2 * (size_t)D.max_m * (lp ? 4 : 8);
+    if (ev0) (void)hipEventRecord(ev0, st);
     if (prec == 0) {
         const double *pool = sd ? D.SD : D.Fd;
         const int64_t *off = sd ? D.sdoff : D.foff;
@@ -364,6 +366,7 @@ void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gathe
         hipLaunchKernelGGL((k_symv<uint16_t, STRIP_B16>), dim3(n_items), dim3(256), shm, st, items, D.m, D.boff, D.Fh,
                            D.foff_lp, (const float *)D.fscale, D.bl_ptr, D.bl_lam, coef, x, gather, D.ypart);
     }
+    if (ev1) (void)hipEventRecord(ev1, st);
     const int R = strip_rows(prec);
     hipLaunchKernelGGL(k_part_reduce, dim3(D.nsub), dim3(256), 0, st, items, lp ? D.item0_lp : D.item0_d,
                        lp ? D.nitems_lp : D.nitems_d, D.m, D.boff, R, D.ypart, D.yb);
@@ -422,6 +425,15 @@ void launch_copy_scalar(const double *src, double *dst, hipStream_t st) {
 
 void launch_symmetrize(double *A, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_symmetrize, dim3(vgrid((int64_t)n * n)), dim3(256), 0, st, A, n);
+}
+
+__global__ void k_copy(const double4 *__restrict__ src, double4 *__restrict__ dst, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void launch_copy(const double *src, double *dst, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, st, (const double4 *)src, (double4 *)dst, n / 4);
 }
 
 void launch_primal(const PrimalDev &P, int nsub, const double *ub, double *u, hipStream_t st) {

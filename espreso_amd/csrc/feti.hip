@@ -72,7 +72,7 @@ struct espreso_feti_s {
     std::vector<double> c;
     hipStream_t st = nullptr;
     std::vector<void *> allocs;
-    bool ready = false, solved = false;
+    bool loaded = false, ready = false, solved = false;
     // grid bookkeeping
     int ndof = 0, nnodes = 0;
     std::vector<std::vector<int>> l2g;
@@ -84,6 +84,8 @@ struct espreso_feti_s {
            *d_Ap = nullptr, *d_b = nullptr, *d_tmp = nullptr, *d_r2 = nullptr, *d_partial = nullptr,
            *d_scal = nullptr;
     double *d_u = nullptr;    // per-subdomain primal (local order), sum n
+    double *d_f = nullptr, *d_Qm = nullptr, *d_GGt = nullptr, *d_c = nullptr;  // resident inputs
+    std::vector<int64_t> h_foff, h_lpoff, h_boff;
     double *d_e = nullptr;    // -R^T f
     std::vector<int64_t> uoff;
     std::vector<double> lambda_h;
@@ -118,10 +120,11 @@ struct espreso_feti_s {
     }
     void sync() { ESP_HIP(hipStreamSynchronize(st)); }
 
-    void setup();
+    void load();
+    void gpu_setup();
     void assemble(const double *d_f, const double *d_Q, const std::vector<int64_t> &foff,
                   const std::vector<int64_t> &lpoff, const std::vector<int64_t> &boff);
-    void coarse(const std::vector<double> &GGt);
+    void coarse();
     void build_items(int prec, std::vector<SymvItem> &items, std::vector<int> &item0, std::vector<int> &nit,
                      int64_t &plen);
     // dual operators on device vectors
@@ -133,7 +136,7 @@ struct espreso_feti_s {
 };
 
 // ----------------------------------------------------------------- setup ----
-void espreso_feti_s::setup() {
+void espreso_feti_s::load() {
     ESP_CHECK(nlam >= 0, ESPRESO_ERR_STATE, "espreso_feti_set_dual not called");
     for (int s = 0; s < nsub; s++) ESP_CHECK(sub[s].set, ESPRESO_ERR_STATE, "subdomain " + std::to_string(s) + " not set");
     double t0 = wall();
@@ -404,21 +407,13 @@ void espreso_feti_s::setup() {
         P.alpha = alloc<double>(ncoarse);
         d_u = alloc<double>(n_off[nsub]);
         uoff = n_off;
-        double *d_f = upload(ff);
-        sync();
-        // ---- assembly of S_D and F_i on the GPU, in waves ----
-        last.time_coarse = 0;
-        double ta = wall();
-        assemble(d_f, d_Q, foff, lpoff, boff);
-        last.time_assembly = wall() - ta;
-        release(d_f);
-        release(d_Q);
+        d_f = upload(ff);
     }
-
-    // ---- coarse problem (GG^T)^-1 on the GPU ----
-    double tc = wall();
-    coarse(GGt);
-    last.time_coarse = wall() - tc;
+    d_Qm = d_Q;
+    d_GGt = upload(GGt);
+    h_foff = foff;
+    h_lpoff = lpoff;
+    h_boff = boff;
 
     // ---- SYMV work lists ----
     {
@@ -439,10 +434,6 @@ void espreso_feti_s::setup() {
         }
         D.ypart = alloc<double>(std::max(plen_d, plen_lp));
     }
-    // K^+ f on the boundary: ubf = F_i hf (FP64)
-    launch_symv(D, false, 0, d_hf, 0, false, st);
-    ESP_HIP(hipMemcpyAsync(d_ubf, D.yb, nb * sizeof(double), hipMemcpyDeviceToDevice, st));
-
     // PCG vectors
     d_x = alloc<double>(nlam); d_r = alloc<double>(nlam); d_w = alloc<double>(nlam); d_y = alloc<double>(nlam);
     d_z = alloc<double>(nlam); d_p = alloc<double>(nlam); d_Ap = alloc<double>(nlam); d_b = alloc<double>(nlam);
@@ -450,14 +441,42 @@ void espreso_feti_s::setup() {
     d_r2 = alloc<double>(nlam);
     d_partial = alloc<double>(1024);
     d_scal = alloc<double>(16);
-    // b = B K^+ f - c
-    double *d_c = upload(c);
+    d_c = upload(c);
+    {
+        const int ssz = prec == 0 ? 8 : (prec == 1 ? 4 : 2);
+        double mat = 0, mat64 = 0, gat = 0, yb = 0;
+        for (int s = 0; s < nsub; s++) {
+            const double m = sub[s].m;
+            mat += ssz * m * (m + 1) / 2;
+            mat64 += 8 * m * (m + 1) / 2;
+            gat += 20.0 * sub[s].lam.size();
+            yb += 8 * m;
+        }
+        last.symv_bytes = mat + gat + yb;
+        last.symv_bytes_sd = mat64 + gat + yb;
+    }
+    sync();
+    loaded = true;
+    last.time_load = wall() - t0;
+}
+
+// GPU part of the setup, repeatable on the resident inputs: local dual
+// operators and Dirichlet blocks, coarse inverse, right-hand side b.
+void espreso_feti_s::gpu_setup() {
+    ESP_CHECK(loaded, ESPRESO_ERR_STATE, "inputs not loaded");
+    double ta = wall();
+    assemble(d_f, d_Qm, h_foff, h_lpoff, h_boff);
+    last.time_assembly = wall() - ta;
+    double tc = wall();
+    coarse();
+    // K^+ f on the boundary: ubf = F_i hf (FP64), then b = B K^+ f - c
+    launch_symv(D, false, 0, d_hf, 0, false, st);
+    ESP_HIP(hipMemcpyAsync(d_ubf, D.yb, D.nbnd * sizeof(double), hipMemcpyDeviceToDevice, st));
     launch_lambda_gather(D, d_ubf, false, d_b, st);
     launch_axpby(d_b, d_b, d_c, nlam, 1.0, -1.0, st);
     sync();
-    release(d_c);
+    last.time_coarse = wall() - tc;
     ready = true;
-    (void)t0;
 }
 
 void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::vector<int64_t> &foff,
@@ -587,10 +606,13 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
     }
 }
 
-void espreso_feti_s::coarse(const std::vector<double> &GGt) {
+void espreso_feti_s::coarse() {
     const int nc = ncoarse;
     const int nt = (nc + TILE - 1) / TILE;
-    double *S = upload(GGt);
+    if (D.GGinv) release(D.GGinv);
+    D.GGinv = nullptr;
+    double *S = alloc<double>((size_t)nc * nc);
+    ESP_HIP(hipMemcpyAsync(S, d_GGt, (size_t)nc * nc * sizeof(double), hipMemcpyDeviceToDevice, st));
     double *W = alloc<double>((size_t)nc * nc);
     double *Dinv = alloc<double>((size_t)nt * TILE * TILE);
     int *info = alloc<int>(1);
@@ -907,7 +929,8 @@ int espreso_feti_setup(espreso_feti *h) {
     ESP_API_BEGIN
     ESP_CHECK(h, ESPRESO_ERR_ARG, "null handle");
     ESP_HIP(hipSetDevice(h->opt.device));
-    h->setup();
+    if (!h->loaded) h->load();
+    h->gpu_setup();
     ESP_API_END
 }
 
@@ -1015,9 +1038,7 @@ int espreso_feti_time_apply(espreso_feti *h, int32_t op, int32_t reps, double *m
     float tot = 0;
     ESP_HIP(hipEventElapsedTime(&tot, e0, e1));
     for (int r = 0; r < reps; r++) {
-        ESP_HIP(hipEventRecord(s0, h->st));
-        launch_symv(h->D, op != 0, op == 0 ? h->opt.sc_precision : 0, h->d_z, 1, op != 0, h->st);
-        ESP_HIP(hipEventRecord(s1, h->st));
+        launch_symv(h->D, op != 0, op == 0 ? h->opt.sc_precision : 0, h->d_z, 1, op != 0, h->st, s0, s1);
         ESP_HIP(hipEventSynchronize(s1));
         float t = 0;
         ESP_HIP(hipEventElapsedTime(&t, s0, s1));
@@ -1029,6 +1050,35 @@ int espreso_feti_time_apply(espreso_feti *h, int32_t op, int32_t reps, double *m
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(s0);
     (void)hipEventDestroy(s1);
+    ESP_API_END
+}
+
+int espreso_debug_copy(int64_t bytes, int32_t reps, double *ms) {
+    ESP_API_BEGIN
+    ESP_CHECK(bytes >= 32 && reps > 0, ESPRESO_ERR_ARG, "copy size must be >= 32 bytes");
+    const int64_t n = bytes / 32 * 4;
+    double *a = nullptr, *b = nullptr;
+    ESP_HIP(hipMalloc(&a, n * sizeof(double)));
+    hipError_t e = hipMalloc(&b, n * sizeof(double));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipMemset(a, 0, n * sizeof(double));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float t = 0;
+    if (e == hipSuccess) {
+        launch_copy(a, b, n, nullptr);
+        (void)hipEventRecord(e0, nullptr);
+        for (int r = 0; r < reps; r++) launch_copy(a, b, n, nullptr);
+        (void)hipEventRecord(e1, nullptr);
+        e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    ESP_HIP(e);
+    *ms = t / reps;
     ESP_API_END
 }
 

@@ -100,6 +100,10 @@ typedef struct {
     double  assembly_flops;     /* flops executed by the assembly kernels      */
     int32_t refinement_steps;   /* outer FP64 refinement steps (mixed precision) */
     int32_t pad;
+    double  time_load;          /* host preprocessing + upload of the inputs (s), not in time_total */
+    double  symv_bytes;         /* algorithmic bytes of one F SYMV launch: packed F_i in storage
+                                   precision + lambda gather (8+4+8 B per B1 entry) + y_b (8 B) */
+    double  symv_bytes_sd;      /* the same for one S_D SYMV launch (FP64)                    */
 } espreso_feti_stats;
 
 typedef struct espreso_feti_s espreso_feti;
@@ -145,7 +149,9 @@ int espreso_feti_set_subdomain(espreso_feti *h, int32_t i, int32_t n,
                                const double *sign, const double *scale);
 int espreso_feti_set_dual(espreso_feti *h, int32_t n_lambda, const double *c);
 
-/* GPU setup: boundary Schur complements, F_i = (K^+)_bb, S_D, coarse problem */
+/* GPU setup: boundary Schur complements, F_i = (K^+)_bb, S_D, coarse problem.
+ * The first call also preprocesses and uploads the inputs; later calls redo
+ * only the GPU part on the resident inputs.                                  */
 int espreso_feti_setup(espreso_feti *h);
 
 /* Projected PCG to the stop criterion, then primal recovery. */
@@ -167,7 +173,8 @@ int espreso_feti_subdomain_boundary(espreso_feti *h, int32_t i, int32_t *m, int3
 
 /* Timed dual-operator applies on device-resident vectors (benchmark use):
  * runs `reps` applies of F (op 0) or of the preconditioner (op 1) and
- * returns the mean time of one apply and of its SYMV kernel in ms.          */
+ * returns the mean time of one apply and of its SYMV kernel alone in ms
+ * (HIP events recorded on the stream the kernels run on).                   */
 int espreso_feti_time_apply(espreso_feti *h, int32_t op, int32_t reps,
                             double *ms_apply, double *ms_symv);
 
@@ -177,6 +184,9 @@ int espreso_feti_time_apply(espreso_feti *h, int32_t op, int32_t reps,
  * espreso_debug_tile_gemm: C = alpha op(A) op(B) + beta C for one tile
  * (M, N <= 64), column-major host arrays.                                     */
 int espreso_debug_mfma(const double *a, const double *b, double *out);
+/* Plain device copy of `bytes` (read + write), `reps` times: mean ms per copy.
+ * Used to calibrate HBM byte counters of the profiler.                      */
+int espreso_debug_copy(int64_t bytes, int32_t reps, double *ms);
 int espreso_debug_tile_gemm(int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const double *A, int32_t lda,
                             const double *B, int32_t ldb, double *C, int32_t ldc, double alpha, double beta);
 

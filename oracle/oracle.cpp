@@ -1283,6 +1283,7 @@ int oracle_cpu_sample(const oracle_grid_problem *p, const oracle_options *o, int
         h.L = make_loads(h.G);
         element_matrices(p->physics, h.G.h, p->poisson, h.Ke1, h.Nint);
         h.sub.assign(h.G.nsub, Subdomain());
+#pragma omp parallel for schedule(dynamic)
         for (int s = 0; s < h.G.nsub; s++) {
             h.assemble(h.sub[s], s);
             h.kernels(h.sub[s]);
