@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--apply-reps", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-symv", action="store_true",
+                    help="only: one setup, --apply-reps F applies and a 1 GiB copy (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -83,6 +85,18 @@ def main():
     def step():
         g.setup()
         return g.solve()
+
+    if args.profile_symv:
+        g.setup()
+        sizes = g.sizes()
+        ms_F, ms_symv = g.time_apply(0, args.apply_reps)
+        ms_copy = core.debug_copy(1 << 30, 3)
+        st = g.solve()
+        print(json.dumps({"profile": "symv", "workload": args.workload, "sc": args.sc, "ms_symv": ms_symv,
+                          "ms_apply": ms_F, "symv_bytes": st["symv_bytes"], "copy_bytes": 1 << 30,
+                          "ms_copy": ms_copy, "lambda": sizes[1]}), flush=True)
+        g.close()
+        return
 
     for w in range(args.warmup):
         st = step()

@@ -27,7 +27,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // C(MxN) = alpha * op(A)(MxK) * op(B)(KxN) + beta * C, M,N <= 64.
 // op(A)(i,k) = TA ? A[k + i*lda] : A[i + k*lda];  op(B)(k,j) = TB ? B[j + k*ldb] : B[k + j*ldb].
 // C may alias A or B when this tile is the only reader of the aliased tile:
-// all reads finish before the first store.
+// all reads finish before the first store.  The next K chunk is fetched into
+// registers while the MFMAs of the current chunk run from LDS.
 template <bool TA, bool TB>
 __device__ __forceinline__ void tile_gemm(double *C, int ldc, const double *A, int lda, const double *B, int ldb,
                                           int M, int N, int K, double alpha, double beta) {
@@ -38,22 +39,40 @@ __device__ __forceinline__ void tile_gemm(double *C, int ldc, const double *A, i
     d4 acc[2][2];
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+    // element (row/col index, k index) handled by this thread in load slot `it`
+    auto aidx = [&](int it, int &i, int &kk) {
+        const int q = tid + 256 * it;
+        if (!TA) { i = q & 63; kk = q >> 6; } else { kk = q & 15; i = q >> 4; }
+    };
+    auto bidx = [&](int it, int &j, int &kk) {
+        const int q = tid + 256 * it;
+        if (!TB) { kk = q & 15; j = q >> 4; } else { j = q & 63; kk = q >> 6; }
+    };
+    double ra[4], rb[4];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int it = 0; it < 4; it++) {
+            int i, kk;
+            aidx(it, i, kk);
+            ra[it] = (i < M && k0 + kk < K) ? (TA ? A[(k0 + kk) + (size_t)i * lda] : A[i + (size_t)(k0 + kk) * lda]) : 0.0;
+            int j;
+            bidx(it, j, kk);
+            rb[it] = (j < N && k0 + kk < K) ? (TB ? B[j + (size_t)(k0 + kk) * ldb] : B[(k0 + kk) + (size_t)j * ldb]) : 0.0;
+        }
+    };
+    if (K > 0) fetch(0);
     for (int k0 = 0; k0 < K; k0 += 16) {
 #pragma unroll
         for (int it = 0; it < 4; it++) {
-            int q = tid + 256 * it;
             int i, kk;
-            if (!TA) { i = q & 63; kk = q >> 6; } else { kk = q & 15; i = q >> 4; }
-            double v = 0.0;
-            if (i < M && k0 + kk < K) v = TA ? A[(k0 + kk) + (size_t)i * lda] : A[i + (size_t)(k0 + kk) * lda];
-            As[kk][i] = v;
+            aidx(it, i, kk);
+            As[kk][i] = ra[it];
             int j;
-            if (!TB) { kk = q & 15; j = q >> 4; } else { j = q & 63; kk = q >> 6; }
-            v = 0.0;
-            if (j < N && k0 + kk < K) v = TB ? B[j + (size_t)(k0 + kk) * ldb] : B[(k0 + kk) + (size_t)j * ldb];
-            Bs[kk][j] = v;
+            bidx(it, j, kk);
+            Bs[kk][j] = rb[it];
         }
         __syncthreads();
+        if (k0 + 16 < K) fetch(k0 + 16);
 #pragma unroll
         for (int ks = 0; ks < 16; ks += 4) {
             double a0 = As[ks + (lane >> 4)][wi + (lane & 15)];
@@ -94,7 +113,9 @@ __device__ __forceinline__ void pick(const DenseDesc &d, int which, double *&A, 
     }
 }
 
-// ---- Cholesky: factor + invert the diagonal tile of step k (one WG each) ----
+// ---- left-looking block Cholesky, step k ----
+// diagonal tile: A_kk -= L_kw L_kw^T over the band window w, then factor the
+// tile and invert its factor (one workgroup per subdomain)
 __global__ void __launch_bounds__(256) k_potrf_diag(const DenseDesc *D, int which, int step) {
     const DenseDesc &d = D[blockIdx.x];
     double *A, *B;
@@ -103,14 +124,22 @@ __global__ void __launch_bounds__(256) k_potrf_diag(const DenseDesc *D, int whic
     const int k0 = step * TILE;
     if (k0 >= n) return;
     const int nb = min(TILE, n - k0);
+    const int j0 = max(0, step - band) * TILE;
+    double *Akk = A + k0 + (size_t)k0 * n;
+    if (k0 > j0) {
+        const double *Lk = A + k0 + (size_t)j0 * n;
+        tile_gemm<false, true>(Akk, n, Lk, n, Lk, n, nb, nb, k0 - j0, -1.0, 1.0);
+        __syncthreads();
+    }
     __shared__ double T[TILE][TILE + 1];
+    __shared__ double xd[TILE];
     __shared__ int bad;
     const int tid = threadIdx.x;
     if (tid == 0) bad = 0;
     for (int q = tid; q < TILE * TILE; q += 256) {
         int i = q & 63, j = q >> 6;
         double v;
-        if (i < nb && j < nb) v = (i >= j) ? A[(k0 + i) + (size_t)(k0 + j) * n] : 0.0;
+        if (i < nb && j < nb) v = (i >= j) ? Akk[i + (size_t)j * n] : 0.0;
         else v = (i == j) ? 1.0 : 0.0;
         T[i][j] = v;
     }
@@ -133,28 +162,30 @@ __global__ void __launch_bounds__(256) k_potrf_diag(const DenseDesc *D, int whic
         }
         __syncthreads();
     }
-    // inverse of the lower-triangular factor, one column per thread, written
-    // column-major straight into Dinv (each thread re-reads only its own column)
-    double *Dinv = d.Dinv + (size_t)step * TILE * TILE;
+    // X = T^-1 (lower): thread c computes column c; X[i][c] for i > c is kept
+    // in the unused strict upper triangle at T[c][i], X[c][c] in xd[c]
     if (tid < TILE) {
         const int c = tid;
-        double *X = Dinv + (size_t)c * TILE;
-        for (int i = 0; i < c; i++) X[i] = 0.0;
-        X[c] = 1.0 / T[c][c];
+        const double dc = 1.0 / T[c][c];
+        xd[c] = dc;
         for (int i = c + 1; i < TILE; i++) {
-            double s = 0.0;
-            for (int k = c; k < i; k++) s += T[i][k] * X[k];
-            X[i] = -s / T[i][i];
+            double s = T[i][c] * dc;
+            for (int k = c + 1; k < i; k++) s += T[i][k] * T[c][k];
+            T[c][i] = -s / T[i][i];
         }
     }
+    __syncthreads();
     if (tid == 0 && bad && d.info[0] == 0) d.info[0] = k0 + bad;
+    double *Dinv = d.Dinv + (size_t)step * TILE * TILE;
     for (int q = tid; q < TILE * TILE; q += 256) {
         int i = q & 63, j = q >> 6;
-        if (i < nb && j < nb && i >= j) A[(k0 + i) + (size_t)(k0 + j) * n] = T[i][j];
+        Dinv[q] = i > j ? T[j][i] : (i == j ? xd[i] : 0.0);
+        if (i < nb && j < nb && i >= j) Akk[i + (size_t)j * n] = T[i][j];
     }
 }
 
-// panel: A_ik <- A_ik * inv(L_kk)^T for the tiles below the diagonal inside the band
+// column below the diagonal: A_ik = (A_ik - L_iw L_kw^T) inv(L_kk)^T for
+// step < i <= step + band, window w = [max(0, i - band), step)
 __global__ void __launch_bounds__(256) k_potrf_panel(const DenseDesc *D, int which, int step) {
     const DenseDesc &d = D[blockIdx.y];
     double *A, *B;
@@ -164,61 +195,38 @@ __global__ void __launch_bounds__(256) k_potrf_panel(const DenseDesc *D, int whi
     if (i >= nt || i > step + band) return;
     const int k0 = step * TILE, i0 = i * TILE;
     const int nb = min(TILE, n - k0), mi = min(TILE, n - i0);
+    const int j0 = max(0, i - band) * TILE;
     double *C = A + i0 + (size_t)k0 * n;
+    if (k0 > j0) {
+        tile_gemm<false, true>(C, n, A + i0 + (size_t)j0 * n, n, A + k0 + (size_t)j0 * n, n, mi, nb, k0 - j0, -1.0,
+                               1.0);
+        __syncthreads();
+    }
     tile_gemm<false, true>(C, n, C, n, d.Dinv + (size_t)step * TILE * TILE, TILE, mi, nb, nb, 1.0, 0.0);
 }
 
-// trailing update: A_ij -= A_ik A_jk^T for step < j <= i <= step + band
-__global__ void __launch_bounds__(256) k_potrf_update(const DenseDesc *D, int which, int step) {
+// ---- left-looking forward substitution X = L^-1 B, block row i ----
+// X_iJ = inv(L_ii) (B_iJ - L_iw X_wJ), window w = [max(first nonzero tile of
+// column tile J, i - band), i)
+__global__ void __launch_bounds__(256) k_trsm_row(const DenseDesc *D, int which, int step) {
     const DenseDesc &d = D[blockIdx.y];
     double *A, *B;
     int n, band, nt, ncols, ntc;
     pick(d, which, A, n, band, nt, B, ncols, ntc);
-    const int q = blockIdx.x;
-    int di = (int)((sqrtf(8.0f * q + 1.0f) - 1.0f) * 0.5f);
-    while ((di + 1) * (di + 2) / 2 <= q) di++;
-    while (di * (di + 1) / 2 > q) di--;
-    const int dj = q - di * (di + 1) / 2;
-    if (di >= band) return;
-    const int i = step + 1 + di, j = step + 1 + dj;
-    if (i >= nt) return;
-    const int k0 = step * TILE, i0 = i * TILE, j0 = j * TILE;
-    const int nb = min(TILE, n - k0), mi = min(TILE, n - i0), mj = min(TILE, n - j0);
-    tile_gemm<false, true>(A + i0 + (size_t)j0 * n, n, A + i0 + (size_t)k0 * n, n, A + j0 + (size_t)k0 * n, n,
-                           mi, mj, nb, -1.0, 1.0);
-}
-
-// forward substitution, diagonal part: B_kJ <- inv(L_kk) B_kJ
-__global__ void __launch_bounds__(256) k_trsm_diag(const DenseDesc *D, int which, int step) {
-    const DenseDesc &d = D[blockIdx.y];
-    double *A, *B;
-    int n, band, nt, ncols, ntc;
-    pick(d, which, A, n, band, nt, B, ncols, ntc);
-    const int J = blockIdx.x;
-    if (J >= ntc || step >= nt) return;
+    const int J = blockIdx.x, i = step;
+    if (J >= ntc || i >= nt) return;
     const int cs = which == 0 ? d.colstart[J] : J;
-    if (cs > step) return;
-    const int k0 = step * TILE, j0 = J * TILE;
-    const int nb = min(TILE, n - k0), nj = min(TILE, ncols - j0);
-    double *C = B + k0 + (size_t)j0 * n;
-    tile_gemm<false, false>(C, n, d.Dinv + (size_t)step * TILE * TILE, TILE, C, n, nb, nj, nb, 1.0, 0.0);
-}
-
-// forward substitution, update: B_iJ -= L_ik B_kJ for step < i <= step + band
-__global__ void __launch_bounds__(256) k_trsm_update(const DenseDesc *D, int which, int step, int max_ntc) {
-    const DenseDesc &d = D[blockIdx.y];
-    double *A, *B;
-    int n, band, nt, ncols, ntc;
-    pick(d, which, A, n, band, nt, B, ncols, ntc);
-    const int J = blockIdx.x % max_ntc;
-    const int i = step + 1 + blockIdx.x / max_ntc;
-    if (J >= ntc || i >= nt || i > step + band) return;
-    const int cs = which == 0 ? d.colstart[J] : J;
-    if (cs > step) return;
-    const int k0 = step * TILE, i0 = i * TILE, j0 = J * TILE;
-    const int nb = min(TILE, n - k0), mi = min(TILE, n - i0), nj = min(TILE, ncols - j0);
-    tile_gemm<false, false>(B + i0 + (size_t)j0 * n, n, A + i0 + (size_t)k0 * n, n, B + k0 + (size_t)j0 * n, n,
-                            mi, nj, nb, -1.0, 1.0);
+    if (cs > i) return;
+    const int i0 = i * TILE, c0 = J * TILE;
+    const int mi = min(TILE, n - i0), nj = min(TILE, ncols - c0);
+    const int j0 = max(cs, i - band) * TILE;
+    double *C = B + i0 + (size_t)c0 * n;
+    if (i0 > j0) {
+        tile_gemm<false, false>(C, n, A + i0 + (size_t)j0 * n, n, B + j0 + (size_t)c0 * n, n, mi, nj, i0 - j0, -1.0,
+                                1.0);
+        __syncthreads();
+    }
+    tile_gemm<false, false>(C, n, d.Dinv + (size_t)i * TILE * TILE, TILE, C, n, mi, nj, mi, 1.0, 0.0);
 }
 
 // lower tiles of  S = S - Y^T Y  (which 0)  or  S = W^T W  (which 1)
@@ -340,9 +348,11 @@ __global__ void k_pack(const DenseDesc *D, T *pool, const int64_t *off, const fl
     const int c = r0 + blockIdx.y * 256 + threadIdx.x;
     if (c >= m) return;
     const double inv = scale ? 1.0 / (double)scale[blockIdx.z] : 1.0;
-    T *dst = pool + off[blockIdx.z] + strip_offset(m, R, t) + (int64_t)(c - r0) * R;
+    constexpr int VEC = 16 / (int)sizeof(T);
+    T *base = pool + off[blockIdx.z];
 #pragma unroll 4
     for (int r = 0; r < R; r++) {
+        T *dst = base + strip_index(m, R, VEC, t, r, c);
         const int row = r0 + r;
         double v = 0.0;
         if (row < m && row <= c) {
@@ -350,9 +360,9 @@ __global__ void k_pack(const DenseDesc *D, T *pool, const int64_t *off, const fl
             if (row == c) v *= 0.5;
         }
         v *= inv;
-        if constexpr (sizeof(T) == 8) dst[r] = (T)v;
-        else if constexpr (sizeof(T) == 4) dst[r] = (T)(float)v;
-        else dst[r] = to_bf16((float)v);
+        if constexpr (sizeof(T) == 8) *dst = (T)v;
+        else if constexpr (sizeof(T) == 4) *dst = (T)(float)v;
+        else *dst = to_bf16((float)v);
     }
 }
 
@@ -377,7 +387,7 @@ __global__ void k_unpack(const T *pool, int64_t off, int m, float scale, double 
         int i = (int)(e % m), j = (int)(e / m);
         int r = min(i, j), c = max(i, j);
         int t = r / R;
-        const T *src = pool + off + strip_offset(m, R, t) + (int64_t)(c - t * R) * R + (r - t * R);
+        const T *src = pool + off + strip_index(m, R, 16 / (int)sizeof(T), t, r - t * R, c);
         double v;
         if constexpr (sizeof(T) == 8) v = (double)*src;
         else if constexpr (sizeof(T) == 4) v = (double)*src;
@@ -453,14 +463,12 @@ void launch_potrf_step(const DenseDesc *desc, int nb, int which, int step, int m
     hipLaunchKernelGGL(k_potrf_diag, dim3(nb), dim3(256), 0, st, desc, which, step);
     if (max_band <= 0) return;
     hipLaunchKernelGGL(k_potrf_panel, dim3(max_band, nb), dim3(256), 0, st, desc, which, step);
-    hipLaunchKernelGGL(k_potrf_update, dim3(max_band * (max_band + 1) / 2, nb), dim3(256), 0, st, desc, which, step);
 }
 
 void launch_trsm_step(const DenseDesc *desc, int nb, int which, int step, int max_band, int max_ntc,
                       hipStream_t st) {
-    hipLaunchKernelGGL(k_trsm_diag, dim3(max_ntc, nb), dim3(256), 0, st, desc, which, step);
-    if (max_band <= 0) return;
-    hipLaunchKernelGGL(k_trsm_update, dim3(max_band * max_ntc, nb), dim3(256), 0, st, desc, which, step, max_ntc);
+    (void)max_band;
+    hipLaunchKernelGGL(k_trsm_row, dim3(max_ntc, nb), dim3(256), 0, st, desc, which, step);
 }
 
 void launch_syrk(const DenseDesc *desc, int nb, int which, int max_ntm, hipStream_t st) {

@@ -92,22 +92,28 @@ __global__ void __launch_bounds__(256) k_symv(const SymvItem *__restrict__ items
             xr[r] = (r0 + r < m) ? xs[r0 + r - j0] : (A)0;
             racc[r] = (A)0;
         }
-        for (int c = lane; c < ncol; c += 64) {
-            const u32x4 *src = (const u32x4 *)(S + (int64_t)c * R);
-            constexpr int NV = R * (int)sizeof(T) / 16;
-            u32x4 raw[NV];
+        constexpr int VEC = 16 / (int)sizeof(T);
+        constexpr int NV = R / VEC;
+        for (int c0 = 0; c0 < ncol; c0 += 64) {
+            const int w = min(64, ncol - c0);
+            if (lane < w) {
+                // chunk [R/VEC][w][VEC]: the q-th 16-byte load of lane l sits at (q*w + l)*VEC
+                const u32x4 *src = (const u32x4 *)(S + (int64_t)c0 * R) + lane;
+                u32x4 raw[NV];
 #pragma unroll
-            for (int q = 0; q < NV; q++) raw[q] = __builtin_nontemporal_load(src + q);
-            const T *a = (const T *)raw;
-            const A xc = xs[r0 + c - j0];
-            A col = (A)0;
+                for (int q = 0; q < NV; q++) raw[q] = __builtin_nontemporal_load(src + (int64_t)q * w);
+                const T *a = (const T *)raw;
+                const int c = c0 + lane;
+                const A xc = xs[r0 + c - j0];
+                A col = (A)0;
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-                const A av = load_cvt<T, A>(a[r]);
-                col = fma(av, xr[r], col);
-                racc[r] = fma(av, xc, racc[r]);
+                for (int r = 0; r < R; r++) {
+                    const A av = load_cvt<T, A>(a[r]);
+                    col = fma(av, xr[r], col);
+                    racc[r] = fma(av, xc, racc[r]);
+                }
+                atomicAdd(&ys[r0 + c - j0], col);
             }
-            atomicAdd(&ys[r0 + c - j0], col);
         }
         const A rs = wave_rows_reduce<R, A>(racc, lane);
         const int row = lane / (64 / R);

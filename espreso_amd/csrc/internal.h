@@ -38,12 +38,21 @@ constexpr int STRIP_B16 = 64;   // BF16 storage
 __host__ __device__ inline int strip_rows(int prec) { return prec == 0 ? STRIP_F64 : (prec == 1 ? STRIP_F32 : STRIP_B16); }
 
 // Packed-strip layout of a symmetric m x m matrix with R rows per strip:
-// strip t holds rows [tR, tR+R) x columns [tR, m), column-major inside the
-// strip (R consecutive values per column), zero below the diagonal, diagonal
-// stored halved so that one pass applies both triangles.
+// strip t holds rows [tR, tR+R) x columns [tR, m) (R*(m-tR) values), zero
+// below the diagonal, diagonal stored halved so that one pass applies both
+// triangles.  Inside a strip the columns go in chunks of 64; a chunk of width
+// w is stored [R/VEC][w][VEC] with VEC = 16 bytes / element size, so that the
+// q-th 16-byte load of wave lane l (column l of the chunk) is contiguous with
+// lane l+1's: every wave-wide load instruction reads one contiguous segment.
 __host__ __device__ inline int64_t strip_offset(int64_t m, int64_t R, int64_t t) { return R * (t * m - R * t * (t - 1) / 2); }
 __host__ __device__ inline int64_t strip_count(int64_t m, int64_t R) { return (m + R - 1) / R; }
 __host__ __device__ inline int64_t strip_size(int64_t m, int64_t R) { return strip_offset(m, R, strip_count(m, R)); }
+// position of element (row tR + r, column c >= tR) of strip t
+__host__ __device__ inline int64_t strip_index(int64_t m, int64_t R, int64_t VEC, int64_t t, int64_t r, int64_t c) {
+    const int64_t cl = c - t * R, k = cl >> 6, ncol = m - t * R;
+    const int64_t w = (ncol - 64 * k) < 64 ? (ncol - 64 * k) : 64;
+    return strip_offset(m, R, t) + 64 * k * R + ((r / VEC) * w + (cl & 63)) * VEC + r % VEC;
+}
 
 // Per-subdomain descriptor of the dense assembly workspace (device memory).
 // Interior DOFs first (ascending local DOF), boundary DOFs after them.
