@@ -381,3 +381,141 @@ class GridFETI(FETISolver):
         g = np.zeros(n, dtype=np.int32)
         _check(lib().espreso_grid_l2g(self.h, i, _ip(g)))
         return g
+
+
+# ---------------------------------------------------------------- FETI4I ----
+_F4_SIGS = {
+    "FETI4ISetDefaultIntegerOptions": (None, [_I]),
+    "FETI4ISetDefaultRealOptions": (None, [_D]),
+    "FETI4ICreateStiffnessMatrix": (None, [C.POINTER(_H), C.c_int32, C.c_int32]),
+    "FETI4IAddElement": (None, [_H, C.c_int32, C.c_int32, _I, C.c_int32, _I, _D]),
+    "FETI4ICreateInstance": (None, [C.POINTER(_H), _H, C.c_int32, _D, _I, C.c_int32, _I, C.c_int32, _I, _D, _I,
+                                    _D]),
+    "FETI4ISolve": (None, [_H, C.c_int32, _D]),
+    "FETI4IUpdateStiffnessMatrix": (None, [_H, _H]),
+    "FETI4IUpdateRhs": (None, [_H, C.c_int32, _D]),
+    "FETI4IUpdateDirichlet": (None, [_H, C.c_int32, _I, _D]),
+    "FETI4IDestroy": (None, [_H]),
+    "FETI4IGetLastStatus": (C.c_int, []),
+    "FETI4IGetLastError": (C.c_char_p, []),
+    "FETI4IInstanceInfo": (C.c_int, [_H, _I]),
+    "FETI4IExportSubdomain": (C.c_int, [_H, C.c_int32, _I, _I, _I, _I, _D, _D, _D, _I, _I, _D, _D]),
+    "FETI4IGetStats": (C.c_int, [_H, _D]),
+}
+_f4_ready = False
+
+
+def f4lib():
+    global _f4_ready
+    L = lib()
+    if not _f4_ready:
+        for name, (res, args) in _F4_SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _f4_ready = True
+    return L
+
+
+def _f4check():
+    st = f4lib().FETI4IGetLastStatus()
+    if st != 0:
+        raise RuntimeError(f"FETI4I error {st}: {f4lib().FETI4IGetLastError().decode()}")
+
+
+class FETI4I:
+    """Thin Python view of the FETI4I C API (include/feti4i.h)."""
+
+    def __init__(self, elements, size, rhs, dirichlet=(), subdomains=4, preconditioner=3, precision=1e-8,
+                 max_iterations=500, index_base=0):
+        L = f4lib()
+        io = np.zeros(10, np.int32)
+        ro = np.zeros(1)
+        L.FETI4ISetDefaultIntegerOptions(_ip(io))
+        L.FETI4ISetDefaultRealOptions(_dp(ro))
+        io[0] = subdomains
+        io[1] = max_iterations
+        io[3] = preconditioner
+        ro[0] = precision
+        m = C.c_void_p()
+        L.FETI4ICreateStiffnessMatrix(C.byref(m), 0, index_base)
+        _f4check()
+        self.m = m
+        for nodes, dofs, K in elements:
+            nodes = np.ascontiguousarray(nodes, np.int32)
+            dofs = np.ascontiguousarray(dofs, np.int32)
+            K = np.ascontiguousarray(K, np.float64)
+            L.FETI4IAddElement(m, 3, len(nodes), _ip(nodes), len(dofs), _ip(dofs), _dp(K))
+        _f4check()
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        l2g = np.arange(size, dtype=np.int32) + index_base
+        di = np.ascontiguousarray([d for d, _ in dirichlet], np.int32)
+        dv = np.ascontiguousarray([v for _, v in dirichlet], np.float64)
+        inst = C.c_void_p()
+        L.FETI4ICreateInstance(C.byref(inst), m, size, _dp(rhs), _ip(l2g), 0, None, len(di), _ip(di), _dp(dv),
+                               _ip(io), _dp(ro))
+        _f4check()
+        self.inst = inst
+        self.size = size
+
+    def info(self):
+        v = np.zeros(6, np.int32)
+        f4lib().FETI4IInstanceInfo(self.inst, _ip(v))
+        _f4check()
+        return dict(subdomains=int(v[0]), n_lambda=int(v[1]), n_coarse=int(v[2]), iterations=int(v[3]),
+                    ndof_node=int(v[4]), max_kdim=int(v[5]))
+
+    def subdomain(self, i):
+        L = f4lib()
+        sz = np.zeros(4, np.int32)
+        nul = None
+        L.FETI4IExportSubdomain(self.inst, i, _ip(sz), nul, nul, nul, nul, nul, nul, nul, nul, nul, nul)
+        _f4check()
+        n, nnz, k, nl = (int(x) for x in sz)
+        g = np.zeros(n, np.int32)
+        rp = np.zeros(n + 1, np.int32)
+        ci = np.zeros(nnz, np.int32)
+        kv = np.zeros(nnz)
+        f = np.zeros(n)
+        R = np.zeros(n * k)
+        lam = np.zeros(nl, np.int32)
+        dof = np.zeros(nl, np.int32)
+        sign = np.zeros(nl)
+        scale = np.zeros(nl)
+        L.FETI4IExportSubdomain(self.inst, i, _ip(sz), _ip(g), _ip(rp), _ip(ci), _dp(kv), _dp(f), _dp(R), _ip(lam),
+                                _ip(dof), _dp(sign), _dp(scale))
+        _f4check()
+        return dict(gdofs=g, rowptr=rp, colidx=ci, vals=kv, f=f, R=R.reshape(k, n).T, lam=lam, dof=dof, sign=sign,
+                    scale=scale)
+
+    def solve(self):
+        u = np.zeros(self.size)
+        f4lib().FETI4ISolve(self.inst, self.size, _dp(u))
+        _f4check()
+        return u
+
+    def stats(self):
+        s = np.zeros(6)
+        f4lib().FETI4IGetStats(self.inst, _dp(s))
+        _f4check()
+        return s
+
+    def update_rhs(self, rhs):
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        f4lib().FETI4IUpdateRhs(self.inst, len(rhs), _dp(rhs))
+        _f4check()
+
+    def close(self):
+        L = f4lib()
+        if getattr(self, "inst", None):
+            L.FETI4IDestroy(self.inst)
+            self.inst = None
+        if getattr(self, "m", None):
+            L.FETI4IDestroy(self.m)
+            self.m = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
