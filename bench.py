@@ -59,6 +59,8 @@ def main():
     ap.add_argument("--workload", default="elasticity_512sub_10e3")
     ap.add_argument("--sc", default="fp64", choices=["fp64", "fp32", "bf16"])
     ap.add_argument("--preconditioner", default="dirichlet")
+    ap.add_argument("--method", default=None, choices=["total", "hybrid"],
+                    help="FETI method (default: the workload's own, hybrid for the 512- and 1000-subdomain cases)")
     ap.add_argument("--apply-reps", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
@@ -75,10 +77,11 @@ def main():
     from espreso_amd import problems as PB
 
     spec = PB.ALL[args.workload]
-    log(f"bench: workload {args.workload}, F_i storage {args.sc}, loading inputs")
+    method = args.method or spec.get("method", "total")
+    log(f"bench: workload {args.workload}, method {method}, F_i storage {args.sc}, loading inputs")
     t0 = time.perf_counter()
     g = core.GridFETI(spec, preconditioner=args.preconditioner, precision=1e-8, sc_precision=args.sc,
-                      max_iterations=1000)
+                      max_iterations=1000, method=method)
     t_build = time.perf_counter() - t0
     log(f"bench: problem built on the host in {t_build:.1f} s; {g.n_sub} subdomains, {g.n_lambda} multipliers")
 
@@ -151,7 +154,7 @@ def main():
         "data": "synthetic (seeded structured HEXA8 elasticity box)",
         "config": {
             "workload": args.workload,
-            "method": "TOTAL_FETI",
+            "method": "HYBRID_FETI" if method == "hybrid" else "TOTAL_FETI",
             "preconditioner": args.preconditioner,
             "subdomains": g.n_sub,
             "elements_per_subdomain": list(spec["elements"]),

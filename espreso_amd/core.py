@@ -101,6 +101,7 @@ SIGNATURES = {
     "espreso_grid_l2g": (C.c_int, [_H, C.c_int32, _I]),
     "espreso_grid_export": (C.c_int, [C.POINTER(GridProblem), C.c_int32, C.c_int32, _I, _I, _I, _D, _D, _D, _I, _I,
                                       _D, _D, _D]),
+    "espreso_grid_export_cluster": (C.c_int, [C.POINTER(GridProblem), C.c_int32, _I, _I, _I, _I, _D]),
     "espreso_grid_nodal_solution": (C.c_int, [_H, _D]),
     "espreso_feti_create": (C.c_int, [C.POINTER(_H), C.c_int32, C.POINTER(Options)]),
     "espreso_feti_destroy": (C.c_int, [_H]),
@@ -108,6 +109,7 @@ SIGNATURES = {
     "espreso_feti_set_subdomain": (C.c_int, [_H, C.c_int32, C.c_int32, _I, _I, _D, _D, C.c_int32, _D,
                                              C.c_int32, _I, _I, _D, _D]),
     "espreso_feti_set_dual": (C.c_int, [_H, C.c_int32, _D]),
+    "espreso_feti_set_cluster": (C.c_int, [_H, C.c_int32, C.c_int32, C.c_int32, _I, _I, _D]),
     "espreso_feti_setup": (C.c_int, [_H]),
     "espreso_feti_solve": (C.c_int, [_H, C.POINTER(Stats)]),
     "espreso_feti_primal": (C.c_int, [_H, C.c_int32, _D]),
@@ -207,10 +209,13 @@ def grid_problem(spec):
     return p
 
 
+METHOD = {"total": 0, "TOTAL_FETI": 0, "hybrid": 1, "HYBRID_FETI": 1}
+
+
 def make_options(preconditioner="dirichlet", precision=1e-8, max_iterations=500, scaling=1, sc_precision="fp64",
-                 device=0, memory_budget_gb=0.0):
+                 device=0, memory_budget_gb=0.0, method="total"):
     o = Options()
-    o.method = 0
+    o.method = METHOD[method] if isinstance(method, str) else int(method)
     o.preconditioner = PREC[preconditioner] if isinstance(preconditioner, str) else int(preconditioner)
     o.max_iterations = max_iterations
     o.scaling = scaling
@@ -243,6 +248,19 @@ def grid_export(spec, i, scaling=1):
                                      _ip(lam), _ip(dof), _dp(sign), _dp(scale), _dp(c)))
     return dict(n=n, rowptr=rp, colidx=ci, vals=kv, f=f, R=R.reshape(k, n).T, lam=lam, dof=dof, sign=sign,
                 scale=scale, c=c, n_sub=int(sz[5]))
+
+
+def grid_export_cluster(spec, i):
+    """Host-only export of subdomain i's cluster id and B0 corner entries."""
+    p = grid_problem(spec)
+    cl = np.zeros(1, np.int32)
+    n0 = np.zeros(1, np.int32)
+    _check(lib().espreso_grid_export_cluster(C.byref(p), i, _ip(cl), _ip(n0), None, None, None))
+    row = np.zeros(int(n0[0]), np.int32)
+    dof = np.zeros(int(n0[0]), np.int32)
+    sign = np.zeros(int(n0[0]))
+    _check(lib().espreso_grid_export_cluster(C.byref(p), i, _ip(cl), _ip(n0), _ip(row), _ip(dof), _dp(sign)))
+    return dict(cluster=int(cl[0]), row=row, dof=dof, sign=sign)
 
 
 class FETISolver:
@@ -287,6 +305,12 @@ class FETISolver:
         Rf = np.ascontiguousarray(R.T).ravel()  # column-major n x k
         _check(lib().espreso_feti_set_subdomain(self.h, i, n, _ip(rp), _ip(ci), _dp(kv), _dp(f), R.shape[1], _dp(Rf),
                                                 len(lam), _ip(lam), _ip(dof), _dp(sign), _dp(scale)))
+
+    def set_cluster(self, i, cluster, row, dof, sign):
+        row = np.ascontiguousarray(row, np.int32)
+        dof = np.ascontiguousarray(dof, np.int32)
+        sign = np.ascontiguousarray(sign, np.float64)
+        _check(lib().espreso_feti_set_cluster(self.h, i, cluster, len(row), _ip(row), _ip(dof), _dp(sign)))
 
     def set_dual(self, c):
         c = np.ascontiguousarray(c, dtype=np.float64)

@@ -425,6 +425,42 @@ __global__ void k_scatter(const DenseDesc *D, const int *rowptr, const int64_t *
     if (pr < ni) d.Kib[pr + (size_t)m * ni] = f[perm_off[b] + r];
 }
 
+// ---- hybrid FETI: corner columns of F_i (lower triangle of S) ----
+// FB0[j + q m] = sign_q F(j, j_q);  Fcc[q1 + q2 nq] = sign_q1 FB0[j_q1 + q2 m]
+__global__ void k_extract_corners(const DenseDesc *D, const int *nq, const int64_t *qoff, const int *qj,
+                                  const double *qsign, const int64_t *fb0off, const int64_t *fccoff, double *FB0,
+                                  double *Fcc) {
+    const DenseDesc &d = D[blockIdx.y];
+    const int b = blockIdx.y, m = d.m, n = nq[b];
+    const int *jq = qj + qoff[b];
+    const double *sg = qsign + qoff[b];
+    double *fb = FB0 + fb0off[b];
+    const int64_t tot = (int64_t)m * n;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(e % m), q = (int)(e / m);
+        const int c = jq[q];
+        const double v = j >= c ? d.S[j + (size_t)c * m] : d.S[c + (size_t)j * m];
+        fb[e] = sg[q] * v;
+    }
+    if (blockIdx.x == 0) {
+        double *fc = Fcc + fccoff[b];
+        for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+            const int q1 = e % n, q2 = e / n;
+            const int c1 = jq[q1], c2 = jq[q2];
+            const double v = c1 >= c2 ? d.S[c1 + (size_t)c2 * m] : d.S[c2 + (size_t)c1 * m];
+            fc[e] = sg[q1] * sg[q2] * v;
+        }
+    }
+}
+
+void launch_extract_corners(const DenseDesc *desc, int nb, const int *nq, const int64_t *qoff, const int *qj,
+                            const double *qsign, const int64_t *fb0off, const int64_t *fccoff, double *FB0,
+                            double *Fcc, int max_m, int max_nq, hipStream_t st) {
+    unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(((int64_t)max_m * max_nq + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_extract_corners, dim3(g, nb), dim3(256), 0, st, desc, nq, qoff, qj, qsign, fb0off, fccoff,
+                       FB0, Fcc);
+}
+
 // ---- diagnostics: one MFMA with lane-indexed operands, one tile GEMM ----
 __global__ void k_probe_mfma(const double *a, const double *b, double *out) {
     const int lane = threadIdx.x;

@@ -55,6 +55,28 @@ struct DualDev {
     double *ypart = nullptr;
     double *yb = nullptr, *xb = nullptr;  // nbnd scratch
     double *cd = nullptr, *ce = nullptr;  // coarse scratch
+    // coarse space: blocks (TFETI: subdomains, HTFETI: clusters); per subdomain
+    // the basis Rt_s (m x ck[s]) restricted to the boundary and N_s (kdim x ck)
+    // mapping block amplitudes to subdomain kernel amplitudes
+    int nblk = 0;
+    int *cblk = nullptr, *ck = nullptr, *blk_ptr = nullptr, *blk_sub = nullptr;
+    int64_t *rtoff = nullptr, *dsub_off = nullptr, *noff = nullptr;
+    double *Rt = nullptr, *N = nullptr, *dsub = nullptr;
+    int64_t *asub = nullptr;          // offset of kernel amplitudes of subdomain s (sum of kdim)
+};
+
+// hybrid FETI clusters (B0 corners, F0, G0, Sa), device side
+struct ClusterDev {
+    int ncl = 0, max_n0 = 0, max_kt = 0;
+    int *n0 = nullptr, *kt = nullptr;       // B0 rows, sum of subdomain kernel dims per cluster
+    int64_t *rowoff = nullptr, *f0off = nullptr, *g0off = nullptr, *saoff = nullptr, *eoff = nullptr;
+    int64_t *row_b1 = nullptr, *row_b2 = nullptr;   // global boundary index of the +1 / -1 entry of each row
+    double *F0inv = nullptr, *G0 = nullptr, *Sainv = nullptr;
+    // per subdomain: B0 entries q (cluster-local row, FB0 column)
+    int *nq = nullptr, *qrow = nullptr;
+    int64_t *qoff = nullptr, *fb0off = nullptr, *asub = nullptr;
+    double *FB0 = nullptr;                  // m x nq per subdomain: sign_q * F_i[:, j_q]
+    double *lambda0 = nullptr, *alphaK = nullptr;
 };
 
 struct PrimalDev {
@@ -62,6 +84,7 @@ struct PrimalDev {
     int64_t *band_off = nullptr, *yf_off = nullptr, *perm_off = nullptr, *rp_off = nullptr, *nz_off = nullptr;
     int64_t *boff = nullptr, *work_off = nullptr, *r_off = nullptr;
     double *band = nullptr, *yf = nullptr, *vals = nullptr, *R = nullptr, *alpha = nullptr, *work = nullptr;
+    int64_t *asub = nullptr;
     int *perm = nullptr, *iperm = nullptr, *rowptr = nullptr, *colidx = nullptr;
 };
 
@@ -76,6 +99,14 @@ void launch_boundary_gather(const DualDev &D, const double *x, bool scaled, doub
 void launch_kbb(const DualDev &D, const double *xb, double *yb, hipStream_t st);
 // mode 0: y = x - G^T GGinv G x; mode 1: y = G^T GGinv x (x coarse); mode 2: y = GGinv G x (y coarse)
 void launch_project(const DualDev &D, const double *x, double *y, int mode, hipStream_t st);
+// out[asub[s] + a] = -R_s^T (B_s^T x) with the subdomain kernel (hybrid FETI e0)
+void launch_subdomain_kernel_dots(const DualDev &D, const double *x, double *out, hipStream_t st);
+// y_b += -K^+ B0^T lambda0 (+ R alphaK when addR) after solving the cluster problems
+void launch_cluster_correction(const DualDev &D, const ClusterDev &C, const double *e0s, double *yb, int addR,
+                               hipStream_t st);
+// alpha_tot(s) = alphaK(s) (hybrid) + N_s alpha_coarse
+void launch_alpha_combine(const DualDev &D, const ClusterDev &C, bool hybrid, const double *alpha_coarse,
+                          double *alpha_tot, hipStream_t st);
 void launch_dot(const double *a, const double *b, int64_t n, double *partial, double *out, hipStream_t st);
 void launch_p_update(double *p, const double *y, int64_t n, const double *num, const double *den, int first,
                      hipStream_t st);

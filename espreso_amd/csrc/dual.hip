@@ -183,13 +183,16 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
     return r;
 }
 
-// d = G x, G = -(B R)^T: one workgroup per subdomain
-__global__ void __launch_bounds__(256) k_apply_G(const int *M, const int64_t *boff, const int *kdim, const int *coff,
-                                                 const int64_t *rboff, const double *Rb, const int64_t *bl_ptr,
-                                                 const int *bl_lam, const double *sign, const double *x, double *d) {
+// per subdomain s: out[off[s] + a] = -sum_j R_s[j, a] (B_s^T x)_j for a < k[s]
+// (R_s: m_s x k_s column-major at R + roff[s]); one workgroup per subdomain.
+// With the coarse basis this is G x = -(B R)^T x per subdomain.
+__global__ void __launch_bounds__(256) k_apply_G(const int *M, const int64_t *boff, const int *kcols,
+                                                 const int64_t *outoff, const int64_t *roff, const double *R,
+                                                 const int64_t *bl_ptr, const int *bl_lam, const double *sign,
+                                                 const double *x, double *out) {
     __shared__ double red[4];
     extern __shared__ double xbs[];
-    const int s = blockIdx.x, m = M[s], k = kdim[s];
+    const int s = blockIdx.x, m = M[s], k = kcols[s];
     const int64_t b0 = boff[s];
     for (int j = threadIdx.x; j < m; j += 256) {
         double v = 0.0;
@@ -197,12 +200,26 @@ __global__ void __launch_bounds__(256) k_apply_G(const int *M, const int64_t *bo
         xbs[j] = v;
     }
     __syncthreads();
-    const double *R = Rb + rboff[s];
+    const double *Rs = R + roff[s];
     for (int a = 0; a < k; a++) {
         double v = 0.0;
-        for (int j = threadIdx.x; j < m; j += 256) v += R[j + (int64_t)a * m] * xbs[j];
+        for (int j = threadIdx.x; j < m; j += 256) v += Rs[j + (int64_t)a * m] * xbs[j];
         v = block_sum(v, red);
-        if (threadIdx.x == 0) d[coff[s] + a] = -v;
+        if (threadIdx.x == 0) out[outoff[s] + a] = -v;
+    }
+}
+
+// d[coff[b] + a] = sum over the subdomains of coarse block b of their partials
+__global__ void k_block_sum(int nblk, const int *blk_ptr, const int *blk_sub, const int64_t *subpart_off,
+                            const int *ck, const int *coff, const double *part, double *d) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nblk; b += gridDim.x * blockDim.x) {
+        const int s0 = blk_sub[blk_ptr[b]];
+        const int k = ck[s0];
+        for (int a = 0; a < k; a++) {
+            double v = 0.0;
+            for (int q = blk_ptr[b]; q < blk_ptr[b + 1]; q++) v += part[subpart_off[blk_sub[q]] + a];
+            d[coff[b] + a] = v;
+        }
     }
 }
 
@@ -217,10 +234,10 @@ __global__ void __launch_bounds__(256) k_coarse_gemv(int nc, const double *Ginv,
     if (threadIdx.x == 0) e[i] = v;
 }
 
-// mode 0: y = x - G^T e ; mode 1: y = G^T e
+// mode 0: y = x - G^T e ; mode 1: y = G^T e   (G^T e)_l = -sum_entries sign Rt_s[j,:] e_blk(s)
 __global__ void k_project_finish(int nlam, const int *cnt, const int64_t *lb, const int *lsub, const double *sign,
-                                 const int64_t *boff, const int *M, const int *kdim, const int *coff,
-                                 const int64_t *rboff, const double *Rb, const double *e, const double *x, double *y,
+                                 const int64_t *boff, const int *M, const int *ck, const int *cblk, const int *coff,
+                                 const int64_t *rtoff, const double *Rt, const double *e, const double *x, double *y,
                                  int mode) {
     for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < nlam; l += gridDim.x * blockDim.x) {
         double g = 0.0;
@@ -228,13 +245,101 @@ __global__ void k_project_finish(int nlam, const int *cnt, const int64_t *lb, co
             const int s = lsub[2 * l + q];
             const int64_t j = lb[2 * l + q] - boff[s];
             const int m = M[s];
-            const double *R = Rb + rboff[s];
-            const double *es = e + coff[s];
+            const double *R = Rt + rtoff[s];
+            const double *es = e + coff[cblk[s]];
             double acc = 0.0;
-            for (int a = 0; a < kdim[s]; a++) acc += R[j + (int64_t)a * m] * es[a];
+            for (int a = 0; a < ck[s]; a++) acc += R[j + (int64_t)a * m] * es[a];
             g -= sign[2 * l + q] * acc;
         }
         y[l] = mode == 0 ? x[l] - g : g;
+    }
+}
+
+// ---- hybrid FETI: cluster correction of K^+ (cluster.cpp:796-916) ----
+// one workgroup per cluster: g0 = B0 y_b, t = F0^-1 g0, alpha = Sa^-1 (G0 t - e0),
+// lambda0 = F0^-1 (g0 - G0^T alpha)
+__global__ void __launch_bounds__(256) k_cluster_solve(const ClusterDev C, const double *yb, const double *e0s,
+                                                       double *lambda0, double *alphaK) {
+    extern __shared__ double sh[];
+    const int c = blockIdx.x;
+    const int n0 = C.n0[c], kt = C.kt[c];
+    double *g0 = sh, *t = g0 + C.max_n0, *t2 = t + C.max_n0, *al = t2 + C.max_kt;
+    const int64_t r0 = C.rowoff[c];
+    for (int r = threadIdx.x; r < n0; r += 256) g0[r] = yb[C.row_b1[r0 + r]] - yb[C.row_b2[r0 + r]];
+    __syncthreads();
+    const double *F0i = C.F0inv + C.f0off[c];
+    const double *G0 = C.G0 + C.g0off[c];      // kt x n0, column-major
+    const double *Sai = C.Sainv + C.saoff[c];
+    const double *e0 = e0s + C.eoff[c];         // cluster-contiguous subdomain partials
+    for (int r = threadIdx.x; r < n0; r += 256) {
+        double v = 0.0;
+        for (int q = 0; q < n0; q++) v += F0i[r + (int64_t)q * n0] * g0[q];
+        t[r] = v;
+    }
+    __syncthreads();
+    for (int a = threadIdx.x; a < kt; a += 256) {
+        double v = 0.0;
+        for (int r = 0; r < n0; r++) v += G0[a + (int64_t)r * kt] * t[r];
+        t2[a] = v - e0[a];
+    }
+    __syncthreads();
+    for (int a = threadIdx.x; a < kt; a += 256) {
+        double v = 0.0;
+        for (int b = 0; b < kt; b++) v += Sai[a + (int64_t)b * kt] * t2[b];
+        al[a] = v;
+        alphaK[C.eoff[c] + a] = v;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < n0; r += 256) {
+        double v = g0[r];
+        for (int a = 0; a < kt; a++) v -= G0[a + (int64_t)r * kt] * al[a];
+        t[r] = v;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < n0; r += 256) {
+        double v = 0.0;
+        for (int q = 0; q < n0; q++) v += F0i[r + (int64_t)q * n0] * t[q];
+        lambda0[r0 + r] = v;
+    }
+}
+
+// per subdomain: y_b += -FB0_s lambda0 + addR * R_b alphaK   (one workgroup per subdomain)
+__global__ void __launch_bounds__(256) k_cluster_apply(const ClusterDev C, const int *M, const int64_t *boff,
+                                                       const int *kdim, const int64_t *rboff, const double *Rb,
+                                                       const double *lambda0, const double *alphaK, int addR,
+                                                       double *yb) {
+    __shared__ double lam[256];
+    const int s = blockIdx.x, m = M[s];
+    const int nq = C.nq[s];
+    const int64_t q0 = C.qoff[s];
+    for (int q = threadIdx.x; q < nq; q += 256) lam[q] = lambda0[C.qrow[q0 + q]];
+    __syncthreads();
+    const double *FB = C.FB0 + C.fb0off[s];
+    const double *R = Rb + rboff[s];
+    const double *al = alphaK + C.asub[s];
+    const int k = kdim[s];
+    for (int j = threadIdx.x; j < m; j += 256) {
+        double v = 0.0;
+        for (int q = 0; q < nq; q++) v -= FB[j + (int64_t)q * m] * lam[q];
+        if (addR)
+            for (int a = 0; a < k; a++) v += R[j + (int64_t)a * m] * al[a];
+        yb[boff[s] + j] += v;
+    }
+}
+
+// alpha_tot(s) = addK * alphaK(s) + N_s alpha_coarse(block(s))
+__global__ void k_alpha_combine(int nsub, const int *kdim, const int *ck, const int *cblk, const int *coff,
+                                const int64_t *asub, const int64_t *noff, const double *N, const double *alphaK,
+                                int addK, const double *alpha_coarse, double *alpha_tot) {
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < nsub; s += gridDim.x * blockDim.x) {
+        const int k = kdim[s], kc = ck[s];
+        const double *Ns = N + noff[s];  // k x kc column-major
+        const double *ac = alpha_coarse + coff[cblk[s]];
+        for (int a = 0; a < k; a++) {
+            double v = addK ? alphaK[asub[s] + a] : 0.0;
+            for (int b = 0; b < kc; b++) v += Ns[a + (int64_t)b * k] * ac[b];
+            alpha_tot[asub[s] + a] = v;
+        }
     }
 }
 
@@ -337,7 +442,7 @@ __global__ void __launch_bounds__(256) k_primal_interior(const PrimalDev P, cons
     }
     // scatter into local DOF order and add R alpha
     const double *R = P.R + P.r_off[s];
-    const double *al = P.alpha + P.coff[s];
+    const double *al = P.alpha + P.asub[s];
     const int k = P.kdim[s];
     for (int r = threadIdx.x; r < n; r += 256) {
         const int lr = perm[r];
@@ -394,15 +499,37 @@ void launch_kbb(const DualDev &D, const double *xb, double *yb, hipStream_t st) 
 
 void launch_project(const DualDev &D, const double *x, double *y, int mode, hipStream_t st) {
     // mode 0: y = P x;  mode 1: y = G^T GGinv x (x coarse);  mode 2: y = GGinv G x (coarse out)
-    if (mode != 1)
-        hipLaunchKernelGGL(k_apply_G, dim3(D.nsub), dim3(256), (size_t)D.max_m * 8, st, D.m, D.boff, D.kdim, D.coff,
-                           D.rboff, D.Rb, D.bl_ptr, D.bl_lam, D.bl_sign, x, D.cd);
+    if (mode != 1) {
+        hipLaunchKernelGGL(k_apply_G, dim3(D.nsub), dim3(256), (size_t)D.max_m * 8, st, D.m, D.boff, D.ck, D.dsub_off,
+                           D.rtoff, D.Rt, D.bl_ptr, D.bl_lam, D.bl_sign, x, D.dsub);
+        hipLaunchKernelGGL(k_block_sum, dim3(vgrid(D.nblk)), dim3(256), 0, st, D.nblk, D.blk_ptr, D.blk_sub,
+                           D.dsub_off, D.ck, D.coff, D.dsub, D.cd);
+    }
     const double *din = mode == 1 ? x : D.cd;
     double *eout = mode == 2 ? y : D.ce;
     hipLaunchKernelGGL(k_coarse_gemv, dim3(D.ncoarse), dim3(256), 0, st, D.ncoarse, D.GGinv, din, eout);
     if (mode == 2) return;
     hipLaunchKernelGGL(k_project_finish, dim3(vgrid(D.nlam)), dim3(256), 0, st, D.nlam, D.le_cnt, D.le_b, D.le_sub,
-                       D.le_sign, D.boff, D.m, D.kdim, D.coff, D.rboff, D.Rb, D.ce, x, y, mode);
+                       D.le_sign, D.boff, D.m, D.ck, D.cblk, D.coff, D.rtoff, D.Rt, D.ce, x, y, mode);
+}
+
+void launch_subdomain_kernel_dots(const DualDev &D, const double *x, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_apply_G, dim3(D.nsub), dim3(256), (size_t)D.max_m * 8, st, D.m, D.boff, D.kdim, D.asub,
+                       D.rboff, D.Rb, D.bl_ptr, D.bl_lam, D.bl_sign, x, out);
+}
+
+void launch_cluster_correction(const DualDev &D, const ClusterDev &C, const double *e0s, double *yb, int addR,
+                               hipStream_t st) {
+    const size_t shm = (2 * (size_t)C.max_n0 + 2 * (size_t)C.max_kt) * 8;
+    hipLaunchKernelGGL(k_cluster_solve, dim3(C.ncl), dim3(256), shm, st, C, yb, e0s, C.lambda0, C.alphaK);
+    hipLaunchKernelGGL(k_cluster_apply, dim3(D.nsub), dim3(256), 0, st, C, D.m, D.boff, D.kdim, D.rboff, D.Rb,
+                       C.lambda0, C.alphaK, addR, yb);
+}
+
+void launch_alpha_combine(const DualDev &D, const ClusterDev &C, bool hybrid, const double *alpha_coarse,
+                          double *alpha_tot, hipStream_t st) {
+    hipLaunchKernelGGL(k_alpha_combine, dim3(vgrid(D.nsub)), dim3(256), 0, st, D.nsub, D.kdim, D.ck, D.cblk, D.coff,
+                       D.asub, D.noff, D.N, C.alphaK, hybrid ? 1 : 0, alpha_coarse, alpha_tot);
 }
 
 void launch_dot(const double *a, const double *b, int64_t n, double *partial, double *out, hipStream_t st) {

@@ -41,7 +41,92 @@ struct SubHost {
     std::vector<int> perm, iperm, bdofs, colstart;
     int band_tiles = 0, bw = 0;
     bool set = false;
+    // hybrid FETI: cluster and B0 (corner) rows touching the subdomain
+    int cluster = -1;
+    std::vector<int> b0row, b0dof;
+    std::vector<double> b0sign;
 };
+
+// symmetric eigen-decomposition (cyclic Jacobi) of the n x n column-major A;
+// eigenvalues in w, eigenvectors in the columns of V
+void jacobi_eigen(int n, std::vector<double> A, std::vector<double> &w, std::vector<double> &V) {
+    V.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) V[i + (size_t)i * n] = 1.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0, tot = 0;
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++) {
+                tot += A[i + (size_t)j * n] * A[i + (size_t)j * n];
+                if (i != j) off += A[i + (size_t)j * n] * A[i + (size_t)j * n];
+            }
+        if (off <= 1e-30 * tot) break;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double apq = A[p + (size_t)q * n];
+                if (apq == 0.0) continue;
+                const double app = A[p + (size_t)p * n], aqq = A[q + (size_t)q * n];
+                const double th = 0.5 * (aqq - app) / apq;
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < n; k++) {
+                    const double akp = A[k + (size_t)p * n], akq = A[k + (size_t)q * n];
+                    A[k + (size_t)p * n] = c * akp - sn * akq;
+                    A[k + (size_t)q * n] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    const double apk = A[p + (size_t)k * n], aqk = A[q + (size_t)k * n];
+                    A[p + (size_t)k * n] = c * apk - sn * aqk;
+                    A[q + (size_t)k * n] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    const double vkp = V[k + (size_t)p * n], vkq = V[k + (size_t)q * n];
+                    V[k + (size_t)p * n] = c * vkp - sn * vkq;
+                    V[k + (size_t)q * n] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    w.resize(n);
+    for (int i = 0; i < n; i++) w[i] = A[i + (size_t)i * n];
+}
+
+// inverse of a symmetric positive definite matrix (column-major, in place)
+bool spd_inverse(int n, std::vector<double> &A) {
+    std::vector<double> L = A;
+    for (int j = 0; j < n; j++) {
+        double d = L[j + (size_t)j * n];
+        for (int k = 0; k < j; k++) d -= L[j + (size_t)k * n] * L[j + (size_t)k * n];
+        if (!(d > 0)) return false;
+        d = std::sqrt(d);
+        L[j + (size_t)j * n] = d;
+        for (int i = j + 1; i < n; i++) {
+            double v = L[i + (size_t)j * n];
+            for (int k = 0; k < j; k++) v -= L[i + (size_t)k * n] * L[j + (size_t)k * n];
+            L[i + (size_t)j * n] = v / d;
+        }
+    }
+    std::vector<double> x(n);
+    for (int c = 0; c < n; c++) {
+        std::fill(x.begin(), x.end(), 0.0);
+        x[c] = 1.0;
+        for (int i = 0; i < n; i++) {
+            double v = x[i];
+            for (int k = 0; k < i; k++) v -= L[i + (size_t)k * n] * x[k];
+            x[i] = v / L[i + (size_t)i * n];
+        }
+        for (int i = n - 1; i >= 0; i--) {
+            double v = x[i];
+            for (int k = i + 1; k < n; k++) v -= L[k + (size_t)i * n] * x[k];
+            x[i] = v / L[i + (size_t)i * n];
+        }
+        for (int i = 0; i < n; i++) A[i + (size_t)c * n] = x[i];
+    }
+    for (int j = 0; j < n; j++)
+        for (int i = j + 1; i < n; i++) {
+            const double v = 0.5 * (A[i + (size_t)j * n] + A[j + (size_t)i * n]);
+            A[i + (size_t)j * n] = A[j + (size_t)i * n] = v;
+        }
+    return true;
+}
 
 // orthonormalise the k columns of V (n x k); false when they are dependent
 bool mgs(int n, int k, double *V) {
@@ -79,6 +164,24 @@ struct espreso_feti_s {
     // device
     DualDev D;
     PrimalDev P;
+    ClusterDev CL;
+    bool hybrid = false;
+    int ncl = 0;
+    std::vector<std::vector<int>> cl_subs;          // subdomains of each cluster
+    std::vector<int64_t> asubv;                     // kernel amplitude offsets per subdomain
+    std::vector<double> Rb_h, e0f_h;                // host copies: R_b, -R_s^T f_s
+    std::vector<int64_t> rboff_h;
+    std::vector<int> le_cnt_h, le_sub_h;
+    std::vector<int64_t> le_b_h;
+    std::vector<double> le_sign_h;
+    std::vector<int> nq_h;
+    std::vector<int64_t> qoff_h, fccoff_h;
+    std::vector<int> qrow_h, qj_h;                  // per B0 entry: cluster row, boundary index
+    std::vector<double> qsign_h;
+    double *d_qj = nullptr, *d_Fcc = nullptr;       // (device) boundary index as double, corner blocks
+    int *d_qjint = nullptr;
+    double *d_qsign = nullptr;
+    double *d_e0f = nullptr, *d_e0s = nullptr, *d_alpha_tot = nullptr;
     double *d_hf = nullptr, *d_ubf = nullptr, *d_fb = nullptr;  // boundary arrays
     double *d_x = nullptr, *d_r = nullptr, *d_w = nullptr, *d_y = nullptr, *d_z = nullptr, *d_p = nullptr,
            *d_Ap = nullptr, *d_b = nullptr, *d_tmp = nullptr, *d_r2 = nullptr, *d_partial = nullptr,
@@ -125,6 +228,17 @@ struct espreso_feti_s {
     void assemble(const double *d_f, const double *d_Q, const std::vector<int64_t> &foff,
                   const std::vector<int64_t> &lpoff, const std::vector<int64_t> &boff);
     void coarse();
+    void coarse_layout_subdomains();
+    void load_clusters();
+    double *d_alpha_c = nullptr;    // coarse amplitudes
+    template <class T>
+    T *replace_upload(T *old, const std::vector<T> &v) {
+        if (old) release(old);
+        return upload(v);
+    }
+    void cluster_setup();
+    void build_ggt(const std::vector<double> &Rt, const std::vector<int64_t> &rtoff, const std::vector<int> &ck,
+                   const std::vector<int> &cblk, const std::vector<int> &coff, std::vector<double> &GGt);
     void build_items(int prec, std::vector<SymvItem> &items, std::vector<int> &item0, std::vector<int> &nit,
                      int64_t &plen);
     // dual operators on device vectors
@@ -149,6 +263,7 @@ void espreso_feti_s::load() {
     for (int s = 0; s < nsub; s++) {
         SubHost &h = sub[s];
         std::vector<int> b(h.dof);
+        b.insert(b.end(), h.b0dof.begin(), h.b0dof.end());
         std::sort(b.begin(), b.end());
         b.erase(std::unique(b.begin(), b.end()), b.end());
         h.bdofs = b;
@@ -192,7 +307,7 @@ void espreso_feti_s::load() {
         coffv[s + 1] = coffv[s] + sub[s].kdim;
         max_m = std::max(max_m, sub[s].m);
     }
-    ncoarse = (int)coffv[nsub];
+    asubv = coffv;
     const int64_t nb = boff[nsub];
 
     // boundary -> lambda CSR and lambda -> entries
@@ -229,7 +344,7 @@ void espreso_feti_s::load() {
     // R_b, Q = orth(R_b), fb, K_bb (boundary CSR), e = -R^T f, GG^T
     std::vector<int64_t> rboff(nsub + 1, 0);
     for (int s = 0; s < nsub; s++) rboff[s + 1] = rboff[s] + (int64_t)sub[s].m * sub[s].kdim;
-    std::vector<double> Rb(rboff[nsub]), Q(rboff[nsub]), fb(nb), e(ncoarse);
+    std::vector<double> Rb(rboff[nsub]), Q(rboff[nsub]), fb(nb), e(coffv[nsub]);
     std::vector<int64_t> kbb_cnt(nsub);
     std::vector<char> rank_ok(nsub, 1);
 #pragma omp parallel for schedule(dynamic)
@@ -277,20 +392,25 @@ void espreso_feti_s::load() {
             }
         }
     }
-    // GG^T (dense, ncoarse^2): sum over lambda of G-column outer products
-    std::vector<double> GGt((size_t)ncoarse * ncoarse, 0.0);
-    for (int l = 0; l < nlam; l++) {
-        for (int a = 0; a < le_cnt[l]; a++)
-            for (int b2 = 0; b2 < le_cnt[l]; b2++) {
-                const int s1 = le_sub[2 * l + a], s2 = le_sub[2 * l + b2];
-                const int64_t j1 = le_b[2 * l + a] - boff[s1], j2 = le_b[2 * l + b2] - boff[s2];
-                const double g1 = le_sign[2 * l + a], g2 = le_sign[2 * l + b2];
-                const int m1 = sub[s1].m, m2 = sub[s2].m;
-                for (int i = 0; i < sub[s1].kdim; i++)
-                    for (int j = 0; j < sub[s2].kdim; j++)
-                        GGt[(coffv[s1] + i) + (size_t)(coffv[s2] + j) * ncoarse] +=
-                            g1 * Rb[rboff[s1] + j1 + (int64_t)i * m1] * g2 * Rb[rboff[s2] + j2 + (int64_t)j * m2];
-            }
+    // keep what the coarse layout and the cluster setup need
+    Rb_h = Rb;
+    rboff_h = rboff;
+    e0f_h = e;
+    le_cnt_h = le_cnt;
+    le_sub_h = le_sub;
+    le_b_h = le_b;
+    le_sign_h = le_sign;
+    hybrid = opt.method == 1;
+    if (hybrid) {
+        ncl = 0;
+        for (int s = 0; s < nsub; s++) {
+            ESP_CHECK(sub[s].cluster >= 0, ESPRESO_ERR_STATE,
+                      "HYBRID_FETI: subdomain " + std::to_string(s) + " has no cluster (espreso_feti_set_cluster)");
+            ncl = std::max(ncl, sub[s].cluster + 1);
+        }
+        cl_subs.assign(ncl, {});
+        for (int s = 0; s < nsub; s++) cl_subs[sub[s].cluster].push_back(s);
+        for (int c = 0; c < ncl; c++) ESP_CHECK(!cl_subs[c].empty(), ESPRESO_ERR_ARG, "empty cluster id");
     }
 
     // ---- device: problem arrays ----
@@ -299,11 +419,11 @@ void espreso_feti_s::load() {
     D.ncoarse = ncoarse;
     D.max_m = max_m;
     D.nbnd = nb;
-    std::vector<int> mv(nsub), kd(nsub), cv(nsub);
-    for (int s = 0; s < nsub; s++) { mv[s] = sub[s].m; kd[s] = sub[s].kdim; cv[s] = (int)coffv[s]; }
+    std::vector<int> mv(nsub), kd(nsub);
+    for (int s = 0; s < nsub; s++) { mv[s] = sub[s].m; kd[s] = sub[s].kdim; }
     D.m = upload(mv);
     D.kdim = upload(kd);
-    D.coff = upload(cv);
+    D.asub = upload(asubv);
     D.boff = upload(boff);
     D.rboff = upload(rboff);
     D.bl_ptr = upload(bl_ptr);
@@ -321,12 +441,13 @@ void espreso_feti_s::load() {
     D.Rb = upload(Rb);
     D.yb = alloc<double>(nb);
     D.xb = alloc<double>(nb);
-    D.cd = alloc<double>(ncoarse);
-    D.ce = alloc<double>(ncoarse);
+
     d_fb = upload(fb);
     d_hf = alloc<double>(nb);
     d_ubf = alloc<double>(nb);
-    d_e = upload(e);
+    d_e0f = upload(e);
+    d_e0s = alloc<double>(coffv[nsub]);
+    d_alpha_tot = alloc<double>(coffv[nsub]);
     double *d_Q = upload(Q);
 
     // packed pools
@@ -392,7 +513,7 @@ void espreso_feti_s::load() {
         P.bw = upload(bwv);
         P.m = D.m;
         P.kdim = D.kdim;
-        P.coff = D.coff;
+        P.asub = D.asub;
         P.boff = D.boff;
         P.nz_off = upload(nz_off);
         P.rp_off = upload(rp_off);
@@ -404,16 +525,17 @@ void espreso_feti_s::load() {
         P.band = alloc<double>(band_off[nsub]);
         P.yf = alloc<double>(yf_off[nsub]);
         P.work = alloc<double>(yf_off[nsub]);
-        P.alpha = alloc<double>(ncoarse);
+        P.alpha = d_alpha_tot;
         d_u = alloc<double>(n_off[nsub]);
         uoff = n_off;
         d_f = upload(ff);
     }
     d_Qm = d_Q;
-    d_GGt = upload(GGt);
     h_foff = foff;
     h_lpoff = lpoff;
     h_boff = boff;
+    if (hybrid) load_clusters();
+    else coarse_layout_subdomains();
 
     // ---- SYMV work lists ----
     {
@@ -468,11 +590,14 @@ void espreso_feti_s::gpu_setup() {
     assemble(d_f, d_Qm, h_foff, h_lpoff, h_boff);
     last.time_assembly = wall() - ta;
     double tc = wall();
+    if (hybrid) cluster_setup();
     coarse();
-    // K^+ f on the boundary: ubf = F_i hf (FP64), then b = B K^+ f - c
+    // K^+ f on the boundary: ubf = F_i hf (FP64); hybrid: + cluster correction;
+    // then b = B K^+ f - c
     launch_symv(D, false, 0, d_hf, 0, false, st);
     ESP_HIP(hipMemcpyAsync(d_ubf, D.yb, D.nbnd * sizeof(double), hipMemcpyDeviceToDevice, st));
-    launch_lambda_gather(D, d_ubf, false, d_b, st);
+    if (hybrid) launch_cluster_correction(D, CL, d_e0f, D.yb, 1, st);
+    launch_lambda_gather(D, D.yb, false, d_b, st);
     launch_axpby(d_b, d_b, d_c, nlam, 1.0, -1.0, st);
     sync();
     last.time_coarse = wall() - tc;
@@ -587,6 +712,27 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         launch_identity(d_desc, nbt, max_m, st);
         for (int k = 0; k < max_ntm; k++) launch_trsm_step(d_desc, nbt, 1, k, max_ntm, max_ntm, st);
         launch_syrk(d_desc, nbt, 1, max_ntm, st);
+        if (hybrid) {
+            std::vector<int> nq(nq_h.begin() + s0, nq_h.begin() + s1);
+            std::vector<int64_t> qo(nbt), fo(nbt), co(nbt);
+            int max_nq = 0;
+            int64_t fb = 0;
+            for (int s = 0; s < s0; s++) fb += (int64_t)sub[s].m * nq_h[s];
+            for (int b = 0; b < nbt; b++) {
+                qo[b] = qoff_h[s0 + b];
+                co[b] = fccoff_h[s0 + b];
+                fo[b] = fb;
+                fb += (int64_t)sub[s0 + b].m * nq_h[s0 + b];
+                max_nq = std::max(max_nq, nq[b]);
+            }
+            int *d_nq = upload(nq);
+            int64_t *d_qo = upload(qo), *d_fo = upload(fo), *d_co = upload(co);
+            if (max_nq > 0)
+                launch_extract_corners(d_desc, nbt, d_nq, d_qo, d_qjint, d_qsign, d_fo, d_co, CL.FB0, d_Fcc, max_m,
+                                       max_nq, st);
+            sync();
+            for (void *p : {(void *)d_nq, (void *)d_qo, (void *)d_fo, (void *)d_co}) release(p);
+        }
         launch_pack(d_desc, nbt, 0, D.Fd, d_f_off, nullptr, max_m, st);
         if (prec == 1) launch_pack(d_desc, nbt, 1, D.Ff, d_lp_off, nullptr, max_m, st);
         if (prec == 2) launch_pack(d_desc, nbt, 2, D.Fh, d_lp_off, D.fscale + s0, max_m, st);
@@ -679,6 +825,11 @@ void espreso_feti_s::build_items(int prec, std::vector<SymvItem> &items, std::ve
 // ------------------------------------------------------ dual operators ----
 void espreso_feti_s::apply_F(const double *x, double *y, int prec) {
     launch_symv(D, false, prec, x, 1, false, st);
+    if (hybrid) {
+        // K~^+ = K^+ - K^+ B0^T lambda0 + R alpha (cluster.cpp:796-916), e0 = -R^T B^T x
+        launch_subdomain_kernel_dots(D, x, d_e0s, st);
+        launch_cluster_correction(D, CL, d_e0s, D.yb, 1, st);
+    }
     launch_lambda_gather(D, D.yb, false, y, st);
 }
 
@@ -805,11 +956,18 @@ void espreso_feti_s::solve(espreso_feti_stats *out) {
     }
     double tp = wall();
     last.time_pcg = tp - t0;
-    // amplitudes alpha = (GG^T)^-1 G r ; primal
-    launch_project(D, d_r, P.alpha, 2, st);
+    // amplitudes alpha = (GG^T)^-1 G r ; primal u = K~^+ (f - B^T lambda) + R alpha
+    launch_project(D, d_r, d_alpha_c, 2, st);
     // u_b = K^+ f|_b - F_i B^T lambda
     launch_symv(D, false, 0, d_x, 1, false, st);
     launch_axpby(D.xb, d_ubf, D.yb, D.nbnd, 1.0, -1.0, st);
+    if (hybrid) {
+        // e0 = -R^T (f - B^T lambda) = e0f - (-R^T B^T lambda)
+        launch_subdomain_kernel_dots(D, d_x, d_e0s, st);
+        launch_axpby(d_e0s, d_e0f, d_e0s, asubv[nsub], 1.0, -1.0, st);
+        launch_cluster_correction(D, CL, d_e0s, D.xb, 0, st);
+    }
+    launch_alpha_combine(D, CL, hybrid, d_alpha_c, d_alpha_tot, st);
     launch_primal(P, nsub, D.xb, d_u, st);
     sync();
     last.time_primal = wall() - tp;
@@ -826,6 +984,333 @@ void espreso_feti_s::solve(espreso_feti_stats *out) {
     last.time_total = last.time_assembly + last.time_coarse + last.time_pcg + last.time_primal;
     solved = true;
     if (out) *out = last;
+}
+
+// --------------------------------------------------- coarse space layout ----
+// GG^T (dense): sum over lambda of the outer products of G columns, G = -(B Rt)^T
+void espreso_feti_s::build_ggt(const std::vector<double> &Rt, const std::vector<int64_t> &rtoff,
+                               const std::vector<int> &ck, const std::vector<int> &cblk, const std::vector<int> &coff,
+                               std::vector<double> &GGt) {
+    const int nc = ncoarse;
+    GGt.assign((size_t)nc * nc, 0.0);
+    const std::vector<int64_t> &boff = h_boff;
+    for (int l = 0; l < nlam; l++)
+        for (int a = 0; a < le_cnt_h[l]; a++)
+            for (int b2 = 0; b2 < le_cnt_h[l]; b2++) {
+                const int s1 = le_sub_h[2 * l + a], s2 = le_sub_h[2 * l + b2];
+                const int64_t j1 = le_b_h[2 * l + a] - boff[s1], j2 = le_b_h[2 * l + b2] - boff[s2];
+                const double g = le_sign_h[2 * l + a] * le_sign_h[2 * l + b2];
+                const int m1 = sub[s1].m, m2 = sub[s2].m;
+                const int o1 = coff[cblk[s1]], o2 = coff[cblk[s2]];
+                for (int i = 0; i < ck[s1]; i++) {
+                    const double r1 = g * Rt[rtoff[s1] + j1 + (int64_t)i * m1];
+                    for (int j = 0; j < ck[s2]; j++)
+                        GGt[(o1 + i) + (size_t)(o2 + j) * nc] += r1 * Rt[rtoff[s2] + j2 + (int64_t)j * m2];
+                }
+            }
+}
+
+// Total FETI: one coarse block per subdomain, basis R_b, N_s = I
+void espreso_feti_s::coarse_layout_subdomains() {
+    std::vector<int> cblk(nsub), ck(nsub), coff(nsub), bptr(nsub + 1), bsub(nsub);
+    std::vector<int64_t> noff(nsub + 1, 0);
+    ncoarse = 0;
+    for (int s = 0; s < nsub; s++) {
+        cblk[s] = s;
+        ck[s] = sub[s].kdim;
+        coff[s] = ncoarse;
+        ncoarse += sub[s].kdim;
+        bptr[s] = s;
+        bsub[s] = s;
+        noff[s + 1] = noff[s] + (int64_t)sub[s].kdim * sub[s].kdim;
+    }
+    bptr[nsub] = nsub;
+    std::vector<double> N(noff[nsub], 0.0);
+    for (int s = 0; s < nsub; s++)
+        for (int a = 0; a < sub[s].kdim; a++) N[noff[s] + a + (int64_t)a * sub[s].kdim] = 1.0;
+    D.nblk = nsub;
+    D.ncoarse = ncoarse;
+    D.cblk = upload(cblk);
+    D.ck = upload(ck);
+    D.coff = upload(coff);
+    D.blk_ptr = upload(bptr);
+    D.blk_sub = upload(bsub);
+    D.Rt = D.Rb;
+    D.rtoff = D.rboff;
+    D.dsub_off = D.asub;
+    D.dsub = alloc<double>(asubv[nsub]);
+    D.N = upload(N);
+    D.noff = upload(noff);
+    D.cd = alloc<double>(ncoarse);
+    D.ce = alloc<double>(ncoarse);
+    d_alpha_c = alloc<double>(ncoarse);
+    std::vector<double> GGt;
+    build_ggt(Rb_h, rboff_h, ck, cblk, coff, GGt);
+    d_GGt = upload(GGt);
+    d_e = upload(e0f_h);
+}
+
+// Hybrid FETI: B0 rows per cluster, corner entries per subdomain (cluster.cpp:1186-1224)
+void espreso_feti_s::load_clusters() {
+    const std::vector<int64_t> &boff = h_boff;
+    std::vector<int> n0(ncl, 0), kt(ncl, 0);
+    std::vector<int64_t> rowoff(ncl + 1, 0), eoff(ncl, 0);
+    for (int c = 0; c < ncl; c++) {
+        const auto &L = cl_subs[c];
+        for (size_t q = 1; q < L.size(); q++)
+            ESP_CHECK(L[q] == L[q - 1] + 1, ESPRESO_ERR_ARG, "HYBRID_FETI: subdomains of a cluster must be consecutive");
+        int mx = -1;
+        for (int s : L)
+            for (int r : sub[s].b0row) mx = std::max(mx, r);
+        n0[c] = mx + 1;
+        rowoff[c + 1] = rowoff[c] + n0[c];
+        for (int s : L) kt[c] += sub[s].kdim;
+        eoff[c] = asubv[L[0]];
+    }
+    std::vector<int64_t> rb1(rowoff[ncl], -1), rb2(rowoff[ncl], -1);
+    nq_h.assign(nsub, 0);
+    qoff_h.assign(nsub + 1, 0);
+    std::vector<int64_t> fb0off(nsub + 1, 0);
+    fccoff_h.assign(nsub + 1, 0);
+    qrow_h.clear();
+    qj_h.clear();
+    qsign_h.clear();
+    int max_nq = 0;
+    for (int s = 0; s < nsub; s++) {
+        const SubHost &h = sub[s];
+        const int c = h.cluster;
+        nq_h[s] = (int)h.b0row.size();
+        ESP_CHECK(nq_h[s] <= 256, ESPRESO_ERR_ARG, "HYBRID_FETI: more than 256 B0 entries in one subdomain");
+        max_nq = std::max(max_nq, nq_h[s]);
+        qoff_h[s + 1] = qoff_h[s] + nq_h[s];
+        fb0off[s + 1] = fb0off[s] + (int64_t)h.m * nq_h[s];
+        fccoff_h[s + 1] = fccoff_h[s] + (int64_t)nq_h[s] * nq_h[s];
+        for (int q = 0; q < nq_h[s]; q++) {
+            const int r = h.b0row[q];
+            ESP_CHECK(r >= 0 && r < n0[c], ESPRESO_ERR_ARG, "B0 row out of range");
+            const int j = h.iperm[h.b0dof[q]] - h.ni;
+            const int64_t row = rowoff[c] + r;
+            qrow_h.push_back((int)row);
+            qj_h.push_back(j);
+            qsign_h.push_back(h.b0sign[q]);
+            if (h.b0sign[q] > 0) {
+                ESP_CHECK(rb1[row] < 0, ESPRESO_ERR_ARG, "B0 row with two +1 entries");
+                rb1[row] = boff[s] + j;
+            } else {
+                ESP_CHECK(rb2[row] < 0, ESPRESO_ERR_ARG, "B0 row with two -1 entries");
+                rb2[row] = boff[s] + j;
+            }
+        }
+    }
+    for (int64_t r = 0; r < rowoff[ncl]; r++)
+        ESP_CHECK(rb1[r] >= 0 && rb2[r] >= 0, ESPRESO_ERR_ARG, "each B0 row needs one +1 and one -1 entry");
+    CL.ncl = ncl;
+    CL.max_n0 = *std::max_element(n0.begin(), n0.end());
+    CL.max_kt = *std::max_element(kt.begin(), kt.end());
+    CL.n0 = upload(n0);
+    CL.kt = upload(kt);
+    CL.rowoff = upload(rowoff);
+    CL.eoff = upload(eoff);
+    CL.row_b1 = upload(rb1);
+    CL.row_b2 = upload(rb2);
+    CL.nq = upload(nq_h);
+    CL.qoff = upload(qoff_h);
+    CL.qrow = upload(qrow_h);
+    CL.fb0off = upload(fb0off);
+    CL.asub = D.asub;
+    CL.FB0 = alloc<double>(fb0off[nsub]);
+    CL.lambda0 = alloc<double>(std::max<int64_t>(1, rowoff[ncl]));
+    CL.alphaK = alloc<double>(asubv[nsub]);
+    d_qjint = upload(qj_h);
+    d_qsign = upload(qsign_h);
+    d_Fcc = alloc<double>(std::max<int64_t>(1, fccoff_h[nsub]));
+    (void)max_nq;
+}
+
+// Hybrid FETI cluster problems (cluster.cpp:1226-2016): F0 = B0 K^+ B0^T from
+// the corner blocks, G0 = -(B0 R)^T, Sa = G0 F0^-1 G0^T regularised on its
+// kernel N = ker G0^T (the cluster rigid modes); the coarse basis of the
+// cluster is R_s N_s on each of its subdomains.
+void espreso_feti_s::cluster_setup() {
+    std::vector<double> Fcc(std::max<int64_t>(1, fccoff_h[nsub]));
+    ESP_HIP(hipMemcpyAsync(Fcc.data(), d_Fcc, Fcc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    sync();
+    std::vector<int> n0(ncl), kt(ncl), kN(ncl);
+    std::vector<int64_t> rowoff(ncl + 1, 0);
+    std::vector<std::vector<double>> F0i(ncl), G0(ncl), Sai(ncl), Nc(ncl);
+    for (int c = 0; c < ncl; c++) {
+        int mx = -1, k = 0;
+        for (int s : cl_subs[c]) {
+            for (int r : sub[s].b0row) mx = std::max(mx, r);
+            k += sub[s].kdim;
+        }
+        n0[c] = mx + 1;
+        kt[c] = k;
+        rowoff[c + 1] = rowoff[c] + n0[c];
+    }
+    std::string err;
+#pragma omp parallel for schedule(dynamic)
+    for (int c = 0; c < ncl; c++) {
+        const int n = n0[c], k = kt[c];
+        std::vector<double> F0((size_t)n * n, 0.0), G((size_t)k * n, 0.0);
+        int aoff = 0;
+        for (int s : cl_subs[c]) {
+            const SubHost &h = sub[s];
+            const int nq = nq_h[s];
+            const int64_t q0 = qoff_h[s];
+            for (int q1 = 0; q1 < nq; q1++)
+                for (int q2 = 0; q2 < nq; q2++)
+                    F0[(qrow_h[q0 + q1] - rowoff[c]) + (size_t)(qrow_h[q0 + q2] - rowoff[c]) * n] +=
+                        Fcc[fccoff_h[s] + q1 + (int64_t)q2 * nq];
+            for (int q = 0; q < nq; q++) {
+                const int r = qrow_h[q0 + q] - (int)rowoff[c];
+                const int j = qj_h[q0 + q];
+                for (int a = 0; a < h.kdim; a++)
+                    G[(aoff + a) + (size_t)r * k] -= qsign_h[q0 + q] * Rb_h[rboff_h[s] + j + (int64_t)a * h.m];
+            }
+            aoff += h.kdim;
+        }
+        if (n > 0 && !spd_inverse(n, F0)) {
+#pragma omp critical
+            err = "HYBRID_FETI: F0 of cluster " + std::to_string(c) + " is not positive definite";
+            continue;
+        }
+        // Sa = G F0^-1 G^T
+        std::vector<double> GF((size_t)k * n, 0.0), Sa((size_t)k * k, 0.0), GGc((size_t)k * k, 0.0);
+        for (int a = 0; a < k; a++)
+            for (int r = 0; r < n; r++) {
+                double v = 0;
+                for (int q = 0; q < n; q++) v += G[a + (size_t)q * k] * F0[q + (size_t)r * n];
+                GF[a + (size_t)r * k] = v;
+            }
+        for (int a = 0; a < k; a++)
+            for (int b = 0; b < k; b++) {
+                double v = 0, w = 0;
+                for (int r = 0; r < n; r++) {
+                    v += GF[a + (size_t)r * k] * G[b + (size_t)r * k];
+                    w += G[a + (size_t)r * k] * G[b + (size_t)r * k];
+                }
+                Sa[a + (size_t)b * k] = v;
+                GGc[a + (size_t)b * k] = w;
+            }
+        // kernel of G^T: eigenvectors of G G^T with (numerically) zero eigenvalue
+        std::vector<double> w, V;
+        jacobi_eigen(k, GGc, w, V);
+        double wmax = 0;
+        for (double x : w) wmax = std::max(wmax, std::fabs(x));
+        std::vector<double> N;
+        int kn = 0;
+        for (int i = 0; i < k; i++)
+            if (std::fabs(w[i]) <= 1e-10 * wmax || wmax == 0.0) {
+                N.insert(N.end(), V.begin() + (size_t)i * k, V.begin() + (size_t)(i + 1) * k);
+                kn++;
+            }
+        if (kn == 0) {
+#pragma omp critical
+            err = "HYBRID_FETI: cluster " + std::to_string(c) + " has no rigid modes left (over-constrained B0)";
+            continue;
+        }
+        double rho = 0;
+        for (int a = 0; a < k; a++) rho = std::max(rho, Sa[a + (size_t)a * k]);
+        if (!(rho > 0)) rho = 1.0;
+        for (int a = 0; a < k; a++)
+            for (int b = 0; b < k; b++) {
+                double v = 0;
+                for (int i = 0; i < kn; i++) v += N[a + (size_t)i * k] * N[b + (size_t)i * k];
+                Sa[a + (size_t)b * k] += rho * v;
+            }
+        if (!spd_inverse(k, Sa)) {
+#pragma omp critical
+            err = "HYBRID_FETI: regularised Sa of cluster " + std::to_string(c) + " is not positive definite";
+            continue;
+        }
+        F0i[c] = std::move(F0);
+        G0[c] = std::move(G);
+        Sai[c] = std::move(Sa);
+        Nc[c] = std::move(N);
+        kN[c] = kn;
+    }
+    ESP_CHECK(err.empty(), ESPRESO_ERR_NUMERIC, err);
+    // device cluster matrices
+    std::vector<int64_t> f0off(ncl + 1, 0), g0off(ncl + 1, 0), saoff(ncl + 1, 0);
+    for (int c = 0; c < ncl; c++) {
+        f0off[c + 1] = f0off[c] + (int64_t)n0[c] * n0[c];
+        g0off[c + 1] = g0off[c] + (int64_t)kt[c] * n0[c];
+        saoff[c + 1] = saoff[c] + (int64_t)kt[c] * kt[c];
+    }
+    std::vector<double> F0all(std::max<int64_t>(1, f0off[ncl])), G0all(std::max<int64_t>(1, g0off[ncl])),
+        Saall(saoff[ncl]);
+    for (int c = 0; c < ncl; c++) {
+        std::copy(F0i[c].begin(), F0i[c].end(), F0all.begin() + f0off[c]);
+        std::copy(G0[c].begin(), G0[c].end(), G0all.begin() + g0off[c]);
+        std::copy(Sai[c].begin(), Sai[c].end(), Saall.begin() + saoff[c]);
+    }
+    CL.F0inv = replace_upload(CL.F0inv, F0all);
+    CL.G0 = replace_upload(CL.G0, G0all);
+    CL.Sainv = replace_upload(CL.Sainv, Saall);
+    CL.f0off = replace_upload(CL.f0off, f0off);
+    CL.g0off = replace_upload(CL.g0off, g0off);
+    CL.saoff = replace_upload(CL.saoff, saoff);
+    // coarse space: one block per cluster, basis Rt_s = R_b,s N_s
+    std::vector<int> cblk(nsub), ck(nsub), coff(ncl), bptr(ncl + 1, 0), bsub;
+    std::vector<int64_t> rtoff(nsub + 1, 0), noff(nsub + 1, 0), dsub_off(nsub + 1, 0);
+    ncoarse = 0;
+    for (int c = 0; c < ncl; c++) {
+        coff[c] = ncoarse;
+        ncoarse += kN[c];
+        for (int s : cl_subs[c]) bsub.push_back(s);
+        bptr[c + 1] = (int)bsub.size();
+    }
+    for (int s = 0; s < nsub; s++) {
+        const int c = sub[s].cluster;
+        cblk[s] = c;
+        ck[s] = kN[c];
+        rtoff[s + 1] = rtoff[s] + (int64_t)sub[s].m * kN[c];
+        noff[s + 1] = noff[s] + (int64_t)sub[s].kdim * kN[c];
+        dsub_off[s + 1] = dsub_off[s] + kN[c];
+    }
+    std::vector<double> Rt(rtoff[nsub], 0.0), Nall(noff[nsub], 0.0), e(ncoarse, 0.0);
+    for (int c = 0; c < ncl; c++) {
+        const int k = kt[c];
+        int aoff = 0;
+        for (int s : cl_subs[c]) {
+            const SubHost &h = sub[s];
+            for (int b = 0; b < kN[c]; b++) {
+                for (int a = 0; a < h.kdim; a++) {
+                    const double nab = Nc[c][(aoff + a) + (size_t)b * k];
+                    Nall[noff[s] + a + (int64_t)b * h.kdim] = nab;
+                    e[coff[c] + b] += nab * e0f_h[asubv[s] + a];
+                    for (int j = 0; j < h.m; j++)
+                        Rt[rtoff[s] + j + (int64_t)b * h.m] += Rb_h[rboff_h[s] + j + (int64_t)a * h.m] * nab;
+                }
+            }
+            aoff += h.kdim;
+        }
+    }
+    D.nblk = ncl;
+    D.ncoarse = ncoarse;
+    D.cblk = replace_upload(D.cblk, cblk);
+    D.ck = replace_upload(D.ck, ck);
+    D.coff = replace_upload(D.coff, coff);
+    D.blk_ptr = replace_upload(D.blk_ptr, bptr);
+    D.blk_sub = replace_upload(D.blk_sub, bsub);
+    D.rtoff = replace_upload(D.rtoff, rtoff);
+    D.Rt = replace_upload(D.Rt, Rt);
+    D.N = replace_upload(D.N, Nall);
+    D.noff = replace_upload(D.noff, noff);
+    D.dsub_off = replace_upload(D.dsub_off, dsub_off);
+    if (D.dsub) release(D.dsub);
+    D.dsub = alloc<double>(std::max<int64_t>(1, dsub_off[nsub]));
+    if (D.cd) release(D.cd);
+    if (D.ce) release(D.ce);
+    if (d_alpha_c) release(d_alpha_c);
+    D.cd = alloc<double>(ncoarse);
+    D.ce = alloc<double>(ncoarse);
+    d_alpha_c = alloc<double>(ncoarse);
+    std::vector<double> GGt;
+    build_ggt(Rt, rtoff, ck, cblk, coff, GGt);
+    d_GGt = replace_upload(d_GGt, GGt);
+    d_e = replace_upload(d_e, e);
 }
 
 // ================================================================ C ABI ====
@@ -914,6 +1399,25 @@ int espreso_feti_set_subdomain(espreso_feti *h, int32_t i, int32_t n, const int3
         ESP_CHECK(q == 0 || s.lam[q] > s.lam[q - 1], ESPRESO_ERR_ARG, "multipliers must be ascending");
     }
     s.set = true;
+    ESP_API_END
+}
+
+int espreso_feti_set_cluster(espreso_feti *h, int32_t i, int32_t cluster, int32_t n0, const int32_t *row,
+                             const int32_t *dof, const double *sign) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && i >= 0 && i < h->nsub && cluster >= 0 && n0 >= 0, ESPRESO_ERR_ARG, "invalid cluster data");
+    ESP_CHECK(!h->loaded, ESPRESO_ERR_STATE, "setup already done");
+    ESP_CHECK(h->sub[i].set, ESPRESO_ERR_STATE, "set the subdomain before its cluster data");
+    SubHost &s = h->sub[i];
+    s.cluster = cluster;
+    s.b0row.assign(row, row + n0);
+    s.b0dof.assign(dof, dof + n0);
+    s.b0sign.assign(sign, sign + n0);
+    for (int q = 0; q < n0; q++) {
+        ESP_CHECK(s.b0dof[q] >= 0 && s.b0dof[q] < s.n, ESPRESO_ERR_ARG, "B0 DOF out of range");
+        ESP_CHECK(s.b0sign[q] == 1.0 || s.b0sign[q] == -1.0, ESPRESO_ERR_ARG, "B0 values must be +1 or -1");
+        ESP_CHECK(s.b0row[q] >= 0, ESPRESO_ERR_ARG, "B0 row must be non-negative");
+    }
     ESP_API_END
 }
 
@@ -1144,6 +1648,11 @@ int espreso_grid_load(espreso_feti *h, const espreso_grid_problem *p) {
                                             sd.f.data(), sd.kdim, sd.R.data(), (int)sd.lam.size(), sd.lam.data(),
                                             sd.dof.data(), sd.sign.data(), sd.scale.data());
         if (rc) return rc;
+        if (h->opt.method == 1) {
+            rc = espreso_feti_set_cluster(h, s, sd.cluster, (int)sd.b0row.size(), sd.b0row.data(), sd.b0dof.data(),
+                                          sd.b0sign.data());
+            if (rc) return rc;
+        }
         h->l2g[s].swap(sd.l2g);
         GridSubdomain().rowptr.swap(sd.rowptr);
         std::vector<int>().swap(sd.colidx);
@@ -1178,6 +1687,22 @@ int espreso_grid_export(const espreso_grid_problem *p, int32_t scaling, int32_t 
     if (sign) std::copy(sd.sign.begin(), sd.sign.end(), sign);
     if (scale) std::copy(sd.scale.begin(), sd.scale.end(), scale);
     if (c) std::copy(g.c.begin(), g.c.end(), c);
+    ESP_API_END
+}
+
+int espreso_grid_export_cluster(const espreso_grid_problem *p, int32_t i, int32_t *cluster, int32_t *n0,
+                                int32_t *row, int32_t *dof, double *sign) {
+    ESP_API_BEGIN
+    ESP_CHECK(p && cluster && n0, ESPRESO_ERR_ARG, "null argument");
+    ESP_CHECK(i >= 0 && i < grid_subdomain_count(*p), ESPRESO_ERR_ARG, "subdomain out of range");
+    GridProblem g;
+    grid_build(*p, 1, g);
+    const GridSubdomain &sd = g.sub[i];
+    *cluster = sd.cluster;
+    *n0 = (int)sd.b0row.size();
+    if (row) std::copy(sd.b0row.begin(), sd.b0row.end(), row);
+    if (dof) std::copy(sd.b0dof.begin(), sd.b0dof.end(), dof);
+    if (sign) std::copy(sd.b0sign.begin(), sd.b0sign.end(), sign);
     ESP_API_END
 }
 

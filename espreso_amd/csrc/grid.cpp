@@ -397,6 +397,79 @@ void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out) {
                 }
             }
     out.nlam = lam;
+
+    // ---- B0 corners per cluster (fetidata.cpp:146-187, equalityconstraints.cpp:504-552) ----
+    // Interface nodes of a cluster are grouped by the full set of subdomains
+    // owning them; a group touching other clusters ("external") gives a corner
+    // every size/3 nodes, any other group its middle node.  Each corner glues
+    // consecutive pairs of the cluster's owners, one row per DOF.
+    const int ncl = p.clusters[0] * p.clusters[1] * p.clusters[2];
+    for (int s = 0; s < ns; s++) out.sub[s].cluster = s / B.dpc;
+#pragma omp parallel for schedule(dynamic)
+    for (int cl = 0; cl < ncl; cl++) {
+        const int cc[3] = {cl % p.clusters[0], (cl / p.clusters[0]) % p.clusters[1], cl / (p.clusters[0] * p.clusters[1])};
+        int lo3[3], hi3[3];
+        for (int d = 0; d < 3; d++) {
+            const int span = p.domains[d] * p.elements[d];
+            lo3[d] = cc[d] * span;
+            hi3[d] = (cc[d] + 1) * span;
+        }
+        std::vector<std::vector<int>> keys;
+        std::vector<std::vector<int>> members;
+        std::vector<int> own, loc;
+        int gg[3];
+        for (gg[2] = lo3[2]; gg[2] <= hi3[2]; gg[2]++)
+            for (gg[1] = lo3[1]; gg[1] <= hi3[1]; gg[1]++)
+                for (gg[0] = lo3[0]; gg[0] <= hi3[0]; gg[0]++) {
+                    own.clear();
+                    int lo[3], hi[3];
+                    for (int d = 0; d < 3; d++) B.owners(d, gg[d], lo[d], hi[d]);
+                    for (int z = lo[2]; z <= hi[2]; z++)
+                        for (int y = lo[1]; y <= hi[1]; y++)
+                            for (int x = lo[0]; x <= hi[0]; x++) own.push_back(B.id_of(x, y, z));
+                    std::sort(own.begin(), own.end());
+                    int nloc_own = 0;
+                    for (int o : own) nloc_own += o / B.dpc == cl;
+                    if (nloc_own < 2) continue;
+                    size_t gi = 0;
+                    while (gi < keys.size() && keys[gi] != own) gi++;
+                    if (gi == keys.size()) {
+                        keys.push_back(own);
+                        members.emplace_back();
+                    }
+                    members[gi].push_back(gg[0] + B.NN[0] * (gg[1] + B.NN[1] * gg[2]));
+                }
+        int row = 0;
+        for (size_t gi = 0; gi < keys.size(); gi++) {
+            loc.clear();
+            for (int o : keys[gi])
+                if (o / B.dpc == cl) loc.push_back(o);
+            const bool external = loc.size() != keys[gi].size();
+            const std::vector<int> &nodes = members[gi];
+            const int sz = (int)nodes.size();
+            std::vector<int> picks;
+            if (external) {
+                const int inc = std::max(1, sz / 3);
+                for (int q = 0; q < sz; q += inc) picks.push_back(nodes[q]);
+            } else {
+                picks.push_back(nodes[sz / 2]);
+            }
+            for (int node : picks) {
+                const int g3[3] = {node % B.NN[0], (node / B.NN[0]) % B.NN[1], node / (B.NN[0] * B.NN[1])};
+                for (size_t a = 0; a + 1 < loc.size(); a++)
+                    for (int c = 0; c < nd; c++) {
+                        GridSubdomain &s1 = out.sub[loc[a]], &s2 = out.sub[loc[a + 1]];
+                        s1.b0row.push_back(row);
+                        s1.b0dof.push_back(local_dof(loc[a], g3, c));
+                        s1.b0sign.push_back(1.0);
+                        s2.b0row.push_back(row);
+                        s2.b0dof.push_back(local_dof(loc[a + 1], g3, c));
+                        s2.b0sign.push_back(-1.0);
+                        row++;
+                    }
+            }
+        }
+    }
 }
 
 }  // namespace esp

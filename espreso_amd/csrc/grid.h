@@ -22,6 +22,10 @@ struct GridSubdomain {
     std::vector<double> R;           // n x kdim, orthonormal columns
     std::vector<int> lam, dof;       // lambda rows touching the subdomain, ascending
     std::vector<double> sign, scale;
+    // hybrid FETI: cluster id and the B0 corner entries (cluster-local row, DOF, +1/-1)
+    int cluster = 0;
+    std::vector<int> b0row, b0dof;
+    std::vector<double> b0sign;
 };
 
 struct GridProblem {

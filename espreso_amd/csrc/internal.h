@@ -96,6 +96,12 @@ void launch_scatter(const DenseDesc *desc, int nb, const int *rowptr, const int6
                     const double *vals, const int64_t *nz_off, const int *iperm, const int64_t *perm_off,
                     const double *f, const int *n, int max_n, hipStream_t st);
 
+// hybrid FETI: FB0 = F_i B0_i^T (m x nq, signed corner columns) and the
+// corner block B0_i F_i B0_i^T (nq x nq), from F_i in the lower triangle of S
+void launch_extract_corners(const DenseDesc *desc, int nb, const int *nq, const int64_t *qoff, const int *qj,
+                            const double *qsign, const int64_t *fb0off, const int64_t *fccoff, double *FB0,
+                            double *Fcc, int max_m, int max_nq, hipStream_t st);
+
 // diagnostics (dense.hip)
 void launch_probe_mfma(const double *a, const double *b, double *out, hipStream_t st);
 void launch_probe_gemm(int ta, int tb, int M, int N, int K, const double *A, int lda, const double *B, int ldb,

@@ -72,7 +72,7 @@ enum { ESPRESO_PREC_NONE = 0, ESPRESO_PREC_LUMPED = 1, ESPRESO_PREC_WEIGHT = 2, 
 enum { ESPRESO_SC_FP64 = 0, ESPRESO_SC_FP32 = 1, ESPRESO_SC_BF16 = 2 };
 
 typedef struct {
-    int32_t method;             /* 0 TOTAL_FETI                                */
+    int32_t method;             /* 0 TOTAL_FETI, 1 HYBRID_FETI                 */
     int32_t preconditioner;     /* ESPRESO_PREC_*                              */
     int32_t max_iterations;
     int32_t scaling;            /* 1: K-diagonal B1 scaling; 0: 1/multiplicity */
@@ -127,6 +127,11 @@ int espreso_grid_load(espreso_feti *h, const espreso_grid_problem *p);
 int espreso_grid_export(const espreso_grid_problem *p, int32_t scaling, int32_t i, int32_t *sizes,
                         int32_t *rowptr, int32_t *colidx, double *values, double *f, double *R,
                         int32_t *lambda, int32_t *dof, double *sign, double *scale, double *c);
+/* Host-only export of the hybrid-FETI data of subdomain i: its cluster and
+ * its B0 corner entries (cluster-local row, local DOF, +1/-1).  *n0 returns
+ * the entry count; pass NULL arrays to query it.                             */
+int espreso_grid_export_cluster(const espreso_grid_problem *p, int32_t i, int32_t *cluster, int32_t *n0,
+                                int32_t *row, int32_t *dof, double *sign);
 /* global node id of every local node of subdomain i (after espreso_grid_load) */
 int espreso_grid_l2g(espreso_feti *h, int32_t i, int32_t *l2g);
 /* merge per-subdomain primal of the last solve into nodal u (average of copies) */
@@ -148,6 +153,14 @@ int espreso_feti_set_subdomain(espreso_feti *h, int32_t i, int32_t n,
                                int32_t nlam, const int32_t *lambda, const int32_t *dof,
                                const double *sign, const double *scale);
 int espreso_feti_set_dual(espreso_feti *h, int32_t n_lambda, const double *c);
+/* Hybrid FETI (options.method = 1): the cluster of subdomain i and the B0
+ * rows touching it (cluster-local row id, local DOF, +1/-1); every B0 row
+ * holds one +1 and one -1 entry of two subdomains of the same cluster, and
+ * the subdomains of a cluster are numbered consecutively.  Replaces
+ * Constraints::B0Corners (equalityconstraints.cpp:504-552) and
+ * ClusterBase::CompressB0 (cluster.cpp:1186-1224) inputs.                    */
+int espreso_feti_set_cluster(espreso_feti *h, int32_t i, int32_t cluster, int32_t n0, const int32_t *row,
+                             const int32_t *dof, const double *sign);
 
 /* GPU setup: boundary Schur complements, F_i = (K^+)_bb, S_D, coarse problem.
  * The first call also preprocesses and uploads the inputs; later calls redo

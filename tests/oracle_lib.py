@@ -277,3 +277,32 @@ def cpu_sample(spec, sample, threads=0, preconditioner=3):
     if rc:
         raise RuntimeError(lib().oracle_last_error().decode())
     return dict(setup_per_sub=t[0], kplus_per_sub=t[1], sd_apply_per_sub=t[2])
+
+
+def hybrid(spec, dirichlet=False):
+    """Numpy HTFETI restatement (oracle/htfeti.py) fed by the C oracle's
+    decomposition: K^+ = K_reg^-1 (fix-point regularisation), analytic R, f,
+    B1, c, corner B0 from the subdomain node maps."""
+    from oracle import htfeti
+    o = Oracle(spec, preconditioner=3 if dirichlet else 0, kplus=0)
+    try:
+        Kp, R, f, B1, l2g, SD = [], [], [], [], [], []
+        for s in range(o.n_sub):
+            Kp.append(np.linalg.inv(o.dense(s, 1)))
+            R.append(np.ascontiguousarray(o.dense(s, 2)))
+            f.append(o.rhs(s))
+            B1.append(o.lambdas(s))
+            l2g.append(o.l2g(s))
+            if dirichlet:
+                SD.append((o.boundary(s), o.dense(s, 3)))
+        c = o.vec_c()
+        ndof, nsub = o.ndof, o.n_sub
+    finally:
+        o.close()
+    dpc = spec["domains"][0] * spec["domains"][1] * spec["domains"][2]
+    B0 = htfeti.corners(l2g, ndof, dpc)
+    H = htfeti.HybridFETI(Kp, R, f, B1, c, B0, SD if dirichlet else None)
+    H.l2g = l2g
+    H.ndof = ndof
+    H.B0data = B0
+    return H
