@@ -461,6 +461,22 @@ void launch_extract_corners(const DenseDesc *desc, int nb, const int *nq, const 
                        FB0, Fcc);
 }
 
+// S(upper) = S(lower)^T for every descriptor
+__global__ void k_symmetrize_desc(const DenseDesc *D) {
+    const DenseDesc &d = D[blockIdx.y];
+    const int m = d.m;
+    const int64_t nn = (int64_t)m * m;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nn; e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % m), j = (int)(e / m);
+        if (i < j) d.S[e] = d.S[j + (int64_t)i * m];
+    }
+}
+
+void launch_symmetrize_desc(const DenseDesc *desc, int nb, int max_m, hipStream_t st) {
+    unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(((int64_t)max_m * max_m + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_symmetrize_desc, dim3(g, nb), dim3(256), 0, st, desc);
+}
+
 // ---- diagnostics: one MFMA with lane-indexed operands, one tile GEMM ----
 __global__ void k_probe_mfma(const double *a, const double *b, double *out) {
     const int lane = threadIdx.x;

@@ -67,15 +67,22 @@ struct DualDev {
 
 // hybrid FETI clusters (B0 corners, F0, G0, Sa), device side
 struct ClusterDev {
-    int ncl = 0, max_n0 = 0, max_kt = 0;
-    int *n0 = nullptr, *kt = nullptr;       // B0 rows, sum of subdomain kernel dims per cluster
-    int64_t *rowoff = nullptr, *f0off = nullptr, *g0off = nullptr, *saoff = nullptr, *eoff = nullptr;
+    int ncl = 0, max_n0 = 0, max_kt = 0, max_kn = 0;
+    int *n0 = nullptr, *kt = nullptr, *kn = nullptr;  // B0 rows, sum of subdomain kernel dims, cluster kernel dim
+    int *sfirst = nullptr, *scount = nullptr;        // consecutive subdomains of each cluster
+    int64_t *rowoff = nullptr, *f0off = nullptr, *g0off = nullptr, *saoff = nullptr, *eoff = nullptr,
+            *ncoff = nullptr;
     int64_t *row_b1 = nullptr, *row_b2 = nullptr;   // global boundary index of the +1 / -1 entry of each row
-    double *F0inv = nullptr, *G0 = nullptr, *Sainv = nullptr;
+    double *F0inv = nullptr;   // n0 x n0 per cluster (full, symmetric)
+    double *G0T = nullptr;     // n0 x kt per cluster: -(B0 R), i.e. G0^T
+    double *H = nullptr;       // n0 x kt per cluster: F0^-1 G0^T
+    double *Sainv = nullptr;   // kt x kt per cluster: (G0 F0^-1 G0^T + rho N N^T)^-1
+    double *Nc = nullptr;      // kt x kn per cluster: orthonormal basis of ker G0^T (cluster rigid modes)
     // per subdomain: B0 entries q (cluster-local row, FB0 column)
     int *nq = nullptr, *qrow = nullptr;
-    int64_t *qoff = nullptr, *fb0off = nullptr, *asub = nullptr;
-    double *FB0 = nullptr;                  // m x nq per subdomain: sign_q * F_i[:, j_q]
+    int64_t *qoff = nullptr, *fb0off = nullptr, *fccoff = nullptr, *asub = nullptr;
+    double *FB0 = nullptr;     // m x nq per subdomain: sign_q * F_i[:, j_q]
+    double *Fcc = nullptr;     // nq x nq per subdomain: sign_q1 sign_q2 F_i[j_q1, j_q2]
     double *lambda0 = nullptr, *alphaK = nullptr;
 };
 
@@ -104,6 +111,10 @@ void launch_subdomain_kernel_dots(const DualDev &D, const double *x, double *out
 // y_b += -K^+ B0^T lambda0 (+ R alphaK when addR) after solving the cluster problems
 void launch_cluster_correction(const DualDev &D, const ClusterDev &C, const double *e0s, double *yb, int addR,
                                hipStream_t st);
+// set-up: F0 (into C.F0inv) from the corner blocks; H = F0^-1 G0^T and the
+// regularised Sa (into Sa) once C.F0inv holds the inverse
+void launch_cluster_setup_f0(const ClusterDev &C, hipStream_t st);
+void launch_cluster_setup_sa(const ClusterDev &C, double *Sa, hipStream_t st);
 // alpha_tot(s) = alphaK(s) (hybrid) + N_s alpha_coarse
 void launch_alpha_combine(const DualDev &D, const ClusterDev &C, bool hybrid, const double *alpha_coarse,
                           double *alpha_tot, hipStream_t st);

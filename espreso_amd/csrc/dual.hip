@@ -256,33 +256,58 @@ __global__ void k_project_finish(int nlam, const int *cnt, const int64_t *lb, co
 }
 
 // ---- hybrid FETI: cluster correction of K^+ (cluster.cpp:796-916) ----
-// one workgroup per cluster: g0 = B0 y_b, t = F0^-1 g0, alpha = Sa^-1 (G0 t - e0),
-// lambda0 = F0^-1 (g0 - G0^T alpha)
-__global__ void __launch_bounds__(256) k_cluster_solve(const ClusterDev C, const double *yb, const double *e0s,
-                                                       double *lambda0, double *alphaK) {
+// g0 = B0 y_b, t = F0^-1 g0, alpha = Sa^-1 (G0 t - e0), lambda0 = F0^-1 (g0 - G0^T alpha) = t - H alpha
+// with H = F0^-1 G0^T precomputed at setup.
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// t = F0^-1 g0: 32 rows per workgroup (8 per wave), each row one wave-wide dot
+// over a column of the symmetric F0^-1; g0 is gathered from the SYMV output.
+__global__ void __launch_bounds__(256) k_cluster_t(const ClusterDev C, const double *yb, double *t) {
+    extern __shared__ double g0[];
+    const int c = blockIdx.y;
+    const int n0 = C.n0[c];
+    const int rb = blockIdx.x * 32;
+    if (rb >= n0) return;  // uniform over the workgroup
+    const int64_t ro = C.rowoff[c];
+    for (int r = threadIdx.x; r < n0; r += 256) g0[r] = yb[C.row_b1[ro + r]] - yb[C.row_b2[ro + r]];
+    __syncthreads();
+    const double *F = C.F0inv + C.f0off[c];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int rr = w; rr < 32; rr += 4) {
+        const int r = rb + rr;
+        if (r >= n0) break;  // uniform over the wave
+        const double *col = F + (int64_t)r * n0;
+        double v = 0.0;
+        for (int q = lane; q < n0; q += 64) v += col[q] * g0[q];
+        v = wave_sum(v);
+        if (lane == 0) t[ro + r] = v;
+    }
+}
+
+// one workgroup per cluster: alpha = Sa^-1 (G0 t - e0), lambda0 = t - H alpha (in place over t)
+__global__ void __launch_bounds__(256) k_cluster_alpha(const ClusterDev C, const double *e0s, double *lam0,
+                                                       double *alphaK) {
     extern __shared__ double sh[];
     const int c = blockIdx.x;
     const int n0 = C.n0[c], kt = C.kt[c];
-    double *g0 = sh, *t = g0 + C.max_n0, *t2 = t + C.max_n0, *al = t2 + C.max_kt;
-    const int64_t r0 = C.rowoff[c];
-    for (int r = threadIdx.x; r < n0; r += 256) g0[r] = yb[C.row_b1[r0 + r]] - yb[C.row_b2[r0 + r]];
+    double *t2 = sh, *al = sh + C.max_kt;
+    const int64_t ro = C.rowoff[c];
+    const double *t = lam0 + ro;
+    const double *G0T = C.G0T + C.g0off[c];  // n0 x kt
+    const double *e0 = e0s + C.eoff[c];      // cluster-contiguous subdomain partials
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int a = w; a < kt; a += 4) {
+        double v = 0.0;
+        for (int r = lane; r < n0; r += 64) v += G0T[r + (int64_t)a * n0] * t[r];
+        v = wave_sum(v);
+        if (lane == 0) t2[a] = v - e0[a];
+    }
     __syncthreads();
-    const double *F0i = C.F0inv + C.f0off[c];
-    const double *G0 = C.G0 + C.g0off[c];      // kt x n0, column-major
     const double *Sai = C.Sainv + C.saoff[c];
-    const double *e0 = e0s + C.eoff[c];         // cluster-contiguous subdomain partials
-    for (int r = threadIdx.x; r < n0; r += 256) {
-        double v = 0.0;
-        for (int q = 0; q < n0; q++) v += F0i[r + (int64_t)q * n0] * g0[q];
-        t[r] = v;
-    }
-    __syncthreads();
-    for (int a = threadIdx.x; a < kt; a += 256) {
-        double v = 0.0;
-        for (int r = 0; r < n0; r++) v += G0[a + (int64_t)r * kt] * t[r];
-        t2[a] = v - e0[a];
-    }
-    __syncthreads();
     for (int a = threadIdx.x; a < kt; a += 256) {
         double v = 0.0;
         for (int b = 0; b < kt; b++) v += Sai[a + (int64_t)b * kt] * t2[b];
@@ -290,16 +315,89 @@ __global__ void __launch_bounds__(256) k_cluster_solve(const ClusterDev C, const
         alphaK[C.eoff[c] + a] = v;
     }
     __syncthreads();
+    const double *H = C.H + C.g0off[c];  // n0 x kt
     for (int r = threadIdx.x; r < n0; r += 256) {
-        double v = g0[r];
-        for (int a = 0; a < kt; a++) v -= G0[a + (int64_t)r * kt] * al[a];
-        t[r] = v;
+        double v = t[r];
+        for (int a = 0; a < kt; a++) v -= H[r + (int64_t)a * n0] * al[a];
+        lam0[ro + r] = v;
     }
+}
+
+// ---- hybrid FETI set-up on the device (cluster.cpp:1293-2016) ----
+// F0 = sum over the cluster's subdomains of their corner blocks; one workgroup
+// per cluster, subdomains in order (rows of one subdomain are distinct).
+__global__ void __launch_bounds__(256) k_f0_assemble(const ClusterDev C, double *F0) {
+    const int c = blockIdx.x;
+    const int n0 = C.n0[c];
+    double *F = F0 + C.f0off[c];
+    const int64_t nn = (int64_t)n0 * n0;
+    for (int64_t e = threadIdx.x; e < nn; e += 256) F[e] = 0.0;
     __syncthreads();
+    const int64_t ro = C.rowoff[c];
+    for (int s = C.sfirst[c]; s < C.sfirst[c] + C.scount[c]; s++) {
+        const int nq = C.nq[s];
+        const int *qr = C.qrow + C.qoff[s];
+        const double *Fc = C.Fcc + C.fccoff[s];
+        for (int e = threadIdx.x; e < nq * nq; e += 256) {
+            const int q1 = e % nq, q2 = e / nq;
+            F[(qr[q1] - ro) + (qr[q2] - ro) * (int64_t)n0] += Fc[e];
+        }
+        __syncthreads();
+    }
+}
+
+// H(:, a) = F0^-1 G0^T(:, a): grid (kt, clusters), F0^-1 symmetric so rows are read along columns
+__global__ void __launch_bounds__(256) k_cluster_h(const ClusterDev C) {
+    extern __shared__ double g[];
+    const int c = blockIdx.y, a = blockIdx.x;
+    const int n0 = C.n0[c], kt = C.kt[c];
+    if (a >= kt || n0 == 0) return;  // uniform
+    const double *G0T = C.G0T + C.g0off[c] + (int64_t)a * n0;
+    for (int r = threadIdx.x; r < n0; r += 256) g[r] = G0T[r];
+    __syncthreads();
+    const double *F = C.F0inv + C.f0off[c];
+    double *H = C.H + C.g0off[c] + (int64_t)a * n0;
     for (int r = threadIdx.x; r < n0; r += 256) {
         double v = 0.0;
-        for (int q = 0; q < n0; q++) v += F0i[r + (int64_t)q * n0] * t[q];
-        lambda0[r0 + r] = v;
+        for (int q = 0; q < n0; q++) v += F[r + (int64_t)q * n0] * g[q];
+        H[r] = v;
+    }
+}
+
+// Sa = G0 H + rho N N^T (rho = max diag of G0 H, 1 if that is not positive); one workgroup per cluster
+__global__ void __launch_bounds__(256) k_cluster_sa(const ClusterDev C, double *Sa) {
+    __shared__ double red[4];
+    __shared__ double rho;
+    const int c = blockIdx.x;
+    const int n0 = C.n0[c], kt = C.kt[c], kn = C.kn[c];
+    const double *G0T = C.G0T + C.g0off[c];
+    const double *H = C.H + C.g0off[c];
+    double *S = Sa + C.saoff[c];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int o = w; o < kt * kt; o += 4) {
+        const int i = o % kt, j = o / kt;
+        double v = 0.0;
+        for (int r = lane; r < n0; r += 64) v += G0T[r + (int64_t)i * n0] * H[r + (int64_t)j * n0];
+        v = wave_sum(v);
+        if (lane == 0) S[o] = v;
+    }
+    __syncthreads();
+    double m = 0.0;
+    for (int i = threadIdx.x; i < kt; i += 256) m = fmax(m, S[i + (int64_t)i * kt]);
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if (lane == 0) red[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        rho = r > 0.0 ? r : 1.0;
+    }
+    __syncthreads();
+    const double *N = C.Nc + C.ncoff[c];  // kt x kn
+    for (int o = threadIdx.x; o < kt * kt; o += 256) {
+        const int i = o % kt, j = o / kt;
+        double v = 0.0;
+        for (int b = 0; b < kn; b++) v += N[i + (int64_t)b * kt] * N[j + (int64_t)b * kt];
+        S[o] += rho * v;
     }
 }
 
@@ -520,10 +618,23 @@ void launch_subdomain_kernel_dots(const DualDev &D, const double *x, double *out
 
 void launch_cluster_correction(const DualDev &D, const ClusterDev &C, const double *e0s, double *yb, int addR,
                                hipStream_t st) {
-    const size_t shm = (2 * (size_t)C.max_n0 + 2 * (size_t)C.max_kt) * 8;
-    hipLaunchKernelGGL(k_cluster_solve, dim3(C.ncl), dim3(256), shm, st, C, yb, e0s, C.lambda0, C.alphaK);
+    if (C.max_n0 > 0)
+        hipLaunchKernelGGL(k_cluster_t, dim3((C.max_n0 + 31) / 32, C.ncl), dim3(256), (size_t)C.max_n0 * 8, st, C, yb,
+                           C.lambda0);
+    hipLaunchKernelGGL(k_cluster_alpha, dim3(C.ncl), dim3(256), 2 * (size_t)std::max(C.max_kt, 1) * 8, st, C, e0s,
+                       C.lambda0, C.alphaK);
     hipLaunchKernelGGL(k_cluster_apply, dim3(D.nsub), dim3(256), 0, st, C, D.m, D.boff, D.kdim, D.rboff, D.Rb,
                        C.lambda0, C.alphaK, addR, yb);
+}
+
+void launch_cluster_setup_f0(const ClusterDev &C, hipStream_t st) {
+    hipLaunchKernelGGL(k_f0_assemble, dim3(C.ncl), dim3(256), 0, st, C, C.F0inv);
+}
+
+void launch_cluster_setup_sa(const ClusterDev &C, double *Sa, hipStream_t st) {
+    if (C.max_n0 > 0 && C.max_kt > 0)
+        hipLaunchKernelGGL(k_cluster_h, dim3(C.max_kt, C.ncl), dim3(256), (size_t)C.max_n0 * 8, st, C);
+    hipLaunchKernelGGL(k_cluster_sa, dim3(C.ncl), dim3(256), 0, st, C, Sa);
 }
 
 void launch_alpha_combine(const DualDev &D, const ClusterDev &C, bool hybrid, const double *alpha_coarse,

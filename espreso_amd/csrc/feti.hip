@@ -89,45 +89,6 @@ void jacobi_eigen(int n, std::vector<double> A, std::vector<double> &w, std::vec
     for (int i = 0; i < n; i++) w[i] = A[i + (size_t)i * n];
 }
 
-// inverse of a symmetric positive definite matrix (column-major, in place)
-bool spd_inverse(int n, std::vector<double> &A) {
-    std::vector<double> L = A;
-    for (int j = 0; j < n; j++) {
-        double d = L[j + (size_t)j * n];
-        for (int k = 0; k < j; k++) d -= L[j + (size_t)k * n] * L[j + (size_t)k * n];
-        if (!(d > 0)) return false;
-        d = std::sqrt(d);
-        L[j + (size_t)j * n] = d;
-        for (int i = j + 1; i < n; i++) {
-            double v = L[i + (size_t)j * n];
-            for (int k = 0; k < j; k++) v -= L[i + (size_t)k * n] * L[j + (size_t)k * n];
-            L[i + (size_t)j * n] = v / d;
-        }
-    }
-    std::vector<double> x(n);
-    for (int c = 0; c < n; c++) {
-        std::fill(x.begin(), x.end(), 0.0);
-        x[c] = 1.0;
-        for (int i = 0; i < n; i++) {
-            double v = x[i];
-            for (int k = 0; k < i; k++) v -= L[i + (size_t)k * n] * x[k];
-            x[i] = v / L[i + (size_t)i * n];
-        }
-        for (int i = n - 1; i >= 0; i--) {
-            double v = x[i];
-            for (int k = i + 1; k < n; k++) v -= L[k + (size_t)i * n] * x[k];
-            x[i] = v / L[i + (size_t)i * n];
-        }
-        for (int i = 0; i < n; i++) A[i + (size_t)c * n] = x[i];
-    }
-    for (int j = 0; j < n; j++)
-        for (int i = j + 1; i < n; i++) {
-            const double v = 0.5 * (A[i + (size_t)j * n] + A[j + (size_t)i * n]);
-            A[i + (size_t)j * n] = A[j + (size_t)i * n] = v;
-        }
-    return true;
-}
-
 // orthonormalise the k columns of V (n x k); false when they are dependent
 bool mgs(int n, int k, double *V) {
     for (int pass = 0; pass < 2; pass++)
@@ -237,6 +198,10 @@ struct espreso_feti_s {
         return upload(v);
     }
     void cluster_setup();
+    void invert_spd_batched(double *pool, const std::vector<int64_t> &off, const std::vector<int> &n,
+                            const char *what);
+    std::vector<int> n0_h, kt_h;
+    std::vector<int64_t> f0off_h, saoff_h;
     void build_ggt(const std::vector<double> &Rt, const std::vector<int64_t> &rtoff, const std::vector<int> &ck,
                    const std::vector<int> &cblk, const std::vector<int> &coff, std::vector<double> &GGt);
     void build_items(int prec, std::vector<SymvItem> &items, std::vector<int> &item0, std::vector<int> &nit,
@@ -1050,10 +1015,15 @@ void espreso_feti_s::coarse_layout_subdomains() {
     d_e = upload(e0f_h);
 }
 
-// Hybrid FETI: B0 rows per cluster, corner entries per subdomain (cluster.cpp:1186-1224)
+// Hybrid FETI, host part (once per load; cluster.cpp:1186-1291, 2018-2159):
+// B0 rows per cluster and corner entries per subdomain, G0 = -(B0 R)^T, the
+// cluster kernel N = ker G0^T (Jacobi on G0 G0^T), the coarse space with one
+// block per cluster and basis R~_s = R_b,s N_s, GG^T and e.
 void espreso_feti_s::load_clusters() {
     const std::vector<int64_t> &boff = h_boff;
-    std::vector<int> n0(ncl, 0), kt(ncl, 0);
+    n0_h.assign(ncl, 0);
+    kt_h.assign(ncl, 0);
+    std::vector<int> kn(ncl, 0), sfirst(ncl), scount(ncl);
     std::vector<int64_t> rowoff(ncl + 1, 0), eoff(ncl, 0);
     for (int c = 0; c < ncl; c++) {
         const auto &L = cl_subs[c];
@@ -1062,10 +1032,12 @@ void espreso_feti_s::load_clusters() {
         int mx = -1;
         for (int s : L)
             for (int r : sub[s].b0row) mx = std::max(mx, r);
-        n0[c] = mx + 1;
-        rowoff[c + 1] = rowoff[c] + n0[c];
-        for (int s : L) kt[c] += sub[s].kdim;
+        n0_h[c] = mx + 1;
+        rowoff[c + 1] = rowoff[c] + n0_h[c];
+        for (int s : L) kt_h[c] += sub[s].kdim;
         eoff[c] = asubv[L[0]];
+        sfirst[c] = L[0];
+        scount[c] = (int)L.size();
     }
     std::vector<int64_t> rb1(rowoff[ncl], -1), rb2(rowoff[ncl], -1);
     nq_h.assign(nsub, 0);
@@ -1075,19 +1047,20 @@ void espreso_feti_s::load_clusters() {
     qrow_h.clear();
     qj_h.clear();
     qsign_h.clear();
-    int max_nq = 0;
     for (int s = 0; s < nsub; s++) {
         const SubHost &h = sub[s];
         const int c = h.cluster;
         nq_h[s] = (int)h.b0row.size();
         ESP_CHECK(nq_h[s] <= 256, ESPRESO_ERR_ARG, "HYBRID_FETI: more than 256 B0 entries in one subdomain");
-        max_nq = std::max(max_nq, nq_h[s]);
         qoff_h[s + 1] = qoff_h[s] + nq_h[s];
         fb0off[s + 1] = fb0off[s] + (int64_t)h.m * nq_h[s];
         fccoff_h[s + 1] = fccoff_h[s] + (int64_t)nq_h[s] * nq_h[s];
+        std::vector<char> seen(n0_h[c], 0);
         for (int q = 0; q < nq_h[s]; q++) {
             const int r = h.b0row[q];
-            ESP_CHECK(r >= 0 && r < n0[c], ESPRESO_ERR_ARG, "B0 row out of range");
+            ESP_CHECK(r >= 0 && r < n0_h[c], ESPRESO_ERR_ARG, "B0 row out of range");
+            ESP_CHECK(!seen[r], ESPRESO_ERR_ARG, "a B0 row may touch a subdomain only once");
+            seen[r] = 1;
             const int j = h.iperm[h.b0dof[q]] - h.ni;
             const int64_t row = rowoff[c] + r;
             qrow_h.push_back((int)row);
@@ -1104,181 +1077,126 @@ void espreso_feti_s::load_clusters() {
     }
     for (int64_t r = 0; r < rowoff[ncl]; r++)
         ESP_CHECK(rb1[r] >= 0 && rb2[r] >= 0, ESPRESO_ERR_ARG, "each B0 row needs one +1 and one -1 entry");
+
+    // G0^T (n0 x kt per cluster) and the cluster kernel
+    std::vector<int64_t> g0off(ncl + 1, 0), ncoff(ncl + 1, 0);
+    f0off_h.assign(ncl + 1, 0);
+    saoff_h.assign(ncl + 1, 0);
+    for (int c = 0; c < ncl; c++) {
+        g0off[c + 1] = g0off[c] + (int64_t)n0_h[c] * kt_h[c];
+        f0off_h[c + 1] = f0off_h[c] + (int64_t)n0_h[c] * n0_h[c];
+        saoff_h[c + 1] = saoff_h[c] + (int64_t)kt_h[c] * kt_h[c];
+    }
+    std::vector<double> G0T(std::max<int64_t>(1, g0off[ncl]), 0.0);
+    std::vector<std::vector<double>> Nc(ncl);
+    std::string err;
+#pragma omp parallel for schedule(dynamic)
+    for (int c = 0; c < ncl; c++) {
+        const int n = n0_h[c], k = kt_h[c];
+        double *G = G0T.data() + g0off[c];
+        int aoff = 0;
+        for (int s : cl_subs[c]) {
+            const SubHost &h = sub[s];
+            for (int64_t q = qoff_h[s]; q < qoff_h[s + 1]; q++) {
+                const int r = qrow_h[q] - (int)rowoff[c];
+                for (int a = 0; a < h.kdim; a++)
+                    G[r + (int64_t)(aoff + a) * n] -= qsign_h[q] * Rb_h[rboff_h[s] + qj_h[q] + (int64_t)a * h.m];
+            }
+            aoff += h.kdim;
+        }
+        std::vector<double> GGc((size_t)k * k, 0.0), w, V;
+        for (int a = 0; a < k; a++)
+            for (int b = 0; b < k; b++) {
+                double v = 0;
+                for (int r = 0; r < n; r++) v += G[r + (int64_t)a * n] * G[r + (int64_t)b * n];
+                GGc[a + (size_t)b * k] = v;
+            }
+        jacobi_eigen(k, GGc, w, V);
+        double wmax = 0;
+        for (double x : w) wmax = std::max(wmax, std::fabs(x));
+        int kk = 0;
+        for (int i = 0; i < k; i++)
+            if (wmax == 0.0 || std::fabs(w[i]) <= 1e-10 * wmax) {
+                Nc[c].insert(Nc[c].end(), V.begin() + (size_t)i * k, V.begin() + (size_t)(i + 1) * k);
+                kk++;
+            }
+        kn[c] = kk;
+        if (kk == 0 && k > 0) {
+#pragma omp critical
+            err = "HYBRID_FETI: cluster " + std::to_string(c) + " has no rigid modes left (over-constrained B0)";
+        }
+    }
+    ESP_CHECK(err.empty(), ESPRESO_ERR_NUMERIC, err);
+    for (int c = 0; c < ncl; c++) ncoff[c + 1] = ncoff[c] + (int64_t)kt_h[c] * kn[c];
+    std::vector<double> Nall(std::max<int64_t>(1, ncoff[ncl]));
+    for (int c = 0; c < ncl; c++) std::copy(Nc[c].begin(), Nc[c].end(), Nall.begin() + ncoff[c]);
+
     CL.ncl = ncl;
-    CL.max_n0 = *std::max_element(n0.begin(), n0.end());
-    CL.max_kt = *std::max_element(kt.begin(), kt.end());
-    CL.n0 = upload(n0);
-    CL.kt = upload(kt);
+    CL.max_n0 = *std::max_element(n0_h.begin(), n0_h.end());
+    ESP_CHECK(CL.max_n0 <= 8192, ESPRESO_ERR_ARG, "HYBRID_FETI: more than 8192 B0 rows in one cluster");
+    CL.max_kt = *std::max_element(kt_h.begin(), kt_h.end());
+    CL.max_kn = *std::max_element(kn.begin(), kn.end());
+    CL.n0 = upload(n0_h);
+    CL.kt = upload(kt_h);
+    CL.kn = upload(kn);
+    CL.sfirst = upload(sfirst);
+    CL.scount = upload(scount);
     CL.rowoff = upload(rowoff);
     CL.eoff = upload(eoff);
+    CL.f0off = upload(f0off_h);
+    CL.g0off = upload(g0off);
+    CL.saoff = upload(saoff_h);
+    CL.ncoff = upload(ncoff);
     CL.row_b1 = upload(rb1);
     CL.row_b2 = upload(rb2);
+    CL.G0T = upload(G0T);
+    CL.H = alloc<double>(std::max<int64_t>(1, g0off[ncl]));
+    CL.Nc = upload(Nall);
+    CL.F0inv = alloc<double>(std::max<int64_t>(1, f0off_h[ncl]));
+    CL.Sainv = alloc<double>(std::max<int64_t>(1, saoff_h[ncl]));
     CL.nq = upload(nq_h);
     CL.qoff = upload(qoff_h);
     CL.qrow = upload(qrow_h);
     CL.fb0off = upload(fb0off);
+    CL.fccoff = upload(fccoff_h);
     CL.asub = D.asub;
     CL.FB0 = alloc<double>(fb0off[nsub]);
+    CL.Fcc = alloc<double>(std::max<int64_t>(1, fccoff_h[nsub]));
     CL.lambda0 = alloc<double>(std::max<int64_t>(1, rowoff[ncl]));
     CL.alphaK = alloc<double>(asubv[nsub]);
     d_qjint = upload(qj_h);
     d_qsign = upload(qsign_h);
-    d_Fcc = alloc<double>(std::max<int64_t>(1, fccoff_h[nsub]));
-    (void)max_nq;
-}
+    d_Fcc = CL.Fcc;
 
-// Hybrid FETI cluster problems (cluster.cpp:1226-2016): F0 = B0 K^+ B0^T from
-// the corner blocks, G0 = -(B0 R)^T, Sa = G0 F0^-1 G0^T regularised on its
-// kernel N = ker G0^T (the cluster rigid modes); the coarse basis of the
-// cluster is R_s N_s on each of its subdomains.
-void espreso_feti_s::cluster_setup() {
-    std::vector<double> Fcc(std::max<int64_t>(1, fccoff_h[nsub]));
-    ESP_HIP(hipMemcpyAsync(Fcc.data(), d_Fcc, Fcc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-    sync();
-    std::vector<int> n0(ncl), kt(ncl), kN(ncl);
-    std::vector<int64_t> rowoff(ncl + 1, 0);
-    std::vector<std::vector<double>> F0i(ncl), G0(ncl), Sai(ncl), Nc(ncl);
-    for (int c = 0; c < ncl; c++) {
-        int mx = -1, k = 0;
-        for (int s : cl_subs[c]) {
-            for (int r : sub[s].b0row) mx = std::max(mx, r);
-            k += sub[s].kdim;
-        }
-        n0[c] = mx + 1;
-        kt[c] = k;
-        rowoff[c + 1] = rowoff[c] + n0[c];
-    }
-    std::string err;
-#pragma omp parallel for schedule(dynamic)
-    for (int c = 0; c < ncl; c++) {
-        const int n = n0[c], k = kt[c];
-        std::vector<double> F0((size_t)n * n, 0.0), G((size_t)k * n, 0.0);
-        int aoff = 0;
-        for (int s : cl_subs[c]) {
-            const SubHost &h = sub[s];
-            const int nq = nq_h[s];
-            const int64_t q0 = qoff_h[s];
-            for (int q1 = 0; q1 < nq; q1++)
-                for (int q2 = 0; q2 < nq; q2++)
-                    F0[(qrow_h[q0 + q1] - rowoff[c]) + (size_t)(qrow_h[q0 + q2] - rowoff[c]) * n] +=
-                        Fcc[fccoff_h[s] + q1 + (int64_t)q2 * nq];
-            for (int q = 0; q < nq; q++) {
-                const int r = qrow_h[q0 + q] - (int)rowoff[c];
-                const int j = qj_h[q0 + q];
-                for (int a = 0; a < h.kdim; a++)
-                    G[(aoff + a) + (size_t)r * k] -= qsign_h[q0 + q] * Rb_h[rboff_h[s] + j + (int64_t)a * h.m];
-            }
-            aoff += h.kdim;
-        }
-        if (n > 0 && !spd_inverse(n, F0)) {
-#pragma omp critical
-            err = "HYBRID_FETI: F0 of cluster " + std::to_string(c) + " is not positive definite";
-            continue;
-        }
-        // Sa = G F0^-1 G^T
-        std::vector<double> GF((size_t)k * n, 0.0), Sa((size_t)k * k, 0.0), GGc((size_t)k * k, 0.0);
-        for (int a = 0; a < k; a++)
-            for (int r = 0; r < n; r++) {
-                double v = 0;
-                for (int q = 0; q < n; q++) v += G[a + (size_t)q * k] * F0[q + (size_t)r * n];
-                GF[a + (size_t)r * k] = v;
-            }
-        for (int a = 0; a < k; a++)
-            for (int b = 0; b < k; b++) {
-                double v = 0, w = 0;
-                for (int r = 0; r < n; r++) {
-                    v += GF[a + (size_t)r * k] * G[b + (size_t)r * k];
-                    w += G[a + (size_t)r * k] * G[b + (size_t)r * k];
-                }
-                Sa[a + (size_t)b * k] = v;
-                GGc[a + (size_t)b * k] = w;
-            }
-        // kernel of G^T: eigenvectors of G G^T with (numerically) zero eigenvalue
-        std::vector<double> w, V;
-        jacobi_eigen(k, GGc, w, V);
-        double wmax = 0;
-        for (double x : w) wmax = std::max(wmax, std::fabs(x));
-        std::vector<double> N;
-        int kn = 0;
-        for (int i = 0; i < k; i++)
-            if (std::fabs(w[i]) <= 1e-10 * wmax || wmax == 0.0) {
-                N.insert(N.end(), V.begin() + (size_t)i * k, V.begin() + (size_t)(i + 1) * k);
-                kn++;
-            }
-        if (kn == 0) {
-#pragma omp critical
-            err = "HYBRID_FETI: cluster " + std::to_string(c) + " has no rigid modes left (over-constrained B0)";
-            continue;
-        }
-        double rho = 0;
-        for (int a = 0; a < k; a++) rho = std::max(rho, Sa[a + (size_t)a * k]);
-        if (!(rho > 0)) rho = 1.0;
-        for (int a = 0; a < k; a++)
-            for (int b = 0; b < k; b++) {
-                double v = 0;
-                for (int i = 0; i < kn; i++) v += N[a + (size_t)i * k] * N[b + (size_t)i * k];
-                Sa[a + (size_t)b * k] += rho * v;
-            }
-        if (!spd_inverse(k, Sa)) {
-#pragma omp critical
-            err = "HYBRID_FETI: regularised Sa of cluster " + std::to_string(c) + " is not positive definite";
-            continue;
-        }
-        F0i[c] = std::move(F0);
-        G0[c] = std::move(G);
-        Sai[c] = std::move(Sa);
-        Nc[c] = std::move(N);
-        kN[c] = kn;
-    }
-    ESP_CHECK(err.empty(), ESPRESO_ERR_NUMERIC, err);
-    // device cluster matrices
-    std::vector<int64_t> f0off(ncl + 1, 0), g0off(ncl + 1, 0), saoff(ncl + 1, 0);
-    for (int c = 0; c < ncl; c++) {
-        f0off[c + 1] = f0off[c] + (int64_t)n0[c] * n0[c];
-        g0off[c + 1] = g0off[c] + (int64_t)kt[c] * n0[c];
-        saoff[c + 1] = saoff[c] + (int64_t)kt[c] * kt[c];
-    }
-    std::vector<double> F0all(std::max<int64_t>(1, f0off[ncl])), G0all(std::max<int64_t>(1, g0off[ncl])),
-        Saall(saoff[ncl]);
-    for (int c = 0; c < ncl; c++) {
-        std::copy(F0i[c].begin(), F0i[c].end(), F0all.begin() + f0off[c]);
-        std::copy(G0[c].begin(), G0[c].end(), G0all.begin() + g0off[c]);
-        std::copy(Sai[c].begin(), Sai[c].end(), Saall.begin() + saoff[c]);
-    }
-    CL.F0inv = replace_upload(CL.F0inv, F0all);
-    CL.G0 = replace_upload(CL.G0, G0all);
-    CL.Sainv = replace_upload(CL.Sainv, Saall);
-    CL.f0off = replace_upload(CL.f0off, f0off);
-    CL.g0off = replace_upload(CL.g0off, g0off);
-    CL.saoff = replace_upload(CL.saoff, saoff);
     // coarse space: one block per cluster, basis Rt_s = R_b,s N_s
     std::vector<int> cblk(nsub), ck(nsub), coff(ncl), bptr(ncl + 1, 0), bsub;
     std::vector<int64_t> rtoff(nsub + 1, 0), noff(nsub + 1, 0), dsub_off(nsub + 1, 0);
     ncoarse = 0;
     for (int c = 0; c < ncl; c++) {
         coff[c] = ncoarse;
-        ncoarse += kN[c];
+        ncoarse += kn[c];
         for (int s : cl_subs[c]) bsub.push_back(s);
         bptr[c + 1] = (int)bsub.size();
     }
     for (int s = 0; s < nsub; s++) {
         const int c = sub[s].cluster;
         cblk[s] = c;
-        ck[s] = kN[c];
-        rtoff[s + 1] = rtoff[s] + (int64_t)sub[s].m * kN[c];
-        noff[s + 1] = noff[s] + (int64_t)sub[s].kdim * kN[c];
-        dsub_off[s + 1] = dsub_off[s] + kN[c];
+        ck[s] = kn[c];
+        rtoff[s + 1] = rtoff[s] + (int64_t)sub[s].m * kn[c];
+        noff[s + 1] = noff[s] + (int64_t)sub[s].kdim * kn[c];
+        dsub_off[s + 1] = dsub_off[s] + kn[c];
     }
-    std::vector<double> Rt(rtoff[nsub], 0.0), Nall(noff[nsub], 0.0), e(ncoarse, 0.0);
+    std::vector<double> Rt(std::max<int64_t>(1, rtoff[nsub]), 0.0), Ns(std::max<int64_t>(1, noff[nsub]), 0.0),
+        e(std::max(ncoarse, 1), 0.0);
     for (int c = 0; c < ncl; c++) {
-        const int k = kt[c];
+        const int k = kt_h[c];
         int aoff = 0;
         for (int s : cl_subs[c]) {
             const SubHost &h = sub[s];
-            for (int b = 0; b < kN[c]; b++) {
+            for (int b = 0; b < kn[c]; b++) {
                 for (int a = 0; a < h.kdim; a++) {
                     const double nab = Nc[c][(aoff + a) + (size_t)b * k];
-                    Nall[noff[s] + a + (int64_t)b * h.kdim] = nab;
+                    Ns[noff[s] + a + (int64_t)b * h.kdim] = nab;
                     e[coff[c] + b] += nab * e0f_h[asubv[s] + a];
                     for (int j = 0; j < h.m; j++)
                         Rt[rtoff[s] + j + (int64_t)b * h.m] += Rb_h[rboff_h[s] + j + (int64_t)a * h.m] * nab;
@@ -1287,30 +1205,94 @@ void espreso_feti_s::cluster_setup() {
             aoff += h.kdim;
         }
     }
+    ESP_CHECK(ncoarse > 0, ESPRESO_ERR_NUMERIC, "HYBRID_FETI: empty coarse space");
     D.nblk = ncl;
     D.ncoarse = ncoarse;
-    D.cblk = replace_upload(D.cblk, cblk);
-    D.ck = replace_upload(D.ck, ck);
-    D.coff = replace_upload(D.coff, coff);
-    D.blk_ptr = replace_upload(D.blk_ptr, bptr);
-    D.blk_sub = replace_upload(D.blk_sub, bsub);
-    D.rtoff = replace_upload(D.rtoff, rtoff);
-    D.Rt = replace_upload(D.Rt, Rt);
-    D.N = replace_upload(D.N, Nall);
-    D.noff = replace_upload(D.noff, noff);
-    D.dsub_off = replace_upload(D.dsub_off, dsub_off);
-    if (D.dsub) release(D.dsub);
+    D.cblk = upload(cblk);
+    D.ck = upload(ck);
+    D.coff = upload(coff);
+    D.blk_ptr = upload(bptr);
+    D.blk_sub = upload(bsub);
+    D.rtoff = upload(rtoff);
+    D.Rt = upload(Rt);
+    D.N = upload(Ns);
+    D.noff = upload(noff);
+    D.dsub_off = upload(dsub_off);
     D.dsub = alloc<double>(std::max<int64_t>(1, dsub_off[nsub]));
-    if (D.cd) release(D.cd);
-    if (D.ce) release(D.ce);
-    if (d_alpha_c) release(d_alpha_c);
     D.cd = alloc<double>(ncoarse);
     D.ce = alloc<double>(ncoarse);
     d_alpha_c = alloc<double>(ncoarse);
     std::vector<double> GGt;
     build_ggt(Rt, rtoff, ck, cblk, coff, GGt);
-    d_GGt = replace_upload(d_GGt, GGt);
-    d_e = replace_upload(d_e, e);
+    d_GGt = upload(GGt);
+    d_e = upload(e);
+}
+
+// In-place inverses of nb symmetric positive definite matrices pool + off[i]
+// (sizes n[i], full storage) with the MFMA Cholesky / forward-substitution /
+// W^T W kernels; sizes 0 are skipped.
+void espreso_feti_s::invert_spd_batched(double *pool, const std::vector<int64_t> &off, const std::vector<int> &n,
+                                        const char *what) {
+    std::vector<int> idx;
+    int64_t wsz = 0, dsz = 0;
+    int max_n = 0, max_nt = 0;
+    for (size_t i = 0; i < n.size(); i++)
+        if (n[i] > 0) {
+            idx.push_back((int)i);
+            const int nt = (n[i] + TILE - 1) / TILE;
+            wsz += (int64_t)n[i] * n[i];
+            dsz += (int64_t)nt * TILE * TILE;
+            max_n = std::max(max_n, n[i]);
+            max_nt = std::max(max_nt, nt);
+        }
+    if (idx.empty()) return;
+    const int nb = (int)idx.size();
+    double *W = alloc<double>(wsz), *Dv = alloc<double>(dsz);
+    int *info = alloc<int>(nb);
+    ESP_HIP(hipMemsetAsync(info, 0, nb * sizeof(int), st));
+    std::vector<DenseDesc> desc(nb);
+    int64_t wo = 0, dofs = 0;
+    for (int b = 0; b < nb; b++) {
+        const int i = idx[b];
+        DenseDesc &d = desc[b];
+        d = DenseDesc{};
+        d.S = pool + off[i];
+        d.W = W + wo;
+        d.Dinv = Dv + dofs;
+        d.info = info + b;
+        d.ni = 0;
+        d.m = n[i];
+        d.ntm = d.ntc = (n[i] + TILE - 1) / TILE;
+        wo += (int64_t)n[i] * n[i];
+        dofs += (int64_t)d.ntm * TILE * TILE;
+    }
+    DenseDesc *dd = upload(desc);
+    for (int k = 0; k < max_nt; k++) launch_potrf_step(dd, nb, 1, k, max_nt, st);
+    launch_identity(dd, nb, max_n, st);
+    for (int k = 0; k < max_nt; k++) launch_trsm_step(dd, nb, 1, k, max_nt, max_nt, st);
+    launch_syrk(dd, nb, 1, max_nt, st);
+    launch_symmetrize_desc(dd, nb, max_n, st);
+    std::vector<int> hinfo(nb);
+    ESP_HIP(hipMemcpyAsync(hinfo.data(), info, nb * sizeof(int), hipMemcpyDeviceToHost, st));
+    sync();
+    for (int b = 0; b < nb; b++)
+        ESP_CHECK(hinfo[b] == 0, ESPRESO_ERR_NUMERIC,
+                  std::string("HYBRID_FETI: ") + what + " of cluster " + std::to_string(idx[b]) +
+                      " is not positive definite");
+    release(W);
+    release(Dv);
+    release(info);
+    release(dd);
+}
+
+// Hybrid FETI, device part (every setup; cluster.cpp:1293-2016): F0 from the
+// corner blocks of the assembly, F0^-1, H = F0^-1 G0^T, Sa = G0 H regularised
+// on the cluster kernel, Sa^-1.
+void espreso_feti_s::cluster_setup() {
+    launch_cluster_setup_f0(CL, st);
+    invert_spd_batched(CL.F0inv, f0off_h, n0_h, "F0");
+    launch_cluster_setup_sa(CL, CL.Sainv, st);
+    invert_spd_batched(CL.Sainv, saoff_h, kt_h, "regularised Sa");
 }
 
 // ================================================================ C ABI ====

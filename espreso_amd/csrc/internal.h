@@ -77,6 +77,7 @@ void launch_trsm_step(const DenseDesc *desc, int nb, int which, int step, int ma
                       hipStream_t st);
 void launch_syrk(const DenseDesc *desc, int nb, int which, int max_ntm, hipStream_t st);
 void launch_identity(const DenseDesc *desc, int nb, int max_m, hipStream_t st);
+void launch_symmetrize_desc(const DenseDesc *desc, int nb, int max_m, hipStream_t st);
 // S += rho Q Q^T with rho = max diag S; Q (m x k, column-major) at q[q_off[b]]
 void launch_kernel_reg(const DenseDesc *desc, int nb, const double *q, const int64_t *q_off,
                        const int *kdim, double *rho, int max_m, hipStream_t st);
