@@ -65,8 +65,8 @@ __global__ void __launch_bounds__(256) k_symv(const SymvItem *__restrict__ items
     const SymvItem it = items[blockIdx.x];
     const int s = it.sub, m = M[s];
     const int j0 = it.t0 * R, len = m - j0;
-    A *xs = (A *)smem;
-    A *ys = xs + len;
+    A *xs = (A *)smem;   // len + R entries: x of the columns, zero padding past m
+    A *ys = xs + len + R;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t b0 = boff[s] + j0;
     for (int j = tid; j < len; j += 256) {
@@ -80,20 +80,25 @@ __global__ void __launch_bounds__(256) k_symv(const SymvItem *__restrict__ items
         xs[j] = (A)v;
         ys[j] = (A)0;
     }
+    if (tid < R) xs[len + tid] = (A)0;
     __syncthreads();
     const T *base = pool + poff[s];
     for (int t = it.t0 + wave; t < it.t1; t += 4) {
         const int r0 = t * R;
         const int ncol = m - r0;
         const T *S = base + strip_offset(m, R, t);
-        A xr[R], racc[R];
+        // FP64 keeps the strip-row x in registers; the low-precision strips
+        // (32/64 rows) read it from LDS to stay below 168 VGPRs
+        constexpr bool XREG = sizeof(T) == 8;
+        A racc[R], xr[XREG ? R : 1];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            xr[r] = (r0 + r < m) ? xs[r0 + r - j0] : (A)0;
             racc[r] = (A)0;
+            if constexpr (XREG) xr[r] = xs[r0 + r - j0];
         }
         constexpr int VEC = 16 / (int)sizeof(T);
         constexpr int NV = R / VEC;
+        const A *xrow = xs + (r0 - j0);  // x of the strip rows (LDS broadcast reads)
         for (int c0 = 0; c0 < ncol; c0 += 64) {
             const int w = min(64, ncol - c0);
             if (lane < w) {
@@ -105,14 +110,16 @@ __global__ void __launch_bounds__(256) k_symv(const SymvItem *__restrict__ items
                 const T *a = (const T *)raw;
                 const int c = c0 + lane;
                 const A xc = xs[r0 + c - j0];
-                A col = (A)0;
+                // four independent chains for the column sum (latency of the FMA chain)
+                A col[4] = {(A)0, (A)0, (A)0, (A)0};
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     const A av = load_cvt<T, A>(a[r]);
-                    col = fma(av, xr[r], col);
+                    if constexpr (XREG) col[r & 3] = fma(av, xr[r], col[r & 3]);
+                    else col[r & 3] = fma(av, xrow[r], col[r & 3]);
                     racc[r] = fma(av, xc, racc[r]);
                 }
-                atomicAdd(&ys[r0 + c - j0], col);
+                atomicAdd(&ys[r0 + c - j0], (col[0] + col[1]) + (col[2] + col[3]));
             }
         }
         const A rs = wave_rows_reduce<R, A>(racc, lane);
@@ -190,36 +197,50 @@ __global__ void __launch_bounds__(256) k_apply_G(const int *M, const int64_t *bo
                                                  const int64_t *outoff, const int64_t *roff, const double *R,
                                                  const int64_t *bl_ptr, const int *bl_lam, const double *sign,
                                                  const double *x, double *out) {
-    __shared__ double red[4];
-    extern __shared__ double xbs[];
+    constexpr int KB = 8;  // columns per pass
+    __shared__ double red[KB][4];
     const int s = blockIdx.x, m = M[s], k = kcols[s];
     const int64_t b0 = boff[s];
-    for (int j = threadIdx.x; j < m; j += 256) {
-        double v = 0.0;
-        for (int64_t q = bl_ptr[b0 + j]; q < bl_ptr[b0 + j + 1]; q++) v += sign[q] * x[bl_lam[q]];
-        xbs[j] = v;
-    }
-    __syncthreads();
     const double *Rs = R + roff[s];
-    for (int a = 0; a < k; a++) {
-        double v = 0.0;
-        for (int j = threadIdx.x; j < m; j += 256) v += Rs[j + (int64_t)a * m] * xbs[j];
-        v = block_sum(v, red);
-        if (threadIdx.x == 0) out[outoff[s] + a] = -v;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int a0 = 0; a0 < k; a0 += KB) {
+        const int kb = min(KB, k - a0);
+        double acc[KB];
+#pragma unroll
+        for (int a = 0; a < KB; a++) acc[a] = 0.0;
+        for (int j = threadIdx.x; j < m; j += 256) {
+            double xb = 0.0;
+            for (int64_t q = bl_ptr[b0 + j]; q < bl_ptr[b0 + j + 1]; q++) xb += sign[q] * x[bl_lam[q]];
+#pragma unroll
+            for (int a = 0; a < KB; a++)
+                if (a < kb) acc[a] += Rs[j + (int64_t)(a0 + a) * m] * xb;
+        }
+#pragma unroll
+        for (int a = 0; a < KB; a++) {
+            double v = acc[a];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0) red[a][w] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < kb)
+            out[outoff[s] + a0 + threadIdx.x] =
+                -((red[threadIdx.x][0] + red[threadIdx.x][1]) + (red[threadIdx.x][2] + red[threadIdx.x][3]));
+        __syncthreads();
     }
 }
 
 // d[coff[b] + a] = sum over the subdomains of coarse block b of their partials
 __global__ void k_block_sum(int nblk, const int *blk_ptr, const int *blk_sub, const int64_t *subpart_off,
                             const int *ck, const int *coff, const double *part, double *d) {
-    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nblk; b += gridDim.x * blockDim.x) {
-        const int s0 = blk_sub[blk_ptr[b]];
-        const int k = ck[s0];
-        for (int a = 0; a < k; a++) {
-            double v = 0.0;
-            for (int q = blk_ptr[b]; q < blk_ptr[b + 1]; q++) v += part[subpart_off[blk_sub[q]] + a];
-            d[coff[b] + a] = v;
-        }
+    // one thread per (block, column); blocks hold at most 64 columns
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nblk * 64; e += gridDim.x * blockDim.x) {
+        const int b = e >> 6, a = e & 63;
+        const int q0 = blk_ptr[b], q1 = blk_ptr[b + 1];
+        const int k = ck[blk_sub[q0]];
+        if (a >= k) continue;
+        double v = 0.0;
+        for (int q = q0; q < q1; q++) v += part[subpart_off[blk_sub[q]] + a];
+        d[coff[b] + a] = v;
     }
 }
 
@@ -264,8 +285,9 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// t = F0^-1 g0: 32 rows per workgroup (8 per wave), each row one wave-wide dot
-// over a column of the symmetric F0^-1; g0 is gathered from the SYMV output.
+// t = F0^-1 g0: 32 rows per workgroup (8 per wave, accumulated together for
+// memory-level parallelism), each row a dot over a column of the symmetric
+// F0^-1; g0 is gathered from the SYMV output.
 __global__ void __launch_bounds__(256) k_cluster_t(const ClusterDev C, const double *yb, double *t) {
     extern __shared__ double g0[];
     const int c = blockIdx.y;
@@ -277,20 +299,26 @@ __global__ void __launch_bounds__(256) k_cluster_t(const ClusterDev C, const dou
     __syncthreads();
     const double *F = C.F0inv + C.f0off[c];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int rr = w; rr < 32; rr += 4) {
-        const int r = rb + rr;
-        if (r >= n0) break;  // uniform over the wave
-        const double *col = F + (int64_t)r * n0;
-        double v = 0.0;
-        for (int q = lane; q < n0; q += 64) v += col[q] * g0[q];
-        v = wave_sum(v);
-        if (lane == 0) t[ro + r] = v;
+    const int r0 = rb + w * 8;
+    double v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = 0.0;
+    for (int q = lane; q < n0; q += 64) {
+        const double gq = g0[q];
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            if (r0 + i < n0) v[i] += F[q + (int64_t)(r0 + i) * n0] * gq;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const double sv = wave_sum(v[i]);
+        if (lane == 0 && r0 + i < n0) t[ro + r0 + i] = sv;
     }
 }
 
 // one workgroup per cluster: alpha = Sa^-1 (G0 t - e0), lambda0 = t - H alpha (in place over t)
-__global__ void __launch_bounds__(256) k_cluster_alpha(const ClusterDev C, const double *e0s, double *lam0,
-                                                       double *alphaK) {
+__global__ void __launch_bounds__(1024) k_cluster_alpha(const ClusterDev C, const double *e0s, double *lam0,
+                                                        double *alphaK) {
     extern __shared__ double sh[];
     const int c = blockIdx.x;
     const int n0 = C.n0[c], kt = C.kt[c];
@@ -300,7 +328,7 @@ __global__ void __launch_bounds__(256) k_cluster_alpha(const ClusterDev C, const
     const double *G0T = C.G0T + C.g0off[c];  // n0 x kt
     const double *e0 = e0s + C.eoff[c];      // cluster-contiguous subdomain partials
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int a = w; a < kt; a += 4) {
+    for (int a = w; a < kt; a += 16) {
         double v = 0.0;
         for (int r = lane; r < n0; r += 64) v += G0T[r + (int64_t)a * n0] * t[r];
         v = wave_sum(v);
@@ -308,7 +336,7 @@ __global__ void __launch_bounds__(256) k_cluster_alpha(const ClusterDev C, const
     }
     __syncthreads();
     const double *Sai = C.Sainv + C.saoff[c];
-    for (int a = threadIdx.x; a < kt; a += 256) {
+    for (int a = threadIdx.x; a < kt; a += 1024) {
         double v = 0.0;
         for (int b = 0; b < kt; b++) v += Sai[a + (int64_t)b * kt] * t2[b];
         al[a] = v;
@@ -316,7 +344,7 @@ __global__ void __launch_bounds__(256) k_cluster_alpha(const ClusterDev C, const
     }
     __syncthreads();
     const double *H = C.H + C.g0off[c];  // n0 x kt
-    for (int r = threadIdx.x; r < n0; r += 256) {
+    for (int r = threadIdx.x; r < n0; r += 1024) {
         double v = t[r];
         for (int a = 0; a < kt; a++) v -= H[r + (int64_t)a * n0] * al[a];
         lam0[ro + r] = v;
@@ -401,28 +429,37 @@ __global__ void __launch_bounds__(256) k_cluster_sa(const ClusterDev C, double *
     }
 }
 
-// per subdomain: y_b += -FB0_s lambda0 + addR * R_b alphaK   (one workgroup per subdomain)
+// y_b += -FB0_s lambda0 + addR * R_b alphaK; grid (row chunks of 256, subdomains)
 __global__ void __launch_bounds__(256) k_cluster_apply(const ClusterDev C, const int *M, const int64_t *boff,
                                                        const int *kdim, const int64_t *rboff, const double *Rb,
                                                        const double *lambda0, const double *alphaK, int addR,
                                                        double *yb) {
     __shared__ double lam[256];
-    const int s = blockIdx.x, m = M[s];
+    const int s = blockIdx.y, m = M[s];
+    if (blockIdx.x * 256 >= m) return;  // uniform
     const int nq = C.nq[s];
     const int64_t q0 = C.qoff[s];
     for (int q = threadIdx.x; q < nq; q += 256) lam[q] = lambda0[C.qrow[q0 + q]];
     __syncthreads();
-    const double *FB = C.FB0 + C.fb0off[s];
-    const double *R = Rb + rboff[s];
-    const double *al = alphaK + C.asub[s];
-    const int k = kdim[s];
-    for (int j = threadIdx.x; j < m; j += 256) {
-        double v = 0.0;
-        for (int q = 0; q < nq; q++) v -= FB[j + (int64_t)q * m] * lam[q];
-        if (addR)
-            for (int a = 0; a < k; a++) v += R[j + (int64_t)a * m] * al[a];
-        yb[boff[s] + j] += v;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= m) return;
+    const double *FB = C.FB0 + C.fb0off[s] + j;
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    int q = 0;
+    for (; q + 4 <= nq; q += 4) {
+        v0 -= FB[(int64_t)q * m] * lam[q];
+        v1 -= FB[(int64_t)(q + 1) * m] * lam[q + 1];
+        v2 -= FB[(int64_t)(q + 2) * m] * lam[q + 2];
+        v3 -= FB[(int64_t)(q + 3) * m] * lam[q + 3];
     }
+    for (; q < nq; q++) v0 -= FB[(int64_t)q * m] * lam[q];
+    double v = (v0 + v1) + (v2 + v3);
+    if (addR) {
+        const double *R = Rb + rboff[s] + j;
+        const double *al = alphaK + C.asub[s];
+        for (int a = 0; a < kdim[s]; a++) v += R[(int64_t)a * m] * al[a];
+    }
+    yb[boff[s] + j] += v;
 }
 
 // alpha_tot(s) = addK * alphaK(s) + N_s alpha_coarse(block(s))
@@ -561,7 +598,7 @@ void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gathe
     const SymvItem *items = lp ? D.items_lp : D.items_d;
     const int n_items = lp ? D.n_items_lp : D.n_items_d;
     const double *coef = scaled ? D.bl_sscale : D.bl_sign;
-    const size_t shm = 2 * (size_t)D.max_m * (lp ? 4 : 8);
+    const size_t shm = (2 * (size_t)D.max_m + strip_rows(prec)) * (lp ? 4 : 8);
     if (ev0) (void)hipEventRecord(ev0, st);
     if (prec == 0) {
         const double *pool = sd ? D.SD : D.Fd;
@@ -598,10 +635,10 @@ void launch_kbb(const DualDev &D, const double *xb, double *yb, hipStream_t st) 
 void launch_project(const DualDev &D, const double *x, double *y, int mode, hipStream_t st) {
     // mode 0: y = P x;  mode 1: y = G^T GGinv x (x coarse);  mode 2: y = GGinv G x (coarse out)
     if (mode != 1) {
-        hipLaunchKernelGGL(k_apply_G, dim3(D.nsub), dim3(256), (size_t)D.max_m * 8, st, D.m, D.boff, D.ck, D.dsub_off,
-                           D.rtoff, D.Rt, D.bl_ptr, D.bl_lam, D.bl_sign, x, D.dsub);
-        hipLaunchKernelGGL(k_block_sum, dim3(vgrid(D.nblk)), dim3(256), 0, st, D.nblk, D.blk_ptr, D.blk_sub,
-                           D.dsub_off, D.ck, D.coff, D.dsub, D.cd);
+        hipLaunchKernelGGL(k_apply_G, dim3(D.nsub), dim3(256), 0, st, D.m, D.boff, D.ck, D.dsub_off, D.rtoff, D.Rt,
+                           D.bl_ptr, D.bl_lam, D.bl_sign, x, D.dsub);
+        hipLaunchKernelGGL(k_block_sum, dim3(vgrid((int64_t)D.nblk * 64)), dim3(256), 0, st, D.nblk, D.blk_ptr,
+                           D.blk_sub, D.dsub_off, D.ck, D.coff, D.dsub, D.cd);
     }
     const double *din = mode == 1 ? x : D.cd;
     double *eout = mode == 2 ? y : D.ce;
@@ -612,8 +649,8 @@ void launch_project(const DualDev &D, const double *x, double *y, int mode, hipS
 }
 
 void launch_subdomain_kernel_dots(const DualDev &D, const double *x, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_apply_G, dim3(D.nsub), dim3(256), (size_t)D.max_m * 8, st, D.m, D.boff, D.kdim, D.asub,
-                       D.rboff, D.Rb, D.bl_ptr, D.bl_lam, D.bl_sign, x, out);
+    hipLaunchKernelGGL(k_apply_G, dim3(D.nsub), dim3(256), 0, st, D.m, D.boff, D.kdim, D.asub, D.rboff, D.Rb,
+                       D.bl_ptr, D.bl_lam, D.bl_sign, x, out);
 }
 
 void launch_cluster_correction(const DualDev &D, const ClusterDev &C, const double *e0s, double *yb, int addR,
@@ -621,10 +658,10 @@ void launch_cluster_correction(const DualDev &D, const ClusterDev &C, const doub
     if (C.max_n0 > 0)
         hipLaunchKernelGGL(k_cluster_t, dim3((C.max_n0 + 31) / 32, C.ncl), dim3(256), (size_t)C.max_n0 * 8, st, C, yb,
                            C.lambda0);
-    hipLaunchKernelGGL(k_cluster_alpha, dim3(C.ncl), dim3(256), 2 * (size_t)std::max(C.max_kt, 1) * 8, st, C, e0s,
+    hipLaunchKernelGGL(k_cluster_alpha, dim3(C.ncl), dim3(1024), 2 * (size_t)std::max(C.max_kt, 1) * 8, st, C, e0s,
                        C.lambda0, C.alphaK);
-    hipLaunchKernelGGL(k_cluster_apply, dim3(D.nsub), dim3(256), 0, st, C, D.m, D.boff, D.kdim, D.rboff, D.Rb,
-                       C.lambda0, C.alphaK, addR, yb);
+    hipLaunchKernelGGL(k_cluster_apply, dim3((D.max_m + 255) / 256, D.nsub), dim3(256), 0, st, C, D.m, D.boff, D.kdim,
+                       D.rboff, D.Rb, C.lambda0, C.alphaK, addR, yb);
 }
 
 void launch_cluster_setup_f0(const ClusterDev &C, hipStream_t st) {

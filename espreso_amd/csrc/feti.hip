@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -757,7 +758,9 @@ void espreso_feti_s::coarse() {
 void espreso_feti_s::build_items(int prec, std::vector<SymvItem> &items, std::vector<int> &item0,
                                  std::vector<int> &nit, int64_t &plen) {
     const int R = strip_rows(prec);
-    const int64_t target = 160 * 1024;  // matrix elements per workgroup
+    // matrix elements per workgroup (ESPRESO_SYMV_ITEM overrides, for tuning)
+    int64_t target = 320 * 1024;
+    if (const char *e = std::getenv("ESPRESO_SYMV_ITEM")) target = std::max<int64_t>(4096, std::atoll(e));
     items.clear();
     item0.assign(nsub, 0);
     nit.assign(nsub, 0);
@@ -1127,7 +1130,10 @@ void espreso_feti_s::load_clusters() {
         }
     }
     ESP_CHECK(err.empty(), ESPRESO_ERR_NUMERIC, err);
-    for (int c = 0; c < ncl; c++) ncoff[c + 1] = ncoff[c] + (int64_t)kt_h[c] * kn[c];
+    for (int c = 0; c < ncl; c++) {
+        ESP_CHECK(kn[c] <= 64, ESPRESO_ERR_NUMERIC, "HYBRID_FETI: cluster kernel dimension above 64");
+        ncoff[c + 1] = ncoff[c] + (int64_t)kt_h[c] * kn[c];
+    }
     std::vector<double> Nall(std::max<int64_t>(1, ncoff[ncl]));
     for (int c = 0; c < ncl; c++) std::copy(Nc[c].begin(), Nc[c].end(), Nall.begin() + ncoff[c]);
 
