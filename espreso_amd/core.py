@@ -45,6 +45,7 @@ class GridProblem(C.Structure):
         ("traction_scale", C.c_double),
         ("young_seed", C.c_uint64),
         ("conductivity_seed", C.c_uint64),
+        ("young_elem", C.POINTER(C.c_double)),
     ]
 
 
@@ -103,6 +104,13 @@ SIGNATURES = {
                                       _D, _D, _D]),
     "espreso_grid_export_cluster": (C.c_int, [C.POINTER(GridProblem), C.c_int32, _I, _I, _I, _I, _D]),
     "espreso_grid_nodal_solution": (C.c_int, [_H, _D]),
+    "espreso_grid_unit_element": (C.c_int, [C.POINTER(GridProblem), _D]),
+    "espreso_grid_set_nodal_rhs": (C.c_int, [_H, _D]),
+    "espreso_grid_element_sensitivity": (C.c_int, [_H, _D, _D, _D]),
+    "espreso_feti_set_rhs": (C.c_int, [_H, C.c_int32, _D]),
+    "espreso_feti_load_work": (C.c_int, [_H, _D]),
+    "espreso_grid_nodal_load": (C.c_int, [_H, _D]),
+    "espreso_feti_resolve": (C.c_int, [_H, C.POINTER(Stats)]),
     "espreso_feti_create": (C.c_int, [C.POINTER(_H), C.c_int32, C.POINTER(Options)]),
     "espreso_feti_destroy": (C.c_int, [_H]),
     "espreso_feti_sizes": (C.c_int, [_H, _I]),
@@ -206,7 +214,25 @@ def grid_problem(spec):
     p.traction_scale = spec.get("traction_scale", 1.0)
     p.young_seed = spec.get("young_seed", 0)
     p.conductivity_seed = spec.get("conductivity_seed", 0)
+    ye = spec.get("young_elem")
+    if ye is not None:
+        ye = np.ascontiguousarray(ye, dtype=np.float64)
+        p._keep = ye  # the array must outlive the struct
+        p.young_elem = _dp(ye)
     return p
+
+
+def n_elements(spec):
+    return int(np.prod([spec["clusters"][d] * spec["domains"][d] * spec["elements"][d] for d in range(3)]))
+
+
+def unit_element(spec):
+    """Unit-modulus element matrix of the grid builder (closed-form integrals)."""
+    p = grid_problem(spec)
+    nd = 3 if spec.get("physics", 0) == 0 else 1
+    Ke = np.zeros((8 * nd) ** 2)
+    _check(lib().espreso_grid_unit_element(C.byref(p), _dp(Ke)))
+    return Ke.reshape(8 * nd, 8 * nd)
 
 
 METHOD = {"total": 0, "TOTAL_FETI": 0, "hybrid": 1, "HYBRID_FETI": 1}
@@ -327,6 +353,23 @@ class FETISolver:
         self.stats = st.as_dict()
         return self.stats
 
+    def set_rhs(self, i, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        _check(lib().espreso_feti_set_rhs(self.h, i, _dp(f)))
+
+    def load_work(self):
+        """f^T u of the last solution (compliance)."""
+        c = np.zeros(1)
+        _check(lib().espreso_feti_load_work(self.h, _dp(c)))
+        return float(c[0])
+
+    def resolve(self):
+        """Solve with the loads given by set_rhs, reusing the assembled operators."""
+        st = Stats()
+        _check(lib().espreso_feti_resolve(self.h, C.byref(st)))
+        self.stats = st.as_dict()
+        return self.stats
+
     def primal(self, i, n):
         u = np.zeros(n)
         _check(lib().espreso_feti_primal(self.h, i, _dp(u)))
@@ -399,6 +442,25 @@ class GridFETI(FETISolver):
         u = np.zeros(self.n_nodes * self.ndof)
         _check(lib().espreso_grid_nodal_solution(self.h, _dp(u)))
         return u.reshape(self.n_nodes, self.ndof)
+
+    def nodal_load(self):
+        g = np.zeros(self.n_nodes * self.ndof)
+        _check(lib().espreso_grid_nodal_load(self.h, _dp(g)))
+        return g.reshape(self.n_nodes, self.ndof)
+
+    def set_nodal_rhs(self, g):
+        """New nodal load (n_nodes x ndof) for resolve()."""
+        g = np.ascontiguousarray(g, dtype=np.float64).reshape(-1)
+        assert g.size == self.n_nodes * self.ndof
+        _check(lib().espreso_grid_set_nodal_rhs(self.h, _dp(g)))
+
+    def element_sensitivity(self, u, w=None):
+        """-w_e^T K_e0 u_e for every element (GPU); w defaults to u."""
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        w = u if w is None else np.ascontiguousarray(w, dtype=np.float64).reshape(-1)
+        out = np.zeros(n_elements(self.spec))
+        _check(lib().espreso_grid_element_sensitivity(self.h, _dp(u), _dp(w), _dp(out)))
+        return out
 
     def l2g(self, i):
         n = (self.spec["elements"][0] + 1) * (self.spec["elements"][1] + 1) * (self.spec["elements"][2] + 1)

@@ -133,8 +133,9 @@ __global__ void __launch_bounds__(256) k_potrf_diag(const DenseDesc *D, int whic
     }
     __shared__ double T[TILE][TILE + 1];
     __shared__ double xd[TILE];
+    __shared__ double tmp[3][16][17];
     __shared__ int bad;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) bad = 0;
     for (int q = tid; q < TILE * TILE; q += 256) {
         int i = q & 63, j = q >> 6;
@@ -144,35 +145,125 @@ __global__ void __launch_bounds__(256) k_potrf_diag(const DenseDesc *D, int whic
         T[i][j] = v;
     }
     __syncthreads();
-    for (int j = 0; j < TILE; j++) {
-        if (tid == 0) {
-            double dd = T[j][j];
-            if (!(dd > 0.0) || !isfinite(dd)) {
-                if (!bad) bad = j + 1;
-                dd = 1.0;
+    // ---- T = L L^T, right-looking over four 16-column blocks ----
+    for (int kb = 0; kb < 4; kb++) {
+        const int c0 = kb * 16;
+        if (wave == 0) {
+            // diagonal block in registers: lane r < 16 holds row r
+            const int r = lane;
+            double v[16];
+#pragma unroll
+            for (int c = 0; c < 16; c++) v[c] = (r < 16 && c <= r) ? T[c0 + r][c0 + c] : 0.0;
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                double piv = __shfl(v[j], j, 64);
+                if (!(piv > 0.0) || !isfinite(piv)) {
+                    if (lane == 0 && bad == 0) bad = c0 + j + 1;
+                    piv = 1.0;
+                }
+                const double dd = sqrt(piv);
+                if (r == j) v[j] = dd;
+                else if (r > j) v[j] = v[j] / dd;
+#pragma unroll
+                for (int c = j + 1; c < 16; c++) {
+                    const double lc = __shfl(v[j], c, 64);
+                    if (r >= c) v[c] -= v[j] * lc;
+                }
             }
-            T[j][j] = sqrt(dd);
+            if (r < 16) {
+#pragma unroll
+                for (int c = 0; c < 16; c++)
+                    if (c <= r) T[c0 + r][c0 + c] = v[c];
+            }
         }
         __syncthreads();
-        if (tid > j && tid < TILE) T[tid][j] /= T[j][j];
+        const int rest = TILE - c0 - 16;
+        if (rest == 0) break;
+        // panel rows below: L_rb = A_rb L_bb^-T, one thread per row
+        if (tid < rest) {
+            const int r = c0 + 16 + tid;
+            double x[16];
+#pragma unroll
+            for (int c = 0; c < 16; c++) {
+                double sv = T[r][c0 + c];
+#pragma unroll
+                for (int i = 0; i < c; i++) sv -= x[i] * T[c0 + c][c0 + i];
+                x[c] = sv / T[c0 + c][c0 + c];
+            }
+#pragma unroll
+            for (int c = 0; c < 16; c++) T[r][c0 + c] = x[c];
+        }
         __syncthreads();
-        for (int q = tid; q < TILE * TILE; q += 256) {
-            int i = q & 63, c = q >> 6;
-            if (c > j && i >= c) T[i][c] -= T[i][j] * T[c][j];
+        // trailing lower block update (rank 16)
+        for (int q = tid; q < rest * rest; q += 256) {
+            const int rr = c0 + 16 + q % rest, cc = c0 + 16 + q / rest;
+            if (rr < cc) continue;
+            double sv = 0.0;
+#pragma unroll
+            for (int i = 0; i < 16; i++) sv += T[rr][c0 + i] * T[cc][c0 + i];
+            T[rr][cc] -= sv;
         }
         __syncthreads();
     }
-    // X = T^-1 (lower): thread c computes column c; X[i][c] for i > c is kept
-    // in the unused strict upper triangle at T[c][i], X[c][c] in xd[c]
-    if (tid < TILE) {
-        const int c = tid;
-        const double dc = 1.0 / T[c][c];
-        xd[c] = dc;
-        for (int i = c + 1; i < TILE; i++) {
-            double s = T[i][c] * dc;
-            for (int k = c + 1; k < i; k++) s += T[i][k] * T[c][k];
-            T[c][i] = -s / T[i][i];
+    // ---- X = L^-1 (lower): X(i,j), i > j, kept at T[j][i]; X(i,i) in xd[i] ----
+    // diagonal 16x16 blocks: wave w inverts block w, lane c < 16 one column
+    {
+        const int c0 = wave * 16, c = lane;
+        if (c < 16) {
+            double x[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                if (i < c) {
+                    x[i] = 0.0;
+                    continue;
+                }
+                double sv = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+                for (int k = 0; k < i; k++)
+                    if (k >= c) sv -= T[c0 + i][c0 + k] * x[k];
+                x[i] = sv / T[c0 + i][c0 + i];
+            }
+            xd[c0 + c] = x[c];
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                if (i > c) T[c0 + c][c0 + i] = x[i];
         }
+    }
+    __syncthreads();
+    // off-diagonal blocks by distance: X_IJ = -X_II sum_{K=J}^{I-1} L_IK X_KJ
+    for (int dist = 1; dist < 4; dist++) {
+        const int J = wave, I = wave + dist;
+        const bool act = I < 4;
+        const int a = lane & 15;
+        if (act) {
+            for (int t = 0; t < 4; t++) {
+                const int bcol = (lane >> 4) + 4 * t;
+                double sv = 0.0;
+                for (int K = J; K < I; K++)
+                    for (int k = 0; k < 16; k++) {
+                        const int row = K * 16 + k, col = J * 16 + bcol;
+                        double xv;
+                        if (row > col) xv = T[col][row];
+                        else xv = (row == col) ? xd[row] : 0.0;
+                        sv += T[I * 16 + a][K * 16 + k] * xv;
+                    }
+                tmp[wave][a][bcol] = sv;
+            }
+        }
+        __syncthreads();
+        if (act) {
+            for (int t = 0; t < 4; t++) {
+                const int bcol = (lane >> 4) + 4 * t;
+                double sv = 0.0;
+                for (int k = 0; k <= a; k++) {
+                    const int row = I * 16 + a, col = I * 16 + k;
+                    const double xv = (row == col) ? xd[row] : T[col][row];
+                    sv += xv * tmp[wave][k][bcol];
+                }
+                T[J * 16 + bcol][I * 16 + a] = -sv;
+            }
+        }
+        __syncthreads();
     }
     __syncthreads();
     if (tid == 0 && bad && d.info[0] == 0) d.info[0] = k0 + bad;
@@ -221,9 +312,19 @@ __global__ void __launch_bounds__(256) k_trsm_row(const DenseDesc *D, int which,
     const int mi = min(TILE, n - i0), nj = min(TILE, ncols - c0);
     const int j0 = max(cs, i - band) * TILE;
     double *C = B + i0 + (size_t)c0 * n;
+    if (which == 1 && J == i) {
+        // W = L^-1 I: the right-hand side is the identity, so the diagonal tile is inv(L_ii)
+        const double *Di = d.Dinv + (size_t)i * TILE * TILE;
+        for (int q = threadIdx.x; q < TILE * TILE; q += 256) {
+            const int r = q & 63, c = q >> 6;
+            if (r < mi && c < nj) C[r + (size_t)c * n] = Di[q];
+        }
+        return;
+    }
     if (i0 > j0) {
+        // which 1: the identity has no entries off the diagonal tile
         tile_gemm<false, false>(C, n, A + i0 + (size_t)j0 * n, n, B + j0 + (size_t)c0 * n, n, mi, nj, i0 - j0, -1.0,
-                                1.0);
+                                which == 1 ? 0.0 : 1.0);
         __syncthreads();
     }
     tile_gemm<false, false>(C, n, d.Dinv + (size_t)i * TILE * TILE, TILE, C, n, mi, nj, mi, 1.0, 0.0);
@@ -259,14 +360,6 @@ __global__ void k_fill(double *p, int64_t n, double v) {
         p[i] = v;
 }
 
-__global__ void k_identity(const DenseDesc *D) {
-    const DenseDesc &d = D[blockIdx.y];
-    const int64_t mm = (int64_t)d.m * d.m;
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < mm; q += (int64_t)gridDim.x * blockDim.x) {
-        int64_t i = q % d.m, j = q / d.m;
-        d.W[q] = (i == j) ? 1.0 : 0.0;
-    }
-}
 
 __device__ __forceinline__ double block_max(double v) {
     __shared__ double red[256];
@@ -398,31 +491,67 @@ __global__ void k_unpack(const T *pool, int64_t off, int m, float scale, double 
     }
 }
 
-__global__ void k_scatter(const DenseDesc *D, const int *rowptr, const int64_t *rp_off, const int *colidx,
-                          const double *vals, const int64_t *nz_off, const int *iperm, const int64_t *perm_off,
-                          const double *f, const int *nloc) {
+
+// Dense inputs of the assembly straight from the CSR rows, zeros included, in
+// one coalesced pass (K is symmetric, so row perm[c] of K holds permuted
+// column c): for interior column c the lower band tiles of K_ii, for boundary
+// column j all of K_ib(:, j) and the rows of S(:, j) from its diagonal tile
+// down, and the permuted f as column m of [K_ib | f].  Every entry a later
+// kernel reads is written here, so the workspace needs no zero fill.
+__global__ void __launch_bounds__(256) k_materialize(const DenseDesc *D, const int *rowptr, const int64_t *rp_off,
+                                                     const int *colidx, const double *vals, const int64_t *nz_off,
+                                                     const int *iperm, const int *perm, const int64_t *perm_off,
+                                                     const double *f, const int *nloc, int cols_per_wg, int cap) {
+    extern __shared__ double colbuf[];
     const DenseDesc &d = D[blockIdx.y];
-    const int b = blockIdx.y;
-    const int n = nloc[b];
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
+    const int b = blockIdx.y, n = nloc[b], ni = d.ni, m = d.m;
+    const int cb = blockIdx.x * cols_per_wg;
+    if (cb > n) return;  // uniform; columns 0..n-1 of K plus n = f
+    const int ce = min(n + 1, cb + cols_per_wg);
     const int *rp = rowptr + rp_off[b];
     const int *ci = colidx + nz_off[b];
     const double *kv = vals + nz_off[b];
     const int *ip = iperm + perm_off[b];
-    const int ni = d.ni, m = d.m;
-    const int pr = ip[r];
-    for (int q = rp[r]; q < rp[r + 1]; q++) {
-        const int pc = ip[ci[q]];
-        const double v = kv[q];
-        if (pr < ni) {
-            if (pc < ni) d.Kii[pr + (size_t)pc * ni] = v;
-            else d.Kib[pr + (size_t)(pc - ni) * ni] = v;
-        } else if (pc >= ni) {
-            d.S[(pr - ni) + (size_t)(pc - ni) * m] = v;
+    const int *pm = perm + perm_off[b];
+    for (int c = cb; c < ce; c++) {
+        if (c == n) {  // f column, permuted
+            double *dst = d.Kib + (size_t)m * ni;
+            for (int r = threadIdx.x; r < ni; r += 256) dst[r] = f[perm_off[b] + pm[r]];
+            continue;
+        }
+        // rows [lo, hi) of the permuted column c, assembled in LDS windows of `cap` rows
+        int lo, hi;
+        if (c < ni) {
+            lo = (c / TILE) * TILE;
+            hi = min(ni, lo + (d.band_ii + 1) * TILE);
+        } else {
+            lo = 0;
+            hi = n;  // [K_ib(:, j) ; S(:, j)] as one permuted column
+        }
+        const int l = pm[c];
+        for (int w0 = lo; w0 < hi; w0 += cap) {
+            const int w1 = min(hi, w0 + cap);
+            for (int r = w0 + threadIdx.x; r < w1; r += 256) colbuf[r - w0] = 0.0;
+            __syncthreads();
+            for (int q = rp[l] + threadIdx.x; q < rp[l + 1]; q += 256) {
+                const int pr = ip[ci[q]];
+                if (pr >= w0 && pr < w1) colbuf[pr - w0] = kv[q];
+            }
+            __syncthreads();
+            if (c < ni) {
+                double *dst = d.Kii + (size_t)c * ni;
+                for (int r = w0 + threadIdx.x; r < w1; r += 256) dst[r] = colbuf[r - w0];
+            } else {
+                const int j = c - ni;
+                const int s0 = ni + (j / TILE) * TILE;  // first S row kept, in the permuted column
+                for (int r = w0 + threadIdx.x; r < w1; r += 256) {
+                    if (r < ni) d.Kib[(size_t)j * ni + r] = colbuf[r - w0];
+                    else if (r >= s0) d.S[(size_t)j * m + (r - ni)] = colbuf[r - w0];
+                }
+            }
+            __syncthreads();
         }
     }
-    if (pr < ni) d.Kib[pr + (size_t)m * ni] = f[perm_off[b] + r];
 }
 
 // ---- hybrid FETI: corner columns of F_i (lower triangle of S) ----
@@ -527,10 +656,6 @@ void launch_syrk(const DenseDesc *desc, int nb, int which, int max_ntm, hipStrea
     hipLaunchKernelGGL(k_syrk, dim3(max_ntm * (max_ntm + 1) / 2, nb), dim3(256), 0, st, desc, which);
 }
 
-void launch_identity(const DenseDesc *desc, int nb, int max_m, hipStream_t st) {
-    unsigned g = std::min<unsigned>(cdiv((int64_t)max_m * max_m, 256), 4096);
-    hipLaunchKernelGGL(k_identity, dim3(g, nb), dim3(256), 0, st, desc);
-}
 
 void launch_kernel_reg(const DenseDesc *desc, int nb, const double *q, const int64_t *q_off, const int *kdim,
                        double *rho, int max_m, hipStream_t st) {
@@ -580,11 +705,14 @@ void launch_unpack(const void *pool, int prec, int64_t off, int m, float scale, 
                            m, scale, out);
 }
 
-void launch_scatter(const DenseDesc *desc, int nb, const int *rowptr, const int64_t *rp_off, const int *colidx,
-                    const double *vals, const int64_t *nz_off, const int *iperm, const int64_t *perm_off,
-                    const double *f, const int *n, int max_n, hipStream_t st) {
-    hipLaunchKernelGGL(k_scatter, dim3(cdiv(max_n, 256), nb), dim3(256), 0, st, desc, rowptr, rp_off, colidx, vals,
-                       nz_off, iperm, perm_off, f, n);
+void launch_materialize(const DenseDesc *desc, int nb, const int *rowptr, const int64_t *rp_off, const int *colidx,
+                        const double *vals, const int64_t *nz_off, const int *iperm, const int *perm,
+                        const int64_t *perm_off, const double *f, const int *n, int max_n, hipStream_t st) {
+    const int cpw = 8;
+    const int cap = std::min(std::max(max_n, 1), 8192);  // LDS window (64 KB)
+    hipLaunchKernelGGL(k_materialize, dim3(cdiv(max_n + 1, cpw), nb), dim3(256), (size_t)cap * 8, st, desc, rowptr,
+                       rp_off, colidx, vals, nz_off, iperm, perm, perm_off, f, n, cpw, cap);
 }
+
 
 }  // namespace esp

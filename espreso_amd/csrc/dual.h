@@ -127,6 +127,11 @@ void launch_axpby(double *y, const double *a, const double *b, int64_t n, double
 void launch_copy_scalar(const double *src, double *dst, hipStream_t st);
 void launch_symmetrize(double *A, int n, hipStream_t st);
 void launch_primal(const PrimalDev &P, int nsub, const double *ub, double *u, hipStream_t st);
+// new loads f (local DOF order, all subdomains): P.yf = L^-1 f_int, hf = f_b - K_bi K_ii^-1 f_int
+void launch_rhs_prepare(const PrimalDev &P, int nsub, const double *f, double *hf, hipStream_t st);
+// sens[e] = -w_e^T Ke0 u_e over the NE[0] x NE[1] x NE[2] HEXA8 grid (nodal fields, ndof nd)
+void launch_element_sensitivity(const int *NE, int nd, const double *Ke0, const double *u, const double *w,
+                                double *sens, hipStream_t st);
 // plain copy of n doubles (n multiple of 4): calibration of HBM byte counters
 void launch_copy(const double *src, double *dst, int64_t n, hipStream_t st);
 

@@ -717,6 +717,97 @@ void launch_copy(const double *src, double *dst, int64_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, st, (const double4 *)src, (double4 *)dst, n / 4);
 }
 
+// ---- new load vector: y_f = L^-1 f_int (kept for the primal recovery) and
+// hf = f_b - K_bi K_ii^-1 f_int, so that (K^+ f)_b = F_i hf; one workgroup per subdomain
+__global__ void __launch_bounds__(256) k_rhs_prepare(const PrimalDev P, const double *f, double *hf) {
+    __shared__ double red[4];
+    __shared__ double piv;
+    const int s = blockIdx.x;
+    const int ni = P.ni[s], m = P.m[s], bw = P.bw[s];
+    const int w = bw + 1;
+    const double *L = P.band + P.band_off[s];
+    double *yf = P.yf + P.yf_off[s];
+    const int *perm = P.perm + P.perm_off[s];
+    const int *iperm = P.iperm + P.perm_off[s];
+    const int *rp = P.rowptr + P.rp_off[s];
+    const int *ci = P.colidx + P.nz_off[s];
+    const double *kv = P.vals + P.nz_off[s];
+    const double *fs = f + P.perm_off[s];
+    double *z = P.work + P.work_off[s];
+    for (int r = threadIdx.x; r < ni; r += 256) z[r] = fs[perm[r]];
+    __syncthreads();
+    for (int c = 0; c < ni; c++) {
+        if (threadIdx.x == 0) {
+            z[c] /= L[(int64_t)c * w];
+            piv = z[c];
+        }
+        __syncthreads();
+        const double zc = piv;
+        for (int d = 1 + threadIdx.x; d <= bw && c + d < ni; d += 256) z[c + d] -= L[(int64_t)c * w + d] * zc;
+        __syncthreads();
+    }
+    for (int r = threadIdx.x; r < ni; r += 256) yf[r] = z[r];
+    __syncthreads();
+    for (int c = ni - 1; c >= 0; c--) {
+        double v = 0.0;
+        for (int d = 1 + threadIdx.x; d <= bw && c + d < ni; d += 256) v += L[(int64_t)c * w + d] * z[c + d];
+        v = block_sum(v, red);
+        if (threadIdx.x == 0) z[c] = (z[c] - v) / L[(int64_t)c * w];
+        __syncthreads();
+    }
+    for (int j = threadIdx.x; j < m; j += 256) {
+        const int lr = perm[ni + j];
+        double v = fs[lr];
+        for (int q = rp[lr]; q < rp[lr + 1]; q++) {
+            const int pc = iperm[ci[q]];
+            if (pc < ni) v -= kv[q] * z[pc];
+        }
+        hf[P.boff[s] + j] = v;
+    }
+}
+
+void launch_rhs_prepare(const PrimalDev &P, int nsub, const double *f, double *hf, hipStream_t st) {
+    hipLaunchKernelGGL(k_rhs_prepare, dim3(nsub), dim3(256), 0, st, P, f, hf);
+}
+
+// ---- structured grid: element sensitivities sens_e = -w_e^T K_e0 u_e ----
+__global__ void __launch_bounds__(256) k_element_sensitivity(int NE0, int NE1, int NE2, int nd, const double *Ke0,
+                                                             const double *u, const double *w, double *sens) {
+    extern __shared__ double K[];
+    const int ne = 8 * nd;
+    for (int q = threadIdx.x; q < ne * ne; q += 256) K[q] = Ke0[q];
+    __syncthreads();
+    const int NN0 = NE0 + 1, NN1 = NE1 + 1;
+    const int64_t nel = (int64_t)NE0 * NE1 * NE2;
+    const int cx[8] = {0, 1, 1, 0, 0, 1, 1, 0}, cy[8] = {0, 0, 1, 1, 0, 0, 1, 1}, cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nel; e += (int64_t)gridDim.x * blockDim.x) {
+        const int ex = (int)(e % NE0), ey = (int)((e / NE0) % NE1), ez = (int)(e / ((int64_t)NE0 * NE1));
+        double ue[24], we[24];
+        for (int a = 0; a < 8; a++) {
+            const int64_t node = (ex + cx[a]) + (int64_t)NN0 * ((ey + cy[a]) + (int64_t)NN1 * (ez + cz[a]));
+            for (int c = 0; c < nd; c++) {
+                ue[a * nd + c] = u[node * nd + c];
+                we[a * nd + c] = w[node * nd + c];
+            }
+        }
+        double v = 0.0;
+        for (int i = 0; i < ne; i++) {
+            double t = 0.0;
+            for (int j = 0; j < ne; j++) t += K[i * ne + j] * ue[j];
+            v += we[i] * t;
+        }
+        sens[e] = -v;
+    }
+}
+
+void launch_element_sensitivity(const int *NE, int nd, const double *Ke0, const double *u, const double *w,
+                                double *sens, hipStream_t st) {
+    const int64_t nel = (int64_t)NE[0] * NE[1] * NE[2];
+    const int ne = 8 * nd;
+    hipLaunchKernelGGL(k_element_sensitivity, dim3(vgrid(nel)), dim3(256), (size_t)ne * ne * 8, st, NE[0], NE[1],
+                       NE[2], nd, Ke0, u, w, sens);
+}
+
 void launch_primal(const PrimalDev &P, int nsub, const double *ub, double *u, hipStream_t st) {
     hipLaunchKernelGGL(k_primal_interior, dim3(nsub), dim3(256), 0, st, P, ub, u);
 }

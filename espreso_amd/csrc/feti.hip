@@ -199,6 +199,13 @@ struct espreso_feti_s {
         return upload(v);
     }
     void cluster_setup();
+    void compute_b();
+    void resolve_rhs();
+    std::vector<double> Ns_h;                        // hybrid: N_s per subdomain (kdim x kN)
+    std::vector<int64_t> noff_h;
+    std::vector<int> ccoff_h, cblk_h;                // coarse block offset, block of each subdomain
+    int gridNE[3] = {0, 0, 0};                       // structured grid front end
+    std::vector<double> gridKe0;
     void invert_spd_batched(double *pool, const std::vector<int64_t> &off, const std::vector<int> &n,
                             const char *what);
     std::vector<int> n0_h, kt_h;
@@ -558,16 +565,53 @@ void espreso_feti_s::gpu_setup() {
     double tc = wall();
     if (hybrid) cluster_setup();
     coarse();
-    // K^+ f on the boundary: ubf = F_i hf (FP64); hybrid: + cluster correction;
-    // then b = B K^+ f - c
+    compute_b();
+    sync();
+    last.time_coarse = wall() - tc;
+    ready = true;
+}
+
+// K^+ f on the boundary: ubf = F_i hf (FP64); hybrid: + cluster correction;
+// then b = B K^+ f - c (cluster.cpp:2660-2769)
+void espreso_feti_s::compute_b() {
     launch_symv(D, false, 0, d_hf, 0, false, st);
     ESP_HIP(hipMemcpyAsync(d_ubf, D.yb, D.nbnd * sizeof(double), hipMemcpyDeviceToDevice, st));
     if (hybrid) launch_cluster_correction(D, CL, d_e0f, D.yb, 1, st);
     launch_lambda_gather(D, D.yb, false, d_b, st);
     launch_axpby(d_b, d_b, d_c, nlam, 1.0, -1.0, st);
+}
+
+// New loads (host sub[s].f) on the assembled operators: e = -R^T f (hybrid:
+// per cluster through N_s), y_f and hf by interior band solves, b.
+void espreso_feti_s::resolve_rhs() {
+    std::vector<double> ff(uoff[nsub]);
+    for (int s = 0; s < nsub; s++) {
+        const SubHost &h = sub[s];
+        std::copy(h.f.begin(), h.f.end(), ff.begin() + uoff[s]);
+        for (int a = 0; a < h.kdim; a++) {
+            double acc = 0;
+            for (int i = 0; i < h.n; i++) acc += h.R[i + (size_t)a * h.n] * h.f[i];
+            e0f_h[asubv[s] + a] = -acc;
+        }
+    }
+    std::vector<double> e;
+    if (hybrid) {
+        e.assign(ncoarse, 0.0);
+        for (int s = 0; s < nsub; s++) {
+            const int c = cblk_h[s], kn = (int)((noff_h[s + 1] - noff_h[s]) / std::max(1, sub[s].kdim));
+            for (int b = 0; b < kn; b++)
+                for (int a = 0; a < sub[s].kdim; a++)
+                    e[ccoff_h[c] + b] += Ns_h[noff_h[s] + a + (int64_t)b * sub[s].kdim] * e0f_h[asubv[s] + a];
+        }
+    } else {
+        e = e0f_h;
+    }
+    ESP_HIP(hipMemcpyAsync(d_f, ff.data(), ff.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    ESP_HIP(hipMemcpyAsync(d_e0f, e0f_h.data(), e0f_h.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    ESP_HIP(hipMemcpyAsync(d_e, e.data(), e.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    launch_rhs_prepare(P, nsub, d_f, d_hf, st);
+    compute_b();
     sync();
-    last.time_coarse = wall() - tc;
-    ready = true;
 }
 
 void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::vector<int64_t> &foff,
@@ -591,7 +635,6 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         ESP_CHECK(bytes <= budget, ESPRESO_ERR_MEMORY, "one subdomain's assembly workspace exceeds the device budget");
         const int nbt = s1 - s0;
         double *pool = alloc<double>((size_t)(bytes / 8));
-        launch_fill(pool, (int64_t)(bytes / 8), 0.0, st);
         std::vector<DenseDesc> desc(nbt);
         std::vector<int> cs_all;
         std::vector<int64_t> cs_off(nbt);
@@ -665,8 +708,8 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         int *d_nloc = upload(nloc), *d_kd = upload(kd), *d_bw = upload(bw);
         double *d_rho = alloc<double>(nbt);
 
-        launch_scatter(d_desc, nbt, P.rowptr, d_rp_off, P.colidx, P.vals, d_nz_off, P.iperm, d_n_off, d_f, d_nloc,
-                       max_n, st);
+        launch_materialize(d_desc, nbt, P.rowptr, d_rp_off, P.colidx, P.vals, d_nz_off, P.iperm, P.perm, d_n_off, d_f,
+                           d_nloc, max_n, st);
         for (int k = 0; k < max_ntii; k++) launch_potrf_step(d_desc, nbt, 0, k, max_band, st);
         for (int k = 0; k < max_ntii; k++) launch_trsm_step(d_desc, nbt, 0, k, max_band, max_ntc, st);
         launch_syrk(d_desc, nbt, 0, max_ntm, st);
@@ -675,7 +718,6 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         launch_pack(d_desc, nbt, 0, D.SD, d_f_off, nullptr, max_m, st);
         launch_kernel_reg(d_desc, nbt, d_Q, d_q_off, d_kd, d_rho, max_m, st);
         for (int k = 0; k < max_ntm; k++) launch_potrf_step(d_desc, nbt, 1, k, max_ntm, st);
-        launch_identity(d_desc, nbt, max_m, st);
         for (int k = 0; k < max_ntm; k++) launch_trsm_step(d_desc, nbt, 1, k, max_ntm, max_ntm, st);
         launch_syrk(d_desc, nbt, 1, max_ntm, st);
         if (hybrid) {
@@ -740,7 +782,6 @@ void espreso_feti_s::coarse() {
     d.ntc = nt;
     DenseDesc *dd = upload(std::vector<DenseDesc>{d});
     for (int k = 0; k < nt; k++) launch_potrf_step(dd, 1, 1, k, nt, st);
-    launch_identity(dd, 1, nc, st);
     for (int k = 0; k < nt; k++) launch_trsm_step(dd, 1, 1, k, nt, nt, st);
     launch_syrk(dd, 1, 1, nt, st);
     launch_symmetrize(S, nc, st);
@@ -1212,6 +1253,10 @@ void espreso_feti_s::load_clusters() {
         }
     }
     ESP_CHECK(ncoarse > 0, ESPRESO_ERR_NUMERIC, "HYBRID_FETI: empty coarse space");
+    Ns_h = Ns;
+    noff_h = noff;
+    ccoff_h = coff;
+    cblk_h = cblk;
     D.nblk = ncl;
     D.ncoarse = ncoarse;
     D.cblk = upload(cblk);
@@ -1274,7 +1319,6 @@ void espreso_feti_s::invert_spd_batched(double *pool, const std::vector<int64_t>
     }
     DenseDesc *dd = upload(desc);
     for (int k = 0; k < max_nt; k++) launch_potrf_step(dd, nb, 1, k, max_nt, st);
-    launch_identity(dd, nb, max_n, st);
     for (int k = 0; k < max_nt; k++) launch_trsm_step(dd, nb, 1, k, max_nt, max_nt, st);
     launch_syrk(dd, nb, 1, max_nt, st);
     launch_symmetrize_desc(dd, nb, max_n, st);
@@ -1426,6 +1470,28 @@ int espreso_feti_setup(espreso_feti *h) {
     ESP_API_END
 }
 
+int espreso_feti_set_rhs(espreso_feti *h, int32_t i, const double *f) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && f && i >= 0 && i < h->nsub, ESPRESO_ERR_ARG, "invalid argument");
+    ESP_CHECK(h->sub[i].set, ESPRESO_ERR_STATE, "subdomain not set");
+    SubHost &s = h->sub[i];
+    for (int q = 0; q < s.n; q++) ESP_CHECK(std::isfinite(f[q]), ESPRESO_ERR_ARG, "non-finite load");
+    s.f.assign(f, f + s.n);
+    ESP_API_END
+}
+
+int espreso_feti_resolve(espreso_feti *h, espreso_feti_stats *st) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && h->ready, ESPRESO_ERR_STATE, "espreso_feti_setup not called");
+    ESP_HIP(hipSetDevice(h->opt.device));
+    const double t0 = wall();
+    h->resolve_rhs();
+    const double trhs = wall() - t0;
+    h->solve(st);
+    if (st) st->time_coarse = trhs;
+    ESP_API_END
+}
+
 int espreso_feti_solve(espreso_feti *h, espreso_feti_stats *st) {
     ESP_API_BEGIN
     ESP_CHECK(h, ESPRESO_ERR_ARG, "null handle");
@@ -1438,6 +1504,17 @@ int espreso_feti_primal(espreso_feti *h, int32_t i, double *u) {
     ESP_API_BEGIN
     ESP_CHECK(h && h->solved && i >= 0 && i < h->nsub, ESPRESO_ERR_STATE, "no solution");
     ESP_HIP(hipMemcpy(u, h->d_u + h->uoff[i], h->sub[i].n * sizeof(double), hipMemcpyDeviceToHost));
+    ESP_API_END
+}
+
+int espreso_feti_load_work(espreso_feti *h, double *c) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && c, ESPRESO_ERR_ARG, "null argument");
+    ESP_CHECK(h->solved, ESPRESO_ERR_STATE, "no solution");
+    ESP_HIP(hipSetDevice(h->opt.device));
+    launch_dot(h->d_f, h->d_u, h->uoff[h->nsub], h->d_partial, h->d_scal + 8, h->st);
+    ESP_HIP(hipMemcpyAsync(c, h->d_scal + 8, sizeof(double), hipMemcpyDeviceToHost, h->st));
+    h->sync();
     ESP_API_END
 }
 
@@ -1629,6 +1706,8 @@ int espreso_grid_load(espreso_feti *h, const espreso_grid_problem *p) {
     grid_build(*p, h->opt.scaling, g);
     h->ndof = g.ndof;
     h->nnodes = g.nnodes;
+    for (int d = 0; d < 3; d++) h->gridNE[d] = g.NN[d] - 1;
+    grid_unit_element(*p, h->gridKe0);
     h->l2g.resize(h->nsub);
     for (int s = 0; s < h->nsub; s++) {
         GridSubdomain &sd = g.sub[s];
@@ -1691,6 +1770,65 @@ int espreso_grid_export_cluster(const espreso_grid_problem *p, int32_t i, int32_
     if (row) std::copy(sd.b0row.begin(), sd.b0row.end(), row);
     if (dof) std::copy(sd.b0dof.begin(), sd.b0dof.end(), dof);
     if (sign) std::copy(sd.b0sign.begin(), sd.b0sign.end(), sign);
+    ESP_API_END
+}
+
+int espreso_grid_nodal_load(espreso_feti *h, double *g) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && g, ESPRESO_ERR_ARG, "null argument");
+    ESP_CHECK((int)h->l2g.size() == h->nsub, ESPRESO_ERR_STATE, "grid not loaded");
+    const int nd = h->ndof;
+    std::fill(g, g + (size_t)h->nnodes * nd, 0.0);
+    for (int s = 0; s < h->nsub; s++)
+        for (size_t ln = 0; ln < h->l2g[s].size(); ln++)
+            for (int c = 0; c < nd; c++) g[(size_t)h->l2g[s][ln] * nd + c] += h->sub[s].f[ln * nd + c];
+    ESP_API_END
+}
+
+int espreso_grid_unit_element(const espreso_grid_problem *p, double *Ke) {
+    ESP_API_BEGIN
+    ESP_CHECK(p && Ke, ESPRESO_ERR_ARG, "null argument");
+    std::vector<double> K;
+    grid_unit_element(*p, K);
+    std::copy(K.begin(), K.end(), Ke);
+    ESP_API_END
+}
+
+int espreso_grid_set_nodal_rhs(espreso_feti *h, const double *g) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && g, ESPRESO_ERR_ARG, "null argument");
+    ESP_CHECK((int)h->l2g.size() == h->nsub, ESPRESO_ERR_STATE, "grid not loaded");
+    const int nd = h->ndof;
+    std::vector<int> mult(h->nnodes, 0);
+    for (int s = 0; s < h->nsub; s++)
+        for (int node : h->l2g[s]) mult[node]++;
+    for (int s = 0; s < h->nsub; s++) {
+        std::vector<double> f(h->sub[s].n);
+        for (size_t ln = 0; ln < h->l2g[s].size(); ln++) {
+            const int node = h->l2g[s][ln];
+            for (int c = 0; c < nd; c++) f[ln * nd + c] = g[(size_t)node * nd + c] / mult[node];
+        }
+        const int rc = espreso_feti_set_rhs(h, s, f.data());
+        if (rc) return rc;
+    }
+    ESP_API_END
+}
+
+int espreso_grid_element_sensitivity(espreso_feti *h, const double *u, const double *w, double *sens) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && u && w && sens, ESPRESO_ERR_ARG, "null argument");
+    ESP_CHECK(h->gridNE[0] > 0 && !h->gridKe0.empty(), ESPRESO_ERR_STATE, "grid not loaded");
+    ESP_HIP(hipSetDevice(h->opt.device));
+    const int64_t nel = (int64_t)h->gridNE[0] * h->gridNE[1] * h->gridNE[2];
+    const int64_t nv = (int64_t)h->nnodes * h->ndof;
+    double *du = h->alloc<double>(nv), *dw = h->alloc<double>(nv), *ds = h->alloc<double>(nel);
+    double *dk = h->upload(h->gridKe0);
+    ESP_HIP(hipMemcpyAsync(du, u, nv * sizeof(double), hipMemcpyHostToDevice, h->st));
+    ESP_HIP(hipMemcpyAsync(dw, w, nv * sizeof(double), hipMemcpyHostToDevice, h->st));
+    launch_element_sensitivity(h->gridNE, h->ndof, dk, du, dw, ds, h->st);
+    ESP_HIP(hipMemcpyAsync(sens, ds, nel * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    h->sync();
+    for (void *q : {(void *)du, (void *)dw, (void *)ds, (void *)dk}) h->release(q);
     ESP_API_END
 }
 

@@ -191,6 +191,10 @@ void mgs_columns(int n, int k, double *V) {
 }  // namespace
 
 int grid_subdomain_count(const espreso_grid_problem &p) { return Box(p).nsub(); }
+void grid_unit_element(const espreso_grid_problem &p, std::vector<double> &Ke) {
+    Box B(p);
+    unit_element(p, B.h, Ke);
+}
 int grid_node_count(const espreso_grid_problem &p) {
     Box b(p);
     return b.NN[0] * b.NN[1] * b.NN[2];
@@ -211,6 +215,8 @@ void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out) {
         SplitMix r(p.young_seed);
         for (int e = 0; e < nel; e++) escale[e] = p.young * (0.1 + 0.9 * r.next01());
     }
+    if (nd == 3 && p.young_elem)
+        for (int e = 0; e < nel; e++) escale[e] = p.young_elem[e];
     if (nd == 1 && p.conductivity_seed) {
         SplitMix r(p.conductivity_seed);
         std::vector<double> ks(ns);

@@ -37,6 +37,8 @@ struct GridProblem {
 
 int grid_subdomain_count(const espreso_grid_problem &p);
 int grid_node_count(const espreso_grid_problem &p);
+// unit-modulus element matrix (row-major, node-major DOFs, reference corner order)
+void grid_unit_element(const espreso_grid_problem &p, std::vector<double> &Ke);
 // scaling: 1 = K-diagonal weights, 0 = 1/multiplicity
 void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out);
 

@@ -60,7 +60,7 @@ struct DenseDesc {
     double *Kii;          // ni x ni, ld ni (lower used); becomes L (K_ii = L L^T)
     double *Kib;          // ni x (m+1), ld ni: [K_ib | f_i]  ->  [Y | y_f] = L^-1 [K_ib | f_i]
     double *S;            // m x m, ld m (lower): K_bb - Y^T Y -> + rho Q Q^T -> L_s -> F
-    double *W;            // m x m, ld m: identity -> L_s^-1
+    double *W;            // m x m, ld m: L_s^-1 (lower tiles)
     double *Dinv;         // max(ntii, ntm) x 64 x 64: inverses of diagonal blocks
     const int *colstart;  // ntc: first tile row of a nonzero in each column tile of K_ib
     int *info;            // factorisation status (0 = ok, else 1-based failing column)
@@ -76,8 +76,12 @@ void launch_potrf_step(const DenseDesc *desc, int nb, int which, int step, int m
 void launch_trsm_step(const DenseDesc *desc, int nb, int which, int step, int max_band, int max_ntc,
                       hipStream_t st);
 void launch_syrk(const DenseDesc *desc, int nb, int which, int max_ntm, hipStream_t st);
-void launch_identity(const DenseDesc *desc, int nb, int max_m, hipStream_t st);
 void launch_symmetrize_desc(const DenseDesc *desc, int nb, int max_m, hipStream_t st);
+// CSR K_i (local DOFs) into the dense K_ii band, [K_ib|f] and K_bb, zeros included;
+// iperm maps local -> internal index, perm internal -> local
+void launch_materialize(const DenseDesc *desc, int nb, const int *rowptr, const int64_t *rp_off, const int *colidx,
+                        const double *vals, const int64_t *nz_off, const int *iperm, const int *perm,
+                        const int64_t *perm_off, const double *f, const int *n, int max_n, hipStream_t st);
 // S += rho Q Q^T with rho = max diag S; Q (m x k, column-major) at q[q_off[b]]
 void launch_kernel_reg(const DenseDesc *desc, int nb, const double *q, const int64_t *q_off,
                        const int *kdim, double *rho, int max_m, hipStream_t st);
@@ -92,11 +96,6 @@ void launch_band_extract(const DenseDesc *desc, int nb, double *band, const int6
                          double *yf, const int64_t *yf_off, int max_ni, hipStream_t st);
 // unpack one strip-packed matrix into dense m x m (column-major, full)
 void launch_unpack(const void *pool, int prec, int64_t off, int m, float scale, double *out, hipStream_t st);
-// scatter CSR K_i (local DOFs) into K_ii, [K_ib|f], K_bb; iperm maps local -> internal index
-void launch_scatter(const DenseDesc *desc, int nb, const int *rowptr, const int64_t *rp_off, const int *colidx,
-                    const double *vals, const int64_t *nz_off, const int *iperm, const int64_t *perm_off,
-                    const double *f, const int *n, int max_n, hipStream_t st);
-
 // hybrid FETI: FB0 = F_i B0_i^T (m x nq, signed corner columns) and the
 // corner block B0_i F_i B0_i^T (nq x nq), from F_i in the lower triangle of S
 void launch_extract_corners(const DenseDesc *desc, int nb, const int *nq, const int64_t *qoff, const int *qj,

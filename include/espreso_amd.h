@@ -65,6 +65,8 @@ typedef struct {
     double  traction_scale;
     uint64_t young_seed;
     uint64_t conductivity_seed;
+    const double *young_elem;   /* optional: Young's modulus per element, index
+                                   ex + NEx*(ey + NEy*ez); overrides young/young_seed */
 } espreso_grid_problem;
 
 /* Values follow the reference enums (src/config/ecf/solver/feti.h). */
@@ -132,10 +134,22 @@ int espreso_grid_export(const espreso_grid_problem *p, int32_t scaling, int32_t 
  * the entry count; pass NULL arrays to query it.                             */
 int espreso_grid_export_cluster(const espreso_grid_problem *p, int32_t i, int32_t *cluster, int32_t *n0,
                                 int32_t *row, int32_t *dof, double *sign);
+/* Unit-modulus element matrix K_e0 of the grid (ndof*8 x ndof*8, row-major,
+ * DOFs node-major in the reference's corner order): K_e = E_e K_e0.          */
+int espreso_grid_unit_element(const espreso_grid_problem *p, double *Ke);
 /* global node id of every local node of subdomain i (after espreso_grid_load) */
 int espreso_grid_l2g(espreso_feti *h, int32_t i, int32_t *l2g);
 /* merge per-subdomain primal of the last solve into nodal u (average of copies) */
 int espreso_grid_nodal_solution(espreso_feti *h, double *u_nodal);
+/* Assembled nodal load of the current subdomain loads (sum over copies). */
+int espreso_grid_nodal_load(espreso_feti *h, double *g_nodal);
+/* New nodal load g (n_nodes x ndof) for espreso_feti_resolve: each node's load
+ * is split evenly over the subdomains that share it.                         */
+int espreso_grid_set_nodal_rhs(espreso_feti *h, const double *g_nodal);
+/* Element sensitivities on the GPU: sens[e] = -w_e^T K_e0 u_e for nodal
+ * fields u, w (n_nodes x ndof); with w = u (the compliance adjoint) this is
+ * d(f^T u)/dE_e.  No counterpart in the reference (adjoint extension).       */
+int espreso_grid_element_sensitivity(espreso_feti *h, const double *u_nodal, const double *w_nodal, double *sens);
 
 /* ---- FETI core --------------------------------------------------------- */
 int espreso_feti_create(espreso_feti **h, int32_t n_subdomains, const espreso_feti_options *o);
@@ -153,6 +167,17 @@ int espreso_feti_set_subdomain(espreso_feti *h, int32_t i, int32_t n,
                                int32_t nlam, const int32_t *lambda, const int32_t *dof,
                                const double *sign, const double *scale);
 int espreso_feti_set_dual(espreso_feti *h, int32_t n_lambda, const double *c);
+/* Replace the load vector of subdomain i (local DOF order) after setup; the
+ * next espreso_feti_resolve solves with it, reusing F_i, S_D, the interior
+ * factors and the coarse space.  FETISolver::update with Matrices::f only
+ * (FETISolver.cpp:59-116).                                                    */
+int espreso_feti_set_rhs(espreso_feti *h, int32_t i, const double *f);
+/* Solve again with the loads set by espreso_feti_set_rhs (no reassembly):
+ * new b and e, projected PCG, primal recovery.                               */
+int espreso_feti_resolve(espreso_feti *h, espreso_feti_stats *stats);
+/* sum over subdomains of f_i^T u_i for the last solution (the compliance
+ * f^T u of the assembled problem)                                            */
+int espreso_feti_load_work(espreso_feti *h, double *c);
 /* Hybrid FETI (options.method = 1): the cluster of subdomain i and the B0
  * rows touching it (cluster-local row id, local DOF, +1/-1); every B0 row
  * holds one +1 and one -1 entry of two subdomains of the same cluster, and

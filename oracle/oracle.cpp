@@ -181,6 +181,8 @@ Loads make_loads(const Grid &G) {
         Rng r(G.p.young_seed);
         for (int e = 0; e < nel; e++) L.elem_scale[e] = G.p.young * (0.1 + 0.9 * r.uniform());
     }
+    if (G.p.physics == 0 && G.p.young_elem)
+        for (int e = 0; e < nel; e++) L.elem_scale[e] = G.p.young_elem[e];
     if (G.p.physics == 1 && G.p.conductivity_seed) {
         Rng r(G.p.conductivity_seed);
         std::vector<double> ks(G.nsub);
@@ -1142,6 +1144,15 @@ int oracle_global_sizes(const oracle_grid_problem *p, int32_t *n_nodes, int32_t 
     Grid G(*p);
     *n_nodes = G.nnodes();
     *ndof = G.ndof;
+    return 0;
+}
+
+// unit-modulus element matrix of the grid (row-major, node-major DOFs)
+int oracle_unit_element(const oracle_grid_problem *p, double *Ke) {
+    Grid G(*p);
+    std::vector<double> K, Nint;
+    element_matrices(p->physics, G.h, p->poisson, K, Nint);
+    std::copy(K.begin(), K.end(), Ke);
     return 0;
 }
 

@@ -56,6 +56,7 @@ typedef struct {
     double  traction_scale;
     uint64_t young_seed;        /* !=0: E_e = young*(0.1+0.9 U(0,1)) per element */
     uint64_t conductivity_seed; /* !=0: k_s = conductivity*10^U(-1,1) per subdomain */
+    const double *young_elem;   /* optional Young's modulus per element (overrides) */
 } oracle_grid_problem;
 
 typedef struct {
@@ -86,6 +87,8 @@ typedef struct {
 /* ---- undecomposed ground truth ---------------------------------------- */
 /* nodal solution u[node*ndof + c], node = i + Nx*(j + Ny*k)               */
 int oracle_global_solve(const oracle_grid_problem *p, double *u_nodal);
+/* unit-modulus (E = 1 / conductivity 1) element matrix, row-major, node-major DOFs */
+int oracle_unit_element(const oracle_grid_problem *p, double *Ke);
 int oracle_global_sizes(const oracle_grid_problem *p, int32_t *n_nodes, int32_t *ndof);
 /* global rhs/K actions for residual checks: y = K x (free rows) ; f */
 int oracle_global_residual(const oracle_grid_problem *p, const double *u_nodal, double *rel_res);

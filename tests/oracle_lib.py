@@ -33,6 +33,7 @@ class OracleProblem(C.Structure):
         ("traction_scale", C.c_double),
         ("young_seed", C.c_uint64),
         ("conductivity_seed", C.c_uint64),
+        ("young_elem", C.POINTER(C.c_double)),
     ]
 
 
@@ -92,6 +93,7 @@ def lib():
             getattr(_lib, name).argtypes = [C.c_void_p, d, d]
         _lib.oracle_solve.argtypes = [C.c_void_p, d, d, C.POINTER(OracleStats)]
         _lib.oracle_global_solve.argtypes = [C.POINTER(OracleProblem), d]
+        _lib.oracle_unit_element.argtypes = [C.POINTER(OracleProblem), d]
         _lib.oracle_global_sizes.argtypes = [C.POINTER(OracleProblem), i32, i32]
         _lib.oracle_global_residual.argtypes = [C.POINTER(OracleProblem), d, d]
         _lib.oracle_cpu_sample.argtypes = [C.POINTER(OracleProblem), C.POINTER(OracleOptions), C.c_int32, d]
@@ -127,7 +129,34 @@ def make_problem(spec):
     p.traction_scale = spec.get("traction_scale", 1.0)
     p.young_seed = spec.get("young_seed", 0)
     p.conductivity_seed = spec.get("conductivity_seed", 0)
+    ye = spec.get("young_elem")
+    if ye is not None:
+        ye = np.ascontiguousarray(ye, dtype=np.float64)
+        p._keep = ye  # the array must outlive the struct
+        p.young_elem = _dp(ye)
     return p
+
+
+def unit_element(spec):
+    """Unit-modulus HEXA8 element matrix (Gauss quadrature), row-major."""
+    p = make_problem(spec)
+    nd = 3 if spec.get("physics", 0) == 0 else 1
+    Ke = np.zeros((8 * nd) ** 2)
+    lib().oracle_unit_element(C.byref(p), _dp(Ke))
+    return Ke.reshape(8 * nd, 8 * nd)
+
+
+def element_dofs(spec):
+    """Global DOF indices (n_elements x 8*ndof) of every element, element index
+    ex + NEx*(ey + NEy*ez), corners in the reference's HEXA8 order."""
+    NE = [spec["clusters"][d] * spec["domains"][d] * spec["elements"][d] for d in range(3)]
+    NN = [n + 1 for n in NE]
+    nd = 3 if spec.get("physics", 0) == 0 else 1
+    corner = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+    ez, ey, ex = np.meshgrid(np.arange(NE[2]), np.arange(NE[1]), np.arange(NE[0]), indexing="ij")
+    ex, ey, ez = ex.ravel(), ey.ravel(), ez.ravel()
+    nodes = np.stack([(ex + c[0]) + NN[0] * ((ey + c[1]) + NN[1] * (ez + c[2])) for c in corner], axis=1)
+    return (nodes[:, :, None] * nd + np.arange(nd)[None, None, :]).reshape(len(ex), 8 * nd)
 
 
 def make_options(preconditioner=3, precision=1e-8, max_iterations=500, scaling=1, kplus=0, threads=0):
@@ -306,3 +335,12 @@ def hybrid(spec, dirichlet=False):
     H.ndof = ndof
     H.B0data = B0
     return H
+
+
+def compliance_gradient(spec):
+    """dc/dE_e = -u_e^T K_e0 u_e of the compliance c = f^T u from the
+    undecomposed direct solve and the Gauss-quadrature element matrix."""
+    u = global_solve(spec).reshape(-1)
+    Ke0 = unit_element(spec)
+    ue = u[element_dofs(spec)]
+    return -np.einsum("ei,ij,ej->e", ue, Ke0, ue), u
