@@ -20,7 +20,7 @@ class FETICompliance(torch.autograd.Function):
     @staticmethod
     def forward(ctx, young, spec, options):
         E = young.detach().to("cpu", torch.float64).numpy()
-        solver = core.GridFETI(dict(spec, young_elem=E), **options)
+        solver = core.GridFETI(dict(spec, material_elem=E), **options)
         try:
             solver.setup()
             solver.solve()

@@ -45,7 +45,8 @@ class GridProblem(C.Structure):
         ("traction_scale", C.c_double),
         ("young_seed", C.c_uint64),
         ("conductivity_seed", C.c_uint64),
-        ("young_elem", C.POINTER(C.c_double)),
+        ("material_elem", C.POINTER(C.c_double)),
+        ("slab_elements", C.POINTER(C.c_int32)),
     ]
 
 
@@ -214,16 +215,43 @@ def grid_problem(spec):
     p.traction_scale = spec.get("traction_scale", 1.0)
     p.young_seed = spec.get("young_seed", 0)
     p.conductivity_seed = spec.get("conductivity_seed", 0)
-    ye = spec.get("young_elem")
+    keep = []  # arrays the struct points to must outlive it
+    ye = spec.get("material_elem")
     if ye is not None:
         ye = np.ascontiguousarray(ye, dtype=np.float64)
-        p._keep = ye  # the array must outlive the struct
-        p.young_elem = _dp(ye)
+        keep.append(ye)
+        p.material_elem = _dp(ye)
+    sl = spec.get("slab_elements")
+    if sl is not None:
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.int32) for v in sl]), dtype=np.int32)
+        assert flat.size == sum(spec["clusters"][d] * spec["domains"][d] for d in range(3))
+        keep.append(flat)
+        p.slab_elements = _ip(flat)
+    p._keep = keep
     return p
 
 
+def slabs(spec):
+    """Elements of every subdomain slab per direction (uniform unless spec
+    has slab_elements)."""
+    if spec.get("slab_elements") is not None:
+        return [list(map(int, v)) for v in spec["slab_elements"]]
+    return [[spec["elements"][d]] * (spec["clusters"][d] * spec["domains"][d]) for d in range(3)]
+
+
+def subdomain_elements(spec, s):
+    """(ex, ey, ez) elements of subdomain s (cluster-major ids, x fastest)."""
+    dom, cl = spec["domains"], spec["clusters"]
+    dpc = dom[0] * dom[1] * dom[2]
+    c, d = s // dpc, s % dpc
+    cc = (c % cl[0], (c // cl[0]) % cl[1], c // (cl[0] * cl[1]))
+    dd = (d % dom[0], (d // dom[0]) % dom[1], d // (dom[0] * dom[1]))
+    sl = slabs(spec)
+    return tuple(sl[q][cc[q] * dom[q] + dd[q]] for q in range(3))
+
+
 def n_elements(spec):
-    return int(np.prod([spec["clusters"][d] * spec["domains"][d] * spec["elements"][d] for d in range(3)]))
+    return int(np.prod([sum(v) for v in slabs(spec)]))
 
 
 def unit_element(spec):
@@ -463,7 +491,8 @@ class GridFETI(FETISolver):
         return out
 
     def l2g(self, i):
-        n = (self.spec["elements"][0] + 1) * (self.spec["elements"][1] + 1) * (self.spec["elements"][2] + 1)
+        ex, ey, ez = subdomain_elements(self.spec, i)
+        n = (ex + 1) * (ey + 1) * (ez + 1)
         g = np.zeros(n, dtype=np.int32)
         _check(lib().espreso_grid_l2g(self.h, i, _ip(g)))
         return g

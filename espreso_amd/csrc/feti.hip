@@ -159,6 +159,7 @@ struct espreso_feti_s {
     double fsc_bytes = 0, sd_bytes = 0, asm_flops = 0;
 
     ~espreso_feti_s() {
+        drop_graphs();
         for (void *p : allocs) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -199,6 +200,16 @@ struct espreso_feti_s {
         return upload(v);
     }
     void cluster_setup();
+    void pcg_step(double *x, double *r, int prec, bool first);
+    void pcg_step_graph(double *x, double *r, int prec);
+    void drop_graphs();
+    struct StepGraph {
+        double *x, *r;
+        int prec;
+        hipGraphExec_t exec;
+    };
+    std::vector<StepGraph> step_graphs;
+    bool use_graphs = true;
     void compute_b();
     void resolve_rhs();
     std::vector<double> Ns_h;                        // hybrid: N_s per subdomain (kdim x kN)
@@ -559,6 +570,8 @@ void espreso_feti_s::load() {
 // operators and Dirichlet blocks, coarse inverse, right-hand side b.
 void espreso_feti_s::gpu_setup() {
     ESP_CHECK(loaded, ESPRESO_ERR_STATE, "inputs not loaded");
+    drop_graphs();
+    if (const char *e = std::getenv("ESPRESO_HIP_GRAPHS")) use_graphs = std::atoi(e) != 0;
     double ta = wall();
     assemble(d_f, d_Qm, h_foff, h_lpoff, h_boff);
     last.time_assembly = wall() - ta;
@@ -868,9 +881,62 @@ void espreso_feti_s::apply_prec(const double *x, double *y) {
 // Projected PCG (itersolver.cpp:1537-1684) on device vectors.  r holds the
 // initial residual on entry.  Scalars stay on the device; the host reads
 // ||w|| once per iteration for the stop test.
+// One iteration of Solve_RegCG (itersolver.cpp:1537-1684) as a launch sequence:
+// w = P r, y = P M w, beta, p, Ap = F p, alpha, x, r, ||w||^2.
+void espreso_feti_s::pcg_step(double *x, double *r, int prec, bool first) {
+    double *s_yw = d_scal + 0, *s_yw_prev = d_scal + 1, *s_pAp = d_scal + 2, *s_ww = d_scal + 3;
+    launch_project(D, r, d_w, 0, st);
+    if (opt.preconditioner == ESPRESO_PREC_NONE) {
+        ESP_HIP(hipMemcpyAsync(d_y, d_w, nlam * sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+        apply_prec(d_w, d_z);
+        launch_project(D, d_z, d_y, 0, st);
+    }
+    if (!first) launch_copy_scalar(s_yw, s_yw_prev, st);
+    launch_dot(d_y, d_w, nlam, d_partial, s_yw, st);
+    launch_p_update(d_p, d_y, nlam, s_yw, s_yw_prev, first, st);
+    apply_F(d_p, d_Ap, prec);
+    launch_dot(d_p, d_Ap, nlam, d_partial, s_pAp, st);
+    launch_xr_update(x, r, d_p, d_Ap, nlam, s_yw, s_pAp, st);
+    launch_dot(d_w, d_w, nlam, d_partial, s_ww, st);
+}
+
+// iterations after the first replay a HIP graph of pcg_step, captured once
+// per (x, r, storage precision); graphs hold device pointers of the current
+// set-up and are dropped when the set-up is redone
+void espreso_feti_s::pcg_step_graph(double *x, double *r, int prec) {
+    for (const StepGraph &g : step_graphs)
+        if (g.x == x && g.r == r && g.prec == prec) {
+            ESP_HIP(hipGraphLaunch(g.exec, st));
+            return;
+        }
+    hipGraph_t graph = nullptr;
+    ESP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        pcg_step(x, r, prec, false);
+    } catch (...) {
+        (void)hipStreamEndCapture(st, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    ESP_HIP(hipStreamEndCapture(st, &graph));
+    StepGraph g{x, r, prec, nullptr};
+    const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    ESP_HIP(e);
+    step_graphs.push_back(g);
+    ESP_HIP(hipGraphLaunch(g.exec, st));
+}
+
+void espreso_feti_s::drop_graphs() {
+    for (StepGraph &g : step_graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    step_graphs.clear();
+}
+
 void espreso_feti_s::pcg(const double *b, double *x, double *r, int prec, int &iters, double &norm, double &tol,
                          bool use_abs, double tol_abs) {
-    double *s_yw = d_scal + 0, *s_yw_prev = d_scal + 1, *s_pAp = d_scal + 2, *s_ww = d_scal + 3, *s_tmp = d_scal + 4;
+    double *s_ww = d_scal + 3, *s_tmp = d_scal + 4;
     // tol = precision * min(||P r0||, ||b||)
     launch_project(D, r, d_w, 0, st);
     launch_dot(d_w, d_w, nlam, d_partial, s_tmp, st);
@@ -883,20 +949,8 @@ void espreso_feti_s::pcg(const double *b, double *x, double *r, int prec, int &i
     iters = 0;
     norm = 0;
     for (int it = 0; it < opt.max_iterations; it++) {
-        launch_project(D, r, d_w, 0, st);
-        if (opt.preconditioner == ESPRESO_PREC_NONE) {
-            ESP_HIP(hipMemcpyAsync(d_y, d_w, nlam * sizeof(double), hipMemcpyDeviceToDevice, st));
-        } else {
-            apply_prec(d_w, d_z);
-            launch_project(D, d_z, d_y, 0, st);
-        }
-        if (it > 0) launch_copy_scalar(s_yw, s_yw_prev, st);
-        launch_dot(d_y, d_w, nlam, d_partial, s_yw, st);
-        launch_p_update(d_p, d_y, nlam, s_yw, s_yw_prev, it == 0, st);
-        apply_F(d_p, d_Ap, prec);
-        launch_dot(d_p, d_Ap, nlam, d_partial, s_pAp, st);
-        launch_xr_update(x, r, d_p, d_Ap, nlam, s_yw, s_pAp, st);
-        launch_dot(d_w, d_w, nlam, d_partial, s_ww, st);
+        if (it == 0 || !use_graphs) pcg_step(x, r, prec, it == 0);
+        else pcg_step_graph(x, r, prec);
         double ww = 0;
         ESP_HIP(hipMemcpyAsync(&ww, s_ww, sizeof(double), hipMemcpyDeviceToHost, st));
         sync();

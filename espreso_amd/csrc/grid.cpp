@@ -36,13 +36,26 @@ struct Box {
     int NS[3], NE[3], NN[3];
     double h[3];
     int dpc;
+    std::vector<int> slab[3], off[3];  // elements per subdomain slab, prefix sums
     explicit Box(const espreso_grid_problem &pp) : p(pp) {
         nd = p.physics == 0 ? 3 : 1;
+        int q = 0;
         for (int d = 0; d < 3; d++) {
-            if (p.clusters[d] < 1 || p.domains[d] < 1 || p.elements[d] < 1)
+            if (p.clusters[d] < 1 || p.domains[d] < 1 || (!p.slab_elements && p.elements[d] < 1))
                 throw std::invalid_argument("grid: clusters, domains and elements must be >= 1");
             NS[d] = p.clusters[d] * p.domains[d];
-            NE[d] = NS[d] * p.elements[d];
+            slab[d].assign(NS[d], p.elements[d]);
+            off[d].assign(NS[d] + 1, 0);
+        }
+        if (p.slab_elements)
+            for (int d = 0; d < 3; d++)
+                for (int i = 0; i < NS[d]; i++) {
+                    slab[d][i] = p.slab_elements[q++];
+                    if (slab[d][i] < 1) throw std::invalid_argument("grid: slab element counts must be >= 1");
+                }
+        for (int d = 0; d < 3; d++) {
+            for (int i = 0; i < NS[d]; i++) off[d][i + 1] = off[d][i] + slab[d][i];
+            NE[d] = off[d][NS[d]];
             NN[d] = NE[d] + 1;
             h[d] = p.length[d] / NE[d];
         }
@@ -62,11 +75,14 @@ struct Box {
         int d[3] = {dd % p.domains[0], (dd / p.domains[0]) % p.domains[1], dd / (p.domains[0] * p.domains[1])};
         for (int q = 0; q < 3; q++) S[q] = c[q] * p.domains[q] + d[q];
     }
+    // slab holding element index e along direction d
+    int slab_of(int d, int e) const {
+        return (int)(std::upper_bound(off[d].begin(), off[d].end(), e) - off[d].begin()) - 1;
+    }
     // subdomain range owning node index g along direction d
     void owners(int d, int g, int &lo, int &hi) const {
-        int e = p.elements[d];
-        hi = std::min(g / e, NS[d] - 1);
-        lo = (g % e == 0 && g > 0) ? g / e - 1 : hi;
+        hi = std::min(slab_of(d, g), NS[d] - 1);
+        lo = (g == off[d][hi] && hi > 0) ? hi - 1 : hi;
     }
     bool dirichlet(const int *g, int comp, double &val) const {
         for (int f = 0; f < 6; f++) {
@@ -215,17 +231,17 @@ void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out) {
         SplitMix r(p.young_seed);
         for (int e = 0; e < nel; e++) escale[e] = p.young * (0.1 + 0.9 * r.next01());
     }
-    if (nd == 3 && p.young_elem)
-        for (int e = 0; e < nel; e++) escale[e] = p.young_elem[e];
     if (nd == 1 && p.conductivity_seed) {
         SplitMix r(p.conductivity_seed);
         std::vector<double> ks(ns);
         for (int s = 0; s < ns; s++) ks[s] = p.conductivity * std::pow(10.0, 2.0 * r.next01() - 1.0);
         for (int e = 0; e < nel; e++) {
             int ei = e % B.NE[0], ej = (e / B.NE[0]) % B.NE[1], ek = e / (B.NE[0] * B.NE[1]);
-            escale[e] = ks[B.id_of(ei / p.elements[0], ej / p.elements[1], ek / p.elements[2])];
+            escale[e] = ks[B.id_of(B.slab_of(0, ei), B.slab_of(1, ej), B.slab_of(2, ek))];
         }
     }
+    if (p.material_elem)
+        for (int e = 0; e < nel; e++) escale[e] = p.material_elem[e];
     std::vector<double> trac;
     if (p.traction_seed) {
         SplitMix r(p.traction_seed);
@@ -236,15 +252,15 @@ void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out) {
     unit_element(p, B.h, Ke);
     const int ne = 8 * nd;
 
-    const int lx = p.elements[0] + 1, ly = p.elements[1] + 1, lz = p.elements[2] + 1;
-    const int nloc = lx * ly * lz;
-
 #pragma omp parallel for schedule(dynamic)
     for (int s = 0; s < ns; s++) {
         GridSubdomain &sd = out.sub[s];
         int S[3];
         B.coord_of(s, S);
-        const int gx0 = S[0] * p.elements[0], gy0 = S[1] * p.elements[1], gz0 = S[2] * p.elements[2];
+        const int ex = B.slab[0][S[0]], ey = B.slab[1][S[1]], ez = B.slab[2][S[2]];
+        const int lx = ex + 1, ly = ey + 1, lz = ez + 1;
+        const int nloc = lx * ly * lz;
+        const int gx0 = B.off[0][S[0]], gy0 = B.off[1][S[1]], gz0 = B.off[2][S[2]];
         sd.n = nloc * nd;
         sd.l2g.resize(nloc);
         // neighbour position table: for node ln, slot of the 27 offsets among its existing neighbours
@@ -284,9 +300,9 @@ void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out) {
                 }
         sd.f.assign(sd.n, 0.0);
         std::vector<double> fe(ne);
-        for (int z = 0; z < p.elements[2]; z++)
-            for (int y = 0; y < p.elements[1]; y++)
-                for (int x = 0; x < p.elements[0]; x++) {
+        for (int z = 0; z < ez; z++)
+            for (int y = 0; y < ey; y++)
+                for (int x = 0; x < ex; x++) {
                     const int ei = gx0 + x, ej = gy0 + y, ek = gz0 + z;
                     const double sc = escale[ei + B.NE[0] * (ej + B.NE[1] * ek)];
                     int ln[8];
@@ -332,7 +348,8 @@ void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out) {
     auto local_dof = [&](int s, const int *g, int c) {
         int S[3];
         B.coord_of(s, S);
-        int x = g[0] - S[0] * p.elements[0], y = g[1] - S[1] * p.elements[1], z = g[2] - S[2] * p.elements[2];
+        const int lx = B.slab[0][S[0]] + 1, ly = B.slab[1][S[1]] + 1;
+        int x = g[0] - B.off[0][S[0]], y = g[1] - B.off[1][S[1]], z = g[2] - B.off[2][S[2]];
         return (x + lx * (y + ly * z)) * nd + c;
     };
     auto diag_of = [&](int s, int dof) {
@@ -416,9 +433,8 @@ void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out) {
         const int cc[3] = {cl % p.clusters[0], (cl / p.clusters[0]) % p.clusters[1], cl / (p.clusters[0] * p.clusters[1])};
         int lo3[3], hi3[3];
         for (int d = 0; d < 3; d++) {
-            const int span = p.domains[d] * p.elements[d];
-            lo3[d] = cc[d] * span;
-            hi3[d] = (cc[d] + 1) * span;
+            lo3[d] = B.off[d][cc[d] * p.domains[d]];
+            hi3[d] = B.off[d][(cc[d] + 1) * p.domains[d]];
         }
         std::vector<std::vector<int>> keys;
         std::vector<std::vector<int>> members;

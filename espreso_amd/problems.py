@@ -48,6 +48,55 @@ ADJOINT_64x10 = dict(
     young_seed=20261205,
 )
 
+# Uneven subdomains (boxes of 4..14 elements per direction, seeded per slab so
+# that neighbouring boxes stay conforming), 2000 subdomains in one cluster,
+# heat with conductivity log-uniform in [0.1, 10] per subdomain; the load
+# cases are seeded U(-1,1) nodal sources applied with resolve().
+def uneven_slabs(counts, seed, lo=4, hi=14):
+    """Seeded element counts per subdomain slab for (NSx, NSy, NSz) slabs."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return tuple(tuple(int(v) for v in rng.integers(lo, hi + 1, size=n)) for n in counts)
+
+
+HEAT_UNEVEN_2000 = dict(
+    name="heat_uneven_2000sub",
+    physics=1, clusters=(1, 1, 1), domains=(10, 10, 20), elements=(9, 9, 9),
+    slab_elements=uneven_slabs((10, 10, 20), 20261203),
+    length=(1.0, 1.0, 2.0), conductivity=1.0, conductivity_seed=20261204,
+    fixed=(0, 0, 0, 0, 1, 0), heat_source=1.0,
+)
+
+ELASTICITY_UNEVEN_512 = dict(
+    name="elasticity_uneven_512sub",
+    physics=0, clusters=(1, 1, 1), domains=(8, 8, 8), elements=(9, 9, 9),
+    slab_elements=uneven_slabs((8, 8, 8), 20261206),
+    length=(1.0, 1.0, 1.0), young=E_STEEL, poisson=0.3, density=7850.0,
+    accel=(0.0, 0.0, -9.81), fixed=(0, 0, 0, 0, 7, 0),
+)
+
+
+def nodal_load_cases(spec, n_cases, seed=20261207):
+    """Seeded U(-1,1) nodal loads, (n_cases, n_nodes, ndof)."""
+    import numpy as np
+    from .core import slabs
+    nn = [sum(v) + 1 for v in slabs(spec)]
+    nd = 3 if spec.get("physics", 0) == 0 else 1
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1.0, 1.0, size=(n_cases, nn[0] * nn[1] * nn[2], nd))
+
+
+def uniform_equivalent(spec):
+    """The same global mesh as one subdomain (for the undecomposed direct solve)."""
+    from .core import slabs
+    s = dict(spec)
+    s.pop("slab_elements", None)
+    s["clusters"] = (1, 1, 1)
+    s["domains"] = (1, 1, 1)
+    s["elements"] = tuple(sum(v) for v in slabs(spec))
+    return s
+
+
 # --- the reference's own benchmark inputs (golden .emr values) -----------
 
 E_PATCH = 2.1e11
@@ -85,8 +134,8 @@ HEAT_SIMPLE = dict(
     fixed=(1, 1, 0, 0, 0, 0), fixed_value=(100.0, 300.0, 0, 0, 0, 0),
 )
 
-ALL = {p["name"]: p for p in (TFETI_64x5, HTFETI_512x10, MIXED_1000x12, ADJOINT_64x10,
-                              PRESSURE_PATCH, GRAVITY, HEAT_SIMPLE)}
+ALL = {p["name"]: p for p in (TFETI_64x5, HTFETI_512x10, MIXED_1000x12, ADJOINT_64x10, HEAT_UNEVEN_2000,
+                              ELASTICITY_UNEVEN_512, PRESSURE_PATCH, GRAVITY, HEAT_SIMPLE)}
 
 
 def scaled(spec, **over):

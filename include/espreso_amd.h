@@ -65,8 +65,12 @@ typedef struct {
     double  traction_scale;
     uint64_t young_seed;
     uint64_t conductivity_seed;
-    const double *young_elem;   /* optional: Young's modulus per element, index
-                                   ex + NEx*(ey + NEy*ez); overrides young/young_seed */
+    const double *material_elem; /* optional, per element (index ex + NEx*(ey + NEy*ez)):
+                                    Young's modulus (elasticity) or conductivity (heat);
+                                    overrides young/young_seed/conductivity(_seed)       */
+    const int32_t *slab_elements; /* optional uneven subdomains: elements of every
+                                    subdomain slab, x slabs then y then z (NSx+NSy+NSz
+                                    values); overrides elements[]                      */
 } espreso_grid_problem;
 
 /* Values follow the reference enums (src/config/ecf/solver/feti.h). */

@@ -181,8 +181,6 @@ Loads make_loads(const Grid &G) {
         Rng r(G.p.young_seed);
         for (int e = 0; e < nel; e++) L.elem_scale[e] = G.p.young * (0.1 + 0.9 * r.uniform());
     }
-    if (G.p.physics == 0 && G.p.young_elem)
-        for (int e = 0; e < nel; e++) L.elem_scale[e] = G.p.young_elem[e];
     if (G.p.physics == 1 && G.p.conductivity_seed) {
         Rng r(G.p.conductivity_seed);
         std::vector<double> ks(G.nsub);
@@ -194,6 +192,8 @@ Loads make_loads(const Grid &G) {
                     L.elem_scale[i + G.NE[0] * (j + G.NE[1] * k)] = ks[G.subid(S)];
                 }
     }
+    if (G.p.material_elem)
+        for (int e = 0; e < nel; e++) L.elem_scale[e] = G.p.material_elem[e];
     if (G.p.traction_seed) {
         Rng r(G.p.traction_seed);
         int nf = G.NN[1] * G.NN[2];
@@ -1057,7 +1057,8 @@ struct Oracle {
 };
 
 // ------------------------------------------------- global direct solve ----
-int global_solve(const oracle_grid_problem &pp, double *u_nodal, double *rel_res_check, const double *u_in) {
+int global_solve(const oracle_grid_problem &pp, double *u_nodal, double *rel_res_check, const double *u_in,
+                 const double *f_nodal = nullptr) {
     Grid G(pp);
     Loads L = make_loads(G);
     std::vector<double> Ke1, Nint;
@@ -1089,6 +1090,7 @@ int global_solve(const oracle_grid_problem &pp, double *u_nodal, double *rel_res
     Profile P;
     P.init(fst);
     std::vector<double> f(n, 0.0);
+    if (f_nodal) std::copy(f_nodal, f_nodal + n, f.begin());
     int ne = 8 * nd;
     std::vector<double> fe(ne);
     // residual mode: y = K u - f on free rows
@@ -1101,10 +1103,10 @@ int global_solve(const oracle_grid_problem &pp, double *u_nodal, double *rel_res
                 int dofs[24];
                 for (int v = 0; v < 8; v++)
                     for (int d = 0; d < nd; d++) dofs[v * nd + d] = G.node(i + HEX[v][0], j + HEX[v][1], k + HEX[v][2]) * nd + d;
-                element_rhs(G, L, Nint, i, j, k, fe.data());
+                if (!f_nodal) element_rhs(G, L, Nint, i, j, k, fe.data());
                 for (int r = 0; r < ne; r++) {
                     int row = dofs[r];
-                    f[row] += fe[r];
+                    if (!f_nodal) f[row] += fe[r];
                     for (int c = 0; c < ne; c++) {
                         int col = dofs[c];
                         double kv = sc * Ke1[r * ne + c];
@@ -1158,6 +1160,10 @@ int oracle_unit_element(const oracle_grid_problem *p, double *Ke) {
 
 int oracle_global_solve(const oracle_grid_problem *p, double *u_nodal) {
     return global_solve(*p, u_nodal, nullptr, nullptr);
+}
+
+int oracle_global_solve_load(const oracle_grid_problem *p, const double *f_nodal, double *u_nodal) {
+    return global_solve(*p, u_nodal, nullptr, nullptr, f_nodal);
 }
 
 int oracle_global_residual(const oracle_grid_problem *p, const double *u_nodal, double *rel_res) {

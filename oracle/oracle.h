@@ -56,7 +56,8 @@ typedef struct {
     double  traction_scale;
     uint64_t young_seed;        /* !=0: E_e = young*(0.1+0.9 U(0,1)) per element */
     uint64_t conductivity_seed; /* !=0: k_s = conductivity*10^U(-1,1) per subdomain */
-    const double *young_elem;   /* optional Young's modulus per element (overrides) */
+    const double *material_elem; /* optional per element: Young's modulus (elasticity) or
+                                    conductivity (heat); overrides the seeded fields      */
 } oracle_grid_problem;
 
 typedef struct {
@@ -87,6 +88,8 @@ typedef struct {
 /* ---- undecomposed ground truth ---------------------------------------- */
 /* nodal solution u[node*ndof + c], node = i + Nx*(j + Ny*k)               */
 int oracle_global_solve(const oracle_grid_problem *p, double *u_nodal);
+/* same with the nodal load f_nodal (n_nodes x ndof) in place of the element loads */
+int oracle_global_solve_load(const oracle_grid_problem *p, const double *f_nodal, double *u_nodal);
 /* unit-modulus (E = 1 / conductivity 1) element matrix, row-major, node-major DOFs */
 int oracle_unit_element(const oracle_grid_problem *p, double *Ke);
 int oracle_global_sizes(const oracle_grid_problem *p, int32_t *n_nodes, int32_t *ndof);

@@ -33,7 +33,7 @@ class OracleProblem(C.Structure):
         ("traction_scale", C.c_double),
         ("young_seed", C.c_uint64),
         ("conductivity_seed", C.c_uint64),
-        ("young_elem", C.POINTER(C.c_double)),
+        ("material_elem", C.POINTER(C.c_double)),
     ]
 
 
@@ -94,6 +94,7 @@ def lib():
         _lib.oracle_solve.argtypes = [C.c_void_p, d, d, C.POINTER(OracleStats)]
         _lib.oracle_global_solve.argtypes = [C.POINTER(OracleProblem), d]
         _lib.oracle_unit_element.argtypes = [C.POINTER(OracleProblem), d]
+        _lib.oracle_global_solve_load.argtypes = [C.POINTER(OracleProblem), d, d]
         _lib.oracle_global_sizes.argtypes = [C.POINTER(OracleProblem), i32, i32]
         _lib.oracle_global_residual.argtypes = [C.POINTER(OracleProblem), d, d]
         _lib.oracle_cpu_sample.argtypes = [C.POINTER(OracleProblem), C.POINTER(OracleOptions), C.c_int32, d]
@@ -129,11 +130,11 @@ def make_problem(spec):
     p.traction_scale = spec.get("traction_scale", 1.0)
     p.young_seed = spec.get("young_seed", 0)
     p.conductivity_seed = spec.get("conductivity_seed", 0)
-    ye = spec.get("young_elem")
+    ye = spec.get("material_elem")
     if ye is not None:
         ye = np.ascontiguousarray(ye, dtype=np.float64)
         p._keep = ye  # the array must outlive the struct
-        p.young_elem = _dp(ye)
+        p.material_elem = _dp(ye)
     return p
 
 
@@ -178,6 +179,21 @@ def global_solve(spec):
     lib().oracle_global_sizes(C.byref(p), C.byref(nn), C.byref(nd))
     u = np.zeros(nn.value * nd.value)
     rc = lib().oracle_global_solve(C.byref(p), _dp(u))
+    if rc:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return u.reshape(nn.value, nd.value)
+
+
+def global_solve_load(spec, f_nodal):
+    """Undecomposed direct solve with a nodal load (n_nodes x ndof)."""
+    p = make_problem(spec)
+    nn = C.c_int32()
+    nd = C.c_int32()
+    lib().oracle_global_sizes(C.byref(p), C.byref(nn), C.byref(nd))
+    f = np.ascontiguousarray(f_nodal, dtype=np.float64).reshape(-1)
+    assert f.size == nn.value * nd.value
+    u = np.zeros(nn.value * nd.value)
+    rc = lib().oracle_global_solve_load(C.byref(p), _dp(f), _dp(u))
     if rc:
         raise RuntimeError(lib().oracle_last_error().decode())
     return u.reshape(nn.value, nd.value)

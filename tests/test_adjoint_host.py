@@ -30,8 +30,8 @@ def test_unit_element_matches_gauss():
         assert np.abs(Kp - Ko).max() <= 1e-12 * np.abs(Ko).max()
 
 
-def test_young_elem_input_matches_oracle():
-    spec = dict(TINY, young_elem=_young(TINY))
+def test_material_elem_input_matches_oracle():
+    spec = dict(TINY, material_elem=_young(TINY))
     spec.pop("young_seed")
     o = O.Oracle(spec, preconditioner=0)
     for s in (0, 5):
@@ -57,16 +57,16 @@ def _compliance(spec):
 
 
 def test_reference_gradient_euler_relation():
-    spec = dict(TINY, young_elem=_young(TINY))
+    spec = dict(TINY, material_elem=_young(TINY))
     g, u = O.compliance_gradient(spec)
     c = _compliance(spec)
     assert c > 0
-    assert abs(spec["young_elem"] @ g + c) <= 1e-10 * c
+    assert abs(spec["material_elem"] @ g + c) <= 1e-10 * c
 
 
 def test_reference_gradient_finite_differences():
     E = _young(TINY)
-    spec = dict(TINY, young_elem=E)
+    spec = dict(TINY, material_elem=E)
     g, _ = O.compliance_gradient(spec)
     rng = np.random.default_rng(5)
     for e in rng.choice(len(E), 6, replace=False):
@@ -74,5 +74,5 @@ def test_reference_gradient_finite_differences():
         Ep, Em = E.copy(), E.copy()
         Ep[e] += h
         Em[e] -= h
-        fd = (_compliance(dict(TINY, young_elem=Ep)) - _compliance(dict(TINY, young_elem=Em))) / (2 * h)
+        fd = (_compliance(dict(TINY, material_elem=Ep)) - _compliance(dict(TINY, material_elem=Em))) / (2 * h)
         assert abs(fd - g[e]) <= 1e-6 * abs(g[e]), (e, fd, g[e])

@@ -45,7 +45,7 @@ def grad_of(spec, E, **opts):
 
 
 def compliance(spec, E, **opts):
-    g = core.GridFETI(dict(spec, young_elem=E), **dict(OPTS, **opts))
+    g = core.GridFETI(dict(spec, material_elem=E), **dict(OPTS, **opts))
     g.setup()
     g.solve()
     c = g.load_work()
@@ -87,7 +87,7 @@ def test_resolve_matches_fresh_solve():
 def test_gradient_vs_oracle(method):
     say("case: gradient vs oracle", method)
     E = young(SMALL)
-    spec = dict(SMALL, young_elem=E)
+    spec = dict(SMALL, material_elem=E)
     spec.pop("young_seed")
     c, g = grad_of(spec, E, method=method)
     gref, _ = O.compliance_gradient(spec)
@@ -100,7 +100,7 @@ def test_gradient_vs_oracle(method):
 def test_gradient_fp32_schur():
     say("case: gradient with FP32 F_i")
     E = young(SMALL)
-    spec = dict(SMALL, young_elem=E)
+    spec = dict(SMALL, material_elem=E)
     _, g64 = grad_of(spec, E)
     _, g32 = grad_of(spec, E, sc_precision="fp32")
     err = np.abs(g32 - g64).max() / np.abs(g64).max()
@@ -111,7 +111,7 @@ def test_gradient_fp32_schur():
 def test_gradient_finite_differences_40cube():
     spec = PB.ADJOINT_64x10
     E = young(spec, seed=int(spec["young_seed"] % 1000))
-    spec = dict(spec, young_elem=E)
+    spec = dict(spec, material_elem=E)
     spec.pop("young_seed")
     say("case: 40^3 gradient vs central differences")
     c, g = grad_of(spec, E)
