@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--apply-reps", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rhs", type=int, default=0,
+                    help="also solve this many seeded U(-1,1) nodal load cases on the assembled operators")
     ap.add_argument("--profile-symv", action="store_true",
                     help="only: one setup, --apply-reps F applies and a 1 GiB copy (counter calibration)")
     args = ap.parse_args()
@@ -120,6 +122,22 @@ def main():
     st = stats[-1]
     tts = t_all / args.steps
 
+    rhs_report = None
+    if args.rhs > 0:
+        loads = PB.nodal_load_cases(spec, args.rhs)
+        its = []
+        t_r = time.perf_counter()
+        for q in range(args.rhs):
+            g.set_nodal_rhs(loads[q])
+            its.append(g.resolve()["iterations"])
+        t_r = time.perf_counter() - t_r
+        rhs_report = {"cases": args.rhs, "s_per_case": round(t_r / args.rhs, 6),
+                      "iterations": [int(v) for v in its],
+                      "note": "load cases solved one after another with espreso_feti_resolve on the assembled "
+                              "operators (b, e, interior band solves, PCG, primal per case)"}
+        log(f"bench: {args.rhs} load cases in {t_r:.3f} s ({t_r / args.rhs * 1e3:.1f} ms per case), "
+            f"iterations {min(its)}..{max(its)}")
+
     # dual operator applies on resident vectors, SYMV kernel timed with HIP events
     ms_F, ms_symv = g.time_apply(0, args.apply_reps)
     ms_M, ms_symv_sd = g.time_apply(1, args.apply_reps)
@@ -151,13 +169,14 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.sc == "fp64" else f"f64+{args.sc}",
-        "data": "synthetic (seeded structured HEXA8 elasticity box)",
+        "data": "synthetic (seeded structured HEXA8 box)",
         "config": {
             "workload": args.workload,
             "method": "HYBRID_FETI" if method == "hybrid" else "TOTAL_FETI",
             "preconditioner": args.preconditioner,
             "subdomains": g.n_sub,
-            "elements_per_subdomain": list(spec["elements"]),
+            "elements_per_subdomain": (list(spec["elements"]) if spec.get("slab_elements") is None
+                                       else "uneven: " + " / ".join(f"{min(v)}..{max(v)}" for v in spec["slab_elements"])),
             "lambda": g.n_lambda,
             "coarse": st["n_coarse"],
             "sc_storage": args.sc,
@@ -190,6 +209,8 @@ def main():
             "seconds": round(st["time_assembly"], 4),
         },
     }
+    if rhs_report:
+        result["load_cases"] = rhs_report
     g.close()
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

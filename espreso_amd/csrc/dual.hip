@@ -10,8 +10,9 @@
 // contiguous range of strips of one subdomain; B_i^T x is gathered straight
 // from lambda space into LDS, row and column contributions accumulate in LDS,
 // and each workgroup writes one partial vector that a second pass sums and
-// scatters back to lambda space in a fixed order (deterministic, no global
-// atomics).
+// scatters back to lambda space in a fixed order (no global atomics; the LDS
+// atomics of the waves inside a workgroup have no fixed order, so results can
+// differ by rounding from run to run).
 #include <hip/hip_runtime.h>
 
 #include "dual.h"
