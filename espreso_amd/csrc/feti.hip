@@ -209,6 +209,8 @@ struct espreso_feti_s {
         hipGraphExec_t exec;
     };
     std::vector<StepGraph> step_graphs;
+    double *asm_pool = nullptr;  // assembly workspace, reused by later set-ups
+    size_t asm_pool_n = 0;
     bool use_graphs = true;
     void compute_b();
     void resolve_rhs();
@@ -637,8 +639,9 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
     };
     size_t freeb = 0, totalb = 0;
     ESP_HIP(hipMemGetInfo(&freeb, &totalb));
-    double budget = opt.memory_budget_gb > 0 ? opt.memory_budget_gb * 1e9 : 0.85 * (double)freeb;
-    budget = std::min(budget, 0.85 * (double)freeb);
+    const double avail = (double)freeb + 8.0 * (double)asm_pool_n;  // the kept workspace is reusable
+    double budget = opt.memory_budget_gb > 0 ? opt.memory_budget_gb * 1e9 : 0.85 * avail;
+    budget = std::min(budget, 0.85 * avail);
     int s0 = 0;
     asm_flops = 0;
     while (s0 < nsub) {
@@ -647,7 +650,17 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         while (s1 < nsub && (s1 == s0 || bytes + 8.0 * ws(sub[s1]) <= budget)) bytes += 8.0 * ws(sub[s1++]);
         ESP_CHECK(bytes <= budget, ESPRESO_ERR_MEMORY, "one subdomain's assembly workspace exceeds the device budget");
         const int nbt = s1 - s0;
-        double *pool = alloc<double>((size_t)(bytes / 8));
+        // the workspace is kept between set-ups (repeated multi-GB hipMalloc/hipFree
+        // of it costs more than the assembly itself on some runs)
+        const size_t need = (size_t)(bytes / 8) + 1;
+        if (asm_pool_n < need) {
+            if (asm_pool) release(asm_pool);
+            asm_pool = nullptr;
+            asm_pool_n = 0;
+            asm_pool = alloc<double>(need);
+            asm_pool_n = need;
+        }
+        double *pool = asm_pool;
         std::vector<DenseDesc> desc(nbt);
         std::vector<int> cs_all;
         std::vector<int64_t> cs_off(nbt);
@@ -764,7 +777,7 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
             ESP_CHECK(info[b] == 0, ESPRESO_ERR_NUMERIC,
                       "dense factorisation breakdown in subdomain " + std::to_string(s0 + b) + " at column " +
                           std::to_string(info[b]));
-        for (void *p : {(void *)pool, (void *)d_cs, (void *)d_info, (void *)d_desc, (void *)d_rp_off, (void *)d_nz_off,
+        for (void *p : {(void *)d_cs, (void *)d_info, (void *)d_desc, (void *)d_rp_off, (void *)d_nz_off,
                         (void *)d_n_off, (void *)d_b_off, (void *)d_q_off, (void *)d_f_off, (void *)d_lp_off,
                         (void *)d_band_off, (void *)d_yf_off, (void *)d_nloc, (void *)d_kd, (void *)d_bw,
                         (void *)d_rho})
