@@ -45,7 +45,8 @@ def cpu_baseline(spec, iterations, n_sub, threads, sample):
         "sample": (f"{sample} of {n_sub} subdomains timed on the host: fix-point-regularised K_i profile "
                    f"Cholesky + Dirichlet S_D (setup), implicit K_i^+ solve + S_D GEMV (per PCG iteration); "
                    f"scaled to {n_sub} subdomains and the GPU run's {iterations} iterations "
-                   f"({wall:.1f} s of host wall time)"),
+                   f"({wall:.1f} s of host wall time); the hybrid cluster problems, projector and vector "
+                   f"updates are not included"),
         "setup_s": setup,
         "per_iteration_s": per_iter,
     }
@@ -62,7 +63,7 @@ def main():
     ap.add_argument("--method", default=None, choices=["total", "hybrid"],
                     help="FETI method (default: the workload's own, hybrid for the 512- and 1000-subdomain cases)")
     ap.add_argument("--apply-reps", type=int, default=20)
-    ap.add_argument("--cpu-sample", type=int, default=32)
+    ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rhs", type=int, default=0,
                     help="also solve this many seeded U(-1,1) nodal load cases on the assembled operators")
