@@ -34,7 +34,7 @@ def run(spec, opts, graphs):
     try:
         g = core.GridFETI(spec, precision=1e-10, **opts)
         out = []
-        for rep in range(2):  # second round replays the graphs captured in the first
+        for rep in range(2):  # the second pass replays the graphs captured in the first
             g.setup()
             st = g.solve()
             out.append((st["iterations"], g.dual().copy(), g.nodal_solution().copy()))
