@@ -59,6 +59,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="elasticity_512sub_10e3")
     ap.add_argument("--sc", default="fp64", choices=["fp64", "fp32", "bf16"])
+    ap.add_argument("--sd", default="fp64", choices=["fp64", "fp32", "bf16"],
+                    help="storage of the Dirichlet preconditioner S_D (fp64 or the same as --sc)")
     ap.add_argument("--preconditioner", default="dirichlet")
     ap.add_argument("--method", default=None, choices=["total", "hybrid"],
                     help="FETI method (default: the workload's own, hybrid for the 512- and 1000-subdomain cases)")
@@ -84,7 +86,7 @@ def main():
     log(f"bench: workload {args.workload}, method {method}, F_i storage {args.sc}, loading inputs")
     t0 = time.perf_counter()
     g = core.GridFETI(spec, preconditioner=args.preconditioner, precision=1e-8, sc_precision=args.sc,
-                      max_iterations=1000, method=method)
+                      max_iterations=1000, method=method, sd_precision=args.sd)
     t_build = time.perf_counter() - t0
     log(f"bench: problem built on the host in {t_build:.1f} s; {g.n_sub} subdomains, {g.n_lambda} multipliers")
 
@@ -169,7 +171,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.sc == "fp64" else f"f64+{args.sc}",
+        "dtype": "f64" if args.sc == "fp64" and args.sd == "fp64" else f"f64 (F_i {args.sc}, S_D {args.sd})",
         "data": "synthetic (seeded structured HEXA8 box)",
         "config": {
             "workload": args.workload,
@@ -181,6 +183,7 @@ def main():
             "lambda": g.n_lambda,
             "coarse": st["n_coarse"],
             "sc_storage": args.sc,
+            "sd_storage": args.sd,
             "precision": 1e-8,
         },
         "iterations": st["iterations"],

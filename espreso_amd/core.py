@@ -60,6 +60,8 @@ class Options(C.Structure):
         ("device", C.c_int32),
         ("precision", C.c_double),
         ("memory_budget_gb", C.c_double),
+        ("sd_precision", C.c_int32),
+        ("pad_", C.c_int32),
     ]
 
 
@@ -267,7 +269,7 @@ METHOD = {"total": 0, "TOTAL_FETI": 0, "hybrid": 1, "HYBRID_FETI": 1}
 
 
 def make_options(preconditioner="dirichlet", precision=1e-8, max_iterations=500, scaling=1, sc_precision="fp64",
-                 device=0, memory_budget_gb=0.0, method="total"):
+                 device=0, memory_budget_gb=0.0, method="total", sd_precision="fp64"):
     o = Options()
     o.method = METHOD[method] if isinstance(method, str) else int(method)
     o.preconditioner = PREC[preconditioner] if isinstance(preconditioner, str) else int(preconditioner)
@@ -277,6 +279,7 @@ def make_options(preconditioner="dirichlet", precision=1e-8, max_iterations=500,
     o.device = device
     o.precision = precision
     o.memory_budget_gb = memory_budget_gb
+    o.sd_precision = SC[sd_precision] if isinstance(sd_precision, str) else int(sd_precision)
     return o
 
 

@@ -47,7 +47,11 @@ struct DualDev {
     float *Ff = nullptr;             // packed F_i, FP32
     uint16_t *Fh = nullptr;          // packed F_i, BF16
     int64_t *foff_lp = nullptr;      // offsets of the low-precision copy
-    double *SD = nullptr;
+    double *SD = nullptr;            // packed S_D, FP64 (sd_prec 0)
+    float *SDf = nullptr;            // packed S_D, FP32 (sd_prec 1), low-precision geometry
+    uint16_t *SDh = nullptr;         // packed S_D, BF16 (sd_prec 2), scaled by sdscale
+    float *sdscale = nullptr;
+    int sd_prec = 0;
     double *GGinv = nullptr;
     // SYMV work lists: FP64 geometry (F_i and S_D) and low-precision geometry
     SymvItem *items_d = nullptr;  int n_items_d = 0;  int *item0_d = nullptr;  int *nitems_d = nullptr;

@@ -594,7 +594,7 @@ static inline unsigned vgrid(int64_t n) { return (unsigned)std::max<int64_t>(1, 
 
 void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gather, bool scaled, hipStream_t st,
                  hipEvent_t ev0, hipEvent_t ev1) {
-    if (sd) prec = 0;
+    if (sd) prec = D.sd_prec;
     const bool lp = prec != 0;
     const SymvItem *items = lp ? D.items_lp : D.items_d;
     const int n_items = lp ? D.n_items_lp : D.n_items_d;
@@ -607,11 +607,13 @@ void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gathe
         hipLaunchKernelGGL((k_symv<double, STRIP_F64>), dim3(n_items), dim3(256), shm, st, items, D.m, D.boff, pool,
                            off, (const float *)nullptr, D.bl_ptr, D.bl_lam, coef, x, gather, D.ypart);
     } else if (prec == 1) {
-        hipLaunchKernelGGL((k_symv<float, STRIP_F32>), dim3(n_items), dim3(256), shm, st, items, D.m, D.boff, D.Ff,
-                           D.foff_lp, (const float *)nullptr, D.bl_ptr, D.bl_lam, coef, x, gather, D.ypart);
+        hipLaunchKernelGGL((k_symv<float, STRIP_F32>), dim3(n_items), dim3(256), shm, st, items, D.m, D.boff,
+                           sd ? D.SDf : D.Ff, D.foff_lp, (const float *)nullptr, D.bl_ptr, D.bl_lam, coef, x, gather,
+                           D.ypart);
     } else {
-        hipLaunchKernelGGL((k_symv<uint16_t, STRIP_B16>), dim3(n_items), dim3(256), shm, st, items, D.m, D.boff, D.Fh,
-                           D.foff_lp, (const float *)D.fscale, D.bl_ptr, D.bl_lam, coef, x, gather, D.ypart);
+        hipLaunchKernelGGL((k_symv<uint16_t, STRIP_B16>), dim3(n_items), dim3(256), shm, st, items, D.m, D.boff,
+                           sd ? D.SDh : D.Fh, D.foff_lp, (const float *)(sd ? D.sdscale : D.fscale), D.bl_ptr,
+                           D.bl_lam, coef, x, gather, D.ypart);
     }
     if (ev1) (void)hipEventRecord(ev1, st);
     const int R = strip_rows(prec);

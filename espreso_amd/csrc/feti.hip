@@ -211,6 +211,7 @@ struct espreso_feti_s {
     std::vector<StepGraph> step_graphs;
     double *asm_pool = nullptr;  // assembly workspace, reused by later set-ups
     size_t asm_pool_n = 0;
+    std::vector<int> wave_plan;  // end subdomain of each assembly wave
     bool use_graphs = true;
     void compute_b();
     void resolve_rhs();
@@ -443,19 +444,25 @@ void espreso_feti_s::load() {
         foff[s + 1] = foff[s] + strip_size(sub[s].m, STRIP_F64);
         lpoff[s + 1] = lpoff[s] + (prec ? strip_size(sub[s].m, Rlp) : 0);
     }
+    ESP_CHECK(opt.sd_precision == 0 || opt.sd_precision == prec, ESPRESO_ERR_ARG,
+              "sd_precision must be FP64 or equal to sc_precision");
+    D.sd_prec = opt.sd_precision;
     D.Fd = alloc<double>(foff[nsub]);
-    D.SD = alloc<double>(foff[nsub]);
     D.foff = upload(foff);
     D.sdoff = D.foff;
     fsc_bytes = (double)foff[nsub] * 8;
-    sd_bytes = (double)foff[nsub] * 8;
     if (prec == 1) { D.Ff = alloc<float>(lpoff[nsub]); fsc_bytes = (double)lpoff[nsub] * 4; }
     if (prec == 2) { D.Fh = alloc<uint16_t>(lpoff[nsub]); fsc_bytes = (double)lpoff[nsub] * 2; }
+    if (D.sd_prec == 0) { D.SD = alloc<double>(foff[nsub]); sd_bytes = (double)foff[nsub] * 8; }
+    if (D.sd_prec == 1) { D.SDf = alloc<float>(lpoff[nsub]); sd_bytes = (double)lpoff[nsub] * 4; }
+    if (D.sd_prec == 2) { D.SDh = alloc<uint16_t>(lpoff[nsub]); sd_bytes = (double)lpoff[nsub] * 2; }
     D.foff_lp = upload(lpoff);
     D.fscale = alloc<float>(nsub);
+    D.sdscale = alloc<float>(nsub);
     {
         std::vector<float> ones(nsub, 1.0f);
         ESP_HIP(hipMemcpyAsync(D.fscale, ones.data(), nsub * sizeof(float), hipMemcpyHostToDevice, st));
+        ESP_HIP(hipMemcpyAsync(D.sdscale, ones.data(), nsub * sizeof(float), hipMemcpyHostToDevice, st));
     }
 
     // primal support
@@ -561,7 +568,7 @@ void espreso_feti_s::load() {
             yb += 8 * m;
         }
         last.symv_bytes = mat + gat + yb;
-        last.symv_bytes_sd = mat64 + gat + yb;
+        last.symv_bytes_sd = (D.sd_prec ? mat : mat64) + gat + yb;
     }
     sync();
     loaded = true;
@@ -637,29 +644,32 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         const int64_t nt = std::max((h.ni + TILE - 1) / TILE, (h.m + TILE - 1) / TILE);
         return (int64_t)h.ni * h.ni + (int64_t)h.ni * (h.m + 1) + 2 * (int64_t)h.m * h.m + nt * TILE * TILE;
     };
-    size_t freeb = 0, totalb = 0;
-    ESP_HIP(hipMemGetInfo(&freeb, &totalb));
-    const double avail = (double)freeb + 8.0 * (double)asm_pool_n;  // the kept workspace is reusable
-    double budget = opt.memory_budget_gb > 0 ? opt.memory_budget_gb * 1e9 : 0.85 * avail;
-    budget = std::min(budget, 0.85 * avail);
+    // wave plan and workspace: planned once from the free memory at the first
+    // set-up and kept (re-planning against a changed free size, and reallocating
+    // a multi-GB workspace, cost more than the assembly on some runs)
+    if (wave_plan.empty()) {
+        size_t freeb = 0, totalb = 0;
+        ESP_HIP(hipMemGetInfo(&freeb, &totalb));
+        double budget = opt.memory_budget_gb > 0 ? opt.memory_budget_gb * 1e9 : 0.85 * (double)freeb;
+        budget = std::min(budget, 0.85 * (double)freeb);
+        size_t most = 0;
+        for (int a = 0; a < nsub;) {
+            int b = a;
+            double bytes = 0;
+            while (b < nsub && (b == a || bytes + 8.0 * ws(sub[b]) <= budget)) bytes += 8.0 * ws(sub[b++]);
+            ESP_CHECK(bytes <= budget, ESPRESO_ERR_MEMORY,
+                      "one subdomain's assembly workspace exceeds the device budget");
+            wave_plan.push_back(b);
+            most = std::max(most, (size_t)(bytes / 8) + 1);
+            a = b;
+        }
+        asm_pool = alloc<double>(most);
+        asm_pool_n = most;
+    }
     int s0 = 0;
     asm_flops = 0;
-    while (s0 < nsub) {
-        int s1 = s0;
-        double bytes = 0;
-        while (s1 < nsub && (s1 == s0 || bytes + 8.0 * ws(sub[s1]) <= budget)) bytes += 8.0 * ws(sub[s1++]);
-        ESP_CHECK(bytes <= budget, ESPRESO_ERR_MEMORY, "one subdomain's assembly workspace exceeds the device budget");
+    for (const int s1 : wave_plan) {
         const int nbt = s1 - s0;
-        // the workspace is kept between set-ups (repeated multi-GB hipMalloc/hipFree
-        // of it costs more than the assembly itself on some runs)
-        const size_t need = (size_t)(bytes / 8) + 1;
-        if (asm_pool_n < need) {
-            if (asm_pool) release(asm_pool);
-            asm_pool = nullptr;
-            asm_pool_n = 0;
-            asm_pool = alloc<double>(need);
-            asm_pool_n = need;
-        }
         double *pool = asm_pool;
         std::vector<DenseDesc> desc(nbt);
         std::vector<int> cs_all;
@@ -741,7 +751,9 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         launch_syrk(d_desc, nbt, 0, max_ntm, st);
         launch_hf(d_desc, nbt, d_fb, d_hf, d_b_off, max_m, st);
         launch_band_extract(d_desc, nbt, P.band, d_band_off, d_bw, P.yf, d_yf_off, max_ni, st);
-        launch_pack(d_desc, nbt, 0, D.SD, d_f_off, nullptr, max_m, st);
+        if (D.sd_prec == 0) launch_pack(d_desc, nbt, 0, D.SD, d_f_off, nullptr, max_m, st);
+        if (D.sd_prec == 1) launch_pack(d_desc, nbt, 1, D.SDf, d_lp_off, nullptr, max_m, st);
+        if (D.sd_prec == 2) launch_pack(d_desc, nbt, 2, D.SDh, d_lp_off, D.sdscale + s0, max_m, st);
         launch_kernel_reg(d_desc, nbt, d_Q, d_q_off, d_kd, d_rho, max_m, st);
         for (int k = 0; k < max_ntm; k++) launch_potrf_step(d_desc, nbt, 1, k, max_ntm, st);
         for (int k = 0; k < max_ntm; k++) launch_trsm_step(d_desc, nbt, 1, k, max_ntm, max_ntm, st);
@@ -1632,8 +1644,18 @@ int espreso_feti_subdomain_dense(espreso_feti *h, int32_t i, int32_t which, doub
     std::vector<int64_t> off(2);
     ESP_HIP(hipMemcpy(off.data(), h->D.foff + i, 2 * sizeof(int64_t), hipMemcpyDeviceToHost));
     if (which == 0) launch_unpack(h->D.Fd, 0, off[0], m, 1.0f, tmp, h->st);
-    else if (which == 1) launch_unpack(h->D.SD, 0, off[0], m, 1.0f, tmp, h->st);
-    else {
+    else if (which == 1) {
+        const int sp = h->D.sd_prec;
+        std::vector<int64_t> lo(1, 0);
+        float sc = 1.0f;
+        if (sp) {
+            ESP_HIP(hipMemcpy(lo.data(), h->D.foff_lp + i, sizeof(int64_t), hipMemcpyDeviceToHost));
+            ESP_HIP(hipMemcpy(&sc, h->D.sdscale + i, sizeof(float), hipMemcpyDeviceToHost));
+        }
+        if (sp == 0) launch_unpack(h->D.SD, 0, off[0], m, 1.0f, tmp, h->st);
+        else if (sp == 1) launch_unpack(h->D.SDf, 1, lo[0], m, 1.0f, tmp, h->st);
+        else launch_unpack(h->D.SDh, 2, lo[0], m, sc, tmp, h->st);
+    } else {
         const int prec = h->opt.sc_precision;
         std::vector<int64_t> lo(1);
         float sc = 1.0f;

@@ -86,6 +86,9 @@ typedef struct {
     int32_t device;             /* HIP device ordinal                          */
     double  precision;          /* stop: ||P r|| < precision*min(||P r0||,||b||) */
     double  memory_budget_gb;   /* 0 = 85% of free device memory              */
+    int32_t sd_precision;       /* storage of the Dirichlet S_D: ESPRESO_SC_FP64, or
+                                   the same low precision as sc_precision        */
+    int32_t pad_;
 } espreso_feti_options;
 
 typedef struct {

@@ -173,3 +173,21 @@ def test_invalid_input_is_reported():
         s.set_subdomain(0, (rp, np.array([0], np.int32), np.array([np.nan])), np.zeros(1), np.ones((1, 1)),
                         [0], [0], [1.0], [1.0])
     s.close()
+
+
+@pytest.mark.parametrize("sc", ["fp32", "bf16"])
+def test_low_precision_dirichlet(sc):
+    say("case: S_D stored in", sc)
+    g = core.GridFETI(MULTI, preconditioner="dirichlet", precision=1e-10, sc_precision=sc, sd_precision=sc)
+    g.setup()
+    st = g.solve()
+    err = rel(g.nodal_solution(), O.global_solve(MULTI))
+    say("low-precision S_D", sc, "err", err, "iters", st["iterations"])
+    assert err < 1e-8
+    o = O.Oracle(MULTI, preconditioner=3, kplus=1)
+    tol = 2 ** -23 if sc == "fp32" else 2 ** -8
+    for s in (0, o.n_sub - 1):
+        Sd = o.dense(s, 3)
+        assert np.abs(g.dense(s, 1) - Sd).max() <= 2 * tol * np.abs(Sd).max()
+    o.close()
+    g.close()
