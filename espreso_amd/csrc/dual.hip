@@ -42,8 +42,11 @@ __device__ __forceinline__ A wave_rows_reduce(A *v, int lane) {
         const int h = n >> 1;
 #pragma unroll
         for (int i = 0; i < h; i++) {
-            A send = up ? v[i] : v[i + h];
-            A keep = up ? v[i + h] : v[i];
+            // select values, not addresses: a conditional over the two array
+            // elements would index the array dynamically and push it to scratch
+            const A lo = v[i], hi = v[i + h];
+            const A send = up ? (A)lo : (A)hi;
+            const A keep = up ? (A)hi : (A)lo;
             v[i] = keep + __shfl_xor(send, o, 64);
         }
         n = h;
@@ -89,7 +92,7 @@ __global__ void __launch_bounds__(256) k_symv(const SymvItem *__restrict__ items
         const int ncol = m - r0;
         const T *S = base + strip_offset(m, R, t);
         // FP64 keeps the strip-row x in registers; the low-precision strips
-        // (32/64 rows) read it from LDS to stay below 168 VGPRs
+        // (32/64 rows) read it from LDS to stay at or below 128 VGPRs
         constexpr bool XREG = sizeof(T) == 8;
         A racc[R], xr[XREG ? R : 1];
 #pragma unroll
