@@ -518,6 +518,7 @@ _F4_SIGS = {
     "FETI4IGetLastError": (C.c_char_p, []),
     "FETI4IInstanceInfo": (C.c_int, [_H, _I]),
     "FETI4IExportSubdomain": (C.c_int, [_H, C.c_int32, _I, _I, _I, _I, _D, _D, _D, _I, _I, _D, _D]),
+    "FETI4IExportCorners": (C.c_int, [_H, C.c_int32, _I, _I, _I, _D]),
     "FETI4IGetStats": (C.c_int, [_H, _D]),
 }
 _f4_ready = False
@@ -545,7 +546,7 @@ class FETI4I:
     """Thin Python view of the FETI4I C API (include/feti4i.h)."""
 
     def __init__(self, elements, size, rhs, dirichlet=(), subdomains=4, preconditioner=3, precision=1e-8,
-                 max_iterations=500, index_base=0):
+                 max_iterations=500, index_base=0, method="total"):
         L = f4lib()
         io = np.zeros(10, np.int32)
         ro = np.zeros(1)
@@ -553,6 +554,7 @@ class FETI4I:
         L.FETI4ISetDefaultRealOptions(_dp(ro))
         io[0] = subdomains
         io[1] = max_iterations
+        io[2] = METHOD[method] if isinstance(method, str) else method
         io[3] = preconditioner
         ro[0] = precision
         m = C.c_void_p()
@@ -577,11 +579,24 @@ class FETI4I:
         self.size = size
 
     def info(self):
-        v = np.zeros(6, np.int32)
+        v = np.zeros(8, np.int32)
         f4lib().FETI4IInstanceInfo(self.inst, _ip(v))
         _f4check()
         return dict(subdomains=int(v[0]), n_lambda=int(v[1]), n_coarse=int(v[2]), iterations=int(v[3]),
-                    ndof_node=int(v[4]), max_kdim=int(v[5]))
+                    ndof_node=int(v[4]), max_kdim=int(v[5]), n_corner_rows=int(v[6]), method=int(v[7]))
+
+    def corners(self, i):
+        """HYBRID_FETI B0 entries of subdomain i: (row, local DOF, sign)."""
+        L = f4lib()
+        n = np.zeros(1, np.int32)
+        L.FETI4IExportCorners(self.inst, i, _ip(n), None, None, None)
+        _f4check()
+        row = np.zeros(int(n[0]), np.int32)
+        dof = np.zeros(int(n[0]), np.int32)
+        sign = np.zeros(int(n[0]))
+        L.FETI4IExportCorners(self.inst, i, _ip(n), _ip(row), _ip(dof), _dp(sign))
+        _f4check()
+        return row, dof, sign
 
     def subdomain(self, i):
         L = f4lib()

@@ -62,6 +62,8 @@ struct Subdomain {
     int kdim = 0;
     std::vector<int> lam, dof;
     std::vector<double> sign, scale;
+    std::vector<int> b0row, b0dof;  // HYBRID_FETI corners (cluster-local row, local DOF)
+    std::vector<double> b0sign;
 };
 
 }  // namespace
@@ -85,7 +87,7 @@ struct FETI4IStructInstance {
     double precision = 1e-5;
     bool built = false;
     std::vector<Subdomain> sub;
-    int nlam = 0, ndof_node = 1, max_kdim = 0;
+    int nlam = 0, ndof_node = 1, max_kdim = 0, n0 = 0;
     std::vector<double> c;
     espreso_feti *h = nullptr;
     bool gpu_ready = false;
@@ -410,6 +412,82 @@ void algebraic_kernel(Subdomain &s) {
     F4_CHECK(kr <= 1e-7 * kn, ESPRESO_ERR_NUMERIC, "algebraic kernel detection failed (K R != 0)");
 }
 
+// ---------------------------------------------------------- B0 corners ----
+// HYBRID_FETI: the instance is one cluster (one cluster per process, as the
+// reference's library interface has one per MPI rank).  Corners follow the
+// reference's CORNERS B0 (fetidata.cpp:146-187, equalityconstraints.cpp:
+// 504-552): interface nodes are grouped by the full set of subdomains owning
+// them, each group gives its middle node (by node number), and a corner glues
+// consecutive pairs of its owners, one B0 row per DOF of the node (+1 in the
+// first subdomain, -1 in the second).  Elements given without nodes group
+// DOFs instead of nodes.
+void corners(FETI4IStructInstance &I, const std::vector<int> &part) {
+    for (auto &S : I.sub) { S.b0row.clear(); S.b0dof.clear(); S.b0sign.clear(); }
+    I.n0 = 0;
+    if (I.method != 1) return;
+    const FETI4IStructMatrix &K = I.K;
+    const int ne = K.nelem(), ns = (int)I.sub.size();
+    // entities: nodes with their DOFs (node-major in the element's DOF list),
+    // or single DOFs when the elements carry no nodes or the DOF count is not
+    // a multiple of the node count
+    bool with_nodes = K.nptr.back() > 0;
+    for (int e = 0; e < ne && with_nodes; e++) {
+        const int64_t kn = K.nptr[e + 1] - K.nptr[e], kd = K.dptr[e + 1] - K.dptr[e];
+        with_nodes = kn > 0 && kd % kn == 0 && kd / kn == I.ndof_node;
+    }
+    const int nd = with_nodes ? I.ndof_node : 1;
+    int nent = 0;
+    if (with_nodes) for (int v : K.nodes) nent = std::max(nent, v + 1);
+    else nent = I.size;
+    std::vector<int> edof((size_t)nent * nd, -1);
+    std::vector<std::vector<int>> own(nent);
+    for (int e = 0; e < ne; e++) {
+        const int64_t kn = with_nodes ? K.nptr[e + 1] - K.nptr[e] : K.dptr[e + 1] - K.dptr[e];
+        for (int64_t a = 0; a < kn; a++) {
+            const int v = with_nodes ? K.nodes[K.nptr[e] + a] : K.dofs[K.dptr[e] + a];
+            for (int c = 0; c < nd; c++) edof[(size_t)v * nd + c] = K.dofs[K.dptr[e] + a * nd + c];
+            own[v].push_back(part[e]);
+        }
+    }
+    std::vector<std::pair<std::vector<int>, int>> keyed;  // (owner set, entity)
+    for (int v = 0; v < nent; v++) {
+        auto &o = own[v];
+        std::sort(o.begin(), o.end());
+        o.erase(std::unique(o.begin(), o.end()), o.end());
+        if (o.size() >= 2) keyed.push_back({o, v});
+    }
+    std::stable_sort(keyed.begin(), keyed.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+    auto ldof = [&](int s, int g) {
+        const auto &v = I.sub[s].gdofs;
+        const auto it = std::lower_bound(v.begin(), v.end(), g);
+        F4_CHECK(it != v.end() && *it == g, ESPRESO_ERR_ARG, "corner DOF missing from its subdomain");
+        return (int)(it - v.begin());
+    };
+    int row = 0;
+    for (size_t g0 = 0; g0 < keyed.size();) {
+        size_t g1 = g0;
+        while (g1 < keyed.size() && keyed[g1].first == keyed[g0].first) g1++;
+        const std::vector<int> &ow = keyed[g0].first;
+        const int v = keyed[g0 + (g1 - g0) / 2].second;  // middle entity of the group (ascending number)
+        for (size_t a = 0; a + 1 < ow.size(); a++)
+            for (int c = 0; c < nd; c++) {
+                const int g = edof[(size_t)v * nd + c];
+                F4_CHECK(g >= 0, ESPRESO_ERR_ARG, "corner node without DOFs");
+                Subdomain &s1 = I.sub[ow[a]], &s2 = I.sub[ow[a + 1]];
+                s1.b0row.push_back(row);
+                s1.b0dof.push_back(ldof(ow[a], g));
+                s1.b0sign.push_back(1.0);
+                s2.b0row.push_back(row);
+                s2.b0dof.push_back(ldof(ow[a + 1], g));
+                s2.b0sign.push_back(-1.0);
+                row++;
+            }
+        g0 = g1;
+    }
+    (void)ns;
+    I.n0 = row;
+}
+
 // --------------------------------------------------------- decomposition ----
 void decompose(FETI4IStructInstance &I) {
     const FETI4IStructMatrix &K = I.K;
@@ -554,6 +632,7 @@ void decompose(FETI4IStructInstance &I) {
     I.nlam = lam;
     for (int s = 0; s < ns; s++)
         F4_CHECK(!I.sub[s].lam.empty(), ESPRESO_ERR_ARG, "subdomain without multipliers (isolated part of the mesh)");
+    corners(I, part);
     I.built = true;
     I.gpu_ready = false;
 }
@@ -566,7 +645,7 @@ void ensure_gpu(FETI4IStructInstance &I) {
         I.h = nullptr;
     }
     espreso_feti_options o{};
-    o.method = 0;
+    o.method = I.method;
     o.preconditioner = I.prec;
     o.max_iterations = I.max_it;
     o.scaling = 1;
@@ -581,6 +660,11 @@ void ensure_gpu(FETI4IStructInstance &I) {
                                         S.vals.data(), S.f.data(), S.kdim, S.R.data(), (int)S.lam.size(),
                                         S.lam.data(), S.dof.data(), S.sign.data(), S.scale.data());
         F4_CHECK(rc == 0, rc, espreso_last_error());
+        if (I.method == 1) {
+            rc = espreso_feti_set_cluster(I.h, (int)s, 0, (int)S.b0row.size(), S.b0row.data(), S.b0dof.data(),
+                                          S.b0sign.data());
+            F4_CHECK(rc == 0, rc, espreso_last_error());
+        }
     }
     rc = espreso_feti_set_dual(I.h, I.nlam, I.c.data());
     F4_CHECK(rc == 0, rc, espreso_last_error());
@@ -596,8 +680,8 @@ void set_options(FETI4IStructInstance &I, const FETI4IInt *io, const FETI4IReal 
     if (io) {
         F4_CHECK(io[FETI4I_SUBDOMAINS] >= 1, ESPRESO_ERR_ARG, "FETI4I_SUBDOMAINS must be positive");
         F4_CHECK(io[FETI4I_MAX_ITERATIONS] >= 1, ESPRESO_ERR_ARG, "FETI4I_MAX_ITERATIONS must be positive");
-        F4_CHECK(io[FETI4I_FETI_METHOD] == 0, ESPRESO_ERR_ARG,
-                 "FETI4I_FETI_METHOD: only TOTAL_FETI (0) is served by this library");
+        F4_CHECK(io[FETI4I_FETI_METHOD] == 0 || io[FETI4I_FETI_METHOD] == 1, ESPRESO_ERR_ARG,
+                 "FETI4I_FETI_METHOD: TOTAL_FETI (0) or HYBRID_FETI (1)");
         F4_CHECK(io[FETI4I_PRECONDITIONER] >= 0 && io[FETI4I_PRECONDITIONER] <= 3, ESPRESO_ERR_ARG,
                  "FETI4I_PRECONDITIONER: NONE, LUMPED, WEIGHT_FUNCTION or DIRICHLET (0..3)");
         F4_CHECK(io[FETI4I_CGSOLVER] == 0, ESPRESO_ERR_ARG, "FETI4I_CGSOLVER: only PCG (0)");
@@ -796,6 +880,8 @@ int FETI4IInstanceInfo(FETI4IInstance instance, FETI4IInt *info) {
         info[3] = instance->st.iterations;
         info[4] = instance->ndof_node;
         info[5] = instance->max_kdim;
+        info[6] = instance->n0;
+        info[7] = instance->method;
     });
     return g_status;
 }
@@ -822,6 +908,20 @@ int FETI4IExportSubdomain(FETI4IInstance instance, FETI4IInt i, FETI4IInt *sizes
         if (dof) std::copy(S.dof.begin(), S.dof.end(), dof);
         if (sign) std::copy(S.sign.begin(), S.sign.end(), sign);
         if (scale) std::copy(S.scale.begin(), S.scale.end(), scale);
+    });
+    return g_status;
+}
+
+int FETI4IExportCorners(FETI4IInstance instance, FETI4IInt i, FETI4IInt *size, FETI4IInt *row, FETI4IInt *dof,
+                        FETI4IReal *sign) {
+    guarded([&] {
+        F4_CHECK(is_instance(instance) && size, ESPRESO_ERR_ARG, "invalid arguments");
+        F4_CHECK(i >= 0 && i < (FETI4IInt)instance->sub.size(), ESPRESO_ERR_ARG, "subdomain out of range");
+        const Subdomain &S = instance->sub[i];
+        *size = (FETI4IInt)S.b0row.size();
+        if (row) std::copy(S.b0row.begin(), S.b0row.end(), row);
+        if (dof) std::copy(S.b0dof.begin(), S.b0dof.end(), dof);
+        if (sign) std::copy(S.b0sign.begin(), S.b0sign.end(), sign);
     });
     return g_status;
 }

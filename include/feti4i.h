@@ -81,8 +81,9 @@ void FETI4IDestroy(void *ptr);
 /* 0 when the last FETI4I call on this thread succeeded, else a negative code */
 int FETI4IGetLastStatus(void);
 const char *FETI4IGetLastError(void);
-/* info[0] subdomains, [1] lambda rows, [2] coarse dimension, [3] last PCG
- * iterations, [4] DOFs per node detected, [5] max kernel dimension          */
+/* info (8 entries): [0] subdomains, [1] lambda rows, [2] coarse dimension,
+ * [3] last PCG iterations, [4] DOFs per node detected, [5] max kernel
+ * dimension, [6] B0 corner rows (HYBRID_FETI, else 0), [7] FETI method     */
 int FETI4IInstanceInfo(FETI4IInstance instance, FETI4IInt *info);
 /* Decomposed data of subdomain i as handed to the FETI core; sizes[0] n,
  * [1] nnz, [2] kernel dimension, [3] lambda rows of i.  NULL arrays query
@@ -91,6 +92,12 @@ int FETI4IExportSubdomain(FETI4IInstance instance, FETI4IInt i, FETI4IInt *sizes
                           FETI4IInt *rowptr, FETI4IInt *colidx, FETI4IReal *values, FETI4IReal *f,
                           FETI4IReal *R, FETI4IInt *lambda, FETI4IInt *dof, FETI4IReal *sign,
                           FETI4IReal *scale);
+/* HYBRID_FETI corners of subdomain i (the instance is one cluster):
+ * *size entries of (cluster B0 row, subdomain-local DOF, +1/-1); NULL arrays
+ * query the size only.  Every row has one +1 and one -1 entry, in two
+ * subdomains, on copies of the same instance DOF.                           */
+int FETI4IExportCorners(FETI4IInstance instance, FETI4IInt i, FETI4IInt *size, FETI4IInt *row, FETI4IInt *dof,
+                        FETI4IReal *sign);
 /* stats[0] iterations, [1] final ||P r||, [2] tolerance, [3] assembly s,
  * [4] PCG s, [5] total GPU s                                                */
 int FETI4IGetStats(FETI4IInstance instance, FETI4IReal *stats);

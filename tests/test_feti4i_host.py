@@ -75,3 +75,50 @@ def test_invalid_options_are_reported(inputs):
         core.FETI4I(elements, size, rhs, dirichlet, subdomains=0)
     with pytest.raises(RuntimeError, match="PRECONDITIONER"):
         core.FETI4I(elements, size, rhs, dirichlet, subdomains=2, preconditioner=7)
+
+
+@pytest.mark.parametrize("subdomains", [2, 6])
+def test_hybrid_corners(inputs, subdomains):
+    """HYBRID_FETI corners: every B0 row holds +1 and -1 on two subdomain
+    copies of the same instance DOF; every group of interface nodes with the
+    same owner set gets one corner node (all DOFs of the node) per
+    consecutive owner pair; rows are unique."""
+    elements, size, rhs, dirichlet, o = inputs
+    f4 = core.FETI4I(elements, size, rhs, dirichlet, subdomains=subdomains, method="hybrid")
+    info = f4.info()
+    assert info["method"] == 1 and info["n_corner_rows"] > 0
+    nd = info["ndof_node"]
+    rows = {}
+    owners = {}
+    for s in range(subdomains):
+        d = f4.subdomain(s)
+        for g in d["gdofs"]:
+            owners.setdefault(int(g) // nd, set()).add(s)
+        for r, dof, sg in zip(*f4.corners(s)):
+            rows.setdefault(int(r), []).append((s, int(d["gdofs"][dof]), float(sg)))
+    assert sorted(rows) == list(range(info["n_corner_rows"]))
+    for r, ent in rows.items():
+        assert len(ent) == 2
+        (s1, g1, a), (s2, g2, b) = ent
+        assert s1 != s2 and g1 == g2 and {a, b} == {1.0, -1.0}
+    # groups by owner set; one middle node per group, glued along consecutive owners
+    groups = {}
+    for node, ow in owners.items():
+        if len(ow) > 1:
+            groups.setdefault(tuple(sorted(ow)), []).append(node)
+    expected = sum((len(k) - 1) * nd for k in groups)
+    assert info["n_corner_rows"] == expected
+    corner_nodes = {g // nd for ent in rows.values() for _, g, _ in ent}
+    for key, nodes in groups.items():
+        nodes.sort()
+        assert nodes[len(nodes) // 2] in corner_nodes
+    f4.close()
+
+
+def test_hybrid_option_is_accepted_and_others_rejected(inputs):
+    elements, size, rhs, dirichlet, o = inputs
+    with pytest.raises(RuntimeError, match="FETI_METHOD"):
+        core.FETI4I(elements, size, rhs, dirichlet, subdomains=2, method=2)
+    f4 = core.FETI4I(elements, size, rhs, dirichlet, subdomains=2, method=0)
+    assert f4.info()["n_corner_rows"] == 0 and f4.info()["method"] == 0
+    f4.close()

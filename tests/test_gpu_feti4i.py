@@ -39,3 +39,39 @@ def test_feti4i_heat():
     assert abs(u.min() - 100) < 1e-7 and abs(u.max() - 300) < 1e-7 and abs(u.mean() - 200) < 1e-7
     assert f4.info()["max_kdim"] == 1
     f4.close()
+
+
+@pytest.mark.parametrize("subdomains,prec", [(6, 3), (8, 1), (2, 3)])
+def test_feti4i_hybrid_matches_global(subdomains, prec):
+    """HYBRID_FETI through FETI4I: the instance is one cluster glued by corners;
+    same solution as the undecomposed direct solve, and as TOTAL_FETI."""
+    spec = PB.scaled(PB.GRAVITY, clusters=(1, 1, 1), domains=(6, 4, 4), elements=(1, 1, 1), traction_seed=4,
+                     traction_scale=1e5)
+    print("case: FETI4I hybrid gravity", subdomains, prec, flush=True)
+    elements, size, rhs, dirichlet, o = element_problem(spec)
+    f4 = core.FETI4I(elements, size, rhs, dirichlet, subdomains=subdomains, preconditioner=prec, precision=1e-10,
+                     method="hybrid")
+    u = f4.solve()
+    ug = O.global_solve(spec).ravel()
+    info = f4.info()
+    print("hybrid: iterations", info["iterations"], "coarse", info["n_coarse"], "corner rows", info["n_corner_rows"],
+          flush=True)
+    assert np.abs(u - ug).max() / np.abs(ug).max() < 1e-8
+    # the cluster's rigid body modes, plus relative rotations of subdomain
+    # pairs that a single corner node glues only at one point
+    assert 6 <= info["n_coarse"] <= 6 * subdomains
+    f4.update_rhs(2.0 * rhs)
+    u2 = f4.solve()
+    assert np.abs(u2 - 2.0 * ug).max() / np.abs(ug).max() < 1e-8
+    f4.close()
+
+
+def test_feti4i_hybrid_heat():
+    spec = PB.scaled(PB.HEAT_SIMPLE, clusters=(1, 1, 1), domains=(8, 6, 4), elements=(1, 1, 1))
+    print("case: FETI4I hybrid heat", flush=True)
+    elements, size, rhs, dirichlet, o = element_problem(spec)
+    f4 = core.FETI4I(elements, size, rhs, dirichlet, subdomains=5, precision=1e-10, method="hybrid")
+    u = f4.solve()
+    assert abs(u.min() - 100) < 1e-7 and abs(u.max() - 300) < 1e-7 and abs(u.mean() - 200) < 1e-7
+    assert f4.info()["n_coarse"] == 1  # one corner fixes the constant mode of a pair
+    f4.close()
