@@ -114,6 +114,7 @@ SIGNATURES = {
     "espreso_feti_load_work": (C.c_int, [_H, _D]),
     "espreso_grid_nodal_load": (C.c_int, [_H, _D]),
     "espreso_feti_resolve": (C.c_int, [_H, C.POINTER(Stats)]),
+    "espreso_feti_resolve_block": (C.c_int, [_H, C.c_int32, _D, _D, _I, _D]),
     "espreso_feti_create": (C.c_int, [C.POINTER(_H), C.c_int32, C.POINTER(Options)]),
     "espreso_feti_destroy": (C.c_int, [_H]),
     "espreso_feti_sizes": (C.c_int, [_H, _I]),
@@ -401,6 +402,18 @@ class FETISolver:
         self.stats = st.as_dict()
         return self.stats
 
+    def resolve_block(self, f):
+        """Solve several load cases together on the assembled operators.
+        f: (nrhs, sum_i n_i) subdomain-local loads (subdomain order).  Returns
+        (u of the same shape, iterations per case, final ||P r|| per case)."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        nrhs = f.shape[0]
+        u = np.zeros_like(f)
+        it = np.zeros(nrhs, np.int32)
+        nr = np.zeros(nrhs)
+        _check(lib().espreso_feti_resolve_block(self.h, nrhs, _dp(f), _dp(u), _ip(it), _dp(nr)))
+        return u, it, nr
+
     def primal(self, i, n):
         u = np.zeros(n)
         _check(lib().espreso_feti_primal(self.h, i, _dp(u)))
@@ -499,6 +512,26 @@ class GridFETI(FETISolver):
         g = np.zeros(n, dtype=np.int32)
         _check(lib().espreso_grid_l2g(self.h, i, _ip(g)))
         return g
+
+    def resolve_block_nodal(self, loads):
+        """Block re-solve of nodal load cases (nrhs, n_nodes, ndof): loads are
+        split over the subdomain copies of a node by multiplicity (as
+        set_nodal_rhs), solutions averaged over the copies (as nodal_solution).
+        Returns (u (nrhs, n_nodes, ndof), iterations, final norms)."""
+        loads = np.asarray(loads, dtype=np.float64).reshape(len(loads), self.n_nodes, self.ndof)
+        maps = [self.l2g(i) for i in range(self.n_sub)]
+        mult = np.zeros(self.n_nodes)
+        for g in maps:
+            np.add.at(mult, g, 1.0)
+        f = np.concatenate([(loads[:, g, :] / mult[g][None, :, None]).reshape(len(loads), -1) for g in maps], axis=1)
+        u, it, nr = self.resolve_block(f)
+        un = np.zeros_like(loads)
+        off = 0
+        for g in maps:
+            n = len(g) * self.ndof
+            np.add.at(un, (slice(None), g), u[:, off:off + n].reshape(len(loads), len(g), self.ndof))
+            off += n
+        return un / mult[None, :, None], it, nr
 
 
 # ---------------------------------------------------------------- FETI4I ----

@@ -15,6 +15,18 @@ struct SymvItem {
     int64_t part_off;
 };
 
+// Block SYMV work list (FP64 geometry, at most BLOCK_STRIPS strips per item)
+// and its scratch: xb (nbnd x Kmax, right-hand sides interleaved) and the
+// partials (plen x Kmax)
+constexpr int BLOCK_STRIPS = 8;
+struct SymvBlock {
+    SymvItem *items = nullptr;
+    int n_items = 0;
+    int *item0 = nullptr, *nitems = nullptr;
+    int64_t plen = 0;
+    double *xb = nullptr, *ypart = nullptr;
+};
+
 struct DualDev {
     int nsub = 0, nlam = 0, ncoarse = 0, max_m = 0;
     int64_t nbnd = 0;                // sum of m_s
@@ -105,6 +117,12 @@ struct PrimalDev {
 // ev0/ev1 (optional) are recorded around the SYMV kernel alone.
 void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gather, bool scaled, hipStream_t st,
                  hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+// K right-hand sides at once (K = 1, 2, 4, 8; FP64 storage): x is lambda-space,
+// K vectors of n_lambda (gathered with sign, or sign*scale when `scaled`);
+// yb receives K boundary vectors of nbnd.  ev0/ev1 bracket the SYMV kernel.
+void launch_symv_block(const DualDev &D, const SymvBlock &B, bool sd, int K, const double *x, bool scaled,
+                       double *yb, hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+size_t block_symv_lds(int K);
 void launch_lambda_gather(const DualDev &D, const double *yb, bool scaled, double *y, hipStream_t st);
 void launch_boundary_gather(const DualDev &D, const double *x, bool scaled, double *xb, hipStream_t st);
 void launch_kbb(const DualDev &D, const double *xb, double *yb, hipStream_t st);

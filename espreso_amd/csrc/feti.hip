@@ -214,7 +214,24 @@ struct espreso_feti_s {
     std::vector<int> wave_plan;  // end subdomain of each assembly wave
     bool use_graphs = true;
     void compute_b();
+    void compute_b_into(const double *hf, double *ubf, const double *e0f, double *b);
+    void rhs_host(const double *f, std::vector<double> &e0f, std::vector<double> &e) const;
     void resolve_rhs();
+    // block re-solve of several load cases (K = 1, 2, 4, 8 per group)
+    struct BlockBuffers {
+        int K = 0;
+        double *x = nullptr, *r = nullptr, *w = nullptr, *z = nullptr, *y = nullptr, *p = nullptr, *Ap = nullptr,
+               *b = nullptr, *e = nullptr, *alpha_c = nullptr, *alpha_tot = nullptr, *e0f = nullptr, *hf = nullptr,
+               *ubf = nullptr, *xbv = nullptr, *yb = nullptr, *f = nullptr, *yf = nullptr, *u = nullptr,
+               *scal = nullptr;
+    } BB;
+    SymvBlock SB;
+    int64_t nyf = 0;  // interior DOFs (length of P.yf)
+    void block_setup(int K);
+    void apply_F_block(int K, const double *x, double *y);
+    void step_block(int K, const std::vector<char> &active, bool first);
+    void resolve_block(int nrhs, const double *f, double *u, int32_t *iters, double *norms);
+    void resolve_group(int K, const double *f, double *u, int32_t *iters, double *norms);
     std::vector<double> Ns_h;                        // hybrid: N_s per subdomain (kdim x kN)
     std::vector<int64_t> noff_h;
     std::vector<int> ccoff_h, cblk_h;                // coarse block offset, block of each subdomain
@@ -518,6 +535,7 @@ void espreso_feti_s::load() {
         P.band = alloc<double>(band_off[nsub]);
         P.yf = alloc<double>(yf_off[nsub]);
         P.work = alloc<double>(yf_off[nsub]);
+        nyf = yf_off[nsub];
         P.alpha = d_alpha_tot;
         d_u = alloc<double>(n_off[nsub]);
         uoff = n_off;
@@ -595,39 +613,49 @@ void espreso_feti_s::gpu_setup() {
 
 // K^+ f on the boundary: ubf = F_i hf (FP64); hybrid: + cluster correction;
 // then b = B K^+ f - c (cluster.cpp:2660-2769)
-void espreso_feti_s::compute_b() {
-    launch_symv(D, false, 0, d_hf, 0, false, st);
-    ESP_HIP(hipMemcpyAsync(d_ubf, D.yb, D.nbnd * sizeof(double), hipMemcpyDeviceToDevice, st));
-    if (hybrid) launch_cluster_correction(D, CL, d_e0f, D.yb, 1, st);
-    launch_lambda_gather(D, D.yb, false, d_b, st);
-    launch_axpby(d_b, d_b, d_c, nlam, 1.0, -1.0, st);
+void espreso_feti_s::compute_b() { compute_b_into(d_hf, d_ubf, d_e0f, d_b); }
+
+void espreso_feti_s::compute_b_into(const double *hf, double *ubf, const double *e0f, double *b) {
+    launch_symv(D, false, 0, hf, 0, false, st);
+    ESP_HIP(hipMemcpyAsync(ubf, D.yb, D.nbnd * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (hybrid) launch_cluster_correction(D, CL, e0f, D.yb, 1, st);
+    launch_lambda_gather(D, D.yb, false, b, st);
+    launch_axpby(b, b, d_c, nlam, 1.0, -1.0, st);
 }
 
-// New loads (host sub[s].f) on the assembled operators: e = -R^T f (hybrid:
-// per cluster through N_s), y_f and hf by interior band solves, b.
-void espreso_feti_s::resolve_rhs() {
-    std::vector<double> ff(uoff[nsub]);
+// host part of new loads f (all subdomains, local order): e0f = -R_s^T f_s per
+// subdomain and the coarse right-hand side e (hybrid: per cluster through N_s)
+void espreso_feti_s::rhs_host(const double *f, std::vector<double> &e0f, std::vector<double> &e) const {
+    e0f.assign(asubv[nsub], 0.0);
     for (int s = 0; s < nsub; s++) {
         const SubHost &h = sub[s];
-        std::copy(h.f.begin(), h.f.end(), ff.begin() + uoff[s]);
+        const double *fs = f + uoff[s];
         for (int a = 0; a < h.kdim; a++) {
             double acc = 0;
-            for (int i = 0; i < h.n; i++) acc += h.R[i + (size_t)a * h.n] * h.f[i];
-            e0f_h[asubv[s] + a] = -acc;
+            for (int i = 0; i < h.n; i++) acc += h.R[i + (size_t)a * h.n] * fs[i];
+            e0f[asubv[s] + a] = -acc;
         }
     }
-    std::vector<double> e;
     if (hybrid) {
         e.assign(ncoarse, 0.0);
         for (int s = 0; s < nsub; s++) {
             const int c = cblk_h[s], kn = (int)((noff_h[s + 1] - noff_h[s]) / std::max(1, sub[s].kdim));
             for (int b = 0; b < kn; b++)
                 for (int a = 0; a < sub[s].kdim; a++)
-                    e[ccoff_h[c] + b] += Ns_h[noff_h[s] + a + (int64_t)b * sub[s].kdim] * e0f_h[asubv[s] + a];
+                    e[ccoff_h[c] + b] += Ns_h[noff_h[s] + a + (int64_t)b * sub[s].kdim] * e0f[asubv[s] + a];
         }
     } else {
-        e = e0f_h;
+        e = e0f;
     }
+}
+
+// New loads (host sub[s].f) on the assembled operators: e = -R^T f (hybrid:
+// per cluster through N_s), y_f and hf by interior band solves, b.
+void espreso_feti_s::resolve_rhs() {
+    std::vector<double> ff(uoff[nsub]);
+    for (int s = 0; s < nsub; s++) std::copy(sub[s].f.begin(), sub[s].f.end(), ff.begin() + uoff[s]);
+    std::vector<double> e;
+    rhs_host(ff.data(), e0f_h, e);
     ESP_HIP(hipMemcpyAsync(d_f, ff.data(), ff.size() * sizeof(double), hipMemcpyHostToDevice, st));
     ESP_HIP(hipMemcpyAsync(d_e0f, e0f_h.data(), e0f_h.size() * sizeof(double), hipMemcpyHostToDevice, st));
     ESP_HIP(hipMemcpyAsync(d_e, e.data(), e.size() * sizeof(double), hipMemcpyHostToDevice, st));
@@ -1072,6 +1100,223 @@ void espreso_feti_s::solve(espreso_feti_stats *out) {
     last.time_total = last.time_assembly + last.time_coarse + last.time_pcg + last.time_primal;
     solved = true;
     if (out) *out = last;
+}
+
+// ------------------------------------------------------ block re-solve ----
+// Several load cases on the assembled operators, solved together: one
+// projected PCG (itersolver.cpp:1537-1684) per case with its own scalars and
+// stop test, advanced in lockstep so that every F and Dirichlet apply reads
+// the F_i and S_D bytes once for the whole group (block SYMV).  Each case
+// follows FETISolver::solve with its own Matrices::f (FETISolver.cpp:59-149).
+void espreso_feti_s::block_setup(int K) {
+    if (!SB.items) {
+        std::vector<SymvItem> items;
+        std::vector<int> item0(nsub), nit(nsub);
+        int64_t plen = 0;
+        for (int s = 0; s < nsub; s++) {
+            const int m = sub[s].m;
+            const int nst = (int)strip_count(m, STRIP_F64);
+            item0[s] = (int)items.size();
+            for (int t = 0; t < nst; t += BLOCK_STRIPS) {
+                SymvItem it{};
+                it.sub = s;
+                it.t0 = t;
+                it.t1 = std::min(nst, t + BLOCK_STRIPS);
+                it.part_off = plen;
+                plen += m - t * STRIP_F64;
+                items.push_back(it);
+            }
+            nit[s] = (int)items.size() - item0[s];
+        }
+        SB.items = upload(items);
+        SB.n_items = (int)items.size();
+        SB.item0 = upload(item0);
+        SB.nitems = upload(nit);
+        SB.plen = plen;
+    }
+    if (BB.K >= K) return;
+    for (double *q : {BB.x, BB.r, BB.w, BB.z, BB.y, BB.p, BB.Ap, BB.b, BB.e, BB.alpha_c, BB.alpha_tot, BB.e0f, BB.hf,
+                      BB.ubf, BB.xbv, BB.yb, BB.f, BB.yf, BB.u, BB.scal, SB.xb, SB.ypart})
+        if (q) release(q);
+    const int64_t nl = nlam, nb = D.nbnd, nc = std::max(ncoarse, 1), na = std::max<int64_t>(asubv[nsub], 1);
+    BB.x = alloc<double>(K * nl);
+    BB.r = alloc<double>(K * nl);
+    BB.w = alloc<double>(K * nl);
+    BB.z = alloc<double>(K * nl);
+    BB.y = alloc<double>(K * nl);
+    BB.p = alloc<double>(K * nl);
+    BB.Ap = alloc<double>(K * nl);
+    BB.b = alloc<double>(K * nl);
+    BB.e = alloc<double>(K * nc);
+    BB.alpha_c = alloc<double>(K * nc);
+    BB.alpha_tot = alloc<double>(K * na);
+    BB.e0f = alloc<double>(K * na);
+    BB.hf = alloc<double>(K * nb);
+    BB.ubf = alloc<double>(K * nb);
+    BB.xbv = alloc<double>(K * nb);
+    BB.yb = alloc<double>(K * nb);
+    BB.f = alloc<double>(K * uoff[nsub]);
+    BB.yf = alloc<double>(K * std::max<int64_t>(nyf, 1));
+    BB.u = alloc<double>(K * uoff[nsub]);
+    BB.scal = alloc<double>(8 * K);
+    SB.xb = alloc<double>(K * nb);
+    SB.ypart = alloc<double>(K * std::max<int64_t>(SB.plen, 1));
+    BB.K = K;
+}
+
+// Y = F X for K lambda-space vectors (one block SYMV, then the hybrid cluster
+// correction and the scatter per vector)
+void espreso_feti_s::apply_F_block(int K, const double *x, double *y) {
+    launch_symv_block(D, SB, false, K, x, false, BB.yb, st);
+    for (int v = 0; v < K; v++) {
+        double *ybv = BB.yb + v * D.nbnd;
+        if (hybrid) {
+            launch_subdomain_kernel_dots(D, x + (int64_t)v * nlam, d_e0s, st);
+            launch_cluster_correction(D, CL, d_e0s, ybv, 1, st);
+        }
+        launch_lambda_gather(D, ybv, false, y + (int64_t)v * nlam, st);
+    }
+}
+
+// one PCG iteration of every active case (the block applies cover all K)
+void espreso_feti_s::step_block(int K, const std::vector<char> &active, bool first) {
+    const int64_t nl = nlam;
+    auto sc = [&](int v, int q) { return BB.scal + 8 * v + q; };  // 0 yw, 1 yw_prev, 2 pAp, 3 ww
+    for (int v = 0; v < K; v++)
+        if (active[v]) launch_project(D, BB.r + v * nl, BB.w + v * nl, 0, st);
+    if (opt.preconditioner == ESPRESO_PREC_NONE) {
+        for (int v = 0; v < K; v++)
+            if (active[v])
+                ESP_HIP(hipMemcpyAsync(BB.y + v * nl, BB.w + v * nl, nl * sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+        if (opt.preconditioner == ESPRESO_PREC_DIRICHLET) {
+            launch_symv_block(D, SB, true, K, BB.w, true, BB.yb, st);
+            for (int v = 0; v < K; v++)
+                if (active[v]) launch_lambda_gather(D, BB.yb + v * D.nbnd, true, BB.z + v * nl, st);
+        } else {
+            for (int v = 0; v < K; v++)
+                if (active[v]) apply_prec(BB.w + v * nl, BB.z + v * nl);
+        }
+        for (int v = 0; v < K; v++)
+            if (active[v]) launch_project(D, BB.z + v * nl, BB.y + v * nl, 0, st);
+    }
+    for (int v = 0; v < K; v++) {
+        if (!active[v]) continue;
+        if (!first) launch_copy_scalar(sc(v, 0), sc(v, 1), st);
+        launch_dot(BB.y + v * nl, BB.w + v * nl, nl, d_partial, sc(v, 0), st);
+        launch_p_update(BB.p + v * nl, BB.y + v * nl, nl, sc(v, 0), sc(v, 1), first, st);
+    }
+    apply_F_block(K, BB.p, BB.Ap);
+    for (int v = 0; v < K; v++) {
+        if (!active[v]) continue;
+        launch_dot(BB.p + v * nl, BB.Ap + v * nl, nl, d_partial, sc(v, 2), st);
+        launch_xr_update(BB.x + v * nl, BB.r + v * nl, BB.p + v * nl, BB.Ap + v * nl, nl, sc(v, 0), sc(v, 2), st);
+        launch_dot(BB.w + v * nl, BB.w + v * nl, nl, d_partial, sc(v, 3), st);
+    }
+}
+
+void espreso_feti_s::resolve_group(int K, const double *f, double *u, int32_t *iters, double *norms) {
+    block_setup(K);
+    const int64_t nl = nlam, nb = D.nbnd, nt = uoff[nsub], na = std::max<int64_t>(asubv[nsub], 1);
+    const int64_t nc = std::max(ncoarse, 1);
+    // loads: e0f, e on the host; f, y_f, hf and b on the device
+    std::vector<double> e0f, e;
+    for (int v = 0; v < K; v++) {
+        rhs_host(f + v * nt, e0f, e);
+        ESP_HIP(hipMemcpyAsync(BB.f + v * nt, f + v * nt, nt * sizeof(double), hipMemcpyHostToDevice, st));
+        if (!e0f.empty())
+            ESP_HIP(hipMemcpyAsync(BB.e0f + v * na, e0f.data(), e0f.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        if (!e.empty())
+            ESP_HIP(hipMemcpyAsync(BB.e + v * nc, e.data(), e.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        PrimalDev Pv = P;
+        Pv.yf = BB.yf + v * nyf;
+        launch_rhs_prepare(Pv, nsub, BB.f + v * nt, BB.hf + v * nb, st);
+        compute_b_into(BB.hf + v * nb, BB.ubf + v * nb, BB.e0f + v * na, BB.b + v * nl);
+        sync();  // host vectors e0f, e are reused by the next case
+    }
+    // lambda0 = G^T (GG^T)^-1 e ; r = b - F lambda0 ; tol = precision * min(||P r0||, ||b||)
+    for (int v = 0; v < K; v++) launch_project(D, BB.e + v * nc, BB.x + v * nl, 1, st);
+    apply_F_block(K, BB.x, BB.Ap);
+    for (int v = 0; v < K; v++) {
+        launch_axpby(BB.r + v * nl, BB.b + v * nl, BB.Ap + v * nl, nl, 1.0, -1.0, st);
+        launch_project(D, BB.r + v * nl, BB.w + v * nl, 0, st);
+        launch_dot(BB.w + v * nl, BB.w + v * nl, nl, d_partial, BB.scal + 8 * v + 4, st);
+        launch_dot(BB.b + v * nl, BB.b + v * nl, nl, d_partial, BB.scal + 8 * v + 5, st);
+    }
+    std::vector<double> hs(8 * K);
+    ESP_HIP(hipMemcpyAsync(hs.data(), BB.scal, hs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    sync();
+    std::vector<double> tol(K), nrm(K, 0.0);
+    std::vector<char> active(K, 1);
+    std::vector<int> it_v(K, 0);
+    for (int v = 0; v < K; v++) tol[v] = opt.precision * std::min(std::sqrt(hs[8 * v + 4]), std::sqrt(hs[8 * v + 5]));
+    for (int it = 0; it < opt.max_iterations; it++) {
+        step_block(K, active, it == 0);
+        ESP_HIP(hipMemcpyAsync(hs.data(), BB.scal, hs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        sync();
+        bool any = false;
+        for (int v = 0; v < K; v++) {
+            if (!active[v]) continue;
+            ESP_CHECK(std::isfinite(hs[8 * v + 3]), ESPRESO_ERR_NUMERIC, "block PCG produced a non-finite residual");
+            nrm[v] = std::sqrt(hs[8 * v + 3]);
+            it_v[v] = it + 1;
+            if (nrm[v] < tol[v]) active[v] = 0;
+            any |= active[v] != 0;
+        }
+        if (!any) break;
+    }
+    // primal per case: u = K~^+ (f - B^T lambda) + R~ alpha
+    for (int v = 0; v < K; v++) launch_project(D, BB.r + v * nl, BB.alpha_c + v * nc, 2, st);
+    launch_symv_block(D, SB, false, K, BB.x, false, BB.yb, st);
+    for (int v = 0; v < K; v++) {
+        double *xb = BB.xbv + v * nb;
+        launch_axpby(xb, BB.ubf + v * nb, BB.yb + v * nb, nb, 1.0, -1.0, st);
+        if (hybrid) {
+            launch_subdomain_kernel_dots(D, BB.x + v * nl, d_e0s, st);
+            launch_axpby(d_e0s, BB.e0f + v * na, d_e0s, asubv[nsub], 1.0, -1.0, st);
+            launch_cluster_correction(D, CL, d_e0s, xb, 0, st);
+        }
+        launch_alpha_combine(D, CL, hybrid, BB.alpha_c + v * nc, BB.alpha_tot + v * na, st);
+        PrimalDev Pv = P;
+        Pv.yf = BB.yf + v * nyf;
+        Pv.alpha = BB.alpha_tot + v * na;
+        launch_primal(Pv, nsub, xb, BB.u + v * nt, st);
+    }
+    ESP_HIP(hipMemcpyAsync(u, BB.u, K * nt * sizeof(double), hipMemcpyDeviceToHost, st));
+    sync();
+    for (int v = 0; v < K; v++) {
+        if (iters) iters[v] = it_v[v];
+        if (norms) norms[v] = nrm[v];
+    }
+}
+
+void espreso_feti_s::resolve_block(int nrhs, const double *f, double *u, int32_t *iters, double *norms) {
+    ESP_CHECK(ready, ESPRESO_ERR_STATE, "espreso_feti_setup not called");
+    const int64_t nt = uoff[nsub];
+    if (opt.sc_precision != 0 || D.sd_prec != 0) {
+        // low-precision storage: the cases one after another (refinement per case)
+        std::vector<std::vector<double>> keep(nsub);
+        for (int s = 0; s < nsub; s++) keep[s] = sub[s].f;
+        for (int v = 0; v < nrhs; v++) {
+            for (int s = 0; s < nsub; s++)
+                std::copy(f + v * nt + uoff[s], f + v * nt + uoff[s] + sub[s].n, sub[s].f.begin());
+            resolve_rhs();
+            espreso_feti_stats sv{};
+            solve(&sv);
+            ESP_HIP(hipMemcpy(u + v * nt, d_u, nt * sizeof(double), hipMemcpyDeviceToHost));
+            if (iters) iters[v] = sv.iterations;
+            if (norms) norms[v] = sv.final_norm;
+        }
+        for (int s = 0; s < nsub; s++) sub[s].f = keep[s];
+        resolve_rhs();
+        return;
+    }
+    for (int done = 0; done < nrhs;) {
+        const int rem = nrhs - done;
+        const int K = rem >= 8 ? 8 : rem >= 4 ? 4 : rem >= 2 ? 2 : 1;
+        resolve_group(K, f + done * nt, u + done * nt, iters ? iters + done : nullptr, norms ? norms + done : nullptr);
+        done += K;
+    }
 }
 
 // --------------------------------------------------- coarse space layout ----
@@ -1568,6 +1813,18 @@ int espreso_feti_resolve(espreso_feti *h, espreso_feti_stats *st) {
     const double trhs = wall() - t0;
     h->solve(st);
     if (st) st->time_coarse = trhs;
+    ESP_API_END
+}
+
+int espreso_feti_resolve_block(espreso_feti *h, int32_t nrhs, const double *f, double *u, int32_t *iterations,
+                               double *final_norms) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && h->ready, ESPRESO_ERR_STATE, "espreso_feti_setup not called");
+    ESP_CHECK(nrhs >= 0 && (nrhs == 0 || (f && u)), ESPRESO_ERR_ARG, "invalid block arguments");
+    ESP_HIP(hipSetDevice(h->opt.device));
+    for (int64_t q = 0; q < (int64_t)nrhs * h->uoff[h->nsub]; q++)
+        ESP_CHECK(std::isfinite(f[q]), ESPRESO_ERR_NUMERIC, "non-finite load");
+    h->resolve_block(nrhs, f, u, iterations, final_norms);
     ESP_API_END
 }
 

@@ -182,6 +182,19 @@ int espreso_feti_set_rhs(espreso_feti *h, int32_t i, const double *f);
 /* Solve again with the loads set by espreso_feti_set_rhs (no reassembly):
  * new b and e, projected PCG, primal recovery.                               */
 int espreso_feti_resolve(espreso_feti *h, espreso_feti_stats *stats);
+/* Block re-solve: nrhs load cases on the assembled operators, solved
+ * together -- one projected PCG per case (own scalars and stop test) in
+ * lockstep, so that each F and Dirichlet apply reads the F_i / S_D bytes once
+ * for a group of up to 8 cases.  f: nrhs consecutive load vectors, each the
+ * subdomain-local loads of all subdomains in subdomain order (n_i values of
+ * subdomain i, as espreso_feti_set_rhs takes them); u: the primal solutions
+ * in the same layout; iterations and final_norms (per case) may be NULL.  The
+ * instance's own loads are left as they were.  With low-precision F_i or S_D
+ * storage the cases are solved one after another.  No counterpart in the
+ * reference, which solves one Matrices::f per FETISolver::solve
+ * (FETISolver.cpp:59-149, itersolver.cpp:1406-1724).                        */
+int espreso_feti_resolve_block(espreso_feti *h, int32_t nrhs, const double *f, double *u, int32_t *iterations,
+                               double *final_norms);
 /* sum over subdomains of f_i^T u_i for the last solution (the compliance
  * f^T u of the assembled problem)                                            */
 int espreso_feti_load_work(espreso_feti *h, double *c);

@@ -1,0 +1,113 @@
+"""GPU block re-solve: several load cases solved together on the assembled
+operators (block SYMV over F_i and S_D, one PCG per case in lockstep).
+
+Bars: every case within 1e-8 relative of the undecomposed direct solve of the
+same global mesh (FP64); the same primal as solving the cases one after
+another through resolve() (1e-9 relative) with iteration counts within
+max(2, 2 %) (block and single SYMV sum in different orders); group sizes
+8, 4, 2, 1 (13 cases), TFETI and HYBRID_FETI, Dirichlet and lumped; the
+low-precision storage path (cases one after another) against FP64.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from espreso_amd import core
+from espreso_amd import problems as PB
+from test_gpu_uneven import with_material
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+GRAV = PB.scaled(PB.GRAVITY, clusters=(2, 1, 1), domains=(2, 2, 2), elements=(3, 3, 3), method="hybrid")
+HEAT = dict(PB.HEAT_UNEVEN_2000, domains=(3, 3, 4), slab_elements=PB.uneven_slabs((3, 3, 4), 5, 3, 7),
+            name="heat_uneven_36")
+
+
+def sequential(g, loads):
+    us, its = [], []
+    for q in range(loads.shape[0]):
+        g.set_nodal_rhs(loads[q])
+        st = g.resolve()
+        us.append(g.nodal_solution())
+        its.append(st["iterations"])
+    return np.array(us), np.array(its)
+
+
+@pytest.mark.parametrize("method,prec,ncases", [("hybrid", "dirichlet", 13), ("total", "dirichlet", 5),
+                                                 ("total", "lumped", 3)])
+def test_block_matches_sequential_and_direct(method, prec, ncases):
+    spec = dict(GRAV, method=method)
+    say("case: block", method, prec, ncases)
+    g = core.GridFETI(spec, preconditioner=prec, precision=1e-11, max_iterations=3000)
+    g.setup()
+    g.solve()
+    loads = PB.nodal_load_cases(spec, ncases, seed=31)
+    ub, itb, nrm = g.resolve_block_nodal(loads)
+    us, its = sequential(g, loads)
+    for q in range(ncases):
+        uq = O.global_solve_load(spec, loads[q])
+        e_direct = rel(ub[q], uq)
+        e_seq = rel(ub[q], us[q])
+        say(f"load case {q}: iterations block {itb[q]} single {its[q]}, vs direct {e_direct:.2e}, "
+            f"vs single {e_seq:.2e}")
+        assert e_direct < 1e-8
+        assert e_seq < 1e-9
+        assert abs(int(itb[q]) - int(its[q])) <= max(2, int(0.02 * its[q]))
+    g.close()
+
+
+def test_block_uneven_heat():
+    spec = with_material(HEAT)  # per-subdomain conductivity as an element field, shared with the direct solve
+    say("case: block uneven heat")
+    g = core.GridFETI(spec, preconditioner="dirichlet", precision=1e-11, max_iterations=3000)
+    g.setup()
+    loads = PB.nodal_load_cases(spec, 8, seed=5)
+    ub, itb, _ = g.resolve_block_nodal(loads)
+    ue = PB.uniform_equivalent(spec)
+    for q in range(8):
+        err = rel(ub[q], O.global_solve_load(ue, loads[q]))
+        say("load case", q, "iterations", int(itb[q]), "err", err)
+        assert err < 1e-8
+    g.close()
+
+
+def test_block_keeps_instance_loads():
+    """The block solve leaves the instance's own loads and b as they were."""
+    spec = dict(GRAV, method="total")
+    say("case: block leaves instance state")
+    g = core.GridFETI(spec, preconditioner="dirichlet", precision=1e-11, max_iterations=3000)
+    g.setup()
+    g.solve()
+    u0 = g.nodal_solution()
+    b0 = g.vec_b()
+    g.resolve_block_nodal(PB.nodal_load_cases(spec, 2, seed=8))
+    assert np.array_equal(g.vec_b(), b0)
+    g.solve()
+    assert rel(g.nodal_solution(), u0) < 1e-10
+    g.close()
+
+
+def test_block_low_precision_storage():
+    spec = dict(GRAV, method="total")
+    say("case: block with FP32 F_i (cases one after another)")
+    loads = PB.nodal_load_cases(spec, 3, seed=12)
+    g = core.GridFETI(spec, preconditioner="dirichlet", precision=1e-11, max_iterations=3000, sc_precision="fp32")
+    g.setup()
+    u32, it32, _ = g.resolve_block_nodal(loads)
+    g.close()
+    for q in range(3):
+        err = rel(u32[q], O.global_solve_load(spec, loads[q]))
+        say("load case", q, "iterations", int(it32[q]), "err", err)
+        assert err < 1e-8
