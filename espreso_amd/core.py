@@ -114,7 +114,7 @@ SIGNATURES = {
     "espreso_feti_load_work": (C.c_int, [_H, _D]),
     "espreso_grid_nodal_load": (C.c_int, [_H, _D]),
     "espreso_feti_resolve": (C.c_int, [_H, C.POINTER(Stats)]),
-    "espreso_feti_resolve_block": (C.c_int, [_H, C.c_int32, _D, _D, _I, _D]),
+    "espreso_feti_resolve_block": (C.c_int, [_H, C.c_int32, _D, _D, _I, _D, C.POINTER(Stats)]),
     "espreso_feti_create": (C.c_int, [C.POINTER(_H), C.c_int32, C.POINTER(Options)]),
     "espreso_feti_destroy": (C.c_int, [_H]),
     "espreso_feti_sizes": (C.c_int, [_H, _I]),
@@ -135,6 +135,7 @@ SIGNATURES = {
     "espreso_feti_time_apply": (C.c_int, [_H, C.c_int32, C.c_int32, _D, _D]),
     "espreso_debug_mfma": (C.c_int, [_D, _D, _D]),
     "espreso_debug_copy": (C.c_int, [C.c_int64, C.c_int32, _D]),
+    "espreso_debug_stream": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, _D]),
     "espreso_debug_tile_gemm": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _D, C.c_int32, _D,
                                           C.c_int32, _D, C.c_int32, C.c_double, C.c_double]),
 }
@@ -151,6 +152,13 @@ def debug_mfma(a, b):
 def debug_copy(nbytes, reps=5):
     ms = C.c_double()
     _check(lib().espreso_debug_copy(int(nbytes), reps, C.byref(ms)))
+    return ms.value
+
+
+def debug_stream(nbytes, reps=5, workgroups=4096):
+    """Read-only HBM stream: mean ms per pass over nbytes."""
+    ms = C.c_double()
+    _check(lib().espreso_debug_stream(int(nbytes), reps, workgroups, C.byref(ms)))
     return ms.value
 
 
@@ -411,7 +419,9 @@ class FETISolver:
         u = np.zeros_like(f)
         it = np.zeros(nrhs, np.int32)
         nr = np.zeros(nrhs)
-        _check(lib().espreso_feti_resolve_block(self.h, nrhs, _dp(f), _dp(u), _ip(it), _dp(nr)))
+        st = Stats()
+        _check(lib().espreso_feti_resolve_block(self.h, nrhs, _dp(f), _dp(u), _ip(it), _dp(nr), C.byref(st)))
+        self.block_stats = st.as_dict()
         return u, it, nr
 
     def primal(self, i, n):
@@ -518,20 +528,20 @@ class GridFETI(FETISolver):
         split over the subdomain copies of a node by multiplicity (as
         set_nodal_rhs), solutions averaged over the copies (as nodal_solution).
         Returns (u (nrhs, n_nodes, ndof), iterations, final norms)."""
-        loads = np.asarray(loads, dtype=np.float64).reshape(len(loads), self.n_nodes, self.ndof)
-        maps = [self.l2g(i) for i in range(self.n_sub)]
-        mult = np.zeros(self.n_nodes)
-        for g in maps:
-            np.add.at(mult, g, 1.0)
-        f = np.concatenate([(loads[:, g, :] / mult[g][None, :, None]).reshape(len(loads), -1) for g in maps], axis=1)
+        nr_, nn, nd = len(loads), self.n_nodes, self.ndof
+        loads = np.asarray(loads, dtype=np.float64).reshape(nr_, nn, nd)
+        if getattr(self, "_gidx", None) is None:
+            self._gidx = np.concatenate([self.l2g(i) for i in range(self.n_sub)]).astype(np.int64)
+            self._mult = np.bincount(self._gidx, minlength=nn).astype(np.float64)
+        gidx, mult = self._gidx, self._mult
+        f = (loads[:, gidx, :] / mult[gidx][None, :, None]).reshape(nr_, -1)
         u, it, nr = self.resolve_block(f)
-        un = np.zeros_like(loads)
-        off = 0
-        for g in maps:
-            n = len(g) * self.ndof
-            np.add.at(un, (slice(None), g), u[:, off:off + n].reshape(len(loads), len(g), self.ndof))
-            off += n
-        return un / mult[None, :, None], it, nr
+        u = u.reshape(nr_, len(gidx), nd)
+        un = np.empty_like(loads)
+        for q in range(nr_):
+            for d in range(nd):
+                un[q, :, d] = np.bincount(gidx, weights=u[q, :, d], minlength=nn) / mult
+        return un, it, nr
 
 
 # ---------------------------------------------------------------- FETI4I ----

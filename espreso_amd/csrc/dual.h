@@ -123,6 +123,9 @@ void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gathe
 void launch_symv_block(const DualDev &D, const SymvBlock &B, bool sd, int K, const double *x, bool scaled,
                        double *yb, hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 size_t block_symv_lds(int K);
+// the same on MFMA for 1 <= nr <= 16 right-hand sides (stride nr in the scratch)
+void launch_symv_mfma(const DualDev &D, const SymvBlock &B, bool sd, int nr, const double *x, bool scaled,
+                      double *yb, hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_lambda_gather(const DualDev &D, const double *yb, bool scaled, double *y, hipStream_t st);
 void launch_boundary_gather(const DualDev &D, const double *x, bool scaled, double *xb, hipStream_t st);
 void launch_kbb(const DualDev &D, const double *xb, double *yb, hipStream_t st);
@@ -151,10 +154,19 @@ void launch_symmetrize(double *A, int n, hipStream_t st);
 void launch_primal(const PrimalDev &P, int nsub, const double *ub, double *u, hipStream_t st);
 // new loads f (local DOF order, all subdomains): P.yf = L^-1 f_int, hf = f_b - K_bi K_ii^-1 f_int
 void launch_rhs_prepare(const PrimalDev &P, int nsub, const double *f, double *hf, hipStream_t st);
+// nr load vectors at once (strides between vectors fstr, hstr, ystr; work:
+// 16 values per interior row, P.work_off layout)
+void launch_rhs_prepare_multi(const PrimalDev &P, int nsub, int nr, const double *f, int64_t fstr, double *hf,
+                              int64_t hstr, double *yf, int64_t ystr, double *work, hipStream_t st);
+void launch_primal_multi(const PrimalDev &P, int nsub, int nr, const double *ub, int64_t bstr, const double *yf,
+                         int64_t ystr, const double *alpha, int64_t astr, double *u, int64_t ustr, double *work,
+                         hipStream_t st);
 // sens[e] = -w_e^T Ke0 u_e over the NE[0] x NE[1] x NE[2] HEXA8 grid (nodal fields, ndof nd)
 void launch_element_sensitivity(const int *NE, int nd, const double *Ke0, const double *u, const double *w,
                                 double *sens, hipStream_t st);
 // plain copy of n doubles (n multiple of 4): calibration of HBM byte counters
 void launch_copy(const double *src, double *dst, int64_t n, hipStream_t st);
+// read-only stream over n doubles (n even) with `wgs` workgroups (out: wgs doubles)
+void launch_stream_read(const double *src, int64_t n, int wgs, double *out, hipStream_t st);
 
 }  // namespace esp

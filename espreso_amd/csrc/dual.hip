@@ -324,6 +324,156 @@ __global__ void k_boundary_gather_block(int64_t nb, const int64_t *bl_ptr, const
     }
 }
 
+// Block SYMV on MFMA (v_mfma_f64_16x16x4f64): up to 16 right-hand sides, the
+// MFMA's N dimension (lanes of right-hand sides past nr carry zeros).  An
+// item is at most BLOCK_STRIPS strips of one subdomain; wave w owns the
+// 16-column blocks w, w+4, ... of the item's column range and visits every
+// strip of the item that holds the block (the 16-column grid is common to all
+// strips since strips start at multiples of 16):
+//   column product  Dc[c][v] += sum_r A[r][c] x[r][v]   (A op: lane (c, row slot))
+//   row product     Dr[r][v] += sum_c A[r][c] x[c][v]   (A op: lane (r, column slot))
+// The 16-byte loads give the first operand layout; a per-wave LDS tile
+// transposes the block for the second.  Column sums are final when the wave
+// leaves the block (written straight to the item's partial, no atomics); row
+// sums stay in registers per strip and meet in LDS at the end.  x and the
+// partials are interleaved by right-hand side with stride nr.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void __launch_bounds__(256) k_symv_mfma(const SymvItem *__restrict__ items, const int *__restrict__ M,
+                                                   const int64_t *__restrict__ boff, const double *__restrict__ pool,
+                                                   const int64_t *__restrict__ poff, const double *__restrict__ xb,
+                                                   int nr, double *__restrict__ ypart) {
+    constexpr int R = STRIP_F64, S = BLOCK_STRIPS, TL = 18;  // TL: leading dimension of the transpose tile
+    __shared__ double xr[S * R * 16], yr[S * R * 16];        // [item row][rhs], stride 16
+    __shared__ double T[4][16 * TL];
+    const SymvItem it = items[blockIdx.x];
+    const int s = it.sub, m = M[s];
+    const int ns = it.t1 - it.t0, r0i = it.t0 * R;
+    const int nrow = min(ns * R, m - r0i);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int q = lane >> 4, l16 = lane & 15;
+    const double *xs = xb + boff[s] * nr;
+    for (int i = tid; i < S * R * 16; i += 256) {
+        const int row = i >> 4, v = i & 15;
+        xr[i] = (row < nrow && v < nr) ? xs[(int64_t)(r0i + row) * nr + v] : 0.0;
+        yr[i] = 0.0;
+    }
+    __syncthreads();
+    const double *bs = pool + poff[s];
+    double *Tw = T[wave];
+    v4d dr[S];
+#pragma unroll
+    for (int t = 0; t < S; t++) dr[t] = (v4d){0.0, 0.0, 0.0, 0.0};
+    const int nblk = (m - r0i + 15) / 16;
+    for (int cb = wave; cb < nblk; cb += 4) {
+        const int c0 = r0i + 16 * cb;
+        // row-product B operand: x[c0 + q + 4j][l16]
+        double xc[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int c = c0 + q + 4 * j;
+            xc[j] = (c < m && l16 < nr) ? xs[(int64_t)c * nr + l16] : 0.0;
+        }
+        v4d dc = {0.0, 0.0, 0.0, 0.0};
+        const int tl = min(ns, cb + 1);  // strips holding the block (upper triangle)
+#pragma unroll
+        for (int tt = 0; tt < S; tt++) {
+            if (tt < tl) {
+                const int t = it.t0 + tt, r0 = t * R, ncol = m - r0;
+                const int cl = c0 - r0;  // >= 0, multiple of 16
+                const int k = cl >> 6, cc = (cl & 63) + l16;
+                const int w = min(64, ncol - 64 * k);
+                const u32x4 *chunk = (const u32x4 *)(bs + strip_offset(m, R, t) + (int64_t)64 * k * R);
+                u32x4 p0 = {0u, 0u, 0u, 0u}, p1 = {0u, 0u, 0u, 0u};
+                if (cc < w) {
+                    p0 = __builtin_nontemporal_load(chunk + q * w + cc);        // rows 2q, 2q+1
+                    p1 = __builtin_nontemporal_load(chunk + (q + 4) * w + cc);  // rows 2q+8, 2q+9
+                }
+                double a[4];
+                __builtin_memcpy(&a[0], &p0, 16);
+                __builtin_memcpy(&a[2], &p1, 16);
+                // column product: row slot q of call (j, e) is row 2q + 8j + e
+                const double *xrow = xr + (r0 - r0i) * 16;
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int e = 0; e < 2; e++)
+                        dc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[2 * j + e], xrow[(2 * q + 8 * j + e) * 16 + l16],
+                                                                  dc, 0, 0, 0);
+                // transpose through the wave's tile: T[c][r]
+                *(u32x4 *)(Tw + l16 * TL + 2 * q) = p0;
+                *(u32x4 *)(Tw + l16 * TL + 2 * q + 8) = p1;
+                wave_lds_sync();
+                // row product: column slot q of call j is column q + 4j
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    dr[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(Tw[(q + 4 * j) * TL + l16], xc[j], dr[tt], 0, 0, 0);
+                wave_lds_sync();
+            }
+        }
+        // column sums of the block: lane holds D[c0 + q + 4i][l16]
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int c = c0 + q + 4 * i;
+            if (c < m && l16 < nr) {
+                const int j = c - r0i;
+                if (j < nrow) atomicAdd(&yr[j * 16 + l16], dc[i]);
+                else ypart[(it.part_off + j) * nr + l16] = dc[i];
+            }
+        }
+    }
+    // row sums: lane holds D[strip row q + 4i][l16] per strip
+#pragma unroll
+    for (int tt = 0; tt < S; tt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = tt * R + q + 4 * i;
+            if (tt < ns && row < nrow && l16 < nr) atomicAdd(&yr[row * 16 + l16], dr[tt][i]);
+        }
+    __syncthreads();
+    for (int i = tid; i < nrow * 16; i += 256) {
+        const int j = i >> 4, v = i & 15;
+        if (v < nr) ypart[(it.part_off + j) * nr + v] = yr[i];
+    }
+}
+
+// yb[v * nbnd + boff + j] = sum of the block partials covering j (fixed order)
+__global__ void k_part_reduce_mfma(const SymvItem *items, const int *item0, const int *nitems, const int *M,
+                                   const int64_t *boff, int64_t nbnd, int nr, const double *ypart, double *yb) {
+    const int s = blockIdx.x;
+    const int m = M[s];
+    const int i0 = item0[s], ni = nitems[s];
+    for (int i = threadIdx.x; i < m * nr; i += blockDim.x) {
+        const int j = i / nr, v = i % nr;
+        double acc = 0.0;
+        for (int q = 0; q < ni; q++) {
+            const SymvItem it = items[i0 + q];
+            const int j0 = it.t0 * STRIP_F64;
+            if (j >= j0) acc += ypart[(it.part_off + j - j0) * nr + v];
+        }
+        yb[v * nbnd + boff[s] + j] = acc;
+    }
+}
+
+// xb[j nr + v] = sum coef * x[v nlam + lambda], v < nr
+__global__ void k_boundary_gather_multi(int64_t nb, int nr, const int64_t *bl_ptr, const int *bl_lam,
+                                        const double *coef, const double *x, int64_t nlam, double *xb) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nb * nr;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = i / nr;
+        const int v = (int)(i % nr);
+        double acc = 0.0;
+        for (int64_t q = bl_ptr[j]; q < bl_ptr[j + 1]; q++) acc += coef[q] * x[v * nlam + bl_lam[q]];
+        xb[i] = acc;
+    }
+}
+
 // lambda-space y = sum_e coef_e * yb[b_e]
 __global__ void k_lambda_gather(int nlam, const int *cnt, const int64_t *lb, const double *coef, const double *yb,
                                 double *y) {
@@ -700,68 +850,6 @@ __global__ void k_symmetrize(double *A, int n) {
     }
 }
 
-// ---- primal recovery: u_int = K_ii^-1 (f_int - K_ib u_b) via the banded factor ----
-// one workgroup per subdomain; work holds ni values per subdomain
-__global__ void __launch_bounds__(256) k_primal_interior(const PrimalDev P, const double *ub, double *u) {
-    __shared__ double red[4];
-    __shared__ double piv;
-    const int s = blockIdx.x;
-    const int ni = P.ni[s], n = P.n[s], bw = P.bw[s];
-    const int w = bw + 1;
-    const double *L = P.band + P.band_off[s];
-    const double *yf = P.yf + P.yf_off[s];
-    const int *perm = P.perm + P.perm_off[s];
-    const int *iperm = P.iperm + P.perm_off[s];
-    const int *rp = P.rowptr + P.rp_off[s];
-    const int *ci = P.colidx + P.nz_off[s];
-    const double *kv = P.vals + P.nz_off[s];
-    const double *ubs = ub + P.boff[s];
-    double *z = P.work + P.work_off[s];
-    double *us = u + P.perm_off[s];
-    // z = K_ib u_b (interior rows)
-    for (int r = threadIdx.x; r < ni; r += 256) {
-        const int lr = perm[r];
-        double v = 0.0;
-        for (int q = rp[lr]; q < rp[lr + 1]; q++) {
-            const int pc = iperm[ci[q]];
-            if (pc >= ni) v += kv[q] * ubs[pc - ni];
-        }
-        z[r] = v;
-    }
-    __syncthreads();
-    // z <- L^-1 z (column sweep)
-    for (int c = 0; c < ni; c++) {
-        if (threadIdx.x == 0) {
-            z[c] /= L[(int64_t)c * w];
-            piv = z[c];
-        }
-        __syncthreads();
-        const double zc = piv;
-        for (int d = 1 + threadIdx.x; d <= bw && c + d < ni; d += 256) z[c + d] -= L[(int64_t)c * w + d] * zc;
-        __syncthreads();
-    }
-    // z <- y_f - z
-    for (int r = threadIdx.x; r < ni; r += 256) z[r] = yf[r] - z[r];
-    __syncthreads();
-    // z <- L^-T z (row sweep from the bottom)
-    for (int c = ni - 1; c >= 0; c--) {
-        double v = 0.0;
-        for (int d = 1 + threadIdx.x; d <= bw && c + d < ni; d += 256) v += L[(int64_t)c * w + d] * z[c + d];
-        v = block_sum(v, red);
-        if (threadIdx.x == 0) z[c] = (z[c] - v) / L[(int64_t)c * w];
-        __syncthreads();
-    }
-    // scatter into local DOF order and add R alpha
-    const double *R = P.R + P.r_off[s];
-    const double *al = P.alpha + P.asub[s];
-    const int k = P.kdim[s];
-    for (int r = threadIdx.x; r < n; r += 256) {
-        const int lr = perm[r];
-        double v = r < ni ? z[r] : ubs[r - ni];
-        for (int a = 0; a < k; a++) v += R[lr + (int64_t)a * n] * al[a];
-        us[lr] = v;
-    }
-}
 
 // ------------------------------------------------------------ launchers ----
 static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
@@ -831,6 +919,23 @@ void launch_symv_block(const DualDev &D, const SymvBlock &B, bool sd, int K, con
         throw Error(ESPRESO_ERR_ARG, "block SYMV: K must be 1, 2, 4 or 8");
     }
 #undef ESP_BLOCK_CASE
+}
+
+void launch_symv_mfma(const DualDev &D, const SymvBlock &B, bool sd, int nr, const double *x, bool scaled,
+                      double *yb, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
+    if (nr < 1 || nr > 16) throw Error(ESPRESO_ERR_ARG, "block SYMV: 1 to 16 right-hand sides");
+    const double *coef = scaled ? D.bl_sscale : D.bl_sign;
+    const double *pool = sd ? D.SD : D.Fd;
+    const int64_t *off = sd ? D.sdoff : D.foff;
+    const unsigned gg = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(D.nbnd * nr, 256), 8192));
+    hipLaunchKernelGGL(k_boundary_gather_multi, dim3(gg), dim3(256), 0, st, D.nbnd, nr, D.bl_ptr, D.bl_lam, coef, x,
+                       (int64_t)D.nlam, B.xb);
+    if (ev0) (void)hipEventRecord(ev0, st);
+    hipLaunchKernelGGL(k_symv_mfma, dim3(B.n_items), dim3(256), 0, st, B.items, D.m, D.boff, pool, off, B.xb, nr,
+                       B.ypart);
+    if (ev1) (void)hipEventRecord(ev1, st);
+    hipLaunchKernelGGL(k_part_reduce_mfma, dim3(D.nsub), dim3(256), 0, st, B.items, B.item0, B.nitems, D.m, D.boff,
+                       D.nbnd, nr, B.ypart, yb);
 }
 
 void launch_lambda_gather(const DualDev &D, const double *yb, bool scaled, double *y, hipStream_t st) {
@@ -928,61 +1033,236 @@ __global__ void k_copy(const double4 *__restrict__ src, double4 *__restrict__ ds
         dst[i] = src[i];
 }
 
+// read-only stream: each thread sums 4 independent 16-byte nontemporal loads
+// per step; one double per workgroup keeps the loads alive
+__global__ void __launch_bounds__(256) k_stream_read(const u32x4 *__restrict__ src, int64_t n, double *out) {
+    __shared__ double red[4];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    unsigned acc = 0;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const u32x4 a = __builtin_nontemporal_load(src + i), b = __builtin_nontemporal_load(src + i + stride),
+                    c = __builtin_nontemporal_load(src + i + 2 * stride),
+                    d = __builtin_nontemporal_load(src + i + 3 * stride);
+        acc ^= (a.x ^ b.y) ^ (c.z ^ d.w);
+    }
+    for (; i < n; i += stride) acc ^= __builtin_nontemporal_load(src + i).x;
+    const double v = block_sum((double)acc, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+
+void launch_stream_read(const double *src, int64_t n, int wgs, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_stream_read, dim3(wgs), dim3(256), 0, st, (const u32x4 *)src, n / 2, out);
+}
+
 void launch_copy(const double *src, double *dst, int64_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, st, (const double4 *)src, (double4 *)dst, n / 4);
 }
 
-// ---- new load vector: y_f = L^-1 f_int (kept for the primal recovery) and
-// hf = f_b - K_bi K_ii^-1 f_int, so that (K^+ f)_b = F_i hf; one workgroup per subdomain
-__global__ void __launch_bounds__(256) k_rhs_prepare(const PrimalDev P, const double *f, double *hf) {
-    __shared__ double red[4];
-    __shared__ double piv;
+// ---- banded interior solves (K_ii = L L^T, L a lower band of width bw stored
+// by columns: L(c + d, c) at L[c w + d], w = bw + 1) for K vectors at once,
+// interleaved z[r K + v].  Blocked by 64 columns: wave 0 solves the diagonal
+// block with shuffles, then all threads apply the block to the rows it reaches.
+template <int K>
+__device__ __forceinline__ void band_lower_solve(const double *__restrict__ L, int w, int bw, int ni, double *z) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int c0 = 0; c0 < ni; c0 += 64) {
+        const int nb = min(64, ni - c0);
+        if (wave == 0) {
+            double zi[K];
+#pragma unroll
+            for (int v = 0; v < K; v++) zi[v] = lane < nb ? z[(int64_t)(c0 + lane) * K + v] : 0.0;
+            for (int c = 0; c < nb; c++) {
+                const int d = lane - c;
+                const double lv = (d > 0 && d <= bw && lane < nb) ? L[(int64_t)(c0 + c) * w + d] : 0.0;
+                const double dg = L[(int64_t)(c0 + c) * w];
+#pragma unroll
+                for (int v = 0; v < K; v++) {
+                    const double zc = __shfl(zi[v], c, 64) / dg;
+                    zi[v] = lane == c ? zc : zi[v] - lv * zc;
+                }
+            }
+            if (lane < nb)
+#pragma unroll
+                for (int v = 0; v < K; v++) z[(int64_t)(c0 + lane) * K + v] = zi[v];
+        }
+        __syncthreads();
+        const int rend = min(ni, c0 + nb + bw);
+        for (int r = c0 + nb + tid; r < rend; r += blockDim.x) {
+            double acc[K];
+#pragma unroll
+            for (int v = 0; v < K; v++) acc[v] = 0.0;
+            for (int c = max(c0, r - bw); c < c0 + nb; c++) {
+                const double l = L[(int64_t)c * w + (r - c)];
+#pragma unroll
+                for (int v = 0; v < K; v++) acc[v] += l * z[(int64_t)c * K + v];
+            }
+#pragma unroll
+            for (int v = 0; v < K; v++) z[(int64_t)r * K + v] -= acc[v];
+        }
+        __syncthreads();
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void band_upper_solve(const double *__restrict__ L, int w, int bw, int ni, double *z) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    for (int c0 = ((ni - 1) / 64) * 64; c0 >= 0; c0 -= 64) {
+        const int nb = min(64, ni - c0);
+        // solved rows below the block: z[c] -= sum_{r >= c0 + nb, r - c <= bw} L(r, c) z[r]
+        for (int cc = wave; cc < nb; cc += nw) {
+            const int c = c0 + cc;
+            const int rl = min(ni - 1, c + bw);
+            double acc[K];
+#pragma unroll
+            for (int v = 0; v < K; v++) acc[v] = 0.0;
+            for (int r = c0 + nb + lane; r <= rl; r += 64) {
+                const double l = L[(int64_t)c * w + (r - c)];
+#pragma unroll
+                for (int v = 0; v < K; v++) acc[v] += l * z[(int64_t)r * K + v];
+            }
+#pragma unroll
+            for (int v = 0; v < K; v++) acc[v] = wave_sum(acc[v]);
+            if (lane == 0)
+#pragma unroll
+                for (int v = 0; v < K; v++) z[(int64_t)c * K + v] -= acc[v];
+        }
+        __syncthreads();
+        if (wave == 0) {
+            double zi[K];
+#pragma unroll
+            for (int v = 0; v < K; v++) zi[v] = lane < nb ? z[(int64_t)(c0 + lane) * K + v] : 0.0;
+            for (int c = nb - 1; c >= 0; c--) {
+                const int d = c - lane;  // L(c, i) of lanes i < c
+                const double lv = (d > 0 && d <= bw) ? L[(int64_t)(c0 + lane) * w + d] : 0.0;
+                const double dg = L[(int64_t)(c0 + c) * w];
+#pragma unroll
+                for (int v = 0; v < K; v++) {
+                    const double zc = __shfl(zi[v], c, 64) / dg;
+                    zi[v] = lane == c ? zc : zi[v] - lv * zc;
+                }
+            }
+            if (lane < nb)
+#pragma unroll
+                for (int v = 0; v < K; v++) z[(int64_t)(c0 + lane) * K + v] = zi[v];
+        }
+        __syncthreads();
+    }
+}
+
+// ---- new load vectors (K at once): y_f = L^-1 f_int (kept for the primal
+// recovery) and hf = f_b - K_bi K_ii^-1 f_int, so that (K^+ f)_b = F_i hf.
+// f, hf and yf hold the K vectors at strides fs, hs, ys; z is the interleaved
+// work area (K values per interior row).  One workgroup per subdomain.
+template <int K>
+__global__ void __launch_bounds__(256) k_rhs_prepare(const PrimalDev P, const double *f, int64_t fstr, double *hf,
+                                                     int64_t hstr, double *yf, int64_t ystr, double *work) {
     const int s = blockIdx.x;
     const int ni = P.ni[s], m = P.m[s], bw = P.bw[s];
     const int w = bw + 1;
     const double *L = P.band + P.band_off[s];
-    double *yf = P.yf + P.yf_off[s];
     const int *perm = P.perm + P.perm_off[s];
     const int *iperm = P.iperm + P.perm_off[s];
     const int *rp = P.rowptr + P.rp_off[s];
     const int *ci = P.colidx + P.nz_off[s];
     const double *kv = P.vals + P.nz_off[s];
     const double *fs = f + P.perm_off[s];
-    double *z = P.work + P.work_off[s];
-    for (int r = threadIdx.x; r < ni; r += 256) z[r] = fs[perm[r]];
+    double *z = work + P.work_off[s] * K;
+    for (int i = threadIdx.x; i < ni * K; i += blockDim.x) z[i] = fs[(i % K) * fstr + perm[i / K]];
     __syncthreads();
-    for (int c = 0; c < ni; c++) {
-        if (threadIdx.x == 0) {
-            z[c] /= L[(int64_t)c * w];
-            piv = z[c];
-        }
-        __syncthreads();
-        const double zc = piv;
-        for (int d = 1 + threadIdx.x; d <= bw && c + d < ni; d += 256) z[c + d] -= L[(int64_t)c * w + d] * zc;
-        __syncthreads();
-    }
-    for (int r = threadIdx.x; r < ni; r += 256) yf[r] = z[r];
+    band_lower_solve<K>(L, w, bw, ni, z);
+    for (int i = threadIdx.x; i < ni * K; i += blockDim.x) yf[(i % K) * ystr + P.yf_off[s] + i / K] = z[i];
     __syncthreads();
-    for (int c = ni - 1; c >= 0; c--) {
-        double v = 0.0;
-        for (int d = 1 + threadIdx.x; d <= bw && c + d < ni; d += 256) v += L[(int64_t)c * w + d] * z[c + d];
-        v = block_sum(v, red);
-        if (threadIdx.x == 0) z[c] = (z[c] - v) / L[(int64_t)c * w];
-        __syncthreads();
-    }
-    for (int j = threadIdx.x; j < m; j += 256) {
+    band_upper_solve<K>(L, w, bw, ni, z);
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
         const int lr = perm[ni + j];
-        double v = fs[lr];
+        double v[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) v[q] = fs[q * fstr + lr];
         for (int q = rp[lr]; q < rp[lr + 1]; q++) {
             const int pc = iperm[ci[q]];
-            if (pc < ni) v -= kv[q] * z[pc];
+            if (pc < ni)
+#pragma unroll
+                for (int r = 0; r < K; r++) v[r] -= kv[q] * z[(int64_t)pc * K + r];
         }
-        hf[P.boff[s] + j] = v;
+#pragma unroll
+        for (int q = 0; q < K; q++) hf[q * hstr + P.boff[s] + j] = v[q];
+    }
+}
+
+// ---- primal recovery (K at once): u_int = K_ii^-1 (f_int - K_ib u_b) via the
+// banded factor, u = [u_int; u_b] + R alpha in local DOF order
+template <int K>
+__global__ void __launch_bounds__(256) k_primal_interior(const PrimalDev P, const double *ub, int64_t bstr,
+                                                         const double *yf, int64_t ystr, const double *alpha,
+                                                         int64_t astr, double *u, int64_t ustr, double *work) {
+    const int s = blockIdx.x;
+    const int ni = P.ni[s], n = P.n[s], bw = P.bw[s];
+    const int w = bw + 1;
+    const double *L = P.band + P.band_off[s];
+    const int *perm = P.perm + P.perm_off[s];
+    const int *iperm = P.iperm + P.perm_off[s];
+    const int *rp = P.rowptr + P.rp_off[s];
+    const int *ci = P.colidx + P.nz_off[s];
+    const double *kv = P.vals + P.nz_off[s];
+    const double *ubs = ub + P.boff[s];
+    double *z = work + P.work_off[s] * K;
+    // z = K_ib u_b (interior rows), then z <- y_f - L^-1 z = L^-1 (f_int - K_ib u_b)
+    // and z <- L^-T z
+    for (int r = threadIdx.x; r < ni; r += blockDim.x) {
+        const int lr = perm[r];
+        double v[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) v[q] = 0.0;
+        for (int q = rp[lr]; q < rp[lr + 1]; q++) {
+            const int pc = iperm[ci[q]];
+            if (pc >= ni)
+#pragma unroll
+                for (int t = 0; t < K; t++) v[t] += kv[q] * ubs[t * bstr + pc - ni];
+        }
+#pragma unroll
+        for (int q = 0; q < K; q++) z[(int64_t)r * K + q] = v[q];
+    }
+    __syncthreads();
+    band_lower_solve<K>(L, w, bw, ni, z);
+    for (int i = threadIdx.x; i < ni * K; i += blockDim.x) z[i] = yf[(i % K) * ystr + P.yf_off[s] + i / K] - z[i];
+    __syncthreads();
+    band_upper_solve<K>(L, w, bw, ni, z);
+    const double *R = P.R + P.r_off[s];
+    const int k = P.kdim[s];
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        const int lr = perm[r];
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const double *al = alpha + q * astr + P.asub[s];
+            double v = r < ni ? z[(int64_t)r * K + q] : ubs[q * bstr + r - ni];
+            for (int a = 0; a < k; a++) v += R[lr + (int64_t)a * n] * al[a];
+            u[q * ustr + P.perm_off[s] + lr] = v;
+        }
     }
 }
 
 void launch_rhs_prepare(const PrimalDev &P, int nsub, const double *f, double *hf, hipStream_t st) {
-    hipLaunchKernelGGL(k_rhs_prepare, dim3(nsub), dim3(256), 0, st, P, f, hf);
+    hipLaunchKernelGGL(k_rhs_prepare<1>, dim3(nsub), dim3(256), 0, st, P, f, (int64_t)0, hf, (int64_t)0, P.yf,
+                       (int64_t)0, P.work);
+}
+
+// K vectors in chunks of 16, 8, 4, 2, 1; work holds 16 values per interior row
+void launch_rhs_prepare_multi(const PrimalDev &P, int nsub, int nr, const double *f, int64_t fstr, double *hf,
+                              int64_t hstr, double *yf, int64_t ystr, double *work, hipStream_t st) {
+    for (int v0 = 0; v0 < nr;) {
+        const int r = nr - v0;
+        const int K = r >= 16 ? 16 : r >= 8 ? 8 : r >= 4 ? 4 : r >= 2 ? 2 : 1;
+        const double *fv = f + v0 * fstr;
+        double *hv = hf + v0 * hstr, *yv = yf + v0 * ystr;
+#define ESP_RHS_CASE(KK)                                                                                          \
+    case KK:                                                                                                      \
+        hipLaunchKernelGGL(k_rhs_prepare<KK>, dim3(nsub), dim3(256), 0, st, P, fv, fstr, hv, hstr, yv, ystr, work); \
+        break;
+        switch (K) { ESP_RHS_CASE(16) ESP_RHS_CASE(8) ESP_RHS_CASE(4) ESP_RHS_CASE(2) ESP_RHS_CASE(1) }
+#undef ESP_RHS_CASE
+        v0 += K;
+    }
 }
 
 // ---- structured grid: element sensitivities sens_e = -w_e^T K_e0 u_e ----
@@ -1024,7 +1304,27 @@ void launch_element_sensitivity(const int *NE, int nd, const double *Ke0, const 
 }
 
 void launch_primal(const PrimalDev &P, int nsub, const double *ub, double *u, hipStream_t st) {
-    hipLaunchKernelGGL(k_primal_interior, dim3(nsub), dim3(256), 0, st, P, ub, u);
+    hipLaunchKernelGGL(k_primal_interior<1>, dim3(nsub), dim3(256), 0, st, P, ub, (int64_t)0, P.yf, (int64_t)0,
+                       P.alpha, (int64_t)0, u, (int64_t)0, P.work);
+}
+
+void launch_primal_multi(const PrimalDev &P, int nsub, int nr, const double *ub, int64_t bstr, const double *yf,
+                         int64_t ystr, const double *alpha, int64_t astr, double *u, int64_t ustr, double *work,
+                         hipStream_t st) {
+    for (int v0 = 0; v0 < nr;) {
+        const int r = nr - v0;
+        const int K = r >= 16 ? 16 : r >= 8 ? 8 : r >= 4 ? 4 : r >= 2 ? 2 : 1;
+        const double *bv = ub + v0 * bstr, *yv = yf + v0 * ystr, *av = alpha + v0 * astr;
+        double *uv = u + v0 * ustr;
+#define ESP_PRIMAL_CASE(KK)                                                                                        \
+    case KK:                                                                                                       \
+        hipLaunchKernelGGL(k_primal_interior<KK>, dim3(nsub), dim3(256), 0, st, P, bv, bstr, yv, ystr, av, astr, uv, \
+                           ustr, work);                                                                            \
+        break;
+        switch (K) { ESP_PRIMAL_CASE(16) ESP_PRIMAL_CASE(8) ESP_PRIMAL_CASE(4) ESP_PRIMAL_CASE(2) ESP_PRIMAL_CASE(1) }
+#undef ESP_PRIMAL_CASE
+        v0 += K;
+    }
 }
 
 }  // namespace esp

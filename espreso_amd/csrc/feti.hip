@@ -217,21 +217,22 @@ struct espreso_feti_s {
     void compute_b_into(const double *hf, double *ubf, const double *e0f, double *b);
     void rhs_host(const double *f, std::vector<double> &e0f, std::vector<double> &e) const;
     void resolve_rhs();
-    // block re-solve of several load cases (K = 1, 2, 4, 8 per group)
+    // block re-solve of several load cases (groups of up to 16)
     struct BlockBuffers {
         int K = 0;
         double *x = nullptr, *r = nullptr, *w = nullptr, *z = nullptr, *y = nullptr, *p = nullptr, *Ap = nullptr,
                *b = nullptr, *e = nullptr, *alpha_c = nullptr, *alpha_tot = nullptr, *e0f = nullptr, *hf = nullptr,
                *ubf = nullptr, *xbv = nullptr, *yb = nullptr, *f = nullptr, *yf = nullptr, *u = nullptr,
-               *scal = nullptr;
+               *scal = nullptr, *work = nullptr;
     } BB;
     SymvBlock SB;
     int64_t nyf = 0;  // interior DOFs (length of P.yf)
     void block_setup(int K);
     void apply_F_block(int K, const double *x, double *y);
     void step_block(int K, const std::vector<char> &active, bool first);
-    void resolve_block(int nrhs, const double *f, double *u, int32_t *iters, double *norms);
-    void resolve_group(int K, const double *f, double *u, int32_t *iters, double *norms);
+    void resolve_block(int nrhs, const double *f, double *u, int32_t *iters, double *norms,
+                       espreso_feti_stats *stats);
+    void resolve_group(int K, const double *f, double *u, int32_t *iters, double *norms, espreso_feti_stats &tot);
     std::vector<double> Ns_h;                        // hybrid: N_s per subdomain (kdim x kN)
     std::vector<int64_t> noff_h;
     std::vector<int> ccoff_h, cblk_h;                // coarse block offset, block of each subdomain
@@ -627,6 +628,7 @@ void espreso_feti_s::compute_b_into(const double *hf, double *ubf, const double 
 // subdomain and the coarse right-hand side e (hybrid: per cluster through N_s)
 void espreso_feti_s::rhs_host(const double *f, std::vector<double> &e0f, std::vector<double> &e) const {
     e0f.assign(asubv[nsub], 0.0);
+#pragma omp parallel for schedule(dynamic, 16)
     for (int s = 0; s < nsub; s++) {
         const SubHost &h = sub[s];
         const double *fs = f + uoff[s];
@@ -1106,7 +1108,8 @@ void espreso_feti_s::solve(espreso_feti_stats *out) {
 // Several load cases on the assembled operators, solved together: one
 // projected PCG (itersolver.cpp:1537-1684) per case with its own scalars and
 // stop test, advanced in lockstep so that every F and Dirichlet apply reads
-// the F_i and S_D bytes once for the whole group (block SYMV).  Each case
+// the F_i and S_D bytes once for the whole group of up to 16 cases (block
+// SYMV on MFMA).  Each case
 // follows FETISolver::solve with its own Matrices::f (FETISolver.cpp:59-149).
 void espreso_feti_s::block_setup(int K) {
     if (!SB.items) {
@@ -1136,7 +1139,7 @@ void espreso_feti_s::block_setup(int K) {
     }
     if (BB.K >= K) return;
     for (double *q : {BB.x, BB.r, BB.w, BB.z, BB.y, BB.p, BB.Ap, BB.b, BB.e, BB.alpha_c, BB.alpha_tot, BB.e0f, BB.hf,
-                      BB.ubf, BB.xbv, BB.yb, BB.f, BB.yf, BB.u, BB.scal, SB.xb, SB.ypart})
+                      BB.ubf, BB.xbv, BB.yb, BB.f, BB.yf, BB.u, BB.scal, BB.work, SB.xb, SB.ypart})
         if (q) release(q);
     const int64_t nl = nlam, nb = D.nbnd, nc = std::max(ncoarse, 1), na = std::max<int64_t>(asubv[nsub], 1);
     BB.x = alloc<double>(K * nl);
@@ -1159,6 +1162,7 @@ void espreso_feti_s::block_setup(int K) {
     BB.yf = alloc<double>(K * std::max<int64_t>(nyf, 1));
     BB.u = alloc<double>(K * uoff[nsub]);
     BB.scal = alloc<double>(8 * K);
+    BB.work = alloc<double>(16 * std::max<int64_t>(nyf, 1));
     SB.xb = alloc<double>(K * nb);
     SB.ypart = alloc<double>(K * std::max<int64_t>(SB.plen, 1));
     BB.K = K;
@@ -1167,7 +1171,7 @@ void espreso_feti_s::block_setup(int K) {
 // Y = F X for K lambda-space vectors (one block SYMV, then the hybrid cluster
 // correction and the scatter per vector)
 void espreso_feti_s::apply_F_block(int K, const double *x, double *y) {
-    launch_symv_block(D, SB, false, K, x, false, BB.yb, st);
+    launch_symv_mfma(D, SB, false, K, x, false, BB.yb, st);
     for (int v = 0; v < K; v++) {
         double *ybv = BB.yb + v * D.nbnd;
         if (hybrid) {
@@ -1190,7 +1194,7 @@ void espreso_feti_s::step_block(int K, const std::vector<char> &active, bool fir
                 ESP_HIP(hipMemcpyAsync(BB.y + v * nl, BB.w + v * nl, nl * sizeof(double), hipMemcpyDeviceToDevice, st));
     } else {
         if (opt.preconditioner == ESPRESO_PREC_DIRICHLET) {
-            launch_symv_block(D, SB, true, K, BB.w, true, BB.yb, st);
+            launch_symv_mfma(D, SB, true, K, BB.w, true, BB.yb, st);
             for (int v = 0; v < K; v++)
                 if (active[v]) launch_lambda_gather(D, BB.yb + v * D.nbnd, true, BB.z + v * nl, st);
         } else {
@@ -1215,25 +1219,25 @@ void espreso_feti_s::step_block(int K, const std::vector<char> &active, bool fir
     }
 }
 
-void espreso_feti_s::resolve_group(int K, const double *f, double *u, int32_t *iters, double *norms) {
+void espreso_feti_s::resolve_group(int K, const double *f, double *u, int32_t *iters, double *norms,
+                                   espreso_feti_stats &tot) {
+    const double t0 = wall();
     block_setup(K);
     const int64_t nl = nlam, nb = D.nbnd, nt = uoff[nsub], na = std::max<int64_t>(asubv[nsub], 1);
     const int64_t nc = std::max(ncoarse, 1);
-    // loads: e0f, e on the host; f, y_f, hf and b on the device
+    // loads: e0f, e on the host; f, y_f, hf (one band-solve launch) and b on the device
     std::vector<double> e0f, e;
+    ESP_HIP(hipMemcpyAsync(BB.f, f, K * nt * sizeof(double), hipMemcpyHostToDevice, st));
     for (int v = 0; v < K; v++) {
         rhs_host(f + v * nt, e0f, e);
-        ESP_HIP(hipMemcpyAsync(BB.f + v * nt, f + v * nt, nt * sizeof(double), hipMemcpyHostToDevice, st));
         if (!e0f.empty())
             ESP_HIP(hipMemcpyAsync(BB.e0f + v * na, e0f.data(), e0f.size() * sizeof(double), hipMemcpyHostToDevice, st));
         if (!e.empty())
             ESP_HIP(hipMemcpyAsync(BB.e + v * nc, e.data(), e.size() * sizeof(double), hipMemcpyHostToDevice, st));
-        PrimalDev Pv = P;
-        Pv.yf = BB.yf + v * nyf;
-        launch_rhs_prepare(Pv, nsub, BB.f + v * nt, BB.hf + v * nb, st);
-        compute_b_into(BB.hf + v * nb, BB.ubf + v * nb, BB.e0f + v * na, BB.b + v * nl);
         sync();  // host vectors e0f, e are reused by the next case
     }
+    launch_rhs_prepare_multi(P, nsub, K, BB.f, nt, BB.hf, nb, BB.yf, nyf, BB.work, st);
+    for (int v = 0; v < K; v++) compute_b_into(BB.hf + v * nb, BB.ubf + v * nb, BB.e0f + v * na, BB.b + v * nl);
     // lambda0 = G^T (GG^T)^-1 e ; r = b - F lambda0 ; tol = precision * min(||P r0||, ||b||)
     for (int v = 0; v < K; v++) launch_project(D, BB.e + v * nc, BB.x + v * nl, 1, st);
     apply_F_block(K, BB.x, BB.Ap);
@@ -1246,6 +1250,7 @@ void espreso_feti_s::resolve_group(int K, const double *f, double *u, int32_t *i
     std::vector<double> hs(8 * K);
     ESP_HIP(hipMemcpyAsync(hs.data(), BB.scal, hs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     sync();
+    const double t1 = wall();
     std::vector<double> tol(K), nrm(K, 0.0);
     std::vector<char> active(K, 1);
     std::vector<int> it_v(K, 0);
@@ -1265,9 +1270,11 @@ void espreso_feti_s::resolve_group(int K, const double *f, double *u, int32_t *i
         }
         if (!any) break;
     }
+    sync();
+    const double t2 = wall();
     // primal per case: u = K~^+ (f - B^T lambda) + R~ alpha
     for (int v = 0; v < K; v++) launch_project(D, BB.r + v * nl, BB.alpha_c + v * nc, 2, st);
-    launch_symv_block(D, SB, false, K, BB.x, false, BB.yb, st);
+    launch_symv_mfma(D, SB, false, K, BB.x, false, BB.yb, st);
     for (int v = 0; v < K; v++) {
         double *xb = BB.xbv + v * nb;
         launch_axpby(xb, BB.ubf + v * nb, BB.yb + v * nb, nb, 1.0, -1.0, st);
@@ -1277,22 +1284,29 @@ void espreso_feti_s::resolve_group(int K, const double *f, double *u, int32_t *i
             launch_cluster_correction(D, CL, d_e0s, xb, 0, st);
         }
         launch_alpha_combine(D, CL, hybrid, BB.alpha_c + v * nc, BB.alpha_tot + v * na, st);
-        PrimalDev Pv = P;
-        Pv.yf = BB.yf + v * nyf;
-        Pv.alpha = BB.alpha_tot + v * na;
-        launch_primal(Pv, nsub, xb, BB.u + v * nt, st);
     }
+    launch_primal_multi(P, nsub, K, BB.xbv, nb, BB.yf, nyf, BB.alpha_tot, na, BB.u, nt, BB.work, st);
     ESP_HIP(hipMemcpyAsync(u, BB.u, K * nt * sizeof(double), hipMemcpyDeviceToHost, st));
     sync();
     for (int v = 0; v < K; v++) {
         if (iters) iters[v] = it_v[v];
         if (norms) norms[v] = nrm[v];
+        tot.iterations = std::max(tot.iterations, it_v[v]);
+        tot.final_norm = std::max(tot.final_norm, nrm[v]);
     }
+    tot.time_coarse += t1 - t0;
+    tot.time_pcg += t2 - t1;
+    tot.time_primal += wall() - t2;
 }
 
-void espreso_feti_s::resolve_block(int nrhs, const double *f, double *u, int32_t *iters, double *norms) {
+void espreso_feti_s::resolve_block(int nrhs, const double *f, double *u, int32_t *iters, double *norms,
+                                   espreso_feti_stats *stats) {
     ESP_CHECK(ready, ESPRESO_ERR_STATE, "espreso_feti_setup not called");
     const int64_t nt = uoff[nsub];
+    espreso_feti_stats tot{};
+    tot.n_lambda = nlam;
+    tot.n_coarse = ncoarse;
+    tot.n_subdomains = nsub;
     if (opt.sc_precision != 0 || D.sd_prec != 0) {
         // low-precision storage: the cases one after another (refinement per case)
         std::vector<std::vector<double>> keep(nsub);
@@ -1306,17 +1320,26 @@ void espreso_feti_s::resolve_block(int nrhs, const double *f, double *u, int32_t
             ESP_HIP(hipMemcpy(u + v * nt, d_u, nt * sizeof(double), hipMemcpyDeviceToHost));
             if (iters) iters[v] = sv.iterations;
             if (norms) norms[v] = sv.final_norm;
+            tot.iterations = std::max(tot.iterations, sv.iterations);
+            tot.final_norm = std::max(tot.final_norm, sv.final_norm);
+            tot.time_pcg += sv.time_pcg;
+            tot.time_primal += sv.time_primal;
         }
         for (int s = 0; s < nsub; s++) sub[s].f = keep[s];
         resolve_rhs();
+        tot.time_total = tot.time_pcg + tot.time_primal;
+        if (stats) *stats = tot;
         return;
     }
     for (int done = 0; done < nrhs;) {
         const int rem = nrhs - done;
-        const int K = rem >= 8 ? 8 : rem >= 4 ? 4 : rem >= 2 ? 2 : 1;
-        resolve_group(K, f + done * nt, u + done * nt, iters ? iters + done : nullptr, norms ? norms + done : nullptr);
+        const int K = std::min(rem, 16);
+        resolve_group(K, f + done * nt, u + done * nt, iters ? iters + done : nullptr, norms ? norms + done : nullptr,
+                      tot);
         done += K;
     }
+    tot.time_total = tot.time_coarse + tot.time_pcg + tot.time_primal;
+    if (stats) *stats = tot;
 }
 
 // --------------------------------------------------- coarse space layout ----
@@ -1817,14 +1840,14 @@ int espreso_feti_resolve(espreso_feti *h, espreso_feti_stats *st) {
 }
 
 int espreso_feti_resolve_block(espreso_feti *h, int32_t nrhs, const double *f, double *u, int32_t *iterations,
-                               double *final_norms) {
+                               double *final_norms, espreso_feti_stats *stats) {
     ESP_API_BEGIN
     ESP_CHECK(h && h->ready, ESPRESO_ERR_STATE, "espreso_feti_setup not called");
     ESP_CHECK(nrhs >= 0 && (nrhs == 0 || (f && u)), ESPRESO_ERR_ARG, "invalid block arguments");
     ESP_HIP(hipSetDevice(h->opt.device));
     for (int64_t q = 0; q < (int64_t)nrhs * h->uoff[h->nsub]; q++)
         ESP_CHECK(std::isfinite(f[q]), ESPRESO_ERR_NUMERIC, "non-finite load");
-    h->resolve_block(nrhs, f, u, iterations, final_norms);
+    h->resolve_block(nrhs, f, u, iterations, final_norms, stats);
     ESP_API_END
 }
 
@@ -1992,6 +2015,36 @@ int espreso_debug_copy(int64_t bytes, int32_t reps, double *ms) {
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(a);
     if (b) (void)hipFree(b);
+    ESP_HIP(e);
+    *ms = t / reps;
+    ESP_API_END
+}
+
+int espreso_debug_stream(int64_t bytes, int32_t reps, int32_t workgroups, double *ms) {
+    ESP_API_BEGIN
+    ESP_CHECK(bytes >= 16 && reps > 0 && workgroups > 0 && workgroups <= 1 << 20, ESPRESO_ERR_ARG,
+              "invalid stream arguments");
+    const int64_t n = bytes / 16 * 2;
+    double *a = nullptr, *o = nullptr;
+    ESP_HIP(hipMalloc(&a, n * sizeof(double)));
+    hipError_t e = hipMalloc(&o, (size_t)workgroups * sizeof(double));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipMemset(a, 0, n * sizeof(double));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float t = 0;
+    if (e == hipSuccess) {
+        launch_stream_read(a, n, workgroups, o, nullptr);
+        (void)hipEventRecord(e0, nullptr);
+        for (int r = 0; r < reps; r++) launch_stream_read(a, n, workgroups, o, nullptr);
+        (void)hipEventRecord(e1, nullptr);
+        e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    if (o) (void)hipFree(o);
     ESP_HIP(e);
     *ms = t / reps;
     ESP_API_END

@@ -185,16 +185,18 @@ int espreso_feti_resolve(espreso_feti *h, espreso_feti_stats *stats);
 /* Block re-solve: nrhs load cases on the assembled operators, solved
  * together -- one projected PCG per case (own scalars and stop test) in
  * lockstep, so that each F and Dirichlet apply reads the F_i / S_D bytes once
- * for a group of up to 8 cases.  f: nrhs consecutive load vectors, each the
+ * for a group of up to 16 cases.  f: nrhs consecutive load vectors, each the
  * subdomain-local loads of all subdomains in subdomain order (n_i values of
  * subdomain i, as espreso_feti_set_rhs takes them); u: the primal solutions
- * in the same layout; iterations and final_norms (per case) may be NULL.  The
+ * in the same layout; iterations and final_norms (per case) and stats may be
+ * NULL (stats: maximum iterations and norm over the cases, times of the load
+ * preparation in time_coarse, of the PCG and of the primal recovery).  The
  * instance's own loads are left as they were.  With low-precision F_i or S_D
  * storage the cases are solved one after another.  No counterpart in the
  * reference, which solves one Matrices::f per FETISolver::solve
  * (FETISolver.cpp:59-149, itersolver.cpp:1406-1724).                        */
 int espreso_feti_resolve_block(espreso_feti *h, int32_t nrhs, const double *f, double *u, int32_t *iterations,
-                               double *final_norms);
+                               double *final_norms, espreso_feti_stats *stats);
 /* sum over subdomains of f_i^T u_i for the last solution (the compliance
  * f^T u of the assembled problem)                                            */
 int espreso_feti_load_work(espreso_feti *h, double *c);
@@ -245,6 +247,10 @@ int espreso_debug_mfma(const double *a, const double *b, double *out);
 /* Plain device copy of `bytes` (read + write), `reps` times: mean ms per copy.
  * Used to calibrate HBM byte counters of the profiler.                      */
 int espreso_debug_copy(int64_t bytes, int32_t reps, double *ms);
+/* Read-only stream over `bytes` with `workgroups` workgroups of 256 threads,
+ * `reps` times: mean ms per pass (the achievable HBM read rate that the
+ * bandwidth-bound kernels are compared with).                               */
+int espreso_debug_stream(int64_t bytes, int32_t reps, int32_t workgroups, double *ms);
 int espreso_debug_tile_gemm(int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const double *A, int32_t lda,
                             const double *B, int32_t ldb, double *C, int32_t ldc, double alpha, double beta);
 

@@ -4,8 +4,8 @@ operators (block SYMV over F_i and S_D, one PCG per case in lockstep).
 Bars: every case within 1e-8 relative of the undecomposed direct solve of the
 same global mesh (FP64); the same primal as solving the cases one after
 another through resolve() (1e-9 relative) with iteration counts within
-max(2, 2 %) (block and single SYMV sum in different orders); group sizes
-8, 4, 2, 1 (13 cases), TFETI and HYBRID_FETI, Dirichlet and lumped; the
+max(2, 2 %) (block and single SYMV sum in different orders); groups of 16
+and 2 (18 cases), 5 and 3, TFETI and HYBRID_FETI, Dirichlet and lumped; the
 low-precision storage path (cases one after another) against FP64.
 """
 import sys
@@ -45,7 +45,7 @@ def sequential(g, loads):
     return np.array(us), np.array(its)
 
 
-@pytest.mark.parametrize("method,prec,ncases", [("hybrid", "dirichlet", 13), ("total", "dirichlet", 5),
+@pytest.mark.parametrize("method,prec,ncases", [("hybrid", "dirichlet", 18), ("total", "dirichlet", 5),
                                                  ("total", "lumped", 3)])
 def test_block_matches_sequential_and_direct(method, prec, ncases):
     spec = dict(GRAV, method=method)
