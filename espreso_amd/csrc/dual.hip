@@ -517,14 +517,18 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 }
 
 // per subdomain s: out[off[s] + a] = -sum_j R_s[j, a] (B_s^T x)_j for a < k[s]
-// (R_s: m_s x k_s column-major at R + roff[s]); one workgroup per subdomain.
+// (R_s: m_s x k_s column-major at R + roff[s]); one workgroup of 1024 threads
+// per subdomain (enough loads in flight for a latency-bound gather).
 // With the coarse basis this is G x = -(B R)^T x per subdomain.
-__global__ void __launch_bounds__(256) k_apply_G(const int *M, const int64_t *boff, const int *kcols,
-                                                 const int64_t *outoff, const int64_t *roff, const double *R,
-                                                 const int64_t *bl_ptr, const int *bl_lam, const double *sign,
-                                                 const double *x, double *out) {
+constexpr int APPLY_G_THREADS = 1024;
+__global__ void __launch_bounds__(APPLY_G_THREADS) k_apply_G(const int *M, const int64_t *boff, const int *kcols,
+                                                             const int64_t *outoff, const int64_t *roff,
+                                                             const double *R, const int64_t *bl_ptr,
+                                                             const int *bl_lam, const double *sign, const double *x,
+                                                             double *out) {
     constexpr int KB = 8;  // columns per pass
-    __shared__ double red[KB][4];
+    constexpr int NW = APPLY_G_THREADS / 64;
+    __shared__ double red[KB][NW];
     const int s = blockIdx.x, m = M[s], k = kcols[s];
     const int64_t b0 = boff[s];
     const double *Rs = R + roff[s];
@@ -534,7 +538,7 @@ __global__ void __launch_bounds__(256) k_apply_G(const int *M, const int64_t *bo
         double acc[KB];
 #pragma unroll
         for (int a = 0; a < KB; a++) acc[a] = 0.0;
-        for (int j = threadIdx.x; j < m; j += 256) {
+        for (int j = threadIdx.x; j < m; j += APPLY_G_THREADS) {
             double xb = 0.0;
             for (int64_t q = bl_ptr[b0 + j]; q < bl_ptr[b0 + j + 1]; q++) xb += sign[q] * x[bl_lam[q]];
 #pragma unroll
@@ -548,9 +552,11 @@ __global__ void __launch_bounds__(256) k_apply_G(const int *M, const int64_t *bo
             if (lane == 0) red[a][w] = v;
         }
         __syncthreads();
-        if (threadIdx.x < kb)
-            out[outoff[s] + a0 + threadIdx.x] =
-                -((red[threadIdx.x][0] + red[threadIdx.x][1]) + (red[threadIdx.x][2] + red[threadIdx.x][3]));
+        if (threadIdx.x < kb) {
+            double v = 0.0;
+            for (int q = 0; q < NW; q++) v += red[threadIdx.x][q];
+            out[outoff[s] + a0 + threadIdx.x] = -v;
+        }
         __syncthreads();
     }
 }
@@ -770,16 +776,18 @@ __global__ void __launch_bounds__(256) k_cluster_apply(const ClusterDev C, const
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= m) return;
     const double *FB = C.FB0 + C.fb0off[s] + j;
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    // eight independent chains: all loads of a group issue before their use
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int q = 0;
-    for (; q + 4 <= nq; q += 4) {
-        v0 -= FB[(int64_t)q * m] * lam[q];
-        v1 -= FB[(int64_t)(q + 1) * m] * lam[q + 1];
-        v2 -= FB[(int64_t)(q + 2) * m] * lam[q + 2];
-        v3 -= FB[(int64_t)(q + 3) * m] * lam[q + 3];
+    for (; q + 8 <= nq; q += 8) {
+        double f[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) f[i] = FB[(int64_t)(q + i) * m];
+#pragma unroll
+        for (int i = 0; i < 8; i++) acc[i] -= f[i] * lam[q + i];
     }
-    for (; q < nq; q++) v0 -= FB[(int64_t)q * m] * lam[q];
-    double v = (v0 + v1) + (v2 + v3);
+    for (; q < nq; q++) acc[q & 7] -= FB[(int64_t)q * m] * lam[q];
+    double v = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     if (addR) {
         const double *R = Rb + rboff[s] + j;
         const double *al = alphaK + C.asub[s];
@@ -955,7 +963,7 @@ void launch_kbb(const DualDev &D, const double *xb, double *yb, hipStream_t st) 
 void launch_project(const DualDev &D, const double *x, double *y, int mode, hipStream_t st) {
     // mode 0: y = P x;  mode 1: y = G^T GGinv x (x coarse);  mode 2: y = GGinv G x (coarse out)
     if (mode != 1) {
-        hipLaunchKernelGGL(k_apply_G, dim3(D.nsub), dim3(256), 0, st, D.m, D.boff, D.ck, D.dsub_off, D.rtoff, D.Rt,
+        hipLaunchKernelGGL(k_apply_G, dim3(D.nsub), dim3(APPLY_G_THREADS), 0, st, D.m, D.boff, D.ck, D.dsub_off, D.rtoff, D.Rt,
                            D.bl_ptr, D.bl_lam, D.bl_sign, x, D.dsub);
         hipLaunchKernelGGL(k_block_sum, dim3(vgrid((int64_t)D.nblk * 64)), dim3(256), 0, st, D.nblk, D.blk_ptr,
                            D.blk_sub, D.dsub_off, D.ck, D.coff, D.dsub, D.cd);
@@ -969,7 +977,7 @@ void launch_project(const DualDev &D, const double *x, double *y, int mode, hipS
 }
 
 void launch_subdomain_kernel_dots(const DualDev &D, const double *x, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_apply_G, dim3(D.nsub), dim3(256), 0, st, D.m, D.boff, D.kdim, D.asub, D.rboff, D.Rb,
+    hipLaunchKernelGGL(k_apply_G, dim3(D.nsub), dim3(APPLY_G_THREADS), 0, st, D.m, D.boff, D.kdim, D.asub, D.rboff, D.Rb,
                        D.bl_ptr, D.bl_lam, D.bl_sign, x, out);
 }
 
@@ -1063,19 +1071,34 @@ void launch_copy(const double *src, double *dst, int64_t n, hipStream_t st) {
 // by columns: L(c + d, c) at L[c w + d], w = bw + 1) for K vectors at once,
 // interleaved z[r K + v].  Blocked by 64 columns: wave 0 solves the diagonal
 // block with shuffles, then all threads apply the block to the rows it reaches.
+constexpr int BAND_DL = 65;  // leading dimension of the staged diagonal block
+
+// stage the lower band of columns [c0, c0 + nb) restricted to the diagonal
+// block into Ds[c * BAND_DL + d] = L(c0 + c + d, c0 + c) (d <= bw, c + d < nb)
+__device__ __forceinline__ void band_stage_diag(const double *__restrict__ L, int w, int bw, int c0, int nb,
+                                                double *Ds) {
+    for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
+        const int c = e >> 6, d = e & 63;
+        Ds[c * BAND_DL + d] = (c < nb && d <= bw && c + d < nb) ? L[(int64_t)(c0 + c) * w + d] : 0.0;
+    }
+}
+
 template <int K>
-__device__ __forceinline__ void band_lower_solve(const double *__restrict__ L, int w, int bw, int ni, double *z) {
+__device__ __forceinline__ void band_lower_solve(const double *__restrict__ L, int w, int bw, int ni, double *z,
+                                                 double *Ds) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int c0 = 0; c0 < ni; c0 += 64) {
         const int nb = min(64, ni - c0);
+        band_stage_diag(L, w, bw, c0, nb, Ds);
+        __syncthreads();
         if (wave == 0) {
             double zi[K];
 #pragma unroll
             for (int v = 0; v < K; v++) zi[v] = lane < nb ? z[(int64_t)(c0 + lane) * K + v] : 0.0;
             for (int c = 0; c < nb; c++) {
                 const int d = lane - c;
-                const double lv = (d > 0 && d <= bw && lane < nb) ? L[(int64_t)(c0 + c) * w + d] : 0.0;
-                const double dg = L[(int64_t)(c0 + c) * w];
+                const double lv = d > 0 ? Ds[c * BAND_DL + d] : 0.0;
+                const double dg = Ds[c * BAND_DL];
 #pragma unroll
                 for (int v = 0; v < K; v++) {
                     const double zc = __shfl(zi[v], c, 64) / dg;
@@ -1092,6 +1115,7 @@ __device__ __forceinline__ void band_lower_solve(const double *__restrict__ L, i
             double acc[K];
 #pragma unroll
             for (int v = 0; v < K; v++) acc[v] = 0.0;
+#pragma unroll 8
             for (int c = max(c0, r - bw); c < c0 + nb; c++) {
                 const double l = L[(int64_t)c * w + (r - c)];
 #pragma unroll
@@ -1105,10 +1129,12 @@ __device__ __forceinline__ void band_lower_solve(const double *__restrict__ L, i
 }
 
 template <int K>
-__device__ __forceinline__ void band_upper_solve(const double *__restrict__ L, int w, int bw, int ni, double *z) {
+__device__ __forceinline__ void band_upper_solve(const double *__restrict__ L, int w, int bw, int ni, double *z,
+                                                 double *Ds) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
     for (int c0 = ((ni - 1) / 64) * 64; c0 >= 0; c0 -= 64) {
         const int nb = min(64, ni - c0);
+        band_stage_diag(L, w, bw, c0, nb, Ds);
         // solved rows below the block: z[c] -= sum_{r >= c0 + nb, r - c <= bw} L(r, c) z[r]
         for (int cc = wave; cc < nb; cc += nw) {
             const int c = c0 + cc;
@@ -1116,6 +1142,7 @@ __device__ __forceinline__ void band_upper_solve(const double *__restrict__ L, i
             double acc[K];
 #pragma unroll
             for (int v = 0; v < K; v++) acc[v] = 0.0;
+#pragma unroll 4
             for (int r = c0 + nb + lane; r <= rl; r += 64) {
                 const double l = L[(int64_t)c * w + (r - c)];
 #pragma unroll
@@ -1134,8 +1161,8 @@ __device__ __forceinline__ void band_upper_solve(const double *__restrict__ L, i
             for (int v = 0; v < K; v++) zi[v] = lane < nb ? z[(int64_t)(c0 + lane) * K + v] : 0.0;
             for (int c = nb - 1; c >= 0; c--) {
                 const int d = c - lane;  // L(c, i) of lanes i < c
-                const double lv = (d > 0 && d <= bw) ? L[(int64_t)(c0 + lane) * w + d] : 0.0;
-                const double dg = L[(int64_t)(c0 + c) * w];
+                const double lv = (d > 0 && lane < nb) ? Ds[lane * BAND_DL + d] : 0.0;
+                const double dg = Ds[c * BAND_DL];
 #pragma unroll
                 for (int v = 0; v < K; v++) {
                     const double zc = __shfl(zi[v], c, 64) / dg;
@@ -1168,12 +1195,13 @@ __global__ void __launch_bounds__(256) k_rhs_prepare(const PrimalDev P, const do
     const double *kv = P.vals + P.nz_off[s];
     const double *fs = f + P.perm_off[s];
     double *z = work + P.work_off[s] * K;
+    __shared__ double Ds[64 * BAND_DL];
     for (int i = threadIdx.x; i < ni * K; i += blockDim.x) z[i] = fs[(i % K) * fstr + perm[i / K]];
     __syncthreads();
-    band_lower_solve<K>(L, w, bw, ni, z);
+    band_lower_solve<K>(L, w, bw, ni, z, Ds);
     for (int i = threadIdx.x; i < ni * K; i += blockDim.x) yf[(i % K) * ystr + P.yf_off[s] + i / K] = z[i];
     __syncthreads();
-    band_upper_solve<K>(L, w, bw, ni, z);
+    band_upper_solve<K>(L, w, bw, ni, z, Ds);
     for (int j = threadIdx.x; j < m; j += blockDim.x) {
         const int lr = perm[ni + j];
         double v[K];
@@ -1207,6 +1235,7 @@ __global__ void __launch_bounds__(256) k_primal_interior(const PrimalDev P, cons
     const double *kv = P.vals + P.nz_off[s];
     const double *ubs = ub + P.boff[s];
     double *z = work + P.work_off[s] * K;
+    __shared__ double Ds[64 * BAND_DL];
     // z = K_ib u_b (interior rows), then z <- y_f - L^-1 z = L^-1 (f_int - K_ib u_b)
     // and z <- L^-T z
     for (int r = threadIdx.x; r < ni; r += blockDim.x) {
@@ -1224,10 +1253,10 @@ __global__ void __launch_bounds__(256) k_primal_interior(const PrimalDev P, cons
         for (int q = 0; q < K; q++) z[(int64_t)r * K + q] = v[q];
     }
     __syncthreads();
-    band_lower_solve<K>(L, w, bw, ni, z);
+    band_lower_solve<K>(L, w, bw, ni, z, Ds);
     for (int i = threadIdx.x; i < ni * K; i += blockDim.x) z[i] = yf[(i % K) * ystr + P.yf_off[s] + i / K] - z[i];
     __syncthreads();
-    band_upper_solve<K>(L, w, bw, ni, z);
+    band_upper_solve<K>(L, w, bw, ni, z, Ds);
     const double *R = P.R + P.r_off[s];
     const int k = P.kdim[s];
     for (int r = threadIdx.x; r < n; r += blockDim.x) {
