@@ -47,6 +47,8 @@ class GridProblem(C.Structure):
         ("conductivity_seed", C.c_uint64),
         ("material_elem", C.POINTER(C.c_double)),
         ("slab_elements", C.POINTER(C.c_int32)),
+        ("device_assembly", C.c_int32),
+        ("pad_", C.c_int32),
     ]
 
 
@@ -115,6 +117,7 @@ SIGNATURES = {
     "espreso_grid_nodal_load": (C.c_int, [_H, _D]),
     "espreso_feti_resolve": (C.c_int, [_H, C.POINTER(Stats)]),
     "espreso_feti_resolve_block": (C.c_int, [_H, C.c_int32, _D, _D, _I, _D, C.POINTER(Stats)]),
+    "espreso_feti_subdomain_values": (C.c_int, [_H, C.c_int32, _D]),
     "espreso_feti_create": (C.c_int, [C.POINTER(_H), C.c_int32, C.POINTER(Options)]),
     "espreso_feti_destroy": (C.c_int, [_H]),
     "espreso_feti_sizes": (C.c_int, [_H, _I]),
@@ -226,6 +229,7 @@ def grid_problem(spec):
     p.traction_scale = spec.get("traction_scale", 1.0)
     p.young_seed = spec.get("young_seed", 0)
     p.conductivity_seed = spec.get("conductivity_seed", 0)
+    p.device_assembly = int(bool(spec.get("device_assembly", False)))
     keep = []  # arrays the struct points to must outlive it
     ye = spec.get("material_elem")
     if ye is not None:
@@ -460,6 +464,12 @@ class FETISolver:
         b = np.zeros(m.value, dtype=np.int32)
         _check(lib().espreso_feti_subdomain_boundary(self.h, i, C.byref(m), _ip(b)))
         return b
+
+    def values(self, i, nnz):
+        """K_i CSR values as the device holds them (after setup)."""
+        v = np.zeros(nnz)
+        _check(lib().espreso_feti_subdomain_values(self.h, i, _dp(v)))
+        return v
 
     def dense(self, i, which):
         m = len(self.boundary(i))

@@ -164,6 +164,13 @@ void launch_primal_multi(const PrimalDev &P, int nsub, int nr, const double *ub,
 // sens[e] = -w_e^T Ke0 u_e over the NE[0] x NE[1] x NE[2] HEXA8 grid (nodal fields, ndof nd)
 void launch_element_sensitivity(const int *NE, int nd, const double *Ke0, const double *u, const double *w,
                                 double *sens, hipStream_t st);
+// structured grid: K_i CSR values on the device (box: 6 ints per subdomain,
+// first global element and elements per direction; NE: global elements)
+void launch_grid_stiffness(int nsub, int nd, const int *box, const int *NE, const double *Ke, const double *escale,
+                           const int *rowptr, const int64_t *rp_off, const int64_t *nz_off, int max_nnz, double *vals,
+                           hipStream_t st);
+// dst[i] = src[pos[i]]
+void launch_gather_values(int64_t n, const int64_t *pos, const double *src, double *dst, hipStream_t st);
 // plain copy of n doubles (n multiple of 4): calibration of HBM byte counters
 void launch_copy(const double *src, double *dst, int64_t n, hipStream_t st);
 // read-only stream over n doubles (n even) with `wgs` workgroups (out: wgs doubles)

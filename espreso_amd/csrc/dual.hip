@@ -1294,6 +1294,59 @@ void launch_rhs_prepare_multi(const PrimalDev &P, int nsub, int nr, const double
     }
 }
 
+// ---- structured grid on the device: K_i values (CSR of the grid builder)
+// from the unit element matrix Ke and the element material scales.  One
+// thread per CSR entry (row = local node x DOF, columns = the existing
+// neighbours of the node in (z, y, x) offset order, DOFs inside); the entry
+// sums scale_e * Ke over the elements holding both nodes in increasing (z, y,
+// x) element order with separately rounded products and sums -- the order
+// and rounding of the host builder's element loop, so the values agree bit
+// for bit (structuralmechanics3d.cpp:353-509, physics.cpp:147-228).
+__device__ __forceinline__ int hexa_corner(int cx, int cy, int cz) { return 4 * cz + (cy ? (cx ? 2 : 3) : (cx ? 1 : 0)); }
+
+__global__ void __launch_bounds__(256) k_grid_stiffness(int nd, const int *box, int NE0, int NE1, const double *Ke,
+                                                        const double *escale, const int *rowptr,
+                                                        const int64_t *rp_off, const int64_t *nz_off, double *vals) {
+    const int s = blockIdx.y;
+    const int *bx = box + 6 * s;
+    const int gx0 = bx[0], gy0 = bx[1], gz0 = bx[2], ex = bx[3], ey = bx[4], ez = bx[5];
+    const int lx = ex + 1, ly = ey + 1, lz = ez + 1;
+    const int n = lx * ly * lz * nd, ne = 8 * nd;
+    const int *rp = rowptr + rp_off[s];
+    const int nnz = rp[n];
+    double *v = vals + nz_off[s];
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nnz; q += gridDim.x * blockDim.x) {
+#pragma clang fp contract(off)
+        int lo = 0, hi = n;  // row r with rp[r] <= q < rp[r + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (rp[mid] <= q) lo = mid;
+            else hi = mid;
+        }
+        const int r = lo, ln = r / nd, i = r % nd;
+        const int pos = q - rp[r], slot = pos / nd, j = pos % nd;
+        const int x = ln % lx, y = (ln / lx) % ly, z = ln / (lx * ly);
+        const int nxo = 1 + (x > 0) + (x < lx - 1), nyo = 1 + (y > 0) + (y < ly - 1);
+        const int x2 = x + slot % nxo - (x > 0), y2 = y + (slot / nxo) % nyo - (y > 0),
+                  z2 = z + slot / (nxo * nyo) - (z > 0);
+        double acc = 0.0;
+        for (int e2 = max(max(z, z2) - 1, 0); e2 <= min(min(z, z2), ez - 1); e2++)
+            for (int e1 = max(max(y, y2) - 1, 0); e1 <= min(min(y, y2), ey - 1); e1++)
+                for (int e0 = max(max(x, x2) - 1, 0); e0 <= min(min(x, x2), ex - 1); e0++) {
+                    const int a = hexa_corner(x - e0, y - e1, z - e2), b = hexa_corner(x2 - e0, y2 - e1, z2 - e2);
+                    const double sc = escale[(gx0 + e0) + (int64_t)NE0 * ((gy0 + e1) + (int64_t)NE1 * (gz0 + e2))];
+                    const double t = sc * Ke[(nd * a + i) * ne + nd * b + j];  // rounded product (no FMA)
+                    acc = acc + t;
+                }
+        v[q] = acc;
+    }
+}
+
+__global__ void k_gather_values(int64_t n, const int64_t *pos, const double *src, double *dst) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[pos[i]];
+}
+
 // ---- structured grid: element sensitivities sens_e = -w_e^T K_e0 u_e ----
 __global__ void __launch_bounds__(256) k_element_sensitivity(int NE0, int NE1, int NE2, int nd, const double *Ke0,
                                                              const double *u, const double *w, double *sens) {
@@ -1330,6 +1383,18 @@ void launch_element_sensitivity(const int *NE, int nd, const double *Ke0, const 
     const int ne = 8 * nd;
     hipLaunchKernelGGL(k_element_sensitivity, dim3(vgrid(nel)), dim3(256), (size_t)ne * ne * 8, st, NE[0], NE[1],
                        NE[2], nd, Ke0, u, w, sens);
+}
+
+void launch_grid_stiffness(int nsub, int nd, const int *box, const int *NE, const double *Ke, const double *escale,
+                           const int *rowptr, const int64_t *rp_off, const int64_t *nz_off, int max_nnz, double *vals,
+                           hipStream_t st) {
+    const unsigned gx = (unsigned)std::max(1, std::min((max_nnz + 255) / 256, 2048));
+    hipLaunchKernelGGL(k_grid_stiffness, dim3(gx, nsub), dim3(256), 0, st, nd, box, NE[0], NE[1], Ke, escale, rowptr,
+                       rp_off, nz_off, vals);
+}
+
+void launch_gather_values(int64_t n, const int64_t *pos, const double *src, double *dst, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_gather_values, dim3(vgrid(n)), dim3(256), 0, st, n, pos, src, dst);
 }
 
 void launch_primal(const PrimalDev &P, int nsub, const double *ub, double *u, hipStream_t st) {

@@ -238,6 +238,10 @@ struct espreso_feti_s {
     std::vector<int> ccoff_h, cblk_h;                // coarse block offset, block of each subdomain
     int gridNE[3] = {0, 0, 0};                       // structured grid front end
     std::vector<double> gridKe0;
+    bool grid_dev = false;                           // K_i values assembled on the device
+    std::vector<int> grid_box;                       // 6 per subdomain
+    std::vector<double> grid_escale;                 // element material scales
+    std::vector<int64_t> h_nzoff;                    // CSR value offsets per subdomain
     void invert_spd_batched(double *pool, const std::vector<int64_t> &off, const std::vector<int> &n,
                             const char *what);
     std::vector<int> n0_h, kt_h;
@@ -375,13 +379,16 @@ void espreso_feti_s::load() {
     for (int s = 0; s < nsub; s++)
         ESP_CHECK(rank_ok[s], ESPRESO_ERR_NUMERIC,
                   "kernel restricted to the boundary of subdomain " + std::to_string(s) + " is rank deficient");
-    std::vector<int64_t> kbb_ptr(nb + 1, 0), kbb_col;
-    std::vector<double> kbb_val;
+    // K_bb in boundary indices: columns and positions of the values in the
+    // concatenated CSR value array (gathered on the device once K is there)
+    std::vector<int64_t> kbb_ptr(nb + 1, 0), kbb_col, kbb_pos;
+    h_nzoff.assign(nsub + 1, 0);
+    for (int s = 0; s < nsub; s++) h_nzoff[s + 1] = h_nzoff[s] + sub[s].rowptr[sub[s].n];
     {
         int64_t tot = 0;
         for (int s = 0; s < nsub; s++) tot += kbb_cnt[s];
         kbb_col.reserve(tot);
-        kbb_val.reserve(tot);
+        kbb_pos.reserve(tot);
         for (int s = 0; s < nsub; s++) {
             const SubHost &h = sub[s];
             for (int j = 0; j < h.m; j++) {
@@ -390,7 +397,7 @@ void espreso_feti_s::load() {
                     const int pc = h.iperm[h.colidx[k]];
                     if (pc >= h.ni) {
                         kbb_col.push_back(boff[s] + pc - h.ni);
-                        kbb_val.push_back(h.vals[k]);
+                        kbb_pos.push_back(h_nzoff[s] + k);
                     }
                 }
                 kbb_ptr[boff[s] + j + 1] = (int64_t)kbb_col.size();
@@ -442,7 +449,8 @@ void espreso_feti_s::load() {
     D.le_sscale = upload(le_ss);
     D.kbb_ptr = upload(kbb_ptr);
     D.kbb_col = upload(kbb_col);
-    D.kbb_val = upload(kbb_val);
+    D.kbb_val = alloc<double>(kbb_pos.size());
+    int64_t *d_kbb_pos = upload(kbb_pos);
     D.Rb = upload(Rb);
     D.yb = alloc<double>(nb);
     D.xb = alloc<double>(nb);
@@ -504,7 +512,7 @@ void espreso_feti_s::load() {
             const SubHost &h = sub[s];
             std::copy(h.rowptr.begin(), h.rowptr.end(), rp.begin() + rp_off[s]);
             std::copy(h.colidx.begin(), h.colidx.end(), ci.begin() + nz_off[s]);
-            std::copy(h.vals.begin(), h.vals.end(), kv.begin() + nz_off[s]);
+            if (!grid_dev) std::copy(h.vals.begin(), h.vals.end(), kv.begin() + nz_off[s]);
             std::copy(h.perm.begin(), h.perm.end(), pm.begin() + n_off[s]);
             std::copy(h.iperm.begin(), h.iperm.end(), ip.begin() + n_off[s]);
             std::copy(h.f.begin(), h.f.end(), ff.begin() + n_off[s]);
@@ -515,7 +523,24 @@ void espreso_feti_s::load() {
         }
         P.rowptr = upload(rp);
         P.colidx = upload(ci);
-        P.vals = upload(kv);
+        int max_nnz = 0;
+        for (int s = 0; s < nsub; s++) max_nnz = std::max<int64_t>(max_nnz, nz_off[s + 1] - nz_off[s]);
+        if (grid_dev) {
+            // structured grid: K_i values from the unit element matrix and the element materials
+            P.vals = alloc<double>(nz_off[nsub]);
+            int *d_box = upload(grid_box);
+            double *d_Ke = upload(gridKe0), *d_esc = upload(grid_escale);
+            int64_t *d_rpo = upload(rp_off), *d_nzo = upload(nz_off);
+            launch_grid_stiffness(nsub, ndof, d_box, gridNE, d_Ke, d_esc, P.rowptr, d_rpo, d_nzo, max_nnz, P.vals, st);
+            sync();
+            for (void *q : {(void *)d_box, (void *)d_Ke, (void *)d_esc, (void *)d_rpo, (void *)d_nzo})
+                release(q);
+        } else {
+            P.vals = upload(kv);
+        }
+        launch_gather_values((int64_t)kbb_col.size(), d_kbb_pos, P.vals, D.kbb_val, st);
+        sync();
+        release(d_kbb_pos);
         P.perm = upload(pm);
         P.iperm = upload(ip);
         P.R = upload(RR);
@@ -1760,10 +1785,12 @@ int espreso_feti_set_subdomain(espreso_feti *h, int32_t i, int32_t n, const int3
     ESP_CHECK(s.rowptr[0] == 0, ESPRESO_ERR_ARG, "CSR must be 0-based");
     const int nnz = s.rowptr[n];
     s.colidx.assign(colidx, colidx + nnz);
-    s.vals.assign(values, values + nnz);
+    ESP_CHECK(values || h->grid_dev, ESPRESO_ERR_ARG, "null K values");
+    if (values) s.vals.assign(values, values + nnz);
+    else s.vals.clear();  // assembled on the device (structured grid)
     for (int q = 0; q < nnz; q++) {
         ESP_CHECK(s.colidx[q] >= 0 && s.colidx[q] < n, ESPRESO_ERR_ARG, "CSR column out of range");
-        ESP_CHECK(std::isfinite(s.vals[q]), ESPRESO_ERR_NUMERIC, "non-finite value in K");
+        ESP_CHECK(!values || std::isfinite(s.vals[q]), ESPRESO_ERR_NUMERIC, "non-finite value in K");
     }
     s.f.assign(f, f + n);
     for (double v : s.f) ESP_CHECK(std::isfinite(v), ESPRESO_ERR_NUMERIC, "non-finite value in f");
@@ -1836,6 +1863,16 @@ int espreso_feti_resolve(espreso_feti *h, espreso_feti_stats *st) {
     const double trhs = wall() - t0;
     h->solve(st);
     if (st) st->time_coarse = trhs;
+    ESP_API_END
+}
+
+int espreso_feti_subdomain_values(espreso_feti *h, int32_t i, double *vals) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && h->ready && vals, ESPRESO_ERR_STATE, "espreso_feti_setup not called");
+    ESP_CHECK(i >= 0 && i < h->nsub, ESPRESO_ERR_ARG, "subdomain out of range");
+    ESP_HIP(hipSetDevice(h->opt.device));
+    const int64_t nnz = h->h_nzoff[i + 1] - h->h_nzoff[i];
+    ESP_HIP(hipMemcpy(vals, h->P.vals + h->h_nzoff[i], nnz * sizeof(double), hipMemcpyDeviceToHost));
     ESP_API_END
 }
 
@@ -2102,15 +2139,20 @@ int espreso_grid_load(espreso_feti *h, const espreso_grid_problem *p) {
     ESP_CHECK(h && p, ESPRESO_ERR_ARG, "null argument");
     ESP_CHECK(grid_subdomain_count(*p) == h->nsub, ESPRESO_ERR_ARG, "subdomain count differs from the instance");
     GridProblem g;
-    grid_build(*p, h->opt.scaling, g);
+    h->grid_dev = p->device_assembly != 0;
+    grid_build(*p, h->opt.scaling, g, !h->grid_dev);
     h->ndof = g.ndof;
     h->nnodes = g.nnodes;
     for (int d = 0; d < 3; d++) h->gridNE[d] = g.NN[d] - 1;
-    grid_unit_element(*p, h->gridKe0);
+    h->gridKe0 = g.Ke;
+    h->grid_box.resize(6 * (size_t)h->nsub);
+    for (int s = 0; s < h->nsub; s++) std::copy(g.sub[s].box, g.sub[s].box + 6, h->grid_box.begin() + 6 * s);
+    if (h->grid_dev) h->grid_escale.swap(g.escale);
     h->l2g.resize(h->nsub);
     for (int s = 0; s < h->nsub; s++) {
         GridSubdomain &sd = g.sub[s];
-        int rc = espreso_feti_set_subdomain(h, s, sd.n, sd.rowptr.data(), sd.colidx.data(), sd.vals.data(),
+        int rc = espreso_feti_set_subdomain(h, s, sd.n, sd.rowptr.data(), sd.colidx.data(),
+                                            h->grid_dev ? nullptr : sd.vals.data(),
                                             sd.f.data(), sd.kdim, sd.R.data(), (int)sd.lam.size(), sd.lam.data(),
                                             sd.dof.data(), sd.sign.data(), sd.scale.data());
         if (rc) return rc;
@@ -2135,7 +2177,7 @@ int espreso_grid_export(const espreso_grid_problem *p, int32_t scaling, int32_t 
     ESP_CHECK(p && sizes, ESPRESO_ERR_ARG, "null argument");
     ESP_CHECK(i >= 0 && i < grid_subdomain_count(*p), ESPRESO_ERR_ARG, "subdomain out of range");
     GridProblem g;
-    grid_build(*p, scaling, g);
+    grid_build(*p, scaling, g, p->device_assembly == 0);
     const GridSubdomain &sd = g.sub[i];
     sizes[0] = sd.n;
     sizes[1] = sd.rowptr[sd.n];
@@ -2145,7 +2187,10 @@ int espreso_grid_export(const espreso_grid_problem *p, int32_t scaling, int32_t 
     sizes[5] = (int)g.sub.size();
     if (rowptr) std::copy(sd.rowptr.begin(), sd.rowptr.end(), rowptr);
     if (colidx) std::copy(sd.colidx.begin(), sd.colidx.end(), colidx);
-    if (values) std::copy(sd.vals.begin(), sd.vals.end(), values);
+    if (values) {
+        if (sd.vals.empty()) std::fill(values, values + sd.rowptr[sd.n], 0.0);  // device-assembled values
+        else std::copy(sd.vals.begin(), sd.vals.end(), values);
+    }
     if (f) std::copy(sd.f.begin(), sd.f.end(), f);
     if (R) std::copy(sd.R.begin(), sd.R.end(), R);
     if (lambda) std::copy(sd.lam.begin(), sd.lam.end(), lambda);

@@ -216,7 +216,7 @@ int grid_node_count(const espreso_grid_problem &p) {
     return b.NN[0] * b.NN[1] * b.NN[2];
 }
 
-void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out) {
+void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out, bool values) {
     Box B(p);
     const int nd = B.nd, ns = B.nsub();
     out.ndof = nd;
@@ -242,6 +242,7 @@ void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out) {
     }
     if (p.material_elem)
         for (int e = 0; e < nel; e++) escale[e] = p.material_elem[e];
+    out.escale = escale;
     std::vector<double> trac;
     if (p.traction_seed) {
         SplitMix r(p.traction_seed);
@@ -251,6 +252,7 @@ void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out) {
     std::vector<double> Ke;
     unit_element(p, B.h, Ke);
     const int ne = 8 * nd;
+    out.Ke = Ke;
 
 #pragma omp parallel for schedule(dynamic)
     for (int s = 0; s < ns; s++) {
@@ -261,6 +263,12 @@ void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out) {
         const int lx = ex + 1, ly = ey + 1, lz = ez + 1;
         const int nloc = lx * ly * lz;
         const int gx0 = B.off[0][S[0]], gy0 = B.off[1][S[1]], gz0 = B.off[2][S[2]];
+        sd.box[0] = gx0;
+        sd.box[1] = gy0;
+        sd.box[2] = gz0;
+        sd.box[3] = ex;
+        sd.box[4] = ey;
+        sd.box[5] = ez;
         sd.n = nloc * nd;
         sd.l2g.resize(nloc);
         // neighbour position table: for node ln, slot of the 27 offsets among its existing neighbours
@@ -284,7 +292,7 @@ void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out) {
             for (int c = 0; c < nd; c++) sd.rowptr[ln * nd + c + 1] = nnbr[ln] * nd;
         for (int r = 0; r < sd.n; r++) sd.rowptr[r + 1] += sd.rowptr[r];
         sd.colidx.resize(sd.rowptr[sd.n]);
-        sd.vals.assign(sd.rowptr[sd.n], 0.0);
+        if (values) sd.vals.assign(sd.rowptr[sd.n], 0.0);
         for (int z = 0; z < lz; z++)
             for (int y = 0; y < ly; y++)
                 for (int x = 0; x < lx; x++) {
@@ -308,7 +316,7 @@ void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out) {
                     int ln[8];
                     for (int a = 0; a < 8; a++)
                         ln[a] = (x + CORNER[a][0]) + lx * ((y + CORNER[a][1]) + ly * (z + CORNER[a][2]));
-                    for (int a = 0; a < 8; a++)
+                    for (int a = 0; a < 8 && values; a++)
                         for (int b = 0; b < 8; b++) {
                             const int da = CORNER[b][0] - CORNER[a][0] + 1, db = CORNER[b][1] - CORNER[a][1] + 1,
                                       dc = CORNER[b][2] - CORNER[a][2] + 1;
@@ -354,9 +362,26 @@ void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out) {
     };
     auto diag_of = [&](int s, int dof) {
         const GridSubdomain &sd = out.sub[s];
-        for (int q = sd.rowptr[dof]; q < sd.rowptr[dof + 1]; q++)
-            if (sd.colidx[q] == dof) return sd.vals[q];
-        return 0.0;
+        if (values) {
+            for (int q = sd.rowptr[dof]; q < sd.rowptr[dof + 1]; q++)
+                if (sd.colidx[q] == dof) return sd.vals[q];
+            return 0.0;
+        }
+        // the same sum as the assembled diagonal: elements around the node in
+        // (z, y, x) order, each adding scale * Ke
+        const int ln = dof / nd, i = dof % nd;
+        const int lx = sd.box[3] + 1, ly = sd.box[4] + 1;
+        const int x = ln % lx, y = (ln / lx) % ly, z = ln / (lx * ly);
+        double v = 0.0;
+        for (int ez0 = std::max(z - 1, 0); ez0 <= std::min(z, sd.box[5] - 1); ez0++)
+            for (int ey0 = std::max(y - 1, 0); ey0 <= std::min(y, sd.box[4] - 1); ey0++)
+                for (int ex0 = std::max(x - 1, 0); ex0 <= std::min(x, sd.box[3] - 1); ex0++) {
+                    int a = 0;
+                    while (CORNER[a][0] != x - ex0 || CORNER[a][1] != y - ey0 || CORNER[a][2] != z - ez0) a++;
+                    const double sc = out.escale[(sd.box[0] + ex0) + B.NE[0] * ((sd.box[1] + ey0) + B.NE[1] * (sd.box[2] + ez0))];
+                    v += sc * Ke[(nd * a + i) * ne + nd * a + i];
+                }
+        return v;
     };
     out.c.clear();
     int lam = 0;

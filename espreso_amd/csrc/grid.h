@@ -22,6 +22,7 @@ struct GridSubdomain {
     std::vector<double> R;           // n x kdim, orthonormal columns
     std::vector<int> lam, dof;       // lambda rows touching the subdomain, ascending
     std::vector<double> sign, scale;
+    int box[6] = {0, 0, 0, 0, 0, 0};  // first global element (x, y, z) and elements per direction
     // hybrid FETI: cluster id and the B0 corner entries (cluster-local row, DOF, +1/-1)
     int cluster = 0;
     std::vector<int> b0row, b0dof;
@@ -31,6 +32,8 @@ struct GridSubdomain {
 struct GridProblem {
     int ndof = 0, nnodes = 0, nlam = 0;
     int NN[3] = {0, 0, 0};
+    std::vector<double> escale;      // material scale of every element (ex + NEx (ey + NEy ez))
+    std::vector<double> Ke;          // unit element matrix (8 nd x 8 nd, row-major)
     std::vector<GridSubdomain> sub;
     std::vector<double> c;
 };
@@ -39,7 +42,9 @@ int grid_subdomain_count(const espreso_grid_problem &p);
 int grid_node_count(const espreso_grid_problem &p);
 // unit-modulus element matrix (row-major, node-major DOFs, reference corner order)
 void grid_unit_element(const espreso_grid_problem &p, std::vector<double> &Ke);
-// scaling: 1 = K-diagonal weights, 0 = 1/multiplicity
-void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out);
+// scaling: 1 = K-diagonal weights, 0 = 1/multiplicity; values = false leaves
+// GridSubdomain::vals empty (structure only: the K_i values are assembled on
+// the device from escale and Ke, in the same element order)
+void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out, bool values = true);
 
 }  // namespace esp

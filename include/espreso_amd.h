@@ -71,6 +71,11 @@ typedef struct {
     const int32_t *slab_elements; /* optional uneven subdomains: elements of every
                                     subdomain slab, x slabs then y then z (NSx+NSy+NSz
                                     values); overrides elements[]                      */
+    int32_t device_assembly;    /* espreso_grid_load: 1 = the K_i values are computed on
+                                   the GPU from the unit element matrix and the element
+                                   materials (the host builds structure, loads, kernels
+                                   and B1/B0 only); 0 = host-assembled values         */
+    int32_t pad_;
 } espreso_grid_problem;
 
 /* Values follow the reference enums (src/config/ecf/solver/feti.h). */
@@ -132,7 +137,8 @@ int espreso_grid_load(espreso_feti *h, const espreso_grid_problem *p);
 /* Host-only export of subdomain i of the builder (no device needed).
  * sizes: [0] n, [1] nnz, [2] kdim, [3] lambda rows of i, [4] total lambda,
  * [5] subdomains.  Pass NULL arrays to query sizes only.  c (total lambda)
- * may be NULL.                                                               */
+ * may be NULL.  With p->device_assembly the builder runs as it does for the
+ * device assembly (structure, loads, kernels, B1 only) and values are 0.    */
 int espreso_grid_export(const espreso_grid_problem *p, int32_t scaling, int32_t i, int32_t *sizes,
                         int32_t *rowptr, int32_t *colidx, double *values, double *f, double *R,
                         int32_t *lambda, int32_t *dof, double *sign, double *scale, double *c);
@@ -226,6 +232,10 @@ int espreso_feti_apply_F(espreso_feti *h, const double *x, double *y);
 int espreso_feti_apply_prec(espreso_feti *h, const double *x, double *y);
 int espreso_feti_project(espreso_feti *h, const double *x, double *y);
 int espreso_feti_vec_b(espreso_feti *h, double *b);
+/* K_i CSR values of subdomain i as the device holds them (after setup; the
+ * CSR of espreso_feti_set_subdomain, or of the grid builder with
+ * device_assembly), for parity checks of the device assembly              */
+int espreso_feti_subdomain_values(espreso_feti *h, int32_t i, double *vals);
 /* dense m_i x m_i (column-major, boundary DOFs in ascending local order):
  * which 0: F_i = (K^+)_bb in FP64, 1: S_D, 2: F_i as stored (after rounding) */
 int espreso_feti_subdomain_dense(espreso_feti *h, int32_t i, int32_t which, double *out);

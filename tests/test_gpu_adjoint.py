@@ -7,7 +7,9 @@ to the oracle's reference -u_e^T K_e0 u_e from the undecomposed direct solve
 refinement); Euler's relation sum_e E_e dc/dE_e = -c (1e-8); central finite
 differences on the 40^3 workload (h = 1e-6 relative): a random direction over
 all elements (1e-5) and 20 single elements, whose tolerance is set by the
-size of the change of c (printed with each case).
+size of the change of c against the accuracy of c itself: the compliances of
+the differences are solved to a relative dual residual of 1e-13, and c is
+taken to be good to 1e-13 c (printed with each case).
 """
 import sys
 
@@ -24,6 +26,7 @@ pytestmark = pytest.mark.gpu
 
 SMALL = PB.scaled(PB.ADJOINT_64x10, domains=(2, 2, 2), elements=(4, 4, 4))
 OPTS = dict(preconditioner="dirichlet", precision=1e-12, max_iterations=2000)
+FD_OPTS = dict(OPTS, precision=1e-13, max_iterations=4000)  # compliances of the finite differences
 
 
 def say(*a):
@@ -121,20 +124,20 @@ def test_gradient_finite_differences_40cube():
     rng = np.random.default_rng(7)
     d = rng.choice([-1.0, 1.0], size=E.size) * E
     h = 1e-6
-    fd = (compliance(spec, E + h * d) - compliance(spec, E - h * d)) / (2 * h)
+    fd = (compliance(spec, E + h * d, **FD_OPTS) - compliance(spec, E - h * d, **FD_OPTS)) / (2 * h)
     ad = g @ d
     say("directional: fd", fd, "adjoint", ad, "rel", abs(fd - ad) / abs(ad))
     assert abs(fd - ad) <= 1e-5 * abs(ad)
-    # single elements: h = 1e-6 E_e changes c by ~2e-6 E_e |g_e|; the solve
-    # reproduces c to ~1e-14 c, which bounds what the difference can resolve
+    # single elements: h = 1e-6 E_e changes c by ~2e-6 E_e |g_e|; the solves
+    # give c to ~1e-13 c, which bounds what the difference can resolve
     errs = []
     for e in rng.choice(E.size, 20, replace=False):
         he = 1e-6 * E[e]
         Ep, Em = E.copy(), E.copy()
         Ep[e] += he
         Em[e] -= he
-        fd = (compliance(spec, Ep) - compliance(spec, Em)) / (2 * he)
-        noise = 1e-14 * c / he
+        fd = (compliance(spec, Ep, **FD_OPTS) - compliance(spec, Em, **FD_OPTS)) / (2 * he)
+        noise = 1e-13 * c / he
         errs.append(abs(fd - g[e]) / abs(g[e]))
         say(f"element {e}: fd {fd:.9e} adjoint {g[e]:.9e} rel {errs[-1]:.2e} (resolvable {noise / abs(g[e]):.1e})")
         assert abs(fd - g[e]) <= 1e-5 * abs(g[e]) + 3 * noise
