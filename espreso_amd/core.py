@@ -118,6 +118,7 @@ SIGNATURES = {
     "espreso_feti_resolve": (C.c_int, [_H, C.POINTER(Stats)]),
     "espreso_feti_resolve_block": (C.c_int, [_H, C.c_int32, _D, _D, _I, _D, C.POINTER(Stats)]),
     "espreso_feti_subdomain_values": (C.c_int, [_H, C.c_int32, _D]),
+    "espreso_grid_resolve_block": (C.c_int, [_H, C.c_int32, _D, _D, _I, _D, C.POINTER(Stats)]),
     "espreso_feti_create": (C.c_int, [C.POINTER(_H), C.c_int32, C.POINTER(Options)]),
     "espreso_feti_destroy": (C.c_int, [_H]),
     "espreso_feti_sizes": (C.c_int, [_H, _I]),
@@ -538,19 +539,14 @@ class GridFETI(FETISolver):
         split over the subdomain copies of a node by multiplicity (as
         set_nodal_rhs), solutions averaged over the copies (as nodal_solution).
         Returns (u (nrhs, n_nodes, ndof), iterations, final norms)."""
-        nr_, nn, nd = len(loads), self.n_nodes, self.ndof
-        loads = np.asarray(loads, dtype=np.float64).reshape(nr_, nn, nd)
-        if getattr(self, "_gidx", None) is None:
-            self._gidx = np.concatenate([self.l2g(i) for i in range(self.n_sub)]).astype(np.int64)
-            self._mult = np.bincount(self._gidx, minlength=nn).astype(np.float64)
-        gidx, mult = self._gidx, self._mult
-        f = (loads[:, gidx, :] / mult[gidx][None, :, None]).reshape(nr_, -1)
-        u, it, nr = self.resolve_block(f)
-        u = u.reshape(nr_, len(gidx), nd)
-        un = np.empty_like(loads)
-        for q in range(nr_):
-            for d in range(nd):
-                un[q, :, d] = np.bincount(gidx, weights=u[q, :, d], minlength=nn) / mult
+        loads = np.ascontiguousarray(loads, dtype=np.float64).reshape(len(loads), self.n_nodes, self.ndof)
+        nrhs = loads.shape[0]
+        un = np.zeros_like(loads)
+        it = np.zeros(nrhs, np.int32)
+        nr = np.zeros(nrhs)
+        st = Stats()
+        _check(lib().espreso_grid_resolve_block(self.h, nrhs, _dp(loads), _dp(un), _ip(it), _dp(nr), C.byref(st)))
+        self.block_stats = st.as_dict()
         return un, it, nr
 
 

@@ -136,6 +136,15 @@ void launch_subdomain_kernel_dots(const DualDev &D, const double *x, double *out
 // y_b += -K^+ B0^T lambda0 (+ R alphaK when addR) after solving the cluster problems
 void launch_cluster_correction(const DualDev &D, const ClusterDev &C, const double *e0s, double *yb, int addR,
                                hipStream_t st);
+// the correction for nr cases of a block (case strides ystr, tstr/lstr, astr):
+// t_v = F0^-1 B0 y_b,v into t (the lambda0 layout), then per case alpha and
+// lambda0 (launch_cluster_alpha), then y_b,v += -F B0^T lambda0_v + R alphaK_v
+void launch_cluster_t_multi(const DualDev &D, const ClusterDev &C, int nr, const double *yb, int64_t ystr, double *t,
+                            int64_t tstr, hipStream_t st);
+void launch_cluster_alpha(const ClusterDev &C, const double *e0s, double *lam0, double *alphaK, hipStream_t st);
+void launch_cluster_apply_multi(const DualDev &D, const ClusterDev &C, int nr, const double *lam0, int64_t lstr,
+                                const double *alphaK, int64_t astr, int addR, double *yb, int64_t ystr,
+                                hipStream_t st);
 // set-up: F0 (into C.F0inv) from the corner blocks; H = F0^-1 G0^T and the
 // regularised Sa (into Sa) once C.F0inv holds the inverse
 void launch_cluster_setup_f0(const ClusterDev &C, hipStream_t st);

@@ -796,6 +796,95 @@ __global__ void __launch_bounds__(256) k_cluster_apply(const ClusterDev C, const
     yb[boff[s] + j] += v;
 }
 
+// ---- the cluster correction for up to 8 cases per pass (block re-solve):
+// F0^-1 and F_i B0^T are read once per pass instead of once per case.
+// t_v = F0^-1 g0_v, g0_v gathered from the case's SYMV output; 4 rows per
+// wave, kc <= 8 cases (g0 of the kc cases in LDS); grid (rows/16, clusters).
+__global__ void __launch_bounds__(256) k_cluster_t_multi(const ClusterDev C, const double *yb, int64_t ystr,
+                                                         int kc, double *t, int64_t tstr) {
+    extern __shared__ double g0[];  // n0 x kc
+    const int c = blockIdx.y;
+    const int n0 = C.n0[c];
+    const int rb = blockIdx.x * 16;
+    if (rb >= n0) return;  // uniform over the workgroup
+    const int64_t ro = C.rowoff[c];
+    for (int e = threadIdx.x; e < n0 * kc; e += 256) {
+        const int r = e / kc, v = e % kc;
+        g0[e] = yb[v * ystr + C.row_b1[ro + r]] - yb[v * ystr + C.row_b2[ro + r]];
+    }
+    __syncthreads();
+    const double *F = C.F0inv + C.f0off[c];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r0 = rb + w * 4;
+    double acc[4][8];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int v = 0; v < 8; v++) acc[i][v] = 0.0;
+    for (int q = lane; q < n0; q += 64) {
+        double f[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) f[i] = r0 + i < n0 ? F[q + (int64_t)(r0 + i) * n0] : 0.0;
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            if (v < kc) {
+                const double gq = g0[q * kc + v];
+#pragma unroll
+                for (int i = 0; i < 4; i++) acc[i][v] += f[i] * gq;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            if (v < kc) {
+                const double sv = wave_sum(acc[i][v]);
+                if (lane == 0 && r0 + i < n0) t[v * tstr + ro + r0 + i] = sv;
+            }
+        }
+}
+
+// y_b,v += -FB0_s lambda0_v + addR * R_b alphaK_v for kc <= 8 cases;
+// grid (row chunks of 256, subdomains)
+__global__ void __launch_bounds__(256) k_cluster_apply_multi(const ClusterDev C, const int *M, const int64_t *boff,
+                                                             const int *kdim, const int64_t *rboff, const double *Rb,
+                                                             const double *lambda0, int64_t lstr,
+                                                             const double *alphaK, int64_t astr, int addR, int kc,
+                                                             double *yb, int64_t ystr) {
+    __shared__ double lam[256 * 8];
+    const int s = blockIdx.y, m = M[s];
+    if (blockIdx.x * 256 >= m) return;  // uniform
+    const int nq = C.nq[s];
+    const int64_t q0 = C.qoff[s];
+    for (int e = threadIdx.x; e < nq * kc; e += 256) {
+        const int q = e / kc, v = e % kc;
+        lam[e] = lambda0[v * lstr + C.qrow[q0 + q]];
+    }
+    __syncthreads();
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= m) return;
+    const double *FB = C.FB0 + C.fb0off[s] + j;
+    double acc[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) acc[v] = 0.0;
+    for (int q = 0; q < nq; q++) {
+        const double f = FB[(int64_t)q * m];
+#pragma unroll
+        for (int v = 0; v < 8; v++)
+            if (v < kc) acc[v] -= f * lam[q * kc + v];
+    }
+    const double *R = Rb + rboff[s] + j;
+    for (int v = 0; v < kc; v++) {
+        double val = acc[v];
+        if (addR) {
+            const double *al = alphaK + v * astr + C.asub[s];
+            for (int a = 0; a < kdim[s]; a++) val += R[(int64_t)a * m] * al[a];
+        }
+        yb[v * ystr + boff[s] + j] += val;
+    }
+}
+
 // alpha_tot(s) = addK * alphaK(s) + N_s alpha_coarse(block(s))
 __global__ void k_alpha_combine(int nsub, const int *kdim, const int *ck, const int *cblk, const int *coff,
                                 const int64_t *asub, const int64_t *noff, const double *N, const double *alphaK,
@@ -990,6 +1079,36 @@ void launch_cluster_correction(const DualDev &D, const ClusterDev &C, const doub
                        C.lambda0, C.alphaK);
     hipLaunchKernelGGL(k_cluster_apply, dim3((D.max_m + 255) / 256, D.nsub), dim3(256), 0, st, C, D.m, D.boff, D.kdim,
                        D.rboff, D.Rb, C.lambda0, C.alphaK, addR, yb);
+}
+
+// t_v = F0^-1 B0 y_b,v for nr cases (case stride ystr in yb, tstr in t)
+void launch_cluster_t_multi(const DualDev &D, const ClusterDev &C, int nr, const double *yb, int64_t ystr, double *t,
+                            int64_t tstr, hipStream_t st) {
+    if (C.max_n0 <= 0) return;
+    const int kc_max = (int)std::max<int64_t>(1, std::min<int64_t>(8, 6144 / C.max_n0));
+    for (int v0 = 0; v0 < nr; v0 += kc_max) {
+        const int kc = std::min(kc_max, nr - v0);
+        hipLaunchKernelGGL(k_cluster_t_multi, dim3((C.max_n0 + 15) / 16, C.ncl), dim3(256),
+                           (size_t)C.max_n0 * kc * 8, st, C, yb + v0 * ystr, ystr, kc, t + v0 * tstr, tstr);
+    }
+    (void)D;
+}
+
+// alpha and lambda0 of one case from its t (in lam0) and e0s
+void launch_cluster_alpha(const ClusterDev &C, const double *e0s, double *lam0, double *alphaK, hipStream_t st) {
+    hipLaunchKernelGGL(k_cluster_alpha, dim3(C.ncl), dim3(1024), 2 * (size_t)std::max(C.max_kt, 1) * 8, st, C, e0s,
+                       lam0, alphaK);
+}
+
+void launch_cluster_apply_multi(const DualDev &D, const ClusterDev &C, int nr, const double *lam0, int64_t lstr,
+                                const double *alphaK, int64_t astr, int addR, double *yb, int64_t ystr,
+                                hipStream_t st) {
+    for (int v0 = 0; v0 < nr; v0 += 8) {
+        const int kc = std::min(8, nr - v0);
+        hipLaunchKernelGGL(k_cluster_apply_multi, dim3((D.max_m + 255) / 256, D.nsub), dim3(256), 0, st, C, D.m,
+                           D.boff, D.kdim, D.rboff, D.Rb, lam0 + v0 * lstr, lstr, alphaK + v0 * astr, astr, addR, kc,
+                           yb + v0 * ystr, ystr);
+    }
 }
 
 void launch_cluster_setup_f0(const ClusterDev &C, hipStream_t st) {

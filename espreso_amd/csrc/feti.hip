@@ -160,6 +160,11 @@ struct espreso_feti_s {
 
     ~espreso_feti_s() {
         drop_graphs();
+        for (Lane &l : lanes) {
+            if (l.ev) (void)hipEventDestroy(l.ev);
+            if (l.s) (void)hipStreamDestroy(l.s);
+        }
+        if (ev_main) (void)hipEventDestroy(ev_main);
         for (void *p : allocs) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -223,12 +228,31 @@ struct espreso_feti_s {
         double *x = nullptr, *r = nullptr, *w = nullptr, *z = nullptr, *y = nullptr, *p = nullptr, *Ap = nullptr,
                *b = nullptr, *e = nullptr, *alpha_c = nullptr, *alpha_tot = nullptr, *e0f = nullptr, *hf = nullptr,
                *ubf = nullptr, *xbv = nullptr, *yb = nullptr, *f = nullptr, *yf = nullptr, *u = nullptr,
-               *scal = nullptr, *work = nullptr;
+               *scal = nullptr, *work = nullptr, *e0s = nullptr, *lam0 = nullptr, *alphaK = nullptr;
     } BB;
     SymvBlock SB;
     int64_t nyf = 0;  // interior DOFs (length of P.yf)
+    int64_t n_dsub = 1, n_lambda0 = 1, n_e0s = 1;  // scratch sizes (per-case lanes)
+    // per-case work of a block runs on NLANE streams (default priority), each
+    // with its own copies of the scratch the projector, the cluster correction
+    // and the dot products use
+    static constexpr int NLANE = 4;
+    struct Lane {
+        hipStream_t s = nullptr;
+        hipEvent_t ev = nullptr;
+        double *dsub = nullptr, *cd = nullptr, *ce = nullptr, *lambda0 = nullptr, *alphaK = nullptr,
+               *e0s = nullptr, *partial = nullptr;
+        DualDev D;
+        ClusterDev C;
+    } lanes[NLANE];
+    hipEvent_t ev_main = nullptr;
+    bool lanes_ready = false;
+    void lanes_setup();
+    void fork_lanes();
+    void join_lanes();
     void block_setup(int K);
     void apply_F_block(int K, const double *x, double *y);
+    void cluster_block(int K, const double *x);
     void step_block(int K, const std::vector<char> &active, bool first);
     void resolve_block(int nrhs, const double *f, double *u, int32_t *iters, double *norms,
                        espreso_feti_stats *stats);
@@ -459,7 +483,8 @@ void espreso_feti_s::load() {
     d_hf = alloc<double>(nb);
     d_ubf = alloc<double>(nb);
     d_e0f = upload(e);
-    d_e0s = alloc<double>(coffv[nsub]);
+    n_e0s = std::max<int64_t>(1, coffv[nsub]);
+    d_e0s = alloc<double>(n_e0s);
     d_alpha_tot = alloc<double>(coffv[nsub]);
     double *d_Q = upload(Q);
 
@@ -1164,7 +1189,8 @@ void espreso_feti_s::block_setup(int K) {
     }
     if (BB.K >= K) return;
     for (double *q : {BB.x, BB.r, BB.w, BB.z, BB.y, BB.p, BB.Ap, BB.b, BB.e, BB.alpha_c, BB.alpha_tot, BB.e0f, BB.hf,
-                      BB.ubf, BB.xbv, BB.yb, BB.f, BB.yf, BB.u, BB.scal, BB.work, SB.xb, SB.ypart})
+                      BB.ubf, BB.xbv, BB.yb, BB.f, BB.yf, BB.u, BB.scal, BB.work, BB.e0s, BB.lam0, BB.alphaK,
+                      SB.xb, SB.ypart})
         if (q) release(q);
     const int64_t nl = nlam, nb = D.nbnd, nc = std::max(ncoarse, 1), na = std::max<int64_t>(asubv[nsub], 1);
     BB.x = alloc<double>(K * nl);
@@ -1188,66 +1214,137 @@ void espreso_feti_s::block_setup(int K) {
     BB.u = alloc<double>(K * uoff[nsub]);
     BB.scal = alloc<double>(8 * K);
     BB.work = alloc<double>(16 * std::max<int64_t>(nyf, 1));
+    BB.e0s = alloc<double>(K * n_e0s);
+    BB.lam0 = alloc<double>(K * n_lambda0);
+    BB.alphaK = alloc<double>(K * na);
     SB.xb = alloc<double>(K * nb);
     SB.ypart = alloc<double>(K * std::max<int64_t>(SB.plen, 1));
     BB.K = K;
 }
 
-// Y = F X for K lambda-space vectors (one block SYMV, then the hybrid cluster
-// correction and the scatter per vector)
-void espreso_feti_s::apply_F_block(int K, const double *x, double *y) {
-    launch_symv_mfma(D, SB, false, K, x, false, BB.yb, st);
-    for (int v = 0; v < K; v++) {
-        double *ybv = BB.yb + v * D.nbnd;
-        if (hybrid) {
-            launch_subdomain_kernel_dots(D, x + (int64_t)v * nlam, d_e0s, st);
-            launch_cluster_correction(D, CL, d_e0s, ybv, 1, st);
+void espreso_feti_s::lanes_setup() {
+    if (!lanes_ready) {
+        for (Lane &l : lanes) {
+            ESP_HIP(hipStreamCreate(&l.s));
+            ESP_HIP(hipEventCreateWithFlags(&l.ev, hipEventDisableTiming));
+            l.dsub = alloc<double>(n_dsub);
+            l.cd = alloc<double>(std::max(ncoarse, 1));
+            l.ce = alloc<double>(std::max(ncoarse, 1));
+            l.lambda0 = alloc<double>(n_lambda0);
+            l.alphaK = alloc<double>(std::max<int64_t>(asubv[nsub], 1));
+            l.e0s = alloc<double>(n_e0s);
+            l.partial = alloc<double>(1024);
         }
-        launch_lambda_gather(D, ybv, false, y + (int64_t)v * nlam, st);
+        ESP_HIP(hipEventCreateWithFlags(&ev_main, hipEventDisableTiming));
+        lanes_ready = true;
+    }
+    // the operators of the current set-up with the lane's own scratch
+    for (Lane &l : lanes) {
+        l.D = D;
+        l.D.dsub = l.dsub;
+        l.D.cd = l.cd;
+        l.D.ce = l.ce;
+        l.C = CL;
+        l.C.lambda0 = l.lambda0;
+        l.C.alphaK = l.alphaK;
     }
 }
 
-// one PCG iteration of every active case (the block applies cover all K)
+// lanes start after the work queued on the main stream; the main stream
+// continues after the lanes' work
+void espreso_feti_s::fork_lanes() {
+    ESP_HIP(hipEventRecord(ev_main, st));
+    for (Lane &l : lanes) ESP_HIP(hipStreamWaitEvent(l.s, ev_main, 0));
+}
+
+void espreso_feti_s::join_lanes() {
+    for (Lane &l : lanes) {
+        ESP_HIP(hipEventRecord(l.ev, l.s));
+        ESP_HIP(hipStreamWaitEvent(st, l.ev, 0));
+    }
+}
+
+// hybrid: the cluster correction of the block SYMV output BB.yb for the K
+// lambda-space vectors x: F0^-1 and F_i B0^T are read once for the group (main
+// stream), the kernel dots and the small alpha solves run per case on the lanes
+void espreso_feti_s::cluster_block(int K, const double *x) {
+    const int64_t na = std::max<int64_t>(asubv[nsub], 1);
+    fork_lanes();
+    for (int v = 0; v < K; v++)
+        launch_subdomain_kernel_dots(lanes[v % NLANE].D, x + (int64_t)v * nlam, BB.e0s + v * n_e0s,
+                                     lanes[v % NLANE].s);
+    join_lanes();
+    launch_cluster_t_multi(D, CL, K, BB.yb, D.nbnd, BB.lam0, n_lambda0, st);
+    fork_lanes();
+    for (int v = 0; v < K; v++)
+        launch_cluster_alpha(CL, BB.e0s + v * n_e0s, BB.lam0 + v * n_lambda0, BB.alphaK + v * na, lanes[v % NLANE].s);
+    join_lanes();
+    launch_cluster_apply_multi(D, CL, K, BB.lam0, n_lambda0, BB.alphaK, na, 1, BB.yb, D.nbnd, st);
+}
+
+// Y = F X for K lambda-space vectors: one block SYMV on the main stream, the
+// cluster correction, then the scatter of each vector on its lane
+void espreso_feti_s::apply_F_block(int K, const double *x, double *y) {
+    launch_symv_mfma(D, SB, false, K, x, false, BB.yb, st);
+    if (hybrid) cluster_block(K, x);
+    fork_lanes();
+    for (int v = 0; v < K; v++)
+        launch_lambda_gather(lanes[v % NLANE].D, BB.yb + v * D.nbnd, false, y + (int64_t)v * nlam, lanes[v % NLANE].s);
+    join_lanes();
+}
+
+// one PCG iteration of every active case (the block applies cover all K);
+// the per-case chains between the block applies run on the lanes
 void espreso_feti_s::step_block(int K, const std::vector<char> &active, bool first) {
     const int64_t nl = nlam;
     auto sc = [&](int v, int q) { return BB.scal + 8 * v + q; };  // 0 yw, 1 yw_prev, 2 pAp, 3 ww
+    auto lane = [&](int v) -> Lane & { return lanes[v % NLANE]; };
+    fork_lanes();
     for (int v = 0; v < K; v++)
-        if (active[v]) launch_project(D, BB.r + v * nl, BB.w + v * nl, 0, st);
-    if (opt.preconditioner == ESPRESO_PREC_NONE) {
+        if (active[v]) launch_project(lane(v).D, BB.r + v * nl, BB.w + v * nl, 0, lane(v).s);
+    join_lanes();
+    const bool dir = opt.preconditioner == ESPRESO_PREC_DIRICHLET;
+    if (dir) {
+        launch_symv_mfma(D, SB, true, K, BB.w, true, BB.yb, st);
+    } else if (opt.preconditioner != ESPRESO_PREC_NONE) {
         for (int v = 0; v < K; v++)
-            if (active[v])
-                ESP_HIP(hipMemcpyAsync(BB.y + v * nl, BB.w + v * nl, nl * sizeof(double), hipMemcpyDeviceToDevice, st));
-    } else {
-        if (opt.preconditioner == ESPRESO_PREC_DIRICHLET) {
-            launch_symv_mfma(D, SB, true, K, BB.w, true, BB.yb, st);
-            for (int v = 0; v < K; v++)
-                if (active[v]) launch_lambda_gather(D, BB.yb + v * D.nbnd, true, BB.z + v * nl, st);
+            if (active[v]) apply_prec(BB.w + v * nl, BB.z + v * nl);  // main stream (shared boundary scratch)
+    }
+    fork_lanes();
+    for (int v = 0; v < K; v++) {
+        if (!active[v]) continue;
+        Lane &L = lane(v);
+        if (opt.preconditioner == ESPRESO_PREC_NONE) {
+            ESP_HIP(hipMemcpyAsync(BB.y + v * nl, BB.w + v * nl, nl * sizeof(double), hipMemcpyDeviceToDevice, L.s));
         } else {
-            for (int v = 0; v < K; v++)
-                if (active[v]) apply_prec(BB.w + v * nl, BB.z + v * nl);
+            if (dir) launch_lambda_gather(L.D, BB.yb + v * D.nbnd, true, BB.z + v * nl, L.s);
+            launch_project(L.D, BB.z + v * nl, BB.y + v * nl, 0, L.s);
         }
-        for (int v = 0; v < K; v++)
-            if (active[v]) launch_project(D, BB.z + v * nl, BB.y + v * nl, 0, st);
+        if (!first) launch_copy_scalar(sc(v, 0), sc(v, 1), L.s);
+        launch_dot(BB.y + v * nl, BB.w + v * nl, nl, L.partial, sc(v, 0), L.s);
+        launch_p_update(BB.p + v * nl, BB.y + v * nl, nl, sc(v, 0), sc(v, 1), first, L.s);
     }
+    join_lanes();
+    launch_symv_mfma(D, SB, false, K, BB.p, false, BB.yb, st);
+    if (hybrid) cluster_block(K, BB.p);
+    fork_lanes();
     for (int v = 0; v < K; v++) {
         if (!active[v]) continue;
-        if (!first) launch_copy_scalar(sc(v, 0), sc(v, 1), st);
-        launch_dot(BB.y + v * nl, BB.w + v * nl, nl, d_partial, sc(v, 0), st);
-        launch_p_update(BB.p + v * nl, BB.y + v * nl, nl, sc(v, 0), sc(v, 1), first, st);
+        Lane &L = lane(v);
+        double *ybv = BB.yb + v * D.nbnd, *Apv = BB.Ap + v * nl;
+        launch_lambda_gather(L.D, ybv, false, Apv, L.s);
+        launch_dot(BB.p + v * nl, Apv, nl, L.partial, sc(v, 2), L.s);
+        launch_xr_update(BB.x + v * nl, BB.r + v * nl, BB.p + v * nl, Apv, nl, sc(v, 0), sc(v, 2), L.s);
+        launch_dot(BB.w + v * nl, BB.w + v * nl, nl, L.partial, sc(v, 3), L.s);
     }
-    apply_F_block(K, BB.p, BB.Ap);
-    for (int v = 0; v < K; v++) {
-        if (!active[v]) continue;
-        launch_dot(BB.p + v * nl, BB.Ap + v * nl, nl, d_partial, sc(v, 2), st);
-        launch_xr_update(BB.x + v * nl, BB.r + v * nl, BB.p + v * nl, BB.Ap + v * nl, nl, sc(v, 0), sc(v, 2), st);
-        launch_dot(BB.w + v * nl, BB.w + v * nl, nl, d_partial, sc(v, 3), st);
-    }
+    join_lanes();
 }
 
 void espreso_feti_s::resolve_group(int K, const double *f, double *u, int32_t *iters, double *norms,
                                    espreso_feti_stats &tot) {
     const double t0 = wall();
     block_setup(K);
+    lanes_setup();
     const int64_t nl = nlam, nb = D.nbnd, nt = uoff[nsub], na = std::max<int64_t>(asubv[nsub], 1);
     const int64_t nc = std::max(ncoarse, 1);
     // loads: e0f, e on the host; f, y_f, hf (one band-solve launch) and b on the device
@@ -1419,7 +1516,8 @@ void espreso_feti_s::coarse_layout_subdomains() {
     D.Rt = D.Rb;
     D.rtoff = D.rboff;
     D.dsub_off = D.asub;
-    D.dsub = alloc<double>(asubv[nsub]);
+    n_dsub = std::max<int64_t>(1, asubv[nsub]);
+    D.dsub = alloc<double>(n_dsub);
     D.N = upload(N);
     D.noff = upload(noff);
     D.cd = alloc<double>(ncoarse);
@@ -1581,7 +1679,8 @@ void espreso_feti_s::load_clusters() {
     CL.asub = D.asub;
     CL.FB0 = alloc<double>(fb0off[nsub]);
     CL.Fcc = alloc<double>(std::max<int64_t>(1, fccoff_h[nsub]));
-    CL.lambda0 = alloc<double>(std::max<int64_t>(1, rowoff[ncl]));
+    n_lambda0 = std::max<int64_t>(1, rowoff[ncl]);
+    CL.lambda0 = alloc<double>(n_lambda0);
     CL.alphaK = alloc<double>(asubv[nsub]);
     d_qjint = upload(qj_h);
     d_qsign = upload(qsign_h);
@@ -1641,7 +1740,8 @@ void espreso_feti_s::load_clusters() {
     D.N = upload(Ns);
     D.noff = upload(noff);
     D.dsub_off = upload(dsub_off);
-    D.dsub = alloc<double>(std::max<int64_t>(1, dsub_off[nsub]));
+    n_dsub = std::max<int64_t>(1, dsub_off[nsub]);
+    D.dsub = alloc<double>(n_dsub);
     D.cd = alloc<double>(ncoarse);
     D.ce = alloc<double>(ncoarse);
     d_alpha_c = alloc<double>(ncoarse);
@@ -1882,8 +1982,11 @@ int espreso_feti_resolve_block(espreso_feti *h, int32_t nrhs, const double *f, d
     ESP_CHECK(h && h->ready, ESPRESO_ERR_STATE, "espreso_feti_setup not called");
     ESP_CHECK(nrhs >= 0 && (nrhs == 0 || (f && u)), ESPRESO_ERR_ARG, "invalid block arguments");
     ESP_HIP(hipSetDevice(h->opt.device));
-    for (int64_t q = 0; q < (int64_t)nrhs * h->uoff[h->nsub]; q++)
-        ESP_CHECK(std::isfinite(f[q]), ESPRESO_ERR_NUMERIC, "non-finite load");
+    const int64_t nf = (int64_t)nrhs * h->uoff[h->nsub];
+    int64_t bad = 0;
+#pragma omp parallel for reduction(+ : bad)
+    for (int64_t q = 0; q < nf; q++) bad += !std::isfinite(f[q]);
+    ESP_CHECK(bad == 0, ESPRESO_ERR_NUMERIC, "non-finite load");
     h->resolve_block(nrhs, f, u, iterations, final_norms, stats);
     ESP_API_END
 }
@@ -2254,6 +2357,52 @@ int espreso_grid_set_nodal_rhs(espreso_feti *h, const double *g) {
         }
         const int rc = espreso_feti_set_rhs(h, s, f.data());
         if (rc) return rc;
+    }
+    ESP_API_END
+}
+
+int espreso_grid_resolve_block(espreso_feti *h, int32_t nrhs, const double *g, double *u_nodal, int32_t *iterations,
+                               double *final_norms, espreso_feti_stats *stats) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && (nrhs == 0 || (g && u_nodal)) && nrhs >= 0, ESPRESO_ERR_ARG, "null argument");
+    ESP_CHECK((int)h->l2g.size() == h->nsub, ESPRESO_ERR_STATE, "grid not loaded");
+    ESP_CHECK(h->ready, ESPRESO_ERR_STATE, "espreso_feti_setup not called");
+    const int nd = h->ndof, nsub = h->nsub;
+    const int64_t nt = h->uoff[nsub], nv = (int64_t)h->nnodes * nd;
+    std::vector<int> mult(h->nnodes, 0);
+    for (int s = 0; s < nsub; s++)
+        for (int node : h->l2g[s]) mult[node]++;
+    // loads split over the subdomain copies of a node (as espreso_grid_set_nodal_rhs)
+    std::vector<double> f((size_t)nrhs * nt), u((size_t)nrhs * nt);
+#pragma omp parallel for collapse(2) schedule(dynamic, 8)
+    for (int v = 0; v < nrhs; v++)
+        for (int s = 0; s < nsub; s++) {
+            double *fs = f.data() + v * nt + h->uoff[s];
+            const double *gv = g + v * nv;
+            for (size_t ln = 0; ln < h->l2g[s].size(); ln++) {
+                const int node = h->l2g[s][ln];
+                for (int c = 0; c < nd; c++) fs[ln * nd + c] = gv[(size_t)node * nd + c] / mult[node];
+            }
+        }
+    int64_t bad = 0;
+#pragma omp parallel for reduction(+ : bad)
+    for (int64_t q = 0; q < (int64_t)f.size(); q++) bad += !std::isfinite(f[q]);
+    ESP_CHECK(bad == 0, ESPRESO_ERR_NUMERIC, "non-finite load");
+    h->resolve_block(nrhs, f.data(), u.data(), iterations, final_norms, stats);
+    // solutions averaged over the subdomain copies (as espreso_grid_nodal_solution)
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < nrhs; v++) {
+        double *un = u_nodal + v * nv;
+        std::fill(un, un + nv, 0.0);
+        for (int s = 0; s < nsub; s++) {
+            const double *us = u.data() + v * nt + h->uoff[s];
+            for (size_t ln = 0; ln < h->l2g[s].size(); ln++) {
+                const int node = h->l2g[s][ln];
+                for (int c = 0; c < nd; c++) un[(size_t)node * nd + c] += us[ln * nd + c];
+            }
+        }
+        for (int node = 0; node < h->nnodes; node++)
+            for (int c = 0; c < nd; c++) un[(size_t)node * nd + c] /= mult[node];
     }
     ESP_API_END
 }

@@ -159,6 +159,13 @@ int espreso_grid_nodal_load(espreso_feti *h, double *g_nodal);
 /* New nodal load g (n_nodes x ndof) for espreso_feti_resolve: each node's load
  * is split evenly over the subdomains that share it.                         */
 int espreso_grid_set_nodal_rhs(espreso_feti *h, const double *g_nodal);
+/* Block re-solve of nodal load cases on the grid instance: g holds nrhs
+ * nodal loads (n_nodes x ndof each) split over the subdomain copies of a node
+ * as espreso_grid_set_nodal_rhs does; u_nodal receives the nodal solutions
+ * averaged over the copies as espreso_grid_nodal_solution does.  The solve is
+ * espreso_feti_resolve_block (iterations, final_norms, stats may be NULL).  */
+int espreso_grid_resolve_block(espreso_feti *h, int32_t nrhs, const double *g, double *u_nodal, int32_t *iterations,
+                               double *final_norms, espreso_feti_stats *stats);
 /* Element sensitivities on the GPU: sens[e] = -w_e^T K_e0 u_e for nodal
  * fields u, w (n_nodes x ndof); with w = u (the compliance adjoint) this is
  * d(f^T u)/dE_e.  No counterpart in the reference (adjoint extension).       */
