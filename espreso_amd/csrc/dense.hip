@@ -104,6 +104,21 @@ __device__ __forceinline__ void tile_gemm(double *C, int ldc, const double *A, i
             }
 }
 
+// Workgroups are dealt to the 8 XCDs (each with its own L2) round-robin by
+// linear id, x fastest.  The tiles of one subdomain share their operand panels,
+// so the linear id is remapped to keep runs of consecutive logical tiles (x
+// fastest, then subdomain) on one XCD: physical p of a full round of 8 goes to
+// logical (p % 8) * (G / 8) + p / 8; the tail past the last full round stays.
+constexpr int NXCD = 8;
+__device__ __forceinline__ void xcd_tile(int &bx, int &by) {
+    const int gx = gridDim.x, G = gx * gridDim.y;
+    const int p = blockIdx.x + gx * blockIdx.y;
+    const int full = G - G % NXCD;
+    const int L = p < full ? (p % NXCD) * (full / NXCD) + p / NXCD : p;
+    bx = L % gx;
+    by = L / gx;
+}
+
 __device__ __forceinline__ void pick(const DenseDesc &d, int which, double *&A, int &n, int &band, int &nt,
                                      double *&B, int &ncols, int &ntc) {
     if (which == 0) {
@@ -278,11 +293,13 @@ __global__ void __launch_bounds__(256) k_potrf_diag(const DenseDesc *D, int whic
 // column below the diagonal: A_ik = (A_ik - L_iw L_kw^T) inv(L_kk)^T for
 // step < i <= step + band, window w = [max(0, i - band), step)
 __global__ void __launch_bounds__(256) k_potrf_panel(const DenseDesc *D, int which, int step) {
-    const DenseDesc &d = D[blockIdx.y];
+    int bx, by;
+    xcd_tile(bx, by);
+    const DenseDesc &d = D[by];
     double *A, *B;
     int n, band, nt, ncols, ntc;
     pick(d, which, A, n, band, nt, B, ncols, ntc);
-    const int i = step + 1 + blockIdx.x;
+    const int i = step + 1 + bx;
     if (i >= nt || i > step + band) return;
     const int k0 = step * TILE, i0 = i * TILE;
     const int nb = min(TILE, n - k0), mi = min(TILE, n - i0);
@@ -300,11 +317,13 @@ __global__ void __launch_bounds__(256) k_potrf_panel(const DenseDesc *D, int whi
 // X_iJ = inv(L_ii) (B_iJ - L_iw X_wJ), window w = [max(first nonzero tile of
 // column tile J, i - band), i)
 __global__ void __launch_bounds__(256) k_trsm_row(const DenseDesc *D, int which, int step) {
-    const DenseDesc &d = D[blockIdx.y];
+    int bx, by;
+    xcd_tile(bx, by);
+    const DenseDesc &d = D[by];
     double *A, *B;
     int n, band, nt, ncols, ntc;
     pick(d, which, A, n, band, nt, B, ncols, ntc);
-    const int J = blockIdx.x, i = step;
+    const int J = bx, i = step;
     if (J >= ntc || i >= nt) return;
     const int cs = which == 0 ? d.colstart[J] : J;
     if (cs > i) return;
@@ -332,8 +351,10 @@ __global__ void __launch_bounds__(256) k_trsm_row(const DenseDesc *D, int which,
 
 // lower tiles of  S = S - Y^T Y  (which 0)  or  S = W^T W  (which 1)
 __global__ void __launch_bounds__(256) k_syrk(const DenseDesc *D, int which) {
-    const DenseDesc &d = D[blockIdx.y];
-    const int q = blockIdx.x;
+    int bx, by;
+    xcd_tile(bx, by);
+    const DenseDesc &d = D[by];
+    const int q = bx;
     int I = (int)((sqrtf(8.0f * q + 1.0f) - 1.0f) * 0.5f);
     while ((I + 1) * (I + 2) / 2 <= q) I++;
     while (I * (I + 1) / 2 > q) I--;
