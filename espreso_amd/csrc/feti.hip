@@ -254,6 +254,14 @@ struct espreso_feti_s {
     void apply_F_block(int K, const double *x, double *y);
     void cluster_block(int K, const double *x);
     void step_block(int K, const std::vector<char> &active, bool first);
+    // iterations with every case of a group active replay a HIP graph of
+    // step_block (the lanes are forked and joined inside the graph)
+    struct BlockGraph {
+        int K;
+        hipGraphExec_t exec;
+    };
+    std::vector<BlockGraph> block_graphs;
+    void step_block_graph(int K);
     void resolve_block(int nrhs, const double *f, double *u, int32_t *iters, double *norms,
                        espreso_feti_stats *stats);
     void resolve_group(int K, const double *f, double *u, int32_t *iters, double *norms, espreso_feti_stats &tot);
@@ -1037,6 +1045,9 @@ void espreso_feti_s::drop_graphs() {
     for (StepGraph &g : step_graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     step_graphs.clear();
+    for (BlockGraph &g : block_graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    block_graphs.clear();
 }
 
 void espreso_feti_s::pcg(const double *b, double *x, double *r, int prec, int &iters, double &norm, double &tol,
@@ -1188,6 +1199,7 @@ void espreso_feti_s::block_setup(int K) {
         SB.plen = plen;
     }
     if (BB.K >= K) return;
+    drop_graphs();  // graphs hold the group's buffers
     for (double *q : {BB.x, BB.r, BB.w, BB.z, BB.y, BB.p, BB.Ap, BB.b, BB.e, BB.alpha_c, BB.alpha_tot, BB.e0f, BB.hf,
                       BB.ubf, BB.xbv, BB.yb, BB.f, BB.yf, BB.u, BB.scal, BB.work, BB.e0s, BB.lam0, BB.alphaK,
                       SB.xb, SB.ypart})
@@ -1340,6 +1352,31 @@ void espreso_feti_s::step_block(int K, const std::vector<char> &active, bool fir
     join_lanes();
 }
 
+void espreso_feti_s::step_block_graph(int K) {
+    for (const BlockGraph &g : block_graphs)
+        if (g.K == K) {
+            ESP_HIP(hipGraphLaunch(g.exec, st));
+            return;
+        }
+    const std::vector<char> all(K, 1);
+    hipGraph_t graph = nullptr;
+    ESP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        step_block(K, all, false);
+    } catch (...) {
+        (void)hipStreamEndCapture(st, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    ESP_HIP(hipStreamEndCapture(st, &graph));
+    BlockGraph g{K, nullptr};
+    const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    ESP_HIP(e);
+    block_graphs.push_back(g);
+    ESP_HIP(hipGraphLaunch(g.exec, st));
+}
+
 void espreso_feti_s::resolve_group(int K, const double *f, double *u, int32_t *iters, double *norms,
                                    espreso_feti_stats &tot) {
     const double t0 = wall();
@@ -1377,8 +1414,10 @@ void espreso_feti_s::resolve_group(int K, const double *f, double *u, int32_t *i
     std::vector<char> active(K, 1);
     std::vector<int> it_v(K, 0);
     for (int v = 0; v < K; v++) tol[v] = opt.precision * std::min(std::sqrt(hs[8 * v + 4]), std::sqrt(hs[8 * v + 5]));
+    bool all_active = true;
     for (int it = 0; it < opt.max_iterations; it++) {
-        step_block(K, active, it == 0);
+        if (it > 0 && all_active && use_graphs) step_block_graph(K);
+        else step_block(K, active, it == 0);
         ESP_HIP(hipMemcpyAsync(hs.data(), BB.scal, hs.size() * sizeof(double), hipMemcpyDeviceToHost, st));
         sync();
         bool any = false;
@@ -1389,6 +1428,7 @@ void espreso_feti_s::resolve_group(int K, const double *f, double *u, int32_t *i
             it_v[v] = it + 1;
             if (nrm[v] < tol[v]) active[v] = 0;
             any |= active[v] != 0;
+            all_active = all_active && active[v];
         }
         if (!any) break;
     }

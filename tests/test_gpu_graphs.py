@@ -57,3 +57,45 @@ def test_graph_replay_matches_eager(name):
         assert abs(ie - ig) <= max(3, 0.02 * ie)
         assert np.abs(ue - ref).max() <= 1e-8 * scale
         assert np.abs(ug - ref).max() <= 1e-8 * scale
+
+
+BLOCK_CASES = {
+    "hybrid_dirichlet": (PB.scaled(PB.GRAVITY, clusters=(2, 1, 1), domains=(2, 2, 2), elements=(3, 3, 3),
+                                   method="hybrid"), dict(preconditioner="dirichlet")),
+    "tfeti_none": (PB.scaled(PB.GRAVITY, traction_seed=9, traction_scale=1e5), dict(preconditioner="none")),
+}
+
+
+def run_block(spec, opts, graphs, loads):
+    os.environ["ESPRESO_HIP_GRAPHS"] = "1" if graphs else "0"
+    try:
+        g = core.GridFETI(spec, precision=1e-10, method=spec.get("method", "total"), **opts)
+        g.setup()
+        out = []
+        for rep in range(2):  # the second call replays the block graph captured in the first
+            u, it, _ = g.resolve_block_nodal(loads)
+            out.append((it.copy(), u.copy()))
+            g.solve()  # eager work of the instance's own between the block calls
+        g.close()
+        return out
+    finally:
+        os.environ.pop("ESPRESO_HIP_GRAPHS", None)
+
+
+@pytest.mark.parametrize("name", sorted(BLOCK_CASES))
+def test_block_graph_replay_matches_eager(name):
+    """The block iteration (main stream plus 4 forked streams) replayed from its
+    graph: every case of both calls within 1e-8 of the direct solve, iteration
+    counts within max(3, 2 %) of the eager block run."""
+    spec, opts = BLOCK_CASES[name]
+    print("case: block graphs", name, flush=True)
+    loads = PB.nodal_load_cases(spec, 6, seed=21)
+    eager = run_block(spec, opts, False, loads)
+    graph = run_block(spec, opts, True, loads)
+    refs = [O.global_solve_load(spec, loads[q]) for q in range(len(loads))]
+    for (ie, ue), (ig, ug) in zip(eager, graph):
+        for q, ref in enumerate(refs):
+            scale = np.abs(ref).max()
+            assert abs(int(ie[q]) - int(ig[q])) <= max(3, 0.02 * ie[q])
+            assert np.abs(ue[q] - ref).max() <= 1e-8 * scale
+            assert np.abs(ug[q] - ref).max() <= 1e-8 * scale
