@@ -28,12 +28,15 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 // op(A)(i,k) = TA ? A[k + i*lda] : A[i + k*lda];  op(B)(k,j) = TB ? B[j + k*ldb] : B[k + j*ldb].
 // C may alias A or B when this tile is the only reader of the aliased tile:
 // all reads finish before the first store.  The next K chunk is fetched into
-// registers while the MFMAs of the current chunk run from LDS.
+// registers while the MFMAs of the current chunk run from LDS, and stored into
+// the second LDS buffer once they are issued (one barrier per chunk).
 template <bool TA, bool TB>
 __device__ __forceinline__ void tile_gemm(double *C, int ldc, const double *A, int lda, const double *B, int ldb,
                                           int M, int N, int K, double alpha, double beta) {
-    __shared__ double As[16][TILE + 1];
-    __shared__ double Bs[16][TILE + 1];
+    // two LDS buffers: chunk c+1 is stored into one while the MFMAs read chunk c
+    // from the other, so each 16-wide K chunk needs one workgroup barrier
+    __shared__ double As[2][16][TILE + 1];
+    __shared__ double Bs[2][16][TILE + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;
     d4 acc[2][2];
@@ -60,31 +63,39 @@ __device__ __forceinline__ void tile_gemm(double *C, int ldc, const double *A, i
             rb[it] = (j < N && k0 + kk < K) ? (TB ? B[j + (size_t)(k0 + kk) * ldb] : B[(k0 + kk) + (size_t)j * ldb]) : 0.0;
         }
     };
-    if (K > 0) fetch(0);
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    auto store = [&](int buf) {
 #pragma unroll
         for (int it = 0; it < 4; it++) {
             int i, kk;
             aidx(it, i, kk);
-            As[kk][i] = ra[it];
+            As[buf][kk][i] = ra[it];
             int j;
             bidx(it, j, kk);
-            Bs[kk][j] = rb[it];
+            Bs[buf][kk][j] = rb[it];
         }
+    };
+    if (K > 0) {
+        fetch(0);
+        store(0);
         __syncthreads();
-        if (k0 + 16 < K) fetch(k0 + 16);
+        if (16 < K) fetch(16);
+    }
+    for (int k0 = 0, cur = 0; k0 < K; k0 += 16, cur ^= 1) {
 #pragma unroll
         for (int ks = 0; ks < 16; ks += 4) {
-            double a0 = As[ks + (lane >> 4)][wi + (lane & 15)];
-            double a1 = As[ks + (lane >> 4)][wi + 16 + (lane & 15)];
-            double b0 = Bs[ks + (lane >> 4)][wj + (lane & 15)];
-            double b1 = Bs[ks + (lane >> 4)][wj + 16 + (lane & 15)];
+            double a0 = As[cur][ks + (lane >> 4)][wi + (lane & 15)];
+            double a1 = As[cur][ks + (lane >> 4)][wi + 16 + (lane & 15)];
+            double b0 = Bs[cur][ks + (lane >> 4)][wj + (lane & 15)];
+            double b1 = Bs[cur][ks + (lane >> 4)][wj + 16 + (lane & 15)];
             acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
             acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
             acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
             acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
         }
+        // buffer cur^1 was last read in the previous chunk, before the barrier that ended it
+        if (k0 + 16 < K) store(cur ^ 1);
         __syncthreads();
+        if (k0 + 32 < K) fetch(k0 + 32);
     }
     // D layout of the 16x16 f64 MFMA: value v of lane l is row (l/16)+4v, column l%16.
 #pragma unroll
