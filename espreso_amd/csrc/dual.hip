@@ -91,18 +91,21 @@ __global__ void __launch_bounds__(256) k_symv(const SymvItem *__restrict__ items
         const int r0 = t * R;
         const int ncol = m - r0;
         const T *S = base + strip_offset(m, R, t);
-        // FP64 keeps the strip-row x in registers; the low-precision strips
-        // (32/64 rows) read it from LDS to stay at or below 128 VGPRs
-        constexpr bool XREG = sizeof(T) == 8;
+        // the strip-row x: FP64 keeps it in registers (R = 16 values per lane);
+        // BF16 (64 rows) keeps it in one register, row r in lane r, broadcast
+        // with readlane (no LDS reads and waits in the chunk loop); FP32
+        // (32 rows) reads it from LDS, which measured faster there
+        constexpr bool XREG = sizeof(T) == 8, XLANE = sizeof(T) == 2;
         A racc[R], xr[XREG ? R : 1];
 #pragma unroll
         for (int r = 0; r < R; r++) {
             racc[r] = (A)0;
             if constexpr (XREG) xr[r] = xs[r0 + r - j0];
         }
+        const float xlane = XLANE ? (lane < R ? (float)xs[r0 + lane - j0] : 0.0f) : 0.0f;
+        const A *xrow = xs + (r0 - j0);  // x of the strip rows (FP32: LDS broadcast reads)
         constexpr int VEC = 16 / (int)sizeof(T);
         constexpr int NV = R / VEC;
-        const A *xrow = xs + (r0 - j0);  // x of the strip rows (LDS broadcast reads)
         for (int c0 = 0; c0 < ncol; c0 += 64) {
             const int w = min(64, ncol - c0);
             if (lane < w) {
@@ -119,8 +122,15 @@ __global__ void __launch_bounds__(256) k_symv(const SymvItem *__restrict__ items
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     const A av = load_cvt<T, A>(a[r]);
-                    if constexpr (XREG) col[r & 3] = fma(av, xr[r], col[r & 3]);
-                    else col[r & 3] = fma(av, xrow[r], col[r & 3]);
+                    if constexpr (XREG) {
+                        col[r & 3] = fma(av, xr[r], col[r & 3]);
+                    } else if constexpr (XLANE) {
+                        const A xrr = __builtin_bit_cast(
+                            float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, xlane), r));
+                        col[r & 3] = fma(av, xrr, col[r & 3]);
+                    } else {
+                        col[r & 3] = fma(av, xrow[r], col[r & 3]);
+                    }
                     racc[r] = fma(av, xc, racc[r]);
                 }
                 atomicAdd(&ys[r0 + c - j0], (col[0] + col[1]) + (col[2] + col[3]));
