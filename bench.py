@@ -210,6 +210,7 @@ def main():
             "sc_storage": args.sc,
             "sd_storage": args.sd,
             "precision": 1e-8,
+            "symv_item_elements": dict(zip(("fp64", "low_precision"), g.symv_items())),
         },
         "iterations": st["iterations"],
         "phases_s": {k: round(st[k], 6) for k in ("time_assembly", "time_coarse", "time_pcg", "time_primal")},

@@ -118,6 +118,7 @@ SIGNATURES = {
     "espreso_feti_resolve": (C.c_int, [_H, C.POINTER(Stats)]),
     "espreso_feti_resolve_block": (C.c_int, [_H, C.c_int32, _D, _D, _I, _D, C.POINTER(Stats)]),
     "espreso_feti_subdomain_values": (C.c_int, [_H, C.c_int32, _D]),
+    "espreso_feti_symv_items": (C.c_int, [_H, C.POINTER(C.c_int64)]),
     "espreso_grid_resolve_block": (C.c_int, [_H, C.c_int32, _D, _D, _I, _D, C.POINTER(Stats)]),
     "espreso_feti_create": (C.c_int, [C.POINTER(_H), C.c_int32, C.POINTER(Options)]),
     "espreso_feti_destroy": (C.c_int, [_H]),
@@ -465,6 +466,12 @@ class FETISolver:
         b = np.zeros(m.value, dtype=np.int32)
         _check(lib().espreso_feti_subdomain_boundary(self.h, i, C.byref(m), _ip(b)))
         return b
+
+    def symv_items(self):
+        """SYMV work-item sizes chosen by the autotuner: (FP64, low precision)."""
+        t = (C.c_int64 * 2)()
+        _check(lib().espreso_feti_symv_items(self.h, t))
+        return int(t[0]), int(t[1])
 
     def values(self, i, nnz):
         """K_i CSR values as the device holds them (after setup)."""

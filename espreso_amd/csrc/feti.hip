@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <cstdlib>
 #include <chrono>
 #include <cmath>
@@ -280,8 +282,15 @@ struct espreso_feti_s {
     std::vector<int64_t> f0off_h, saoff_h;
     void build_ggt(const std::vector<double> &Rt, const std::vector<int64_t> &rtoff, const std::vector<int> &ck,
                    const std::vector<int> &cblk, const std::vector<int> &coff, std::vector<double> &GGt);
-    void build_items(int prec, std::vector<SymvItem> &items, std::vector<int> &item0, std::vector<int> &nit,
-                     int64_t &plen);
+    void build_items(int prec, int64_t target, std::vector<SymvItem> &items, std::vector<int> &item0,
+                     std::vector<int> &nit, int64_t &plen);
+    // SYMV work-item size (matrix elements per workgroup) per storage
+    // geometry: 0 FP64, 1 the low-precision one; chosen by tune_symv
+    int64_t symv_target[2] = {SYMV_DEFAULT_TARGET, SYMV_DEFAULT_TARGET};
+    bool symv_tuned = false;
+    int max_m_h = 0;
+    void upload_items(int geo, int64_t target);
+    void tune_symv();
     // dual operators on device vectors
     void apply_F(const double *x, double *y, int prec);
     void apply_prec(const double *x, double *y);
@@ -607,24 +616,24 @@ void espreso_feti_s::load() {
     if (hybrid) load_clusters();
     else coarse_layout_subdomains();
 
-    // ---- SYMV work lists ----
+    // ---- SYMV work lists (the item size is tuned at the first set-up) ----
     {
+        max_m_h = max_m;
+        if (const char *e = std::getenv("ESPRESO_SYMV_ITEM")) {
+            symv_target[0] = symv_target[1] = std::max<int64_t>(4096, std::atoll(e));
+            symv_tuned = true;
+        }
+        // partial vectors sized for the smallest item size any candidate uses
         std::vector<SymvItem> it;
         std::vector<int> i0, ni;
-        int64_t plen_d = 0, plen_lp = 0;
-        build_items(0, it, i0, ni, plen_d);
-        D.items_d = upload(it);
-        D.n_items_d = (int)it.size();
-        D.item0_d = upload(i0);
-        D.nitems_d = upload(ni);
-        if (prec) {
-            build_items(prec, it, i0, ni, plen_lp);
-            D.items_lp = upload(it);
-            D.n_items_lp = (int)it.size();
-            D.item0_lp = upload(i0);
-            D.nitems_lp = upload(ni);
+        int64_t plen_max = 1, pl = 0;
+        for (int geo = 0; geo < (prec ? 2 : 1); geo++) {
+            build_items(geo ? prec : 0, std::min(symv_target[geo], SYMV_TUNE_TARGETS[0]), it, i0, ni, pl);
+            plen_max = std::max(plen_max, pl);
         }
-        D.ypart = alloc<double>(std::max(plen_d, plen_lp));
+        D.ypart = alloc<double>(plen_max);
+        upload_items(0, symv_target[0]);
+        if (prec) upload_items(1, symv_target[1]);
     }
     // PCG vectors
     d_x = alloc<double>(nlam); d_r = alloc<double>(nlam); d_w = alloc<double>(nlam); d_y = alloc<double>(nlam);
@@ -667,6 +676,7 @@ void espreso_feti_s::gpu_setup() {
     compute_b();
     sync();
     last.time_coarse = wall() - tc;
+    tune_symv();  // first set-up only (outside the timed phases)
     ready = true;
 }
 
@@ -922,12 +932,9 @@ void espreso_feti_s::coarse() {
     release(dd);
 }
 
-void espreso_feti_s::build_items(int prec, std::vector<SymvItem> &items, std::vector<int> &item0,
+void espreso_feti_s::build_items(int prec, int64_t target, std::vector<SymvItem> &items, std::vector<int> &item0,
                                  std::vector<int> &nit, int64_t &plen) {
     const int R = strip_rows(prec);
-    // matrix elements per workgroup (ESPRESO_SYMV_ITEM overrides, for tuning)
-    int64_t target = 320 * 1024;
-    if (const char *e = std::getenv("ESPRESO_SYMV_ITEM")) target = std::max<int64_t>(4096, std::atoll(e));
     items.clear();
     item0.assign(nsub, 0);
     nit.assign(nsub, 0);
@@ -954,6 +961,78 @@ void espreso_feti_s::build_items(int prec, std::vector<SymvItem> &items, std::ve
             t = t1;
         }
         nit[s] = (int)items.size() - item0[s];
+    }
+}
+
+// (re)build and upload the SYMV work list of a storage geometry
+void espreso_feti_s::upload_items(int geo, int64_t target) {
+    const int prec = geo ? std::max(opt.sc_precision, D.sd_prec) : 0;
+    std::vector<SymvItem> it;
+    std::vector<int> i0, ni;
+    int64_t plen = 0;
+    build_items(prec, target, it, i0, ni, plen);
+    SymvItem *&items = geo ? D.items_lp : D.items_d;
+    int *&item0 = geo ? D.item0_lp : D.item0_d;
+    int *&nitems = geo ? D.nitems_lp : D.nitems_d;
+    for (void *q : {(void *)items, (void *)item0, (void *)nitems})
+        if (q) release(q);
+    items = upload(it);
+    item0 = upload(i0);
+    nitems = upload(ni);
+    (geo ? D.n_items_lp : D.n_items_d) = (int)it.size();
+    symv_target[geo] = target;
+}
+
+// Autotuner of the SYMV work-item size, keyed on the subdomain size: at the
+// first set-up every candidate is timed on the assembled F_i (HIP events, 3
+// applies each) and the fastest kept; the choice is cached for the process by
+// (storage precision, largest interface size rounded to 64).  ESPRESO_SYMV_ITEM
+// fixes the size, ESPRESO_AUTOTUNE=0 keeps the default.
+void espreso_feti_s::tune_symv() {
+    if (symv_tuned) return;
+    symv_tuned = true;
+    if (const char *e = std::getenv("ESPRESO_AUTOTUNE"))
+        if (std::atoi(e) == 0) return;
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, int64_t> cache;
+    const int prec_lp = std::max(opt.sc_precision, D.sd_prec);
+    for (int geo = 0; geo < (prec_lp ? 2 : 1); geo++) {
+        const int prec = geo ? prec_lp : 0;
+        const std::pair<int, int> key{prec, (max_m_h + 63) / 64};
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            auto f = cache.find(key);
+            if (f != cache.end()) {
+                if (f->second != symv_target[geo]) upload_items(geo, f->second);
+                continue;
+            }
+        }
+        // time on the F_i of this precision if stored, else on S_D
+        const bool sd = geo == 1 ? opt.sc_precision == 0 : false;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ESP_HIP(hipEventCreate(&e0));
+        ESP_HIP(hipEventCreate(&e1));
+        int64_t best = symv_target[geo];
+        float best_ms = 1e30f;
+        for (const int64_t target : SYMV_TUNE_TARGETS) {
+            upload_items(geo, target);
+            launch_symv(D, sd, prec, d_b, 1, false, st);  // warm
+            ESP_HIP(hipEventRecord(e0, st));
+            for (int r = 0; r < 3; r++) launch_symv(D, sd, prec, d_b, 1, false, st);
+            ESP_HIP(hipEventRecord(e1, st));
+            ESP_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            ESP_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = target;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        upload_items(geo, best);
+        std::lock_guard<std::mutex> lk(mu);
+        cache[key] = best;
     }
 }
 
@@ -2003,6 +2082,14 @@ int espreso_feti_resolve(espreso_feti *h, espreso_feti_stats *st) {
     const double trhs = wall() - t0;
     h->solve(st);
     if (st) st->time_coarse = trhs;
+    ESP_API_END
+}
+
+int espreso_feti_symv_items(espreso_feti *h, int64_t *targets) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && targets, ESPRESO_ERR_ARG, "null argument");
+    targets[0] = h->symv_target[0];
+    targets[1] = h->symv_target[1];
     ESP_API_END
 }
 

@@ -34,6 +34,9 @@ constexpr int TILE = 64;        // dense tile edge of the assembly kernels
 constexpr int STRIP_F64 = 16;   // rows per packed strip, FP64 storage
 constexpr int STRIP_F32 = 32;   // FP32 storage
 constexpr int STRIP_B16 = 64;   // BF16 storage
+// SYMV work-item sizes (matrix elements per workgroup): default and autotuner candidates
+constexpr int64_t SYMV_DEFAULT_TARGET = 320 * 1024;
+constexpr int64_t SYMV_TUNE_TARGETS[5] = {96 * 1024, 192 * 1024, 320 * 1024, 640 * 1024, 1280 * 1024};
 
 __host__ __device__ inline int strip_rows(int prec) { return prec == 0 ? STRIP_F64 : (prec == 1 ? STRIP_F32 : STRIP_B16); }
 

@@ -239,6 +239,12 @@ int espreso_feti_apply_F(espreso_feti *h, const double *x, double *y);
 int espreso_feti_apply_prec(espreso_feti *h, const double *x, double *y);
 int espreso_feti_project(espreso_feti *h, const double *x, double *y);
 int espreso_feti_vec_b(espreso_feti *h, double *b);
+/* SYMV work-item sizes in matrix elements per workgroup: [0] FP64 storage,
+ * [1] FP32/BF16 storage.  The first espreso_feti_setup picks them by timing
+ * candidates on the assembled operators (cached per process by storage
+ * precision and subdomain interface size); ESPRESO_SYMV_ITEM fixes them and
+ * ESPRESO_AUTOTUNE=0 keeps the default (327680).                            */
+int espreso_feti_symv_items(espreso_feti *h, int64_t *targets);
 /* K_i CSR values of subdomain i as the device holds them (after setup; the
  * CSR of espreso_feti_set_subdomain, or of the grid builder with
  * device_assembly), for parity checks of the device assembly              */
