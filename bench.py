@@ -210,7 +210,7 @@ def main():
             "sc_storage": args.sc,
             "sd_storage": args.sd,
             "precision": 1e-8,
-            "symv_item_elements": dict(zip(("fp64", "low_precision"), g.symv_items())),
+            "symv_item_elements": {k: v for k, v in zip(("fp64", "fp32", "bf16"), g.symv_items()) if v},
         },
         "iterations": st["iterations"],
         "phases_s": {k: round(st[k], 6) for k in ("time_assembly", "time_coarse", "time_pcg", "time_primal")},

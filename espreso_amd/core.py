@@ -468,10 +468,11 @@ class FETISolver:
         return b
 
     def symv_items(self):
-        """SYMV work-item sizes chosen by the autotuner: (FP64, low precision)."""
-        t = (C.c_int64 * 2)()
+        """SYMV work-item sizes chosen by the autotuner: (FP64, FP32, BF16),
+        0 for a precision nothing is stored in."""
+        t = (C.c_int64 * 3)()
         _check(lib().espreso_feti_symv_items(self.h, t))
-        return int(t[0]), int(t[1])
+        return tuple(int(v) for v in t)
 
     def values(self, i, nnz):
         """K_i CSR values as the device holds them (after setup)."""

@@ -58,16 +58,19 @@ struct DualDev {
     double *Fd = nullptr;            // packed F_i, FP64 (always kept)
     float *Ff = nullptr;             // packed F_i, FP32
     uint16_t *Fh = nullptr;          // packed F_i, BF16
-    int64_t *foff_lp = nullptr;      // offsets of the low-precision copy
     double *SD = nullptr;            // packed S_D, FP64 (sd_prec 0)
     float *SDf = nullptr;            // packed S_D, FP32 (sd_prec 1), low-precision geometry
     uint16_t *SDh = nullptr;         // packed S_D, BF16 (sd_prec 2), scaled by sdscale
+    // offsets of the packed pools per storage precision (FP64, FP32, BF16 strip
+    // geometry R = 16, 32, 64); poff[0] == foff, unused precisions null
+    int64_t *poff[3] = {nullptr, nullptr, nullptr};
     float *sdscale = nullptr;
     int sd_prec = 0;
     double *GGinv = nullptr;
-    // SYMV work lists: FP64 geometry (F_i and S_D) and low-precision geometry
-    SymvItem *items_d = nullptr;  int n_items_d = 0;  int *item0_d = nullptr;  int *nitems_d = nullptr;
-    SymvItem *items_lp = nullptr; int n_items_lp = 0; int *item0_lp = nullptr; int *nitems_lp = nullptr;
+    // SYMV work lists per storage precision (F_i and S_D of one precision share them)
+    SymvItem *items[3] = {nullptr, nullptr, nullptr};
+    int n_items[3] = {0, 0, 0};
+    int *item0[3] = {nullptr, nullptr, nullptr}, *nitems[3] = {nullptr, nullptr, nullptr};
     double *ypart = nullptr;
     double *yb = nullptr, *xb = nullptr;  // nbnd scratch
     double *cd = nullptr, *ce = nullptr;  // coarse scratch
@@ -98,6 +101,7 @@ struct ClusterDev {
     int *nq = nullptr, *qrow = nullptr;
     int64_t *qoff = nullptr, *fb0off = nullptr, *fccoff = nullptr, *asub = nullptr;
     double *FB0 = nullptr;     // m x nq per subdomain: sign_q * F_i[:, j_q]
+    float *FB0f = nullptr;     // FB0 in FP32 (FP32/BF16 storage of F_i only)
     double *Fcc = nullptr;     // nq x nq per subdomain: sign_q1 sign_q2 F_i[j_q1, j_q2]
     double *lambda0 = nullptr, *alphaK = nullptr;
 };
@@ -134,8 +138,11 @@ void launch_project(const DualDev &D, const double *x, double *y, int mode, hipS
 // out[asub[s] + a] = -R_s^T (B_s^T x) with the subdomain kernel (hybrid FETI e0)
 void launch_subdomain_kernel_dots(const DualDev &D, const double *x, double *out, hipStream_t st);
 // y_b += -K^+ B0^T lambda0 (+ R alphaK when addR) after solving the cluster problems
+// lowp: apply -F_i B0^T lambda0 with FB0f (the inner iterations on the FP32/BF16 F_i)
 void launch_cluster_correction(const DualDev &D, const ClusterDev &C, const double *e0s, double *yb, int addR,
-                               hipStream_t st);
+                               hipStream_t st, bool lowp = false);
+// FB0f = FB0 rounded to FP32 (for the inner iterations on FP32/BF16 storage)
+void launch_fb0_round(const DualDev &D, const ClusterDev &C, hipStream_t st);
 // the correction for nr cases of a block (case strides ystr, tstr/lstr, astr):
 // t_v = F0^-1 B0 y_b,v into t (the lambda0 layout), then per case alpha and
 // lambda0 (launch_cluster_alpha), then y_b,v += -F B0^T lambda0_v + R alphaK_v

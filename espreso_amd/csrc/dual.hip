@@ -771,11 +771,13 @@ __global__ void __launch_bounds__(256) k_cluster_sa(const ClusterDev C, double *
     }
 }
 
-// y_b += -FB0_s lambda0 + addR * R_b alphaK; grid (row chunks of 256, subdomains)
-__global__ void __launch_bounds__(256) k_cluster_apply(const ClusterDev C, const int *M, const int64_t *boff,
-                                                       const int *kdim, const int64_t *rboff, const double *Rb,
-                                                       const double *lambda0, const double *alphaK, int addR,
-                                                       double *yb) {
+// y_b += -FB0_s lambda0 + addR * R_b alphaK; grid (row chunks of 256, subdomains);
+// FB0 is C.FB0 (FP64) or C.FB0f (the low-precision F_i's columns)
+template <typename T>
+__global__ void __launch_bounds__(256) k_cluster_apply(const ClusterDev C, const T *FB0, const int *M,
+                                                       const int64_t *boff, const int *kdim, const int64_t *rboff,
+                                                       const double *Rb, const double *lambda0, const double *alphaK,
+                                                       int addR, double *yb) {
     __shared__ double lam[256];
     const int s = blockIdx.y, m = M[s];
     if (blockIdx.x * 256 >= m) return;  // uniform
@@ -785,18 +787,18 @@ __global__ void __launch_bounds__(256) k_cluster_apply(const ClusterDev C, const
     __syncthreads();
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= m) return;
-    const double *FB = C.FB0 + C.fb0off[s] + j;
+    const T *FB = FB0 + C.fb0off[s] + j;
     // eight independent chains: all loads of a group issue before their use
     double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int q = 0;
     for (; q + 8 <= nq; q += 8) {
-        double f[8];
+        T f[8];
 #pragma unroll
         for (int i = 0; i < 8; i++) f[i] = FB[(int64_t)(q + i) * m];
 #pragma unroll
-        for (int i = 0; i < 8; i++) acc[i] -= f[i] * lam[q + i];
+        for (int i = 0; i < 8; i++) acc[i] -= (double)f[i] * lam[q + i];
     }
-    for (; q < nq; q++) acc[q & 7] -= FB[(int64_t)q * m] * lam[q];
+    for (; q < nq; q++) acc[q & 7] -= (double)FB[(int64_t)q * m] * lam[q];
     double v = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     if (addR) {
         const double *R = Rb + rboff[s] + j;
@@ -804,6 +806,24 @@ __global__ void __launch_bounds__(256) k_cluster_apply(const ClusterDev C, const
         for (int a = 0; a < kdim[s]; a++) v += R[(int64_t)a * m] * al[a];
     }
     yb[boff[s] + j] += v;
+}
+
+// FB0f = FB0 rounded to FP32.  With FP32 storage these are exactly the columns
+// of the stored F_i, so the inner operator stays symmetric.  With BF16 storage
+// they are not rounded to BF16: F0 = B0 F_i B0^T is built from the FP64 F_i,
+// and the correction cancels the corner directions only when one of its two
+// factors matches F0; a BF16-rounded pair leaves an indefinite error of the
+// BF16 rounding on them (PCG then stalls).  grid (chunks, subdomains)
+__global__ void __launch_bounds__(256) k_fb0_round(const ClusterDev C, const int *M) {
+    const int s = blockIdx.y;
+    const int64_t n = (int64_t)M[s] * C.nq[s];
+    const double *src = C.FB0 + C.fb0off[s];
+    float *dst = C.FB0f + C.fb0off[s];
+    for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) dst[e] = (float)src[e];
+}
+
+void launch_fb0_round(const DualDev &D, const ClusterDev &C, hipStream_t st) {
+    hipLaunchKernelGGL(k_fb0_round, dim3(64, D.nsub), dim3(256), 0, st, C, D.m);
 }
 
 // ---- the cluster correction for up to 8 cases per pass (block re-solve):
@@ -966,8 +986,8 @@ void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gathe
                  hipEvent_t ev0, hipEvent_t ev1) {
     if (sd) prec = D.sd_prec;
     const bool lp = prec != 0;
-    const SymvItem *items = lp ? D.items_lp : D.items_d;
-    const int n_items = lp ? D.n_items_lp : D.n_items_d;
+    const SymvItem *items = D.items[prec];
+    const int n_items = D.n_items[prec];
     const double *coef = scaled ? D.bl_sscale : D.bl_sign;
     const size_t shm = (2 * (size_t)D.max_m + strip_rows(prec)) * (lp ? 4 : 8);
     if (ev0) (void)hipEventRecord(ev0, st);
@@ -978,17 +998,17 @@ void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gathe
                            off, (const float *)nullptr, D.bl_ptr, D.bl_lam, coef, x, gather, D.ypart);
     } else if (prec == 1) {
         hipLaunchKernelGGL((k_symv<float, STRIP_F32>), dim3(n_items), dim3(256), shm, st, items, D.m, D.boff,
-                           sd ? D.SDf : D.Ff, D.foff_lp, (const float *)nullptr, D.bl_ptr, D.bl_lam, coef, x, gather,
+                           sd ? D.SDf : D.Ff, D.poff[1], (const float *)nullptr, D.bl_ptr, D.bl_lam, coef, x, gather,
                            D.ypart);
     } else {
         hipLaunchKernelGGL((k_symv<uint16_t, STRIP_B16>), dim3(n_items), dim3(256), shm, st, items, D.m, D.boff,
-                           sd ? D.SDh : D.Fh, D.foff_lp, (const float *)(sd ? D.sdscale : D.fscale), D.bl_ptr,
+                           sd ? D.SDh : D.Fh, D.poff[2], (const float *)(sd ? D.sdscale : D.fscale), D.bl_ptr,
                            D.bl_lam, coef, x, gather, D.ypart);
     }
     if (ev1) (void)hipEventRecord(ev1, st);
     const int R = strip_rows(prec);
-    hipLaunchKernelGGL(k_part_reduce, dim3(D.nsub), dim3(256), 0, st, items, lp ? D.item0_lp : D.item0_d,
-                       lp ? D.nitems_lp : D.nitems_d, D.m, D.boff, R, D.ypart, D.yb);
+    hipLaunchKernelGGL(k_part_reduce, dim3(D.nsub), dim3(256), 0, st, items, D.item0[prec], D.nitems[prec], D.m,
+                       D.boff, R, D.ypart, D.yb);
 }
 
 int block_symv_window(int K) { return K >= 8 ? 2 : 4; }
@@ -1081,14 +1101,18 @@ void launch_subdomain_kernel_dots(const DualDev &D, const double *x, double *out
 }
 
 void launch_cluster_correction(const DualDev &D, const ClusterDev &C, const double *e0s, double *yb, int addR,
-                               hipStream_t st) {
+                               hipStream_t st, bool lowp) {
     if (C.max_n0 > 0)
         hipLaunchKernelGGL(k_cluster_t, dim3((C.max_n0 + 31) / 32, C.ncl), dim3(256), (size_t)C.max_n0 * 8, st, C, yb,
                            C.lambda0);
     hipLaunchKernelGGL(k_cluster_alpha, dim3(C.ncl), dim3(1024), 2 * (size_t)std::max(C.max_kt, 1) * 8, st, C, e0s,
                        C.lambda0, C.alphaK);
-    hipLaunchKernelGGL(k_cluster_apply, dim3((D.max_m + 255) / 256, D.nsub), dim3(256), 0, st, C, D.m, D.boff, D.kdim,
-                       D.rboff, D.Rb, C.lambda0, C.alphaK, addR, yb);
+    if (lowp && C.FB0f)
+        hipLaunchKernelGGL(k_cluster_apply<float>, dim3((D.max_m + 255) / 256, D.nsub), dim3(256), 0, st, C,
+                           (const float *)C.FB0f, D.m, D.boff, D.kdim, D.rboff, D.Rb, C.lambda0, C.alphaK, addR, yb);
+    else
+        hipLaunchKernelGGL(k_cluster_apply<double>, dim3((D.max_m + 255) / 256, D.nsub), dim3(256), 0, st, C,
+                           (const double *)C.FB0, D.m, D.boff, D.kdim, D.rboff, D.Rb, C.lambda0, C.alphaK, addR, yb);
 }
 
 // t_v = F0^-1 B0 y_b,v for nr cases (case stride ystr in yb, tstr in t)

@@ -152,7 +152,7 @@ struct espreso_feti_s {
            *d_scal = nullptr;
     double *d_u = nullptr;    // per-subdomain primal (local order), sum n
     double *d_f = nullptr, *d_Qm = nullptr, *d_GGt = nullptr, *d_c = nullptr;  // resident inputs
-    std::vector<int64_t> h_foff, h_lpoff, h_boff;
+    std::vector<int64_t> h_poff[3], h_boff;
     double *d_e = nullptr;    // -R^T f
     std::vector<int64_t> uoff;
     std::vector<double> lambda_h;
@@ -195,8 +195,8 @@ struct espreso_feti_s {
 
     void load();
     void gpu_setup();
-    void assemble(const double *d_f, const double *d_Q, const std::vector<int64_t> &foff,
-                  const std::vector<int64_t> &lpoff, const std::vector<int64_t> &boff);
+    void assemble(const double *d_f, const double *d_Q, const std::vector<int64_t> *poff,
+                  const std::vector<int64_t> &boff);
     void coarse();
     void coarse_layout_subdomains();
     void load_clusters();
@@ -285,11 +285,13 @@ struct espreso_feti_s {
     void build_items(int prec, int64_t target, std::vector<SymvItem> &items, std::vector<int> &item0,
                      std::vector<int> &nit, int64_t &plen);
     // SYMV work-item size (matrix elements per workgroup) per storage
-    // geometry: 0 FP64, 1 the low-precision one; chosen by tune_symv
-    int64_t symv_target[2] = {SYMV_DEFAULT_TARGET, SYMV_DEFAULT_TARGET};
+    // precision (FP64, FP32, BF16); 0 for a precision nothing is stored in;
+    // chosen by tune_symv
+    int64_t symv_target[3] = {0, 0, 0};
     bool symv_tuned = false;
     int max_m_h = 0;
-    void upload_items(int geo, int64_t target);
+    bool prec_used(int p) const { return p == 0 || p == opt.sc_precision || p == opt.sd_precision; }
+    void upload_items(int prec, int64_t target);
     void tune_symv();
     // dual operators on device vectors
     void apply_F(const double *x, double *y, int prec);
@@ -505,26 +507,29 @@ void espreso_feti_s::load() {
     d_alpha_tot = alloc<double>(coffv[nsub]);
     double *d_Q = upload(Q);
 
-    // packed pools
-    std::vector<int64_t> foff(nsub + 1, 0), lpoff(nsub + 1, 0);
-    const int Rlp = strip_rows(prec);
-    for (int s = 0; s < nsub; s++) {
-        foff[s + 1] = foff[s] + strip_size(sub[s].m, STRIP_F64);
-        lpoff[s + 1] = lpoff[s] + (prec ? strip_size(sub[s].m, Rlp) : 0);
-    }
-    ESP_CHECK(opt.sd_precision == 0 || opt.sd_precision == prec, ESPRESO_ERR_ARG,
-              "sd_precision must be FP64 or equal to sc_precision");
+    // packed pools: one strip geometry per storage precision in use (F_i and
+    // S_D may be stored in different precisions)
+    ESP_CHECK(opt.sd_precision >= 0 && opt.sd_precision <= 2, ESPRESO_ERR_ARG, "sd_precision must be 0, 1 or 2");
     D.sd_prec = opt.sd_precision;
+    std::vector<int64_t> poff[3];
+    for (int p = 0; p < 3; p++) {
+        if (!prec_used(p)) continue;
+        poff[p].assign(nsub + 1, 0);
+        for (int s = 0; s < nsub; s++) poff[p][s + 1] = poff[p][s] + strip_size(sub[s].m, strip_rows(p));
+        D.poff[p] = upload(poff[p]);
+    }
+    const std::vector<int64_t> &foff = poff[0];
+    const double esize[3] = {8, 4, 2};
     D.Fd = alloc<double>(foff[nsub]);
-    D.foff = upload(foff);
+    D.foff = D.poff[0];
     D.sdoff = D.foff;
-    fsc_bytes = (double)foff[nsub] * 8;
-    if (prec == 1) { D.Ff = alloc<float>(lpoff[nsub]); fsc_bytes = (double)lpoff[nsub] * 4; }
-    if (prec == 2) { D.Fh = alloc<uint16_t>(lpoff[nsub]); fsc_bytes = (double)lpoff[nsub] * 2; }
-    if (D.sd_prec == 0) { D.SD = alloc<double>(foff[nsub]); sd_bytes = (double)foff[nsub] * 8; }
-    if (D.sd_prec == 1) { D.SDf = alloc<float>(lpoff[nsub]); sd_bytes = (double)lpoff[nsub] * 4; }
-    if (D.sd_prec == 2) { D.SDh = alloc<uint16_t>(lpoff[nsub]); sd_bytes = (double)lpoff[nsub] * 2; }
-    D.foff_lp = upload(lpoff);
+    fsc_bytes = (double)poff[prec][nsub] * esize[prec];
+    sd_bytes = (double)poff[D.sd_prec][nsub] * esize[D.sd_prec];
+    if (prec == 1) D.Ff = alloc<float>(poff[1][nsub]);
+    if (prec == 2) D.Fh = alloc<uint16_t>(poff[2][nsub]);
+    if (D.sd_prec == 0) D.SD = alloc<double>(foff[nsub]);
+    if (D.sd_prec == 1) D.SDf = alloc<float>(poff[1][nsub]);
+    if (D.sd_prec == 2) D.SDh = alloc<uint16_t>(poff[2][nsub]);
     D.fscale = alloc<float>(nsub);
     D.sdscale = alloc<float>(nsub);
     {
@@ -610,8 +615,7 @@ void espreso_feti_s::load() {
         d_f = upload(ff);
     }
     d_Qm = d_Q;
-    h_foff = foff;
-    h_lpoff = lpoff;
+    for (int p = 0; p < 3; p++) h_poff[p] = poff[p];
     h_boff = boff;
     if (hybrid) load_clusters();
     else coarse_layout_subdomains();
@@ -619,21 +623,23 @@ void espreso_feti_s::load() {
     // ---- SYMV work lists (the item size is tuned at the first set-up) ----
     {
         max_m_h = max_m;
+        int64_t target = SYMV_DEFAULT_TARGET;
         if (const char *e = std::getenv("ESPRESO_SYMV_ITEM")) {
-            symv_target[0] = symv_target[1] = std::max<int64_t>(4096, std::atoll(e));
+            target = std::max<int64_t>(4096, std::atoll(e));
             symv_tuned = true;
         }
         // partial vectors sized for the smallest item size any candidate uses
         std::vector<SymvItem> it;
         std::vector<int> i0, ni;
         int64_t plen_max = 1, pl = 0;
-        for (int geo = 0; geo < (prec ? 2 : 1); geo++) {
-            build_items(geo ? prec : 0, std::min(symv_target[geo], SYMV_TUNE_TARGETS[0]), it, i0, ni, pl);
+        for (int p = 0; p < 3; p++) {
+            if (!prec_used(p)) continue;
+            build_items(p, std::min(target, SYMV_TUNE_TARGETS[0]), it, i0, ni, pl);
             plen_max = std::max(plen_max, pl);
         }
         D.ypart = alloc<double>(plen_max);
-        upload_items(0, symv_target[0]);
-        if (prec) upload_items(1, symv_target[1]);
+        for (int p = 0; p < 3; p++)
+            if (prec_used(p)) upload_items(p, target);
     }
     // PCG vectors
     d_x = alloc<double>(nlam); d_r = alloc<double>(nlam); d_w = alloc<double>(nlam); d_y = alloc<double>(nlam);
@@ -668,7 +674,7 @@ void espreso_feti_s::gpu_setup() {
     drop_graphs();
     if (const char *e = std::getenv("ESPRESO_HIP_GRAPHS")) use_graphs = std::atoi(e) != 0;
     double ta = wall();
-    assemble(d_f, d_Qm, h_foff, h_lpoff, h_boff);
+    assemble(d_f, d_Qm, h_poff, h_boff);
     last.time_assembly = wall() - ta;
     double tc = wall();
     if (hybrid) cluster_setup();
@@ -734,8 +740,8 @@ void espreso_feti_s::resolve_rhs() {
     sync();
 }
 
-void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::vector<int64_t> &foff,
-                              const std::vector<int64_t> &lpoff, const std::vector<int64_t> &boff) {
+void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::vector<int64_t> *poff,
+                              const std::vector<int64_t> &boff) {
     const int prec = opt.sc_precision;
     // per-subdomain workspace (doubles)
     auto ws = [&](const SubHost &h) {
@@ -811,8 +817,10 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         }
         DenseDesc *d_desc = upload(desc);
         // wave-local offset tables
-        std::vector<int64_t> rp_off(nbt), nz_off(nbt), n_off(nbt), b_off(nbt), q_off(nbt), f_off(nbt),
-            lp_off(nbt), band_off(nbt), yf_off(nbt);
+        std::vector<int64_t> rp_off(nbt), nz_off(nbt), n_off(nbt), b_off(nbt), q_off(nbt), p_off[3], band_off(nbt),
+            yf_off(nbt);
+        for (int p = 0; p < 3; p++)
+            if (!poff[p].empty()) p_off[p].assign(poff[p].begin() + s0, poff[p].begin() + s1);
         std::vector<int> nloc(nbt), kd(nbt), bw(nbt);
         {
             // global offsets of this wave's subdomains in the persistent pools
@@ -828,7 +836,6 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
             for (int b = 0; b < nbt; b++) {
                 const SubHost &h = sub[s0 + b];
                 rp_off[b] = rp; nz_off[b] = nz; n_off[b] = nn; b_off[b] = boff[s0 + b]; q_off[b] = qo;
-                f_off[b] = foff[s0 + b]; lp_off[b] = lpoff[s0 + b];
                 band_off[b] = band; yf_off[b] = yf;
                 rp += h.n + 1; nz += h.rowptr[h.n]; nn += h.n; qo += (int64_t)h.m * h.kdim;
                 band += (int64_t)h.ni * (h.bw + 1); yf += h.ni;
@@ -837,8 +844,11 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
             }
         }
         int64_t *d_rp_off = upload(rp_off), *d_nz_off = upload(nz_off), *d_n_off = upload(n_off),
-                *d_b_off = upload(b_off), *d_q_off = upload(q_off), *d_f_off = upload(f_off),
-                *d_lp_off = upload(lp_off), *d_band_off = upload(band_off), *d_yf_off = upload(yf_off);
+                *d_b_off = upload(b_off), *d_q_off = upload(q_off), *d_band_off = upload(band_off),
+                *d_yf_off = upload(yf_off);
+        int64_t *d_p_off[3] = {nullptr, nullptr, nullptr};
+        for (int p = 0; p < 3; p++)
+            if (!p_off[p].empty()) d_p_off[p] = upload(p_off[p]);
         int *d_nloc = upload(nloc), *d_kd = upload(kd), *d_bw = upload(bw);
         double *d_rho = alloc<double>(nbt);
 
@@ -849,9 +859,9 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         launch_syrk(d_desc, nbt, 0, max_ntm, st);
         launch_hf(d_desc, nbt, d_fb, d_hf, d_b_off, max_m, st);
         launch_band_extract(d_desc, nbt, P.band, d_band_off, d_bw, P.yf, d_yf_off, max_ni, st);
-        if (D.sd_prec == 0) launch_pack(d_desc, nbt, 0, D.SD, d_f_off, nullptr, max_m, st);
-        if (D.sd_prec == 1) launch_pack(d_desc, nbt, 1, D.SDf, d_lp_off, nullptr, max_m, st);
-        if (D.sd_prec == 2) launch_pack(d_desc, nbt, 2, D.SDh, d_lp_off, D.sdscale + s0, max_m, st);
+        if (D.sd_prec == 0) launch_pack(d_desc, nbt, 0, D.SD, d_p_off[0], nullptr, max_m, st);
+        if (D.sd_prec == 1) launch_pack(d_desc, nbt, 1, D.SDf, d_p_off[1], nullptr, max_m, st);
+        if (D.sd_prec == 2) launch_pack(d_desc, nbt, 2, D.SDh, d_p_off[2], D.sdscale + s0, max_m, st);
         launch_kernel_reg(d_desc, nbt, d_Q, d_q_off, d_kd, d_rho, max_m, st);
         for (int k = 0; k < max_ntm; k++) launch_potrf_step(d_desc, nbt, 1, k, max_ntm, st);
         for (int k = 0; k < max_ntm; k++) launch_trsm_step(d_desc, nbt, 1, k, max_ntm, max_ntm, st);
@@ -877,9 +887,9 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
             sync();
             for (void *p : {(void *)d_nq, (void *)d_qo, (void *)d_fo, (void *)d_co}) release(p);
         }
-        launch_pack(d_desc, nbt, 0, D.Fd, d_f_off, nullptr, max_m, st);
-        if (prec == 1) launch_pack(d_desc, nbt, 1, D.Ff, d_lp_off, nullptr, max_m, st);
-        if (prec == 2) launch_pack(d_desc, nbt, 2, D.Fh, d_lp_off, D.fscale + s0, max_m, st);
+        launch_pack(d_desc, nbt, 0, D.Fd, d_p_off[0], nullptr, max_m, st);
+        if (prec == 1) launch_pack(d_desc, nbt, 1, D.Ff, d_p_off[1], nullptr, max_m, st);
+        if (prec == 2) launch_pack(d_desc, nbt, 2, D.Fh, d_p_off[2], D.fscale + s0, max_m, st);
         std::vector<int> info(nbt);
         ESP_HIP(hipMemcpyAsync(info.data(), d_info, nbt * sizeof(int), hipMemcpyDeviceToHost, st));
         sync();
@@ -888,12 +898,14 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
                       "dense factorisation breakdown in subdomain " + std::to_string(s0 + b) + " at column " +
                           std::to_string(info[b]));
         for (void *p : {(void *)d_cs, (void *)d_info, (void *)d_desc, (void *)d_rp_off, (void *)d_nz_off,
-                        (void *)d_n_off, (void *)d_b_off, (void *)d_q_off, (void *)d_f_off, (void *)d_lp_off,
-                        (void *)d_band_off, (void *)d_yf_off, (void *)d_nloc, (void *)d_kd, (void *)d_bw,
-                        (void *)d_rho})
+                        (void *)d_n_off, (void *)d_b_off, (void *)d_q_off, (void *)d_band_off, (void *)d_yf_off,
+                        (void *)d_nloc, (void *)d_kd, (void *)d_bw, (void *)d_rho})
             release(p);
+        for (int64_t *q : d_p_off)
+            if (q) release(q);
         s0 = s1;
     }
+    if (hybrid && CL.FB0f) launch_fb0_round(D, CL, st);
 }
 
 void espreso_feti_s::coarse() {
@@ -964,28 +976,24 @@ void espreso_feti_s::build_items(int prec, int64_t target, std::vector<SymvItem>
     }
 }
 
-// (re)build and upload the SYMV work list of a storage geometry
-void espreso_feti_s::upload_items(int geo, int64_t target) {
-    const int prec = geo ? std::max(opt.sc_precision, D.sd_prec) : 0;
+// (re)build and upload the SYMV work list of a storage precision
+void espreso_feti_s::upload_items(int prec, int64_t target) {
     std::vector<SymvItem> it;
     std::vector<int> i0, ni;
     int64_t plen = 0;
     build_items(prec, target, it, i0, ni, plen);
-    SymvItem *&items = geo ? D.items_lp : D.items_d;
-    int *&item0 = geo ? D.item0_lp : D.item0_d;
-    int *&nitems = geo ? D.nitems_lp : D.nitems_d;
-    for (void *q : {(void *)items, (void *)item0, (void *)nitems})
+    for (void *q : {(void *)D.items[prec], (void *)D.item0[prec], (void *)D.nitems[prec]})
         if (q) release(q);
-    items = upload(it);
-    item0 = upload(i0);
-    nitems = upload(ni);
-    (geo ? D.n_items_lp : D.n_items_d) = (int)it.size();
-    symv_target[geo] = target;
+    D.items[prec] = upload(it);
+    D.item0[prec] = upload(i0);
+    D.nitems[prec] = upload(ni);
+    D.n_items[prec] = (int)it.size();
+    symv_target[prec] = target;
 }
 
 // Autotuner of the SYMV work-item size, keyed on the subdomain size: at the
-// first set-up every candidate is timed on the assembled F_i (HIP events, 3
-// applies each) and the fastest kept; the choice is cached for the process by
+// first set-up every candidate is timed on the assembled F_i (HIP events, 2
+// applies each after one warm-up apply) and the fastest kept; the choice is cached for the process by
 // (storage precision, largest interface size rounded to 64).  ESPRESO_SYMV_ITEM
 // fixes the size, ESPRESO_AUTOTUNE=0 keeps the default.
 void espreso_feti_s::tune_symv() {
@@ -995,30 +1003,29 @@ void espreso_feti_s::tune_symv() {
         if (std::atoi(e) == 0) return;
     static std::mutex mu;
     static std::map<std::pair<int, int>, int64_t> cache;
-    const int prec_lp = std::max(opt.sc_precision, D.sd_prec);
-    for (int geo = 0; geo < (prec_lp ? 2 : 1); geo++) {
-        const int prec = geo ? prec_lp : 0;
+    for (int prec = 0; prec < 3; prec++) {
+        if (!prec_used(prec)) continue;
         const std::pair<int, int> key{prec, (max_m_h + 63) / 64};
         {
             std::lock_guard<std::mutex> lk(mu);
             auto f = cache.find(key);
             if (f != cache.end()) {
-                if (f->second != symv_target[geo]) upload_items(geo, f->second);
+                if (f->second != symv_target[prec]) upload_items(prec, f->second);
                 continue;
             }
         }
-        // time on the F_i of this precision if stored, else on S_D
-        const bool sd = geo == 1 ? opt.sc_precision == 0 : false;
+        // time on the F_i of this precision if stored in it, else on S_D
+        const bool sd = prec != 0 && prec != opt.sc_precision;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         ESP_HIP(hipEventCreate(&e0));
         ESP_HIP(hipEventCreate(&e1));
-        int64_t best = symv_target[geo];
+        int64_t best = symv_target[prec];
         float best_ms = 1e30f;
+        launch_symv(D, sd, prec, d_b, 1, false, st);  // warm
         for (const int64_t target : SYMV_TUNE_TARGETS) {
-            upload_items(geo, target);
-            launch_symv(D, sd, prec, d_b, 1, false, st);  // warm
+            upload_items(prec, target);
             ESP_HIP(hipEventRecord(e0, st));
-            for (int r = 0; r < 3; r++) launch_symv(D, sd, prec, d_b, 1, false, st);
+            for (int r = 0; r < 2; r++) launch_symv(D, sd, prec, d_b, 1, false, st);
             ESP_HIP(hipEventRecord(e1, st));
             ESP_HIP(hipEventSynchronize(e1));
             float ms = 0;
@@ -1030,7 +1037,7 @@ void espreso_feti_s::tune_symv() {
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        upload_items(geo, best);
+        upload_items(prec, best);
         std::lock_guard<std::mutex> lk(mu);
         cache[key] = best;
     }
@@ -1042,7 +1049,7 @@ void espreso_feti_s::apply_F(const double *x, double *y, int prec) {
     if (hybrid) {
         // K~^+ = K^+ - K^+ B0^T lambda0 + R alpha (cluster.cpp:796-916), e0 = -R^T B^T x
         launch_subdomain_kernel_dots(D, x, d_e0s, st);
-        launch_cluster_correction(D, CL, d_e0s, D.yb, 1, st);
+        launch_cluster_correction(D, CL, d_e0s, D.yb, 1, st, prec != 0);
     }
     launch_lambda_gather(D, D.yb, false, y, st);
 }
@@ -1797,6 +1804,7 @@ void espreso_feti_s::load_clusters() {
     CL.fccoff = upload(fccoff_h);
     CL.asub = D.asub;
     CL.FB0 = alloc<double>(fb0off[nsub]);
+    if (opt.sc_precision) CL.FB0f = alloc<float>(fb0off[nsub]);
     CL.Fcc = alloc<double>(std::max<int64_t>(1, fccoff_h[nsub]));
     n_lambda0 = std::max<int64_t>(1, rowoff[ncl]);
     CL.lambda0 = alloc<double>(n_lambda0);
@@ -2088,8 +2096,7 @@ int espreso_feti_resolve(espreso_feti *h, espreso_feti_stats *st) {
 int espreso_feti_symv_items(espreso_feti *h, int64_t *targets) {
     ESP_API_BEGIN
     ESP_CHECK(h && targets, ESPRESO_ERR_ARG, "null argument");
-    targets[0] = h->symv_target[0];
-    targets[1] = h->symv_target[1];
+    for (int p = 0; p < 3; p++) targets[p] = h->symv_target[p];
     ESP_API_END
 }
 
@@ -2196,7 +2203,7 @@ int espreso_feti_subdomain_dense(espreso_feti *h, int32_t i, int32_t which, doub
         std::vector<int64_t> lo(1, 0);
         float sc = 1.0f;
         if (sp) {
-            ESP_HIP(hipMemcpy(lo.data(), h->D.foff_lp + i, sizeof(int64_t), hipMemcpyDeviceToHost));
+            ESP_HIP(hipMemcpy(lo.data(), h->D.poff[sp] + i, sizeof(int64_t), hipMemcpyDeviceToHost));
             ESP_HIP(hipMemcpy(&sc, h->D.sdscale + i, sizeof(float), hipMemcpyDeviceToHost));
         }
         if (sp == 0) launch_unpack(h->D.SD, 0, off[0], m, 1.0f, tmp, h->st);
@@ -2207,7 +2214,7 @@ int espreso_feti_subdomain_dense(espreso_feti *h, int32_t i, int32_t which, doub
         std::vector<int64_t> lo(1);
         float sc = 1.0f;
         if (prec) {
-            ESP_HIP(hipMemcpy(lo.data(), h->D.foff_lp + i, sizeof(int64_t), hipMemcpyDeviceToHost));
+            ESP_HIP(hipMemcpy(lo.data(), h->D.poff[prec] + i, sizeof(int64_t), hipMemcpyDeviceToHost));
             ESP_HIP(hipMemcpy(&sc, h->D.fscale + i, sizeof(float), hipMemcpyDeviceToHost));
         }
         if (prec == 0) launch_unpack(h->D.Fd, 0, off[0], m, 1.0f, tmp, h->st);

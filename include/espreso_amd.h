@@ -91,8 +91,10 @@ typedef struct {
     int32_t device;             /* HIP device ordinal                          */
     double  precision;          /* stop: ||P r|| < precision*min(||P r0||,||b||) */
     double  memory_budget_gb;   /* 0 = 85% of free device memory              */
-    int32_t sd_precision;       /* storage of the Dirichlet S_D: ESPRESO_SC_FP64, or
-                                   the same low precision as sc_precision        */
+    int32_t sd_precision;       /* storage of the Dirichlet S_D: ESPRESO_SC_*, any
+                                   of the three whatever sc_precision is (S_D is the
+                                   preconditioner: it changes the iteration count,
+                                   not the converged solution)                    */
     int32_t pad_;
 } espreso_feti_options;
 
@@ -239,10 +241,11 @@ int espreso_feti_apply_F(espreso_feti *h, const double *x, double *y);
 int espreso_feti_apply_prec(espreso_feti *h, const double *x, double *y);
 int espreso_feti_project(espreso_feti *h, const double *x, double *y);
 int espreso_feti_vec_b(espreso_feti *h, double *b);
-/* SYMV work-item sizes in matrix elements per workgroup: [0] FP64 storage,
- * [1] FP32/BF16 storage.  The first espreso_feti_setup picks them by timing
- * candidates on the assembled operators (cached per process by storage
- * precision and subdomain interface size); ESPRESO_SYMV_ITEM fixes them and
+/* SYMV work-item sizes in matrix elements per workgroup, targets[3]: FP64,
+ * FP32 and BF16 storage (0 where neither F_i nor S_D is stored in that
+ * precision).  The first espreso_feti_setup picks them by timing candidates
+ * on the assembled operators (cached per process by storage precision and
+ * subdomain interface size); ESPRESO_SYMV_ITEM fixes them and
  * ESPRESO_AUTOTUNE=0 keeps the default (327680).                            */
 int espreso_feti_symv_items(espreso_feti *h, int64_t *targets);
 /* K_i CSR values of subdomain i as the device holds them (after setup; the

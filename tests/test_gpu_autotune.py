@@ -40,10 +40,11 @@ def test_autotuned_items(sc):
     tuned, ut = solve(True, sc_precision=sc)
     fixed, uf = solve(False, sc_precision=sc)
     print("tuned", tuned, "fixed", fixed, flush=True)
+    p = core.SC[sc]
     assert tuned[0] in CANDIDATES
-    if sc != "fp64":
-        assert tuned[1] in CANDIDATES
-    assert fixed == (320 * 1024, 320 * 1024)
+    assert tuned[p] in CANDIDATES
+    assert all(tuned[q] == 0 for q in (1, 2) if q != p)
+    assert fixed == tuple(320 * 1024 if q in (0, p) else 0 for q in range(3))
     ref = O.global_solve(SPEC)
     assert np.abs(ut - uf).max() <= 1e-9 * np.abs(uf).max()
     assert np.abs(ut - ref).max() <= 1e-8 * np.abs(ref).max()

@@ -175,6 +175,32 @@ def test_invalid_input_is_reported():
     s.close()
 
 
+@pytest.mark.parametrize("sc,sd", [("fp64", "bf16"), ("fp32", "bf16"), ("bf16", "fp32"), ("fp64", "fp32")])
+def test_mixed_storage_precisions(sc, sd):
+    """F_i and S_D stored in different precisions: each keeps its own strip
+    geometry; the solution stays within 1e-8 of the direct solve and the stored
+    S_D within its rounding of the oracle's."""
+    say("case: F_i", sc, "S_D", sd)
+    g = core.GridFETI(MULTI, preconditioner="dirichlet", precision=1e-10, sc_precision=sc, sd_precision=sd)
+    g.setup()
+    st = g.solve()
+    err = rel(g.nodal_solution(), O.global_solve(MULTI))
+    items = g.symv_items()
+    say("F_i", sc, "S_D", sd, "err", err, "iters", st["iterations"], "items", items)
+    assert err < 1e-8
+    used = {0, core.SC[sc], core.SC[sd]}
+    assert all((items[p] > 0) == (p in used) for p in range(3))
+    o = O.Oracle(MULTI, preconditioner=3, kplus=1)
+    tol = {"fp64": 1e-12, "fp32": 2 ** -23, "bf16": 2 ** -8}
+    for s in (0, o.n_sub - 1):
+        Sd = o.dense(s, 3)
+        assert np.abs(g.dense(s, 1) - Sd).max() <= 2 * tol[sd] * np.abs(Sd).max()
+        F = g.dense(s, 0)  # FP64 F_i; which=2 is F_i as stored
+        assert np.abs(g.dense(s, 2) - F).max() <= 2 * tol[sc] * np.abs(F).max()
+    o.close()
+    g.close()
+
+
 @pytest.mark.parametrize("sc", ["fp32", "bf16"])
 def test_low_precision_dirichlet(sc):
     say("case: S_D stored in", sc)
