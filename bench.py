@@ -14,6 +14,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -50,6 +52,14 @@ def cpu_baseline(spec, iterations, n_sub, threads, sample):
         "setup_s": setup,
         "per_iteration_s": per_iter,
     }
+
+
+def oracle_residual(spec, u):
+    """Relative residual of the GPU's nodal solution in the oracle's FP64 global
+    assembly (no factorisation: one element-by-element K u)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    return float(O.global_residual(O.undecomposed(spec), u))
 
 
 def main():
@@ -132,6 +142,7 @@ def main():
     t_all = time.perf_counter() - t_all
     st = stats[-1]
     tts = t_all / args.steps
+    u_gpu = g.nodal_solution()  # before any load case replaces the right-hand side
 
     rhs_report = None
     if args.rhs > 0:
@@ -245,10 +256,26 @@ def main():
     if rhs_report:
         result["load_cases"] = rhs_report
     g.close()
+    accuracy = {"dual_rel_residual": st["final_norm"] / st["tol"] * 1e-8,
+                "dual_note": "||P r|| / min(||P r0||, ||b||) at the stop (stop test < 1e-8)"}
+    if args.sc != "fp64" or args.sd != "fp64":
+        # the same workload on the FP64 path (whose parity with the oracle the tests establish)
+        g64 = core.GridFETI(spec, preconditioner=args.preconditioner, precision=1e-8, max_iterations=1000,
+                            method=method)
+        g64.setup()
+        g64.solve()
+        u64 = g64.nodal_solution()
+        g64.close()
+        accuracy["primal_rel_diff_vs_fp64_gpu"] = float(np.abs(u_gpu - u64).max() / np.abs(u64).max())
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"bench: CPU baseline on {threads} host threads, sample {args.cpu_sample} subdomains")
         result["cpu_baseline"] = cpu_baseline(spec, st["iterations"], st["n_subdomains"], threads, args.cpu_sample)
+        accuracy["primal_rel_residual_oracle"] = oracle_residual(spec, u_gpu)
+        accuracy["primal_note"] = ("||K u - f|| / ||f|| over the free DOFs of the undecomposed problem, K and f "
+                                   "assembled element by element by the oracle in FP64")
+    result["accuracy"] = accuracy
+    log("bench: accuracy " + json.dumps(accuracy))
     print(json.dumps(result), flush=True)
 
 

@@ -1058,7 +1058,7 @@ struct Oracle {
 
 // ------------------------------------------------- global direct solve ----
 int global_solve(const oracle_grid_problem &pp, double *u_nodal, double *rel_res_check, const double *u_in,
-                 const double *f_nodal = nullptr) {
+                 const double *f_nodal = nullptr, double *res_out = nullptr) {
     Grid G(pp);
     Loads L = make_loads(G);
     std::vector<double> Ke1, Nint;
@@ -1088,7 +1088,7 @@ int global_solve(const oracle_grid_problem &pp, double *u_nodal, double *rel_res
     for (int g = 0; g < n; g++)
         if (fixed[g]) fst[g] = g;
     Profile P;
-    P.init(fst);
+    if (!u_in) P.init(fst);  // the residual mode assembles K u only
     std::vector<double> f(n, 0.0);
     if (f_nodal) std::copy(f_nodal, f_nodal + n, f.begin());
     int ne = 8 * nd;
@@ -1120,6 +1120,7 @@ int global_solve(const oracle_grid_problem &pp, double *u_nodal, double *rel_res
     if (u_in) {
         double num = 0, den = 0;
         for (int g = 0; g < n; g++) {
+            if (res_out) res_out[g] = fixed[g] ? 0.0 : Ku[g] - f[g];
             if (fixed[g]) continue;
             num += (Ku[g] - f[g]) * (Ku[g] - f[g]);
             den += f[g] * f[g];
@@ -1168,6 +1169,19 @@ int oracle_global_solve_load(const oracle_grid_problem *p, const double *f_nodal
 
 int oracle_global_residual(const oracle_grid_problem *p, const double *u_nodal, double *rel_res) {
     return global_solve(*p, nullptr, rel_res, u_nodal);
+}
+
+// the per-subdomain conductivities that conductivity_seed draws (subdomain ids
+// cluster-major, x fastest), for meshes whose subdomains are not uniform boxes
+void oracle_seeded_conductivity(uint64_t seed, int32_t nsub, double base, double *out) {
+    Rng r(seed);
+    for (int s = 0; s < nsub; s++) out[s] = base * std::pow(10.0, 2.0 * r.uniform() - 1.0);
+}
+
+// K u - f per DOF (0 on fixed DOFs), nodal layout
+int oracle_global_residual_vector(const oracle_grid_problem *p, const double *u_nodal, double *res) {
+    double r = 0;
+    return global_solve(*p, nullptr, &r, u_nodal, nullptr, res);
 }
 
 void *oracle_create(const oracle_grid_problem *p, const oracle_options *o) {

@@ -95,6 +95,10 @@ int oracle_unit_element(const oracle_grid_problem *p, double *Ke);
 int oracle_global_sizes(const oracle_grid_problem *p, int32_t *n_nodes, int32_t *ndof);
 /* global rhs/K actions for residual checks: y = K x (free rows) ; f */
 int oracle_global_residual(const oracle_grid_problem *p, const double *u_nodal, double *rel_res);
+/* the per-subdomain conductivities conductivity_seed draws (cluster-major ids) */
+void oracle_seeded_conductivity(uint64_t seed, int32_t nsub, double base, double *out);
+/* K u - f per DOF of the undecomposed problem (0 on fixed DOFs) */
+int oracle_global_residual_vector(const oracle_grid_problem *p, const double *u_nodal, double *res);
 
 /* ---- decomposed TFETI --------------------------------------------------- */
 void *oracle_create(const oracle_grid_problem *p, const oracle_options *o);

@@ -97,6 +97,9 @@ def lib():
         _lib.oracle_global_solve_load.argtypes = [C.POINTER(OracleProblem), d, d]
         _lib.oracle_global_sizes.argtypes = [C.POINTER(OracleProblem), i32, i32]
         _lib.oracle_global_residual.argtypes = [C.POINTER(OracleProblem), d, d]
+        _lib.oracle_global_residual_vector.argtypes = [C.POINTER(OracleProblem), d, d]
+        _lib.oracle_seeded_conductivity.argtypes = [C.c_uint64, C.c_int32, C.c_double, d]
+        _lib.oracle_seeded_conductivity.restype = None
         _lib.oracle_cpu_sample.argtypes = [C.POINTER(OracleProblem), C.POINTER(OracleOptions), C.c_int32, d]
         _lib.oracle_last_error.restype = C.c_char_p
     return _lib
@@ -205,6 +208,39 @@ def global_residual(spec, u):
     r = C.c_double()
     lib().oracle_global_residual(C.byref(p), _dp(u), C.byref(r))
     return r.value
+
+
+def undecomposed(spec):
+    """The spec as one subdomain over the same global mesh, for the undecomposed
+    oracle solve and residual.  A seeded per-subdomain conductivity becomes an
+    explicit element field first, so that uneven subdomains keep their values."""
+    from espreso_amd import core
+    from espreso_amd import problems as PB
+    s = dict(spec)
+    if s.get("physics") == 1 and s.get("conductivity_seed") and s.get("material_elem") is None:
+        cl, dom = s["clusters"], s["domains"]
+        ns = [cl[d] * dom[d] for d in range(3)]
+        ks = np.zeros(ns[0] * ns[1] * ns[2])
+        lib().oracle_seeded_conductivity(s["conductivity_seed"], ks.size, s.get("conductivity", 1.0), _dp(ks))
+        off = [np.concatenate([[0], np.cumsum(v)]) for v in core.slabs(s)]
+        ez, ey, ex = np.meshgrid(*(np.arange(off[d][-1]) for d in (2, 1, 0)), indexing="ij")
+        S = [np.searchsorted(off[d], e.ravel(), side="right") - 1 for d, e in ((0, ex), (1, ey), (2, ez))]
+        # subdomain id: cluster-major, x fastest within the cluster and across clusters
+        c = [S[d] // dom[d] for d in range(3)]
+        q = [S[d] % dom[d] for d in range(3)]
+        sid = (c[0] + cl[0] * (c[1] + cl[1] * c[2])) * (dom[0] * dom[1] * dom[2]) + q[0] + dom[0] * (q[1] + dom[1] * q[2])
+        s["material_elem"] = ks[sid]
+        s.pop("conductivity_seed")
+    return PB.uniform_equivalent(s)
+
+
+def global_residual_vector(spec, u):
+    """K u - f per nodal DOF (0 on fixed DOFs), K and f of the undecomposed problem."""
+    p = make_problem(spec)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    r = np.zeros(u.size)
+    lib().oracle_global_residual_vector(C.byref(p), _dp(u.ravel()), _dp(r))
+    return r.reshape(u.shape)
 
 
 class Oracle:

@@ -49,6 +49,23 @@ def with_material(spec, seed=3):
     return s
 
 
+def test_uneven_seeded_conductivity():
+    """The builder's seeded per-subdomain conductivity on uneven subdomains,
+    against the direct solve with the same values as an element field."""
+    say("case: uneven heat, seeded conductivity")
+    g = core.GridFETI(HEAT, preconditioner="dirichlet", precision=1e-11, max_iterations=3000)
+    g.setup()
+    g.solve()
+    u = g.nodal_solution()
+    ref = O.undecomposed(HEAT)
+    err = rel(u, O.global_solve(ref))
+    res = O.global_residual(ref, u)
+    say("uneven seeded err", err, "residual", res)
+    assert err < 1e-8
+    assert res < 1e-6
+    g.close()
+
+
 @pytest.mark.parametrize("name", ["heat", "elasticity"])
 def test_uneven_solve_and_load_cases(name):
     spec = with_material(HEAT if name == "heat" else ELAST)

@@ -95,6 +95,38 @@ def test_global_solve_vs_numpy_dense():
     assert np.linalg.norm(ug - u) / np.linalg.norm(u) < 1e-12
 
 
+def test_global_residual_vs_numpy_dense():
+    """oracle_global_residual (element-by-element K u, no factorisation) against
+    the dense K and f summed from the subdomain matrices, on a random u and on
+    the direct solution."""
+    spec = PB.scaled(PB.GRAVITY, clusters=(1, 1, 1), domains=(2, 1, 2), elements=(2, 3, 2),
+                     traction_seed=3, traction_scale=1e5)
+    o, K, f = _dense_global(spec)
+    n = K.shape[0]
+    fixed = np.zeros(n, bool)
+    nn = [2 * 2 + 1, 3 + 1, 2 * 2 + 1]
+    fixed[:3 * nn[0] * nn[1]] = True  # z = 0 face, node-major
+    free = ~fixed
+    u = np.random.default_rng(5).standard_normal(n) * 1e-6
+    u[fixed] = 0.0
+    want = np.linalg.norm((K @ u - f)[free]) / np.linalg.norm(f[free])
+    got = O.global_residual(spec, u.reshape(-1, 3))
+    assert abs(got - want) <= 1e-12 * want
+    assert O.global_residual(spec, O.global_solve(spec)) < 1e-12
+
+
+def test_undecomposed_seeded_conductivity():
+    """undecomposed() turns the seeded per-subdomain conductivity into an element
+    field; on uniform subdomains (several clusters) it must give the oracle's own
+    seeded assignment, so the global solutions agree exactly."""
+    spec = dict(PB.HEAT_UNEVEN_2000, clusters=(2, 1, 2), domains=(1, 2, 2), elements=(2, 2, 2), name="seeded")
+    spec.pop("slab_elements")
+    u1 = O.global_solve(spec)
+    u2 = O.global_solve(O.undecomposed(spec))
+    assert np.array_equal(u1, u2)
+    assert "conductivity_seed" not in O.undecomposed(spec)
+
+
 @pytest.mark.parametrize("spec,prec,scaling,kplus", [
     (PB.GRAVITY, 3, 1, 0), (PB.GRAVITY, 1, 1, 0), (PB.GRAVITY, 3, 0, 1), (PB.GRAVITY, 2, 1, 1),
     (PB.HEAT_SIMPLE, 1, 1, 0), (PB.HEAT_SIMPLE, 3, 0, 1),
