@@ -121,13 +121,10 @@ struct PrimalDev {
 // ev0/ev1 (optional) are recorded around the SYMV kernel alone.
 void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gather, bool scaled, hipStream_t st,
                  hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-// K right-hand sides at once (K = 1, 2, 4, 8; FP64 storage): x is lambda-space,
-// K vectors of n_lambda (gathered with sign, or sign*scale when `scaled`);
-// yb receives K boundary vectors of nbnd.  ev0/ev1 bracket the SYMV kernel.
-void launch_symv_block(const DualDev &D, const SymvBlock &B, bool sd, int K, const double *x, bool scaled,
-                       double *yb, hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-size_t block_symv_lds(int K);
-// the same on MFMA for 1 <= nr <= 16 right-hand sides (stride nr in the scratch)
+// nr right-hand sides at once on MFMA (1 <= nr <= 16; FP64 storage): x is
+// lambda-space, nr vectors of n_lambda (gathered with sign, or sign*scale when
+// `scaled`); yb receives nr boundary vectors of nbnd; stride nr in the scratch.
+// ev0/ev1 bracket the SYMV kernel.
 void launch_symv_mfma(const DualDev &D, const SymvBlock &B, bool sd, int nr, const double *x, bool scaled,
                       double *yb, hipStream_t st, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_lambda_gather(const DualDev &D, const double *yb, bool scaled, double *y, hipStream_t st);

@@ -162,178 +162,6 @@ __global__ void k_part_reduce(const SymvItem *items, const int *item0, const int
     }
 }
 
-// ---------------------------------------------------------------------------
-// Block SYMV: K right-hand sides against the FP64 packed matrices in one pass
-// over their bytes (load cases solved together).  Work items are groups of
-// at most BLOCK_STRIPS strips of one subdomain.  An item's columns are walked
-// in windows of `nk` chunks (64 columns each, counted per strip): x and the
-// column sums of the window live in an LDS ring of W = 64 nk + (strips-1) R
-// slots indexed by (column - first item row) mod W, so LDS stays small for
-// any subdomain size.  After each window the columns no later window touches
-// are final and go to the item's partial vector; the item's own rows (row
-// sums and the columns of its diagonal blocks) stay in LDS to the end.
-// Lanes: G row groups of R/G rows; a group covers 64/G columns of a chunk.
-template <int NV, int W, typename A>
-__device__ __forceinline__ void group_reduce(A *v, int lane) {
-    // transpose-reduce NV per-lane values over the W lanes of an aligned group;
-    // afterwards lane l holds totals of value numbers (see block_row_slot)
-    int n = NV;
-#pragma unroll
-    for (int o = W / 2; o >= 1; o >>= 1) {
-        if (n > 1) {
-            const bool up = lane & o;
-            const int h = n >> 1;
-#pragma unroll
-            for (int i = 0; i < h; i++) {
-                const A lo = v[i], hi = v[i + h];
-                const A send = up ? (A)lo : (A)hi;
-                const A keep = up ? (A)hi : (A)lo;
-                v[i] = keep + __shfl_xor(send, o, 64);
-            }
-            n = h;
-        } else {
-            v[0] += __shfl_xor(v[0], o, 64);
-        }
-    }
-}
-
-template <int K, int G>
-__global__ void __launch_bounds__(256) k_symv_block(const SymvItem *__restrict__ items, const int *__restrict__ M,
-                                                    const int64_t *__restrict__ boff, const double *__restrict__ pool,
-                                                    const int64_t *__restrict__ poff, const double *__restrict__ xb,
-                                                    int nk, double *__restrict__ ypart) {
-    constexpr int R = STRIP_F64, RG = R / G, CW = 64 / G, NLD = RG / 2, NV = RG * K;
-    constexpr int SR = BLOCK_STRIPS * R;
-    constexpr int PER = NV >= CW ? NV / CW : 1;  // totals per lane after group_reduce
-    constexpr int SPREAD = NV >= CW ? 1 : CW / NV;
-    extern __shared__ double sm[];
-    const SymvItem it = items[blockIdx.x];
-    const int s = it.sub, m = M[s];
-    const int r0i = it.t0 * R, ns = it.t1 - it.t0;
-    const int nrow = min(ns * R, m - r0i);
-    const int W = 64 * nk + (ns - 1) * R;
-    double *xr = sm, *yr = xr + SR * K, *xw = yr + SR * K, *yw = xw + (size_t)W * K;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, g = lane / CW, cl = lane % CW;
-    const double *xs = xb + boff[s] * K;
-    for (int i = tid; i < SR * K; i += 256) {
-        xr[i] = i < nrow * K ? xs[(int64_t)r0i * K + i] : 0.0;
-        yr[i] = 0.0;
-    }
-    for (int i = tid; i < W * K; i += 256) yw[i] = 0.0;
-    const int nwin = ((m - r0i + 63) / 64 + nk - 1) / nk;
-    const double *bs = pool + poff[s];
-    for (int wi = 0; wi < nwin; wi++) {
-        const int base = r0i + 64 * nk * wi;  // first column of the window
-        const int slot0 = (64 * nk * wi) % W;  // its ring slot
-        const int wlen = min(W, m - base);
-        for (int i = tid; i < wlen * K; i += 256) {
-            int sl = slot0 + i / K;
-            if (sl >= W) sl -= W;
-            xw[sl * K + i % K] = xs[(int64_t)base * K + i];
-        }
-        __syncthreads();
-        for (int t = it.t0 + wave; t < it.t1; t += 4) {
-            const int r0 = t * R, ncol = m - r0;
-            const int k0 = nk * wi, k1 = min((ncol + 63) / 64, k0 + nk);
-            if (k0 >= k1) continue;
-            const double *S = bs + strip_offset(m, R, t);
-            const double *xrow = xr + (r0 - r0i + g * RG) * K;
-            double racc[NV];
-#pragma unroll
-            for (int e = 0; e < NV; e++) racc[e] = 0.0;
-            for (int k = k0; k < k1; k++) {
-                const int w = min(64, ncol - 64 * k);
-                const u32x4 *chunk = (const u32x4 *)(S + (int64_t)64 * k * R);
-#pragma unroll
-                for (int sc = 0; sc < G; sc++) {
-                    const int cc = sc * CW + cl;
-                    if (cc < w) {
-                        // 16-byte load q of column cc holds rows 2q, 2q+1 at u32x4 index q*w + cc
-                        const u32x4 *src = chunk + g * NLD * w + cc;
-                        u32x4 raw[NLD];
-#pragma unroll
-                        for (int j = 0; j < NLD; j++) raw[j] = __builtin_nontemporal_load(src + (int64_t)j * w);
-                        const double *a = (const double *)raw;
-                        int sl = slot0 + (r0 + 64 * k + cc - base);
-                        if (sl >= W) sl -= W;
-                        double xc[K], col[K];
-#pragma unroll
-                        for (int v = 0; v < K; v++) {
-                            xc[v] = xw[sl * K + v];
-                            col[v] = 0.0;
-                        }
-#pragma unroll
-                        for (int r = 0; r < RG; r++)
-#pragma unroll
-                            for (int v = 0; v < K; v++) {
-                                col[v] = fma(a[r], xrow[r * K + v], col[v]);
-                                racc[r * K + v] = fma(a[r], xc[v], racc[r * K + v]);
-                            }
-#pragma unroll
-                        for (int v = 0; v < K; v++) atomicAdd(&yw[sl * K + v], col[v]);
-                    }
-                }
-            }
-            group_reduce<NV, CW>(racc, lane);
-            if (cl % SPREAD == 0) {
-#pragma unroll
-                for (int i = 0; i < PER; i++) {
-                    const int e = (cl / SPREAD) * PER + i;  // value number r*K + v
-                    const int row = r0 - r0i + g * RG + e / K;
-                    if (r0i + row < m) atomicAdd(&yr[row * K + e % K], racc[i]);
-                }
-            }
-        }
-        __syncthreads();
-        // columns [base, base + 64 nk) are final; the item's own rows stay in yr
-        const int fin = min(64 * nk, m - base);
-        for (int i = tid; i < fin * K; i += 256) {
-            int sl = slot0 + i / K;
-            if (sl >= W) sl -= W;
-            const int j = base - r0i + i / K;
-            const double val = yw[sl * K + i % K];
-            yw[sl * K + i % K] = 0.0;
-            if (j < nrow) yr[j * K + i % K] += val;
-            else ypart[(it.part_off + j) * K + i % K] = val;
-        }
-        __syncthreads();
-    }
-    for (int i = tid; i < nrow * K; i += 256) ypart[it.part_off * K + i] = yr[i];
-}
-
-// yb[v * nbnd + boff + j] = sum of the block partials covering j (fixed order)
-template <int K>
-__global__ void k_part_reduce_block(const SymvItem *items, const int *item0, const int *nitems, const int *M,
-                                    const int64_t *boff, int64_t nbnd, const double *ypart, double *yb) {
-    const int s = blockIdx.x;
-    const int m = M[s];
-    const int i0 = item0[s], ni = nitems[s];
-    for (int i = threadIdx.x; i < m * K; i += blockDim.x) {
-        const int j = i / K, v = i % K;
-        double acc = 0.0;
-        for (int q = 0; q < ni; q++) {
-            const SymvItem it = items[i0 + q];
-            const int j0 = it.t0 * STRIP_F64;
-            if (j >= j0) acc += ypart[(it.part_off + j - j0) * K + v];
-        }
-        yb[v * nbnd + boff[s] + j] = acc;
-    }
-}
-
-// K right-hand sides: xb[j K + v] = sum coef * x[v nlam + lambda]
-template <int K>
-__global__ void k_boundary_gather_block(int64_t nb, const int64_t *bl_ptr, const int *bl_lam, const double *coef,
-                                        const double *x, int64_t nlam, double *xb) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nb * K;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = i / K;
-        const int v = (int)(i % K);
-        double acc = 0.0;
-        for (int64_t q = bl_ptr[j]; q < bl_ptr[j + 1]; q++) acc += coef[q] * x[v * nlam + bl_lam[q]];
-        xb[i] = acc;
-    }
-}
-
 // Block SYMV on MFMA (v_mfma_f64_16x16x4f64): up to 16 right-hand sides, the
 // MFMA's N dimension (lanes of right-hand sides past nr carry zeros).  An
 // item is at most BLOCK_STRIPS strips of one subdomain; wave w owns the
@@ -1009,43 +837,6 @@ void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gathe
     const int R = strip_rows(prec);
     hipLaunchKernelGGL(k_part_reduce, dim3(D.nsub), dim3(256), 0, st, items, D.item0[prec], D.nitems[prec], D.m,
                        D.boff, R, D.ypart, D.yb);
-}
-
-int block_symv_window(int K) { return K >= 8 ? 2 : 4; }
-
-size_t block_symv_lds(int K) {
-    const int W = 64 * block_symv_window(K) + (BLOCK_STRIPS - 1) * STRIP_F64;
-    return (size_t)(2 * BLOCK_STRIPS * STRIP_F64 + 2 * W) * K * sizeof(double);
-}
-
-void launch_symv_block(const DualDev &D, const SymvBlock &B, bool sd, int K, const double *x, bool scaled,
-                       double *yb, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1) {
-    const double *coef = scaled ? D.bl_sscale : D.bl_sign;
-    const double *pool = sd ? D.SD : D.Fd;
-    const int64_t *off = sd ? D.sdoff : D.foff;
-    const int nk = block_symv_window(K);
-    const size_t shm = block_symv_lds(K);
-    const unsigned gg = (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(D.nbnd * K, 256), 8192));
-#define ESP_BLOCK_CASE(KK, GG)                                                                                      \
-    case KK:                                                                                                        \
-        hipLaunchKernelGGL(k_boundary_gather_block<KK>, dim3(gg), dim3(256), 0, st, D.nbnd, D.bl_ptr, D.bl_lam,     \
-                           coef, x, (int64_t)D.nlam, B.xb);                                                         \
-        if (ev0) (void)hipEventRecord(ev0, st);                                                                     \
-        hipLaunchKernelGGL((k_symv_block<KK, GG>), dim3(B.n_items), dim3(256), shm, st, B.items, D.m, D.boff, pool, \
-                           off, B.xb, nk, B.ypart);                                                                 \
-        if (ev1) (void)hipEventRecord(ev1, st);                                                                     \
-        hipLaunchKernelGGL(k_part_reduce_block<KK>, dim3(D.nsub), dim3(256), 0, st, B.items, B.item0, B.nitems,     \
-                           D.m, D.boff, D.nbnd, B.ypart, yb);                                                       \
-        break;
-    switch (K) {
-        ESP_BLOCK_CASE(1, 1)
-        ESP_BLOCK_CASE(2, 1)
-        ESP_BLOCK_CASE(4, 2)
-        ESP_BLOCK_CASE(8, 4)
-    default:
-        throw Error(ESPRESO_ERR_ARG, "block SYMV: K must be 1, 2, 4 or 8");
-    }
-#undef ESP_BLOCK_CASE
 }
 
 void launch_symv_mfma(const DualDev &D, const SymvBlock &B, bool sd, int nr, const double *x, bool scaled,
