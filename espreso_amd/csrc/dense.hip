@@ -24,6 +24,15 @@ namespace esp {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// Loads through the global address space: the operand pointers of the tile
+// routine live in descriptors, so the compiler cannot prove them global and
+// would emit flat loads, which also count in lgkmcnt; the wait for the LDS
+// operand reads before the MFMAs would then wait for the prefetch of the next
+// chunk as well.
+__device__ __forceinline__ double gload(const double *p) {
+    return *(const __attribute__((address_space(1))) double *)p;
+}
+
 // C(MxN) = alpha * op(A)(MxK) * op(B)(KxN) + beta * C, M,N <= 64.
 // op(A)(i,k) = TA ? A[k + i*lda] : A[i + k*lda];  op(B)(k,j) = TB ? B[j + k*ldb] : B[k + j*ldb].
 // C may alias A or B when this tile is the only reader of the aliased tile:
@@ -52,15 +61,22 @@ __device__ __forceinline__ void tile_gemm(double *C, int ldc, const double *A, i
         if (!TB) { kk = q & 15; j = q >> 4; } else { j = q & 63; kk = q >> 6; }
     };
     double ra[4], rb[4];
+    int fk = 0;  // K offset of the chunk held in ra/rb
+    // out-of-range elements load element 0 of the operand (in bounds), with no
+    // branch around the loads; store() replaces them by zero, so the wait for
+    // the loads falls after the MFMAs of the chunk in between
     auto fetch = [&](int k0) {
+        fk = k0;
 #pragma unroll
         for (int it = 0; it < 4; it++) {
             int i, kk;
             aidx(it, i, kk);
-            ra[it] = (i < M && k0 + kk < K) ? (TA ? A[(k0 + kk) + (size_t)i * lda] : A[i + (size_t)(k0 + kk) * lda]) : 0.0;
+            const bool va = i < M && k0 + kk < K;
+            ra[it] = gload(A + (va ? (TA ? (k0 + kk) + (size_t)i * lda : i + (size_t)(k0 + kk) * lda) : 0));
             int j;
             bidx(it, j, kk);
-            rb[it] = (j < N && k0 + kk < K) ? (TB ? B[j + (size_t)(k0 + kk) * ldb] : B[(k0 + kk) + (size_t)j * ldb]) : 0.0;
+            const bool vb = j < N && k0 + kk < K;
+            rb[it] = gload(B + (vb ? (TB ? j + (size_t)(k0 + kk) * ldb : (k0 + kk) + (size_t)j * ldb) : 0));
         }
     };
     auto store = [&](int buf) {
@@ -68,10 +84,10 @@ __device__ __forceinline__ void tile_gemm(double *C, int ldc, const double *A, i
         for (int it = 0; it < 4; it++) {
             int i, kk;
             aidx(it, i, kk);
-            As[buf][kk][i] = ra[it];
+            As[buf][kk][i] = (i < M && fk + kk < K) ? ra[it] : 0.0;
             int j;
             bidx(it, j, kk);
-            Bs[buf][kk][j] = rb[it];
+            Bs[buf][kk][j] = (j < N && fk + kk < K) ? rb[it] : 0.0;
         }
     };
     if (K > 0) {
