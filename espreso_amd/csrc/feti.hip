@@ -650,17 +650,18 @@ void espreso_feti_s::load() {
     d_scal = alloc<double>(16);
     d_c = upload(c);
     {
-        const int ssz = prec == 0 ? 8 : (prec == 1 ? 4 : 2);
-        double mat = 0, mat64 = 0, gat = 0, yb = 0;
+        // algorithmic bytes of one SYMV: the packed triangle in the storage
+        // precision, 20 B per B1 entry (index, coefficient, lambda value), y_b
+        const double esz[3] = {8, 4, 2};
+        double tri = 0, gat = 0, yb = 0;
         for (int s = 0; s < nsub; s++) {
             const double m = sub[s].m;
-            mat += ssz * m * (m + 1) / 2;
-            mat64 += 8 * m * (m + 1) / 2;
+            tri += m * (m + 1) / 2;
             gat += 20.0 * sub[s].lam.size();
             yb += 8 * m;
         }
-        last.symv_bytes = mat + gat + yb;
-        last.symv_bytes_sd = (D.sd_prec ? mat : mat64) + gat + yb;
+        last.symv_bytes = esz[prec] * tri + gat + yb;
+        last.symv_bytes_sd = esz[D.sd_prec] * tri + gat + yb;
     }
     sync();
     loaded = true;
