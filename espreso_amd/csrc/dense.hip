@@ -131,6 +131,138 @@ __device__ __forceinline__ void tile_gemm(double *C, int ldc, const double *A, i
             }
 }
 
+// ---- fused tile steps: C = Dinv * (beta C - op(A) op(B)), the update kept in registers ----
+// The 64 x 64 tile is split into column strips: wave w owns columns 16w..16w+15
+// and all 64 rows (row blocks rb = 0..3), so the update R of its strip stays in
+// the MFMA accumulators.  Value v of acc[rb] in lane l is R(16 rb + l/16 + 4 v,
+// 16 w + l%16), which is the B operand of a 16x16x4 MFMA whose k-step covers
+// rows 16 rb + 4 v + (0..3): the multiply by the staged diagonal inverse takes
+// R straight from the accumulators, without a round trip of C through memory.
+constexpr int FUSED_LDS = 2 * 2 * 16 * (TILE + 1);  // doubles: two operand buffers, or one padded 64 x 64 tile
+
+// acc[rb] += op(A)(rows of block rb, :K) op(B)(:K, the wave's columns), M x N result, K > 0
+template <bool TA, bool TB>
+__device__ __forceinline__ void strip_gemm(d4 (&acc)[4], double *lds, const double *A, int lda, const double *B,
+                                           int ldb, int M, int N, int K) {
+    double(*As)[16][TILE + 1] = reinterpret_cast<double(*)[16][TILE + 1]>(lds);
+    double(*Bs)[16][TILE + 1] = reinterpret_cast<double(*)[16][TILE + 1]>(lds + 2 * 16 * (TILE + 1));
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    auto aidx = [&](int it, int &i, int &kk) {
+        const int q = tid + 256 * it;
+        if (!TA) { i = q & 63; kk = q >> 6; } else { kk = q & 15; i = q >> 4; }
+    };
+    auto bidx = [&](int it, int &j, int &kk) {
+        const int q = tid + 256 * it;
+        if (!TB) { kk = q & 15; j = q >> 4; } else { j = q & 63; kk = q >> 6; }
+    };
+    double ra[4], rb[4];
+    int fk = 0;
+    auto fetch = [&](int k0) {
+        fk = k0;
+#pragma unroll
+        for (int it = 0; it < 4; it++) {
+            int i, kk;
+            aidx(it, i, kk);
+            const bool va = i < M && k0 + kk < K;
+            ra[it] = gload(A + (va ? (TA ? (k0 + kk) + (size_t)i * lda : i + (size_t)(k0 + kk) * lda) : 0));
+            int j;
+            bidx(it, j, kk);
+            const bool vb = j < N && k0 + kk < K;
+            rb[it] = gload(B + (vb ? (TB ? j + (size_t)(k0 + kk) * ldb : (k0 + kk) + (size_t)j * ldb) : 0));
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int it = 0; it < 4; it++) {
+            int i, kk;
+            aidx(it, i, kk);
+            As[buf][kk][i] = (i < M && fk + kk < K) ? ra[it] : 0.0;
+            int j;
+            bidx(it, j, kk);
+            Bs[buf][kk][j] = (j < N && fk + kk < K) ? rb[it] : 0.0;
+        }
+    };
+    fetch(0);
+    store(0);
+    __syncthreads();
+    if (16 < K) fetch(16);
+    for (int k0 = 0, cur = 0; k0 < K; k0 += 16, cur ^= 1) {
+#pragma unroll
+        for (int ks = 0; ks < 16; ks += 4) {
+            const double b = Bs[cur][ks + (lane >> 4)][16 * wave + (lane & 15)];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const double a = As[cur][ks + (lane >> 4)][16 * r + (lane & 15)];
+                acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[r], 0, 0, 0);
+            }
+        }
+        if (k0 + 16 < K) store(cur ^ 1);
+        __syncthreads();  // after the last chunk: the operand buffers are free
+        if (k0 + 32 < K) fetch(k0 + 32);
+    }
+}
+
+// R = beta * C0 - acc on the rows < M, columns < N of the tile (zero elsewhere),
+// then C = Dinv(:, :M) R.  TR: the tile's element (r, c) is C[c + r ldc] (the
+// transposed step), else C[r + c ldc].  Dinv is 64 x 64, column-major.  The
+// operand buffers must be free (no wave still reads them).
+template <bool TR>
+__device__ __forceinline__ void strip_finish(d4 (&acc)[4], double *lds, double *C, int ldc, int M, int N,
+                                             const double *Dinv, double beta) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c = 16 * wave + (lane & 15);
+    auto at = [&](int r) { return TR ? (size_t)c + (size_t)r * ldc : (size_t)r + (size_t)c * ldc; };
+    // stage Dinv(i, k) at lds[k (TILE + 1) + i], columns k >= M zero; staged
+    // before C is read, so that its registers are free again (occupancy)
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        double dv[8];
+#pragma unroll
+        for (int it = 0; it < 8; it++) dv[it] = gload(Dinv + tid + 256 * (8 * h + it));
+#pragma unroll
+        for (int it = 0; it < 8; it++) {
+            const int e = tid + 256 * (8 * h + it), i = e & 63, k = e >> 6;
+            lds[k * (TILE + 1) + i] = k < M ? dv[it] : 0.0;
+        }
+    }
+    if (beta != 0.0) {
+#pragma unroll
+        for (int rbk = 0; rbk < 4; rbk++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = 16 * rbk + (lane >> 4) + 4 * v;
+                const bool ok = r < M && c < N;
+                const double c0 = gload(C + (ok ? at(r) : 0));
+                acc[rbk][v] = ok ? beta * c0 - acc[rbk][v] : 0.0;
+            }
+    } else {
+#pragma unroll
+        for (int rbk = 0; rbk < 4; rbk++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = 16 * rbk + (lane >> 4) + 4 * v;
+                acc[rbk][v] = (r < M && c < N) ? -acc[rbk][v] : 0.0;
+            }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++) {
+        d4 out = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int rbk = 0; rbk < 4; rbk++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const double a = lds[(16 * rbk + 4 * v + (lane >> 4)) * (TILE + 1) + 16 * ob + (lane & 15)];
+                out = __builtin_amdgcn_mfma_f64_16x16x4f64(a, acc[rbk][v], out, 0, 0, 0);
+            }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int r = 16 * ob + (lane >> 4) + 4 * v;
+            if (r < M && c < N) C[at(r)] = out[v];
+        }
+    }
+}
+
 // Workgroups are dealt to the 8 XCDs (each with its own L2) round-robin by
 // linear id, x fastest.  The tiles of one subdomain share their operand panels,
 // so the linear id is remapped to keep runs of consecutive logical tiles (x
@@ -332,18 +464,18 @@ __global__ void __launch_bounds__(256) k_potrf_panel(const DenseDesc *D, int whi
     const int nb = min(TILE, n - k0), mi = min(TILE, n - i0);
     const int j0 = max(0, i - band) * TILE;
     double *C = A + i0 + (size_t)k0 * n;
-    if (k0 > j0) {
-        tile_gemm<false, true>(C, n, A + i0 + (size_t)j0 * n, n, A + k0 + (size_t)j0 * n, n, mi, nb, k0 - j0, -1.0,
-                               1.0);
-        __syncthreads();
-    }
-    tile_gemm<false, true>(C, n, C, n, d.Dinv + (size_t)step * TILE * TILE, TILE, mi, nb, nb, 1.0, 0.0);
+    // transposed: T = L_kw L_iw^T (nb x mi), C^T = Dinv_k (C^T - T), i.e. C = (C - L_iw L_kw^T) Dinv_k^T
+    __shared__ double lds[FUSED_LDS];
+    d4 acc[4] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
+    if (k0 > j0)
+        strip_gemm<false, true>(acc, lds, A + k0 + (size_t)j0 * n, n, A + i0 + (size_t)j0 * n, n, nb, mi, k0 - j0);
+    strip_finish<true>(acc, lds, C, n, nb, mi, d.Dinv + (size_t)step * TILE * TILE, 1.0);
 }
 
 // ---- left-looking forward substitution X = L^-1 B, block row i ----
 // X_iJ = inv(L_ii) (B_iJ - L_iw X_wJ), window w = [max(first nonzero tile of
 // column tile J, i - band), i)
-__global__ void __launch_bounds__(256) k_trsm_row(const DenseDesc *D, int which, int step) {
+__global__ void __launch_bounds__(256, 4) k_trsm_row(const DenseDesc *D, int which, int step) {
     int bx, by;
     xcd_tile(bx, by);
     const DenseDesc &d = D[by];
@@ -367,13 +499,12 @@ __global__ void __launch_bounds__(256) k_trsm_row(const DenseDesc *D, int which,
         }
         return;
     }
-    if (i0 > j0) {
-        // which 1: the identity has no entries off the diagonal tile
-        tile_gemm<false, false>(C, n, A + i0 + (size_t)j0 * n, n, B + j0 + (size_t)c0 * n, n, mi, nj, i0 - j0, -1.0,
-                                which == 1 ? 0.0 : 1.0);
-        __syncthreads();
-    }
-    tile_gemm<false, false>(C, n, d.Dinv + (size_t)i * TILE * TILE, TILE, C, n, mi, nj, mi, 1.0, 0.0);
+    __shared__ double lds[FUSED_LDS];
+    d4 acc[4] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
+    if (i0 > j0)
+        strip_gemm<false, false>(acc, lds, A + i0 + (size_t)j0 * n, n, B + j0 + (size_t)c0 * n, n, mi, nj, i0 - j0);
+    // which 1: the identity has no entries off the diagonal tile (beta 0)
+    strip_finish<false>(acc, lds, C, n, mi, nj, d.Dinv + (size_t)i * TILE * TILE, which == 1 ? 0.0 : 1.0);
 }
 
 // lower tiles of  S = S - Y^T Y  (which 0)  or  S = W^T W  (which 1)
