@@ -196,6 +196,9 @@ def main():
             traffic = None
 
     asm_tflops = st["assembly_flops"] / st["time_assembly"] / 1e12
+    # what the FP64 MFMA instruction of the tile routine sustains on this GPU with
+    # register operands only (no memory, no LDS): the practical ceiling
+    mfma_loop = core.debug_mfma_rate(0, 8)
     result = {
         "metric": METRIC,
         "value": round(tts, 6),
@@ -251,6 +254,10 @@ def main():
             "frac": round(asm_tflops / FP64_PEAK_TFLOPS, 4),
             "flops": st["assembly_flops"],
             "seconds": round(st["time_assembly"], 4),
+            "mfma_loop_TFLOPs": round(mfma_loop, 2),
+            "frac_of_mfma_loop": round(asm_tflops / mfma_loop, 4),
+            "note": "achieved = executed assembly flops / assembly time; mfma_loop = v_mfma_f64_16x16x4f64 "
+                    "issue rate with register operands only, measured in this run (espreso_debug_mfma_rate)",
         },
     }
     if rhs_report:

@@ -141,6 +141,7 @@ SIGNATURES = {
     "espreso_debug_mfma": (C.c_int, [_D, _D, _D]),
     "espreso_debug_copy": (C.c_int, [C.c_int64, C.c_int32, _D]),
     "espreso_debug_stream": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, _D]),
+    "espreso_debug_mfma_rate": (C.c_int, [C.c_int32, C.c_int32, _D]),
     "espreso_debug_tile_gemm": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _D, C.c_int32, _D,
                                           C.c_int32, _D, C.c_int32, C.c_double, C.c_double]),
 }
@@ -165,6 +166,13 @@ def debug_stream(nbytes, reps=5, workgroups=4096):
     ms = C.c_double()
     _check(lib().espreso_debug_stream(int(nbytes), reps, workgroups, C.byref(ms)))
     return ms.value
+
+
+def debug_mfma_rate(shape=0, chains=8):
+    """FP64 MFMA issue rate in TFLOP/s (shape 0: 16x16x4, 1: 4x4x4 with 4 blocks)."""
+    t = C.c_double()
+    _check(lib().espreso_debug_mfma_rate(shape, chains, C.byref(t)))
+    return t.value
 
 
 def debug_tile_gemm(A, B, C, ta=False, tb=False, alpha=1.0, beta=0.0):

@@ -802,6 +802,44 @@ __global__ void __launch_bounds__(256) k_probe_gemm(int ta, int tb, int M, int N
     else tile_gemm<true, true>(C, ldc, A, lda, B, ldb, M, N, K, alpha, beta);
 }
 
+// FP64 MFMA issue rate: CH independent accumulator chains per wave, `iter`
+// steps each, register operands only (one store per lane at the end).  SHAPE 0:
+// v_mfma_f64_16x16x4f64 (2048 flops), 1: v_mfma_f64_4x4x4f64 with 4 blocks (512).
+template <int SHAPE, int CH>
+__global__ void __launch_bounds__(256) k_mfma_rate(double *out, int iter) {
+    const double a = 1.0 + threadIdx.x * 1e-9, b = 1.0 - threadIdx.x * 1e-9;
+    double s = 0.0;
+    if constexpr (SHAPE == 0) {
+        d4 acc[CH];
+#pragma unroll
+        for (int c = 0; c < CH; c++) acc[c] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < iter; i++)
+#pragma unroll
+            for (int c = 0; c < CH; c++) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < CH; c++) s += (acc[c][0] + acc[c][1]) + (acc[c][2] + acc[c][3]);
+    } else {
+        double acc[CH];
+#pragma unroll
+        for (int c = 0; c < CH; c++) acc[c] = 0.0;
+        for (int i = 0; i < iter; i++)
+#pragma unroll
+            for (int c = 0; c < CH; c++) acc[c] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, acc[c], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < CH; c++) s += acc[c];
+    }
+    out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+double mfma_rate_flops(int shape) { return shape == 0 ? 2048.0 : 512.0; }
+
+void launch_mfma_rate(int shape, int chains, int blocks, int iter, double *out, hipStream_t st) {
+    if (shape == 0 && chains == 4) hipLaunchKernelGGL((k_mfma_rate<0, 4>), dim3(blocks), dim3(256), 0, st, out, iter);
+    else if (shape == 0) hipLaunchKernelGGL((k_mfma_rate<0, 8>), dim3(blocks), dim3(256), 0, st, out, iter);
+    else if (chains == 4) hipLaunchKernelGGL((k_mfma_rate<1, 4>), dim3(blocks), dim3(256), 0, st, out, iter);
+    else hipLaunchKernelGGL((k_mfma_rate<1, 8>), dim3(blocks), dim3(256), 0, st, out, iter);
+}
+
 void launch_probe_mfma(const double *a, const double *b, double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_probe_mfma, dim3(1), dim3(64), 0, st, a, b, out);
 }

@@ -2325,6 +2325,34 @@ int espreso_debug_stream(int64_t bytes, int32_t reps, int32_t workgroups, double
     ESP_API_END
 }
 
+int espreso_debug_mfma_rate(int32_t shape, int32_t chains, double *tflops) {
+    ESP_API_BEGIN
+    ESP_CHECK((shape == 0 || shape == 1) && (chains == 4 || chains == 8) && tflops, ESPRESO_ERR_ARG,
+              "shape 0 or 1, chains 4 or 8");
+    const int blocks = 4096, iter = 2000;
+    double *o = nullptr;
+    ESP_HIP(hipMalloc(&o, (size_t)blocks * 256 * sizeof(double)));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float t = 0;
+    if (e == hipSuccess) {
+        launch_mfma_rate(shape, chains, blocks, 16, o, nullptr);  // warm
+        (void)hipEventRecord(e0, nullptr);
+        launch_mfma_rate(shape, chains, blocks, iter, o, nullptr);
+        (void)hipEventRecord(e1, nullptr);
+        e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(o);
+    ESP_HIP(e);
+    ESP_CHECK(t > 0, ESPRESO_ERR_STATE, "MFMA rate loop not timed");
+    *tflops = (double)blocks * 4 * iter * chains * mfma_rate_flops(shape) / (t * 1e-3) / 1e12;
+    ESP_API_END
+}
+
 // ---- diagnostics of the MFMA tile routine ----
 int espreso_debug_mfma(const double *a, const double *b, double *out) {
     ESP_API_BEGIN

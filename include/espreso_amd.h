@@ -277,6 +277,11 @@ int espreso_debug_copy(int64_t bytes, int32_t reps, double *ms);
  * `reps` times: mean ms per pass (the achievable HBM read rate that the
  * bandwidth-bound kernels are compared with).                               */
 int espreso_debug_stream(int64_t bytes, int32_t reps, int32_t workgroups, double *ms);
+/* FP64 MFMA issue rate with register operands only (4096 workgroups of 4 waves,
+ * `chains` = 4 or 8 independent accumulators per wave): shape 0
+ * v_mfma_f64_16x16x4f64, 1 v_mfma_f64_4x4x4f64 (4 blocks); TFLOP/s.  The
+ * ceiling that the assembly kernels' MFMA rate is compared with.            */
+int espreso_debug_mfma_rate(int32_t shape, int32_t chains, double *tflops);
 int espreso_debug_tile_gemm(int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const double *A, int32_t lda,
                             const double *B, int32_t ldb, double *C, int32_t ldc, double alpha, double beta);
 

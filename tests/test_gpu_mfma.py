@@ -39,3 +39,13 @@ def test_tile_gemm(ta, tb, shape):
     ref = -1.5 * ((A.T if ta else A) @ (B.T if tb else B)) + 0.5 * C
     got = core.debug_tile_gemm(A, B, C, ta=ta, tb=tb, alpha=-1.5, beta=0.5)
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("shape", [0, 1])
+def test_mfma_rate_loop(shape):
+    """The FP64 MFMA issue-rate diagnostic runs and gives a plausible rate
+    (published FP64 matrix peak of the part: 78.6 TFLOP/s)."""
+    t = core.debug_mfma_rate(shape, 8)
+    print("mfma rate shape", shape, t, "TFLOP/s", flush=True)
+    assert 5.0 < t < 120.0
+
