@@ -284,6 +284,7 @@ struct espreso_feti_s {
                    const std::vector<int> &cblk, const std::vector<int> &coff, std::vector<double> &GGt);
     void build_items(int prec, int64_t target, std::vector<SymvItem> &items, std::vector<int> &item0,
                      std::vector<int> &nit, int64_t &plen);
+    double tile_flops(const SubHost &h) const;
     // SYMV work-item size (matrix elements per workgroup) per storage
     // precision (FP64, FP32, BF16); 0 for a precision nothing is stored in;
     // chosen by tune_symv
@@ -804,10 +805,7 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
             max_n = std::max(max_n, h.n);
             max_m = std::max(max_m, h.m);
             max_ni = std::max(max_ni, h.ni);
-            // executed flops (banded/profile aware estimate for reporting)
-            asm_flops += (double)h.ni * std::pow((double)std::min(h.ni, (h.band_tiles + 1) * TILE), 2) +
-                         2.0 * h.ni * std::min(h.ni, (h.band_tiles + 1) * TILE) * (h.m + 1) + (double)h.ni * h.m * h.m +
-                         (double)h.m * h.m * h.m;
+            asm_flops += tile_flops(h);
         }
         int *d_cs = upload(cs_all);
         int *d_info = alloc<int>(nbt);
@@ -943,6 +941,44 @@ void espreso_feti_s::coarse() {
     release(Dinv);
     release(info);
     release(dd);
+}
+
+// Flops of the assembly tiles of one subdomain, following the kernels' own tile
+// rules (band windows, the leading zero tiles of [K_ib f] that the TRSM and SYRK
+// skip, triangular ranges in the S chain), partial tiles at their real size:
+// the figure the reported assembly rate is computed from.  MFMA instructions
+// issue slightly more (partial tiles are padded to 16).
+double espreso_feti_s::tile_flops(const SubHost &h) const {
+    double f = 0;
+    for (int which = 0; which < 2; which++) {
+        const int n = which ? h.m : h.ni, nt = (n + TILE - 1) / TILE;
+        const int band = which ? nt : h.band_tiles;
+        const int ncols = which ? h.m : h.m + 1, ntc = (ncols + TILE - 1) / TILE;
+        for (int k = 0; k < nt; k++) {
+            const double k0 = (double)k * TILE, nb = std::min(TILE, n - k * TILE);
+            const double j0 = (double)std::max(0, k - band) * TILE;
+            f += 2 * nb * nb * (k0 - j0) + 2.0 / 3.0 * nb * nb * nb;  // diagonal update, factor and inverse
+            for (int i = k + 1; i < nt && i <= k + band; i++) {        // panel
+                const double mi = std::min(TILE, n - i * TILE), w0 = (double)std::max(0, i - band) * TILE;
+                f += 2 * mi * nb * std::max(0.0, k0 - w0) + 2 * mi * nb * nb;
+            }
+            for (int J = 0; J < ntc; J++) {  // forward substitution, block row k
+                const int cs = which ? J : h.colstart[J];
+                if (cs > k || (which && J == k)) continue;
+                const double nj = std::min(TILE, ncols - J * TILE), w0 = (double)std::max(cs, k - band) * TILE;
+                f += 2 * nb * nj * std::max(0.0, k0 - w0) + 2 * nb * nj * nb;
+            }
+        }
+        const int ntm = (h.m + TILE - 1) / TILE;
+        for (int I = 0; I < ntm; I++)  // S -= Y^T Y  or  F = W^T W, lower tiles
+            for (int J = 0; J <= I; J++) {
+                const double mi = std::min(TILE, h.m - I * TILE), mj = std::min(TILE, h.m - J * TILE);
+                const double ks = which ? (double)I * TILE : (double)std::max(h.colstart[I], h.colstart[J]) * TILE;
+                const double len = (which ? h.m : h.ni) - ks;
+                if (len > 0) f += 2 * mi * mj * len;
+            }
+    }
+    return f;
 }
 
 void espreso_feti_s::build_items(int prec, int64_t target, std::vector<SymvItem> &items, std::vector<int> &item0,
