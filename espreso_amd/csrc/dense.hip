@@ -561,19 +561,28 @@ __global__ void k_rho(const DenseDesc *D, double *rho) {
     if (threadIdx.x == 0) rho[blockIdx.x] = v;
 }
 
-__global__ void k_kernel_reg(const DenseDesc *D, const double *Q, const int64_t *q_off, const int *kdim,
-                             const double *rho) {
+// S += rho Q Q^T on the lower triangle: columns j over workgroups, rows i >= j
+// over threads (coalesced, no index division); Q(j, :) in registers for k <= 8
+__global__ void __launch_bounds__(256) k_kernel_reg(const DenseDesc *D, const double *Q, const int64_t *q_off,
+                                                    const int *kdim, const double *rho) {
+    constexpr int KR = 8;
     const DenseDesc &d = D[blockIdx.y];
     const int m = d.m, k = kdim[blockIdx.y];
     const double r = rho[blockIdx.y];
     const double *q = Q + q_off[blockIdx.y];
-    const int64_t mm = (int64_t)m * m;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < mm; e += (int64_t)gridDim.x * blockDim.x) {
-        int i = (int)(e % m), j = (int)(e / m);
-        if (i < j) continue;
-        double s = 0.0;
-        for (int a = 0; a < k; a++) s += q[i + (size_t)a * m] * q[j + (size_t)a * m];
-        d.S[e] += r * s;
+    for (int j = blockIdx.x; j < m; j += gridDim.x) {
+        double qj[KR];
+#pragma unroll
+        for (int a = 0; a < KR; a++) qj[a] = a < k ? q[j + (size_t)a * m] : 0.0;
+        double *col = d.S + (size_t)j * m;
+        for (int i = j + threadIdx.x; i < m; i += 256) {
+            double s = 0.0;
+#pragma unroll
+            for (int a = 0; a < KR; a++)
+                if (a < k) s += q[i + (size_t)a * m] * qj[a];
+            for (int a = KR; a < k; a++) s += q[i + (size_t)a * m] * q[j + (size_t)a * m];
+            col[i] += r * s;
+        }
     }
 }
 
@@ -677,59 +686,70 @@ __global__ void k_unpack(const T *pool, int64_t off, int m, float scale, double 
 // column j all of K_ib(:, j) and the rows of S(:, j) from its diagonal tile
 // down, and the permuted f as column m of [K_ib | f].  Every entry a later
 // kernel reads is written here, so the workspace needs no zero fill.
+// One wave per column (4 columns per workgroup), each wave assembling its column
+// in its own LDS window of MAT_W rows; only wave-level synchronisation, so the
+// waves of a workgroup proceed independently.
+constexpr int MAT_W = 1024;
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ void __launch_bounds__(256) k_materialize(const DenseDesc *D, const int *rowptr, const int64_t *rp_off,
                                                      const int *colidx, const double *vals, const int64_t *nz_off,
                                                      const int *iperm, const int *perm, const int64_t *perm_off,
-                                                     const double *f, const int *nloc, int cols_per_wg, int cap) {
-    extern __shared__ double colbuf[];
+                                                     const double *f, const int *nloc) {
+    __shared__ double win[4][MAT_W];
     const DenseDesc &d = D[blockIdx.y];
     const int b = blockIdx.y, n = nloc[b], ni = d.ni, m = d.m;
-    const int cb = blockIdx.x * cols_per_wg;
-    if (cb > n) return;  // uniform; columns 0..n-1 of K plus n = f
-    const int ce = min(n + 1, cb + cols_per_wg);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = blockIdx.x * 4 + wave;
+    if (c > n) return;  // columns 0..n-1 of K plus n = f; no workgroup barrier follows
     const int *rp = rowptr + rp_off[b];
     const int *ci = colidx + nz_off[b];
     const double *kv = vals + nz_off[b];
     const int *ip = iperm + perm_off[b];
     const int *pm = perm + perm_off[b];
-    for (int c = cb; c < ce; c++) {
-        if (c == n) {  // f column, permuted
-            double *dst = d.Kib + (size_t)m * ni;
-            for (int r = threadIdx.x; r < ni; r += 256) dst[r] = f[perm_off[b] + pm[r]];
-            continue;
+    if (c == n) {  // f column, permuted
+        double *dst = d.Kib + (size_t)m * ni;
+        for (int r = lane; r < ni; r += 64) dst[r] = f[perm_off[b] + pm[r]];
+        return;
+    }
+    // rows [lo, hi) of the permuted column c
+    int lo, hi;
+    if (c < ni) {
+        lo = (c / TILE) * TILE;
+        hi = min(ni, lo + (d.band_ii + 1) * TILE);
+    } else {
+        lo = 0;
+        hi = n;  // [K_ib(:, j) ; S(:, j)] as one permuted column
+    }
+    const int l = pm[c];
+    const int q0 = rp[l], q1 = rp[l + 1];
+    double *cb = win[wave];
+    for (int w0 = lo; w0 < hi; w0 += MAT_W) {
+        const int w1 = min(hi, w0 + MAT_W);
+        for (int r = w0 + lane; r < w1; r += 64) cb[r - w0] = 0.0;
+        wave_lds_sync();
+        for (int q = q0 + lane; q < q1; q += 64) {
+            const int pr = ip[ci[q]];
+            if (pr >= w0 && pr < w1) cb[pr - w0] = kv[q];
         }
-        // rows [lo, hi) of the permuted column c, assembled in LDS windows of `cap` rows
-        int lo, hi;
+        wave_lds_sync();
         if (c < ni) {
-            lo = (c / TILE) * TILE;
-            hi = min(ni, lo + (d.band_ii + 1) * TILE);
+            double *dst = d.Kii + (size_t)c * ni;
+            for (int r = w0 + lane; r < w1; r += 64) dst[r] = cb[r - w0];
         } else {
-            lo = 0;
-            hi = n;  // [K_ib(:, j) ; S(:, j)] as one permuted column
-        }
-        const int l = pm[c];
-        for (int w0 = lo; w0 < hi; w0 += cap) {
-            const int w1 = min(hi, w0 + cap);
-            for (int r = w0 + threadIdx.x; r < w1; r += 256) colbuf[r - w0] = 0.0;
-            __syncthreads();
-            for (int q = rp[l] + threadIdx.x; q < rp[l + 1]; q += 256) {
-                const int pr = ip[ci[q]];
-                if (pr >= w0 && pr < w1) colbuf[pr - w0] = kv[q];
+            const int j = c - ni;
+            const int s0 = ni + (j / TILE) * TILE;  // first S row kept, in the permuted column
+            for (int r = w0 + lane; r < w1; r += 64) {
+                if (r < ni) d.Kib[(size_t)j * ni + r] = cb[r - w0];
+                else if (r >= s0) d.S[(size_t)j * m + (r - ni)] = cb[r - w0];
             }
-            __syncthreads();
-            if (c < ni) {
-                double *dst = d.Kii + (size_t)c * ni;
-                for (int r = w0 + threadIdx.x; r < w1; r += 256) dst[r] = colbuf[r - w0];
-            } else {
-                const int j = c - ni;
-                const int s0 = ni + (j / TILE) * TILE;  // first S row kept, in the permuted column
-                for (int r = w0 + threadIdx.x; r < w1; r += 256) {
-                    if (r < ni) d.Kib[(size_t)j * ni + r] = colbuf[r - w0];
-                    else if (r >= s0) d.S[(size_t)j * m + (r - ni)] = colbuf[r - w0];
-                }
-            }
-            __syncthreads();
         }
+        wave_lds_sync();  // the window is read before the next one clears it
     }
 }
 
@@ -877,7 +897,7 @@ void launch_syrk(const DenseDesc *desc, int nb, int which, int max_ntm, hipStrea
 void launch_kernel_reg(const DenseDesc *desc, int nb, const double *q, const int64_t *q_off, const int *kdim,
                        double *rho, int max_m, hipStream_t st) {
     hipLaunchKernelGGL(k_rho, dim3(nb), dim3(256), 0, st, desc, rho);
-    unsigned g = std::min<unsigned>(cdiv((int64_t)max_m * max_m, 256), 4096);
+    const unsigned g = (unsigned)std::max(1, std::min(max_m, 4096));
     hipLaunchKernelGGL(k_kernel_reg, dim3(g, nb), dim3(256), 0, st, desc, q, q_off, kdim, rho);
 }
 
@@ -925,10 +945,8 @@ void launch_unpack(const void *pool, int prec, int64_t off, int m, float scale, 
 void launch_materialize(const DenseDesc *desc, int nb, const int *rowptr, const int64_t *rp_off, const int *colidx,
                         const double *vals, const int64_t *nz_off, const int *iperm, const int *perm,
                         const int64_t *perm_off, const double *f, const int *n, int max_n, hipStream_t st) {
-    const int cpw = 8;
-    const int cap = std::min(std::max(max_n, 1), 8192);  // LDS window (64 KB)
-    hipLaunchKernelGGL(k_materialize, dim3(cdiv(max_n + 1, cpw), nb), dim3(256), (size_t)cap * 8, st, desc, rowptr,
-                       rp_off, colidx, vals, nz_off, iperm, perm, perm_off, f, n, cpw, cap);
+    hipLaunchKernelGGL(k_materialize, dim3(cdiv(max_n + 1, 4), nb), dim3(256), 0, st, desc, rowptr, rp_off, colidx,
+                       vals, nz_off, iperm, perm, perm_off, f, n);
 }
 
 
