@@ -877,10 +877,18 @@ void launch_fill(double *p, int64_t n, double v, hipStream_t st) {
     hipLaunchKernelGGL(k_fill, dim3(g), dim3(256), 0, st, p, n, v);
 }
 
-void launch_potrf_step(const DenseDesc *desc, int nb, int which, int step, int max_band, hipStream_t st) {
+void launch_potrf_diag(const DenseDesc *desc, int nb, int which, int step, hipStream_t st) {
     hipLaunchKernelGGL(k_potrf_diag, dim3(nb), dim3(256), 0, st, desc, which, step);
+}
+
+void launch_potrf_panel(const DenseDesc *desc, int nb, int which, int step, int max_band, hipStream_t st) {
     if (max_band <= 0) return;
     hipLaunchKernelGGL(k_potrf_panel, dim3(max_band, nb), dim3(256), 0, st, desc, which, step);
+}
+
+void launch_potrf_step(const DenseDesc *desc, int nb, int which, int step, int max_band, hipStream_t st) {
+    launch_potrf_diag(desc, nb, which, step, st);
+    launch_potrf_panel(desc, nb, which, step, max_band, st);
 }
 
 void launch_trsm_step(const DenseDesc *desc, int nb, int which, int step, int max_band, int max_ntc,

@@ -167,6 +167,9 @@ struct espreso_feti_s {
             if (l.s) (void)hipStreamDestroy(l.s);
         }
         if (ev_main) (void)hipEventDestroy(ev_main);
+        for (hipEvent_t e : asm_ev)
+            if (e) (void)hipEventDestroy(e);
+        if (asm_st) (void)hipStreamDestroy(asm_st);
         for (void *p : allocs) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -248,6 +251,11 @@ struct espreso_feti_s {
         ClusterDev C;
     } lanes[NLANE];
     hipEvent_t ev_main = nullptr;
+    // assembly: forward substitution on a second stream (default priority), one
+    // event per Cholesky step plus the join
+    hipStream_t asm_st = nullptr;
+    std::vector<hipEvent_t> asm_ev;
+    void factor_solve(const DenseDesc *desc, int nb, int which, int nt, int band, int ntc);
     bool lanes_ready = false;
     void lanes_setup();
     void fork_lanes();
@@ -853,8 +861,7 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
 
         launch_materialize(d_desc, nbt, P.rowptr, d_rp_off, P.colidx, P.vals, d_nz_off, P.iperm, P.perm, d_n_off, d_f,
                            d_nloc, max_n, st);
-        for (int k = 0; k < max_ntii; k++) launch_potrf_step(d_desc, nbt, 0, k, max_band, st);
-        for (int k = 0; k < max_ntii; k++) launch_trsm_step(d_desc, nbt, 0, k, max_band, max_ntc, st);
+        factor_solve(d_desc, nbt, 0, max_ntii, max_band, max_ntc);
         launch_syrk(d_desc, nbt, 0, max_ntm, st);
         launch_hf(d_desc, nbt, d_fb, d_hf, d_b_off, max_m, st);
         launch_band_extract(d_desc, nbt, P.band, d_band_off, d_bw, P.yf, d_yf_off, max_ni, st);
@@ -862,8 +869,7 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         if (D.sd_prec == 1) launch_pack(d_desc, nbt, 1, D.SDf, d_p_off[1], nullptr, max_m, st);
         if (D.sd_prec == 2) launch_pack(d_desc, nbt, 2, D.SDh, d_p_off[2], D.sdscale + s0, max_m, st);
         launch_kernel_reg(d_desc, nbt, d_Q, d_q_off, d_kd, d_rho, max_m, st);
-        for (int k = 0; k < max_ntm; k++) launch_potrf_step(d_desc, nbt, 1, k, max_ntm, st);
-        for (int k = 0; k < max_ntm; k++) launch_trsm_step(d_desc, nbt, 1, k, max_ntm, max_ntm, st);
+        factor_solve(d_desc, nbt, 1, max_ntm, max_ntm, max_ntm);
         launch_syrk(d_desc, nbt, 1, max_ntm, st);
         if (hybrid) {
             std::vector<int> nq(nq_h.begin() + s0, nq_h.begin() + s1);
@@ -941,6 +947,33 @@ void espreso_feti_s::coarse() {
     release(Dinv);
     release(info);
     release(dd);
+}
+
+// Left-looking block Cholesky of A (which 0: K_ii, 1: S) on the main stream and
+// the forward substitution X = L^-1 B on asm_st.  Block row k of the substitution
+// needs the diagonal inverse of step k and the factor's rows k of earlier steps,
+// all done once the diagonal kernel of step k is; it writes only B, which the
+// Cholesky steps do not touch.  So it starts behind an event after that kernel
+// and fills the GPU while the later, latency-bound diagonal steps run.
+void espreso_feti_s::factor_solve(const DenseDesc *desc, int nb, int which, int nt, int band, int ntc) {
+    if (!asm_st) ESP_HIP(hipStreamCreate(&asm_st));
+    while ((int)asm_ev.size() < nt + 1) {
+        hipEvent_t e = nullptr;
+        ESP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        asm_ev.push_back(e);
+    }
+    // the second stream starts behind everything the main stream has queued
+    ESP_HIP(hipEventRecord(asm_ev[nt], st));
+    ESP_HIP(hipStreamWaitEvent(asm_st, asm_ev[nt], 0));
+    for (int k = 0; k < nt; k++) {
+        launch_potrf_diag(desc, nb, which, k, st);
+        ESP_HIP(hipEventRecord(asm_ev[k], st));
+        launch_potrf_panel(desc, nb, which, k, band, st);
+        ESP_HIP(hipStreamWaitEvent(asm_st, asm_ev[k], 0));
+        launch_trsm_step(desc, nb, which, k, band, ntc, asm_st);
+    }
+    ESP_HIP(hipEventRecord(asm_ev[nt], asm_st));
+    ESP_HIP(hipStreamWaitEvent(st, asm_ev[nt], 0));
 }
 
 // Flops of the assembly tiles of one subdomain, following the kernels' own tile

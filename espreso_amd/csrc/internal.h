@@ -76,6 +76,9 @@ struct DenseDesc {
 // which = 0: operate on K_ii / [K_ib|f];  which = 1: on S / W.
 void launch_fill(double *p, int64_t n, double v, hipStream_t st);
 void launch_potrf_step(const DenseDesc *desc, int nb, int which, int step, int max_band, hipStream_t st);
+// the two halves of a step: factor and invert the diagonal tile, then the panel below it
+void launch_potrf_diag(const DenseDesc *desc, int nb, int which, int step, hipStream_t st);
+void launch_potrf_panel(const DenseDesc *desc, int nb, int which, int step, int max_band, hipStream_t st);
 void launch_trsm_step(const DenseDesc *desc, int nb, int which, int step, int max_band, int max_ntc,
                       hipStream_t st);
 void launch_syrk(const DenseDesc *desc, int nb, int which, int max_ntm, hipStream_t st);
