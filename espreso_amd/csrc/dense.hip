@@ -263,6 +263,124 @@ __device__ __forceinline__ void strip_finish(d4 (&acc)[4], double *lds, double *
     }
 }
 
+// ---- two column tiles per workgroup (forward substitution) ----
+// The tiles (i, J) and (i, J+1) of one block row share the A operand (rows i of
+// the factor) and the diagonal inverse: each wave owns 16 columns of both
+// tiles.  A tile's B rows before its own window start (kfirst[s]) are read as
+// zero: the other tile may start its window earlier.
+constexpr int FUSED2_LDS = 2 * 16 * (TILE + 1) + 2 * 16 * (2 * TILE + 1);  // doubles
+
+__device__ __forceinline__ void strip2_gemm(d4 (&acc)[2][4], double *lds, const double *A, int lda, const double *B,
+                                            int ldb, int M, int N, int K, const int (&kfirst)[2]) {
+    double(*As)[16][TILE + 1] = reinterpret_cast<double(*)[16][TILE + 1]>(lds);
+    double(*Bs)[16][2 * TILE + 1] = reinterpret_cast<double(*)[16][2 * TILE + 1]>(lds + 2 * 16 * (TILE + 1));
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double ra[4], rb[8];
+    int fk = 0;
+    auto bok = [&](int j, int k) { return j < N && k < K && k >= kfirst[j >> 6]; };
+    auto fetch = [&](int k0) {
+        fk = k0;
+#pragma unroll
+        for (int it = 0; it < 4; it++) {
+            const int q = tid + 256 * it, i = q & 63, kk = q >> 6;
+            const bool va = i < M && k0 + kk < K;
+            ra[it] = gload(A + (va ? i + (size_t)(k0 + kk) * lda : 0));
+        }
+#pragma unroll
+        for (int it = 0; it < 8; it++) {
+            const int q = tid + 256 * it, kk = q & 15, j = q >> 4;
+            rb[it] = gload(B + (bok(j, k0 + kk) ? (k0 + kk) + (size_t)j * ldb : 0));
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int it = 0; it < 4; it++) {
+            const int q = tid + 256 * it, i = q & 63, kk = q >> 6;
+            As[buf][kk][i] = (i < M && fk + kk < K) ? ra[it] : 0.0;
+        }
+#pragma unroll
+        for (int it = 0; it < 8; it++) {
+            const int q = tid + 256 * it, kk = q & 15, j = q >> 4;
+            Bs[buf][kk][j] = bok(j, fk + kk) ? rb[it] : 0.0;
+        }
+    };
+    fetch(0);
+    store(0);
+    __syncthreads();
+    if (16 < K) fetch(16);
+    for (int k0 = 0, cur = 0; k0 < K; k0 += 16, cur ^= 1) {
+#pragma unroll
+        for (int ks = 0; ks < 16; ks += 4) {
+            const int kr = ks + (lane >> 4);
+            double a[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) a[r] = As[cur][kr][16 * r + (lane & 15)];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const double b = Bs[cur][kr][TILE * t + 16 * wave + (lane & 15)];
+#pragma unroll
+                for (int r = 0; r < 4; r++) acc[t][r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b, acc[t][r], 0, 0, 0);
+            }
+        }
+        if (k0 + 16 < K) store(cur ^ 1);
+        __syncthreads();  // after the last chunk: the operand buffers are free
+        if (k0 + 32 < K) fetch(k0 + 32);
+    }
+}
+
+// R_t = beta C0 - acc_t on rows < M, columns < N of tile t (columns 64 t..64 t+63
+// of C, element (r, c) at C[r + c ldc]), then C = Dinv(:, :M) R_t for the valid tiles
+__device__ __forceinline__ void strip2_finish(d4 (&acc)[2][4], double *lds, double *C, int ldc, int M, int N,
+                                              const double *Dinv, double beta, const bool (&valid)[2]) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        double dv[8];
+#pragma unroll
+        for (int it = 0; it < 8; it++) dv[it] = gload(Dinv + tid + 256 * (8 * h + it));
+#pragma unroll
+        for (int it = 0; it < 8; it++) {
+            const int e = tid + 256 * (8 * h + it), i = e & 63, k = e >> 6;
+            lds[k * (TILE + 1) + i] = k < M ? dv[it] : 0.0;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const int c = TILE * t + 16 * wave + (lane & 15);
+#pragma unroll
+        for (int rbk = 0; rbk < 4; rbk++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = 16 * rbk + (lane >> 4) + 4 * v;
+                const bool ok = valid[t] && r < M && c < N;
+                const double c0 = (beta != 0.0) ? gload(C + (ok ? (size_t)r + (size_t)c * ldc : 0)) : 0.0;
+                acc[t][rbk][v] = ok ? beta * c0 - acc[t][rbk][v] : 0.0;
+            }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++) {
+        d4 out[2] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+        for (int rbk = 0; rbk < 4; rbk++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const double a = lds[(16 * rbk + 4 * v + (lane >> 4)) * (TILE + 1) + 16 * ob + (lane & 15)];
+#pragma unroll
+                for (int t = 0; t < 2; t++) out[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, acc[t][rbk][v], out[t], 0, 0, 0);
+            }
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int c = TILE * t + 16 * wave + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = 16 * ob + (lane >> 4) + 4 * v;
+                if (valid[t] && r < M && c < N) C[(size_t)r + (size_t)c * ldc] = out[t][v];
+            }
+        }
+    }
+}
+
 // Workgroups are dealt to the 8 XCDs (each with its own L2) round-robin by
 // linear id, x fastest.  The tiles of one subdomain share their operand panels,
 // so the linear id is remapped to keep runs of consecutive logical tiles (x
@@ -475,36 +593,53 @@ __global__ void __launch_bounds__(256) k_potrf_panel(const DenseDesc *D, int whi
 // ---- left-looking forward substitution X = L^-1 B, block row i ----
 // X_iJ = inv(L_ii) (B_iJ - L_iw X_wJ), window w = [max(first nonzero tile of
 // column tile J, i - band), i)
-__global__ void __launch_bounds__(256, 4) k_trsm_row(const DenseDesc *D, int which, int step) {
+__global__ void __launch_bounds__(256, 3) k_trsm_row(const DenseDesc *D, int which, int step) {
     int bx, by;
     xcd_tile(bx, by);
     const DenseDesc &d = D[by];
     double *A, *B;
     int n, band, nt, ncols, ntc;
     pick(d, which, A, n, band, nt, B, ncols, ntc);
-    const int J = bx, i = step;
-    if (J >= ntc || i >= nt) return;
-    const int cs = which == 0 ? d.colstart[J] : J;
-    if (cs > i) return;
-    const int i0 = i * TILE, c0 = J * TILE;
-    const int mi = min(TILE, n - i0), nj = min(TILE, ncols - c0);
-    const int j0 = max(cs, i - band) * TILE;
-    double *C = B + i0 + (size_t)c0 * n;
-    if (which == 1 && J == i) {
-        // W = L^-1 I: the right-hand side is the identity, so the diagonal tile is inv(L_ii)
-        const double *Di = d.Dinv + (size_t)i * TILE * TILE;
-        for (int q = threadIdx.x; q < TILE * TILE; q += 256) {
-            const int r = q & 63, c = q >> 6;
-            if (r < mi && c < nj) C[r + (size_t)c * n] = Di[q];
+    const int J0 = 2 * bx, i = step;
+    if (J0 >= ntc || i >= nt) return;
+    const int i0 = i * TILE, c0 = J0 * TILE;
+    const int mi = min(TILE, n - i0), N = min(2 * TILE, ncols - c0);
+    const double *Di = d.Dinv + (size_t)i * TILE * TILE;
+    bool valid[2];
+    int jst[2], jmin = n;
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const int J = J0 + t;
+        valid[t] = J < ntc;
+        const int cs = which == 0 ? (valid[t] ? d.colstart[J] : i + 1) : J;
+        if (cs > i) valid[t] = false;
+        if (valid[t] && which == 1 && J == i) {
+            // W = L^-1 I: the right-hand side is the identity, so the diagonal tile is inv(L_ii)
+            const int nj = min(TILE, ncols - J * TILE);
+            double *Cd = B + i0 + (size_t)J * TILE * n;
+            for (int q = threadIdx.x; q < TILE * TILE; q += 256) {
+                const int r = q & 63, c = q >> 6;
+                if (r < mi && c < nj) Cd[r + (size_t)c * n] = Di[q];
+            }
+            valid[t] = false;
         }
-        return;
+        jst[t] = max(cs, i - band) * TILE;
+        if (valid[t]) jmin = min(jmin, jst[t]);
     }
-    __shared__ double lds[FUSED_LDS];
-    d4 acc[4] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
-    if (i0 > j0)
-        strip_gemm<false, false>(acc, lds, A + i0 + (size_t)j0 * n, n, B + j0 + (size_t)c0 * n, n, mi, nj, i0 - j0);
+    if (!valid[0] && !valid[1]) return;  // uniform over the workgroup
+    __shared__ double lds[FUSED2_LDS];
+    d4 acc[2][4];
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[t][r] = d4{0.0, 0.0, 0.0, 0.0};
+    double *C = B + i0 + (size_t)c0 * n;
+    if (i0 > jmin) {
+        const int kfirst[2] = {valid[0] ? jst[0] - jmin : i0 - jmin, valid[1] ? jst[1] - jmin : i0 - jmin};
+        strip2_gemm(acc, lds, A + i0 + (size_t)jmin * n, n, B + jmin + (size_t)c0 * n, n, mi, N, i0 - jmin, kfirst);
+    }
     // which 1: the identity has no entries off the diagonal tile (beta 0)
-    strip_finish<false>(acc, lds, C, n, mi, nj, d.Dinv + (size_t)i * TILE * TILE, which == 1 ? 0.0 : 1.0);
+    strip2_finish(acc, lds, C, n, mi, N, Di, which == 1 ? 0.0 : 1.0, valid);
 }
 
 // lower tiles of  S = S - Y^T Y  (which 0)  or  S = W^T W  (which 1)
@@ -894,7 +1029,7 @@ void launch_potrf_step(const DenseDesc *desc, int nb, int which, int step, int m
 void launch_trsm_step(const DenseDesc *desc, int nb, int which, int step, int max_band, int max_ntc,
                       hipStream_t st) {
     (void)max_band;
-    hipLaunchKernelGGL(k_trsm_row, dim3(max_ntc, nb), dim3(256), 0, st, desc, which, step);
+    hipLaunchKernelGGL(k_trsm_row, dim3((max_ntc + 1) / 2, nb), dim3(256), 0, st, desc, which, step);
 }
 
 void launch_syrk(const DenseDesc *desc, int nb, int which, int max_ntm, hipStream_t st) {
