@@ -642,6 +642,39 @@ __global__ void __launch_bounds__(256, 3) k_trsm_row(const DenseDesc *D, int whi
     strip2_finish(acc, lds, C, n, mi, N, Di, which == 1 ? 0.0 : 1.0, valid);
 }
 
+// one column tile per workgroup (few subdomains: twice the workgroups of the pairs)
+__global__ void __launch_bounds__(256, 4) k_trsm_row1(const DenseDesc *D, int which, int step) {
+    int bx, by;
+    xcd_tile(bx, by);
+    const DenseDesc &d = D[by];
+    double *A, *B;
+    int n, band, nt, ncols, ntc;
+    pick(d, which, A, n, band, nt, B, ncols, ntc);
+    const int J = bx, i = step;
+    if (J >= ntc || i >= nt) return;
+    const int cs = which == 0 ? d.colstart[J] : J;
+    if (cs > i) return;
+    const int i0 = i * TILE, c0 = J * TILE;
+    const int mi = min(TILE, n - i0), nj = min(TILE, ncols - c0);
+    const int j0 = max(cs, i - band) * TILE;
+    double *C = B + i0 + (size_t)c0 * n;
+    if (which == 1 && J == i) {
+        // W = L^-1 I: the right-hand side is the identity, so the diagonal tile is inv(L_ii)
+        const double *Di = d.Dinv + (size_t)i * TILE * TILE;
+        for (int q = threadIdx.x; q < TILE * TILE; q += 256) {
+            const int r = q & 63, c = q >> 6;
+            if (r < mi && c < nj) C[r + (size_t)c * n] = Di[q];
+        }
+        return;
+    }
+    __shared__ double lds[FUSED_LDS];
+    d4 acc[4] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
+    if (i0 > j0)
+        strip_gemm<false, false>(acc, lds, A + i0 + (size_t)j0 * n, n, B + j0 + (size_t)c0 * n, n, mi, nj, i0 - j0);
+    // which 1: the identity has no entries off the diagonal tile (beta 0)
+    strip_finish<false>(acc, lds, C, n, mi, nj, d.Dinv + (size_t)i * TILE * TILE, which == 1 ? 0.0 : 1.0);
+}
+
 // lower tiles of  S = S - Y^T Y  (which 0)  or  S = W^T W  (which 1)
 __global__ void __launch_bounds__(256) k_syrk(const DenseDesc *D, int which) {
     int bx, by;
@@ -1029,7 +1062,11 @@ void launch_potrf_step(const DenseDesc *desc, int nb, int which, int step, int m
 void launch_trsm_step(const DenseDesc *desc, int nb, int which, int step, int max_band, int max_ntc,
                       hipStream_t st) {
     (void)max_band;
-    hipLaunchKernelGGL(k_trsm_row, dim3((max_ntc + 1) / 2, nb), dim3(256), 0, st, desc, which, step);
+    // pairs of column tiles when they still fill the GPU (256 CUs, 3 workgroups each)
+    if ((int64_t)nb * ((max_ntc + 1) / 2) >= 768)
+        hipLaunchKernelGGL(k_trsm_row, dim3((max_ntc + 1) / 2, nb), dim3(256), 0, st, desc, which, step);
+    else
+        hipLaunchKernelGGL(k_trsm_row1, dim3(max_ntc, nb), dim3(256), 0, st, desc, which, step);
 }
 
 void launch_syrk(const DenseDesc *desc, int nb, int which, int max_ntm, hipStream_t st) {

@@ -217,3 +217,23 @@ def test_low_precision_dirichlet(sc):
         assert np.abs(g.dense(s, 1) - Sd).max() <= 2 * tol * np.abs(Sd).max()
     o.close()
     g.close()
+
+
+@pytest.mark.parametrize("method", ["total", "hybrid"])
+def test_many_subdomains_paired_substitution(method):
+    """512 subdomains of 4^3 elements: enough column tiles that the assembly's
+    forward substitution takes two tiles per workgroup (the smaller problems
+    above take one).  Checked through size-independent properties: the dual
+    residual reaches 1e-10 and the primal solution's residual in the oracle's
+    FP64 global assembly is below 1e-7."""
+    spec = PB.scaled(PB.HTFETI_512x10, elements=(4, 4, 4))
+    say("case: 512 subdomains of 4^3,", method)
+    g = core.GridFETI(spec, preconditioner="dirichlet", precision=1e-10, method=method, max_iterations=2000)
+    g.setup()
+    st = g.solve()
+    u = g.nodal_solution()
+    res = O.global_residual(O.undecomposed(spec), u)
+    say("paired substitution", method, "iterations", st["iterations"], "residual", res)
+    assert st["final_norm"] < st["tol"]
+    assert res < 1e-7
+    g.close()
