@@ -104,6 +104,7 @@ struct ClusterDev {
     float *FB0f = nullptr;     // FB0 in FP32 (FP32/BF16 storage of F_i only)
     double *Fcc = nullptr;     // nq x nq per subdomain: sign_q1 sign_q2 F_i[j_q1, j_q2]
     double *lambda0 = nullptr, *alphaK = nullptr;
+    double *rho_s = nullptr;   // per subdomain: max diag S_i (the scale of the kernel regularisation)
 };
 
 struct PrimalDev {

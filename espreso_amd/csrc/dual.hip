@@ -587,6 +587,11 @@ __global__ void __launch_bounds__(256) k_cluster_sa(const ClusterDev C, double *
     __syncthreads();
     if (threadIdx.x == 0) {
         double r = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        if (!(r > 0.0)) {
+            // no corner rows (G0 empty): the scale of the cluster's own S_i, so
+            // that the kernel term R Sa^-1 R^T stays within the size of K^+
+            for (int q = 0; q < C.scount[c]; q++) r = fmax(r, C.rho_s[C.sfirst[c] + q]);
+        }
         rho = r > 0.0 ? r : 1.0;
     }
     __syncthreads();

@@ -881,6 +881,8 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         if (D.sd_prec == 1) launch_pack(d_desc, nbt, 1, D.SDf, d_p_off[1], nullptr, max_m, st);
         if (D.sd_prec == 2) launch_pack(d_desc, nbt, 2, D.SDh, d_p_off[2], D.sdscale + s0, max_m, st);
         launch_kernel_reg(d_desc, nbt, d_Q, d_q_off, d_kd, d_rho, max_m, st);
+        if (hybrid)
+            ESP_HIP(hipMemcpyAsync(CL.rho_s + s0, d_rho, nbt * sizeof(double), hipMemcpyDeviceToDevice, st));
         factor_solve(d_desc, nbt, 1, max_ntm, max_ntm, max_ntm);
         launch_syrk(d_desc, nbt, 1, max_ntm, st);
         if (hybrid) {
@@ -1913,6 +1915,7 @@ void espreso_feti_s::load_clusters() {
     n_lambda0 = std::max<int64_t>(1, rowoff[ncl]);
     CL.lambda0 = alloc<double>(n_lambda0);
     CL.alphaK = alloc<double>(asubv[nsub]);
+    CL.rho_s = alloc<double>(nsub);
     d_qjint = upload(qj_h);
     d_qsign = upload(qsign_h);
     d_Fcc = CL.Fcc;

@@ -237,3 +237,19 @@ def test_many_subdomains_paired_substitution(method):
     assert st["final_norm"] < st["tol"]
     assert res < 1e-7
     g.close()
+
+
+@pytest.mark.parametrize("method", ["total", "hybrid"])
+def test_single_subdomain(method):
+    """One subdomain: the multipliers are only the Dirichlet rows of B1 and,
+    with HYBRID_FETI, its cluster has no corner rows (n0 = 0)."""
+    spec = PB.scaled(PB.GRAVITY, clusters=(1, 1, 1), domains=(1, 1, 1), elements=(4, 4, 4), traction_seed=5,
+                     traction_scale=1e5)
+    say("case: single subdomain,", method)
+    g = core.GridFETI(spec, preconditioner="dirichlet", precision=1e-10, method=method)
+    g.setup()
+    st = g.solve()
+    err = rel(g.nodal_solution(), O.global_solve(spec))
+    say("single subdomain", method, "iterations", st["iterations"], "err", err)
+    assert err < 1e-8
+    g.close()
