@@ -239,17 +239,28 @@ def test_many_subdomains_paired_substitution(method):
     g.close()
 
 
+SINGLE = {
+    "elasticity": PB.scaled(PB.GRAVITY, clusters=(1, 1, 1), domains=(1, 1, 1), elements=(4, 4, 4), traction_seed=5,
+                            traction_scale=1e5),
+    "heat": PB.scaled(PB.HEAT_SIMPLE, clusters=(1, 1, 1), domains=(1, 1, 1), elements=(5, 4, 3)),
+    # eight clusters of one subdomain each: no corner rows anywhere, clusters glued by B1 only
+    "clusters_of_one": PB.scaled(PB.GRAVITY, clusters=(2, 2, 2), domains=(1, 1, 1), elements=(3, 3, 3),
+                                 traction_seed=6, traction_scale=1e5),
+}
+
+
 @pytest.mark.parametrize("method", ["total", "hybrid"])
-def test_single_subdomain(method):
-    """One subdomain: the multipliers are only the Dirichlet rows of B1 and,
-    with HYBRID_FETI, its cluster has no corner rows (n0 = 0)."""
-    spec = PB.scaled(PB.GRAVITY, clusters=(1, 1, 1), domains=(1, 1, 1), elements=(4, 4, 4), traction_seed=5,
-                     traction_scale=1e5)
-    say("case: single subdomain,", method)
+@pytest.mark.parametrize("name", list(SINGLE))
+def test_single_subdomain(method, name):
+    """One subdomain per cluster: the multipliers of a lone subdomain are only
+    the Dirichlet rows of B1 and, with HYBRID_FETI, a cluster of one subdomain
+    has no corner rows (n0 = 0)."""
+    spec = SINGLE[name]
+    say("case: single subdomain,", name, method)
     g = core.GridFETI(spec, preconditioner="dirichlet", precision=1e-10, method=method)
     g.setup()
     st = g.solve()
     err = rel(g.nodal_solution(), O.global_solve(spec))
-    say("single subdomain", method, "iterations", st["iterations"], "err", err)
+    say("single subdomain", name, method, "iterations", st["iterations"], "err", err)
     assert err < 1e-8
     g.close()
