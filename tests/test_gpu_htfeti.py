@@ -125,3 +125,37 @@ def test_hybrid_operator_symmetric(sc):
     say("hybrid", sc, "asymmetry", asym)
     assert asym < (1e-12 if sc == "fp64" else 1e-5)
     g.close()
+
+
+@pytest.mark.parametrize("prec", ["none", "lumped", "weight", "dirichlet"])
+def test_clusters_of_eight_preconditioners(prec):
+    """HTFETI with every preconditioner reaches the direct solve."""
+    spec = PB.scaled(PB.HTFETI_512x10, clusters=(2, 2, 2), elements=(4, 4, 4))
+    say("case: hybrid 8 clusters, preconditioner", prec)
+    g = core.GridFETI(spec, method="hybrid", preconditioner=prec, precision=1e-10, max_iterations=3000)
+    g.setup()
+    st = g.solve()
+    err = rel(g.nodal_solution(), O.global_solve(spec))
+    say("hybrid preconditioner", prec, "iterations", st["iterations"], "err", err)
+    assert err < 1e-8
+    g.close()
+
+
+def test_iteration_limit():
+    """An unreachable tolerance stops at max_iterations and reports it (the
+    stop test runs one iteration ahead; the last iterate is the limit's)."""
+    spec = PB.scaled(PB.HTFETI_512x10, clusters=(2, 2, 2), elements=(4, 4, 4))
+    say("case: iteration limit")
+    g = core.GridFETI(spec, method="hybrid", preconditioner="dirichlet", precision=1e-30, max_iterations=7)
+    g.setup()
+    st = g.solve()
+    say("iteration limit", st["iterations"], st["final_norm"], st["tol"])
+    assert st["iterations"] == 7
+    assert st["final_norm"] > st["tol"]
+    # the same 7 iterations without the limit being the cause: a looser run that needs more
+    g2 = core.GridFETI(spec, method="hybrid", preconditioner="dirichlet", precision=1e-10, max_iterations=3000)
+    g2.setup()
+    st2 = g2.solve()
+    assert st2["iterations"] > 7
+    g.close()
+    g2.close()
