@@ -111,3 +111,20 @@ def test_block_low_precision_storage():
         err = rel(u32[q], O.global_solve_load(spec, loads[q]))
         say("load case", q, "iterations", int(it32[q]), "err", err)
         assert err < 1e-8
+
+
+def test_block_clusters_without_corners():
+    """Block re-solve on clusters of one subdomain (no corner rows: the
+    multi-case cluster kernels see n0 = 0) against the direct solve."""
+    spec = PB.scaled(PB.GRAVITY, clusters=(2, 2, 1), domains=(1, 1, 1), elements=(3, 3, 3), method="hybrid")
+    say("case: block, clusters of one subdomain")
+    g = core.GridFETI(spec, preconditioner="dirichlet", precision=1e-11, max_iterations=3000)
+    g.setup()
+    g.solve()
+    loads = PB.nodal_load_cases(spec, 4, seed=33)
+    ub, itb, nrm = g.resolve_block_nodal(loads)
+    for q in range(4):
+        err = rel(ub[q], O.global_solve_load(spec, loads[q]))
+        say("corner-free block case", q, "iterations", int(itb[q]), "err", err)
+        assert err < 1e-8
+    g.close()

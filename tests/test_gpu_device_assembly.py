@@ -64,3 +64,21 @@ def test_device_assembly(name, prec):
         f"vs direct {rel(ud, ug):.2e}")
     assert rel(ud, uh) < 1e-9
     assert rel(ud, ug) < 1e-8
+
+
+def test_device_assembly_hybrid_mixed_storage():
+    """K_i assembled on the device, HYBRID_FETI, F_i in FP32 and S_D in BF16."""
+    spec = dict(PB.scaled(PB.HTFETI_512x10, clusters=(2, 2, 1), elements=(4, 4, 4)), device_assembly=True)
+    print("case: device assembly, hybrid, fp32/bf16", flush=True)
+    g = core.GridFETI(spec, method="hybrid", preconditioner="dirichlet", precision=1e-10, sc_precision="fp32",
+                      sd_precision="bf16")
+    g.setup()
+    g.solve()
+    ref = dict(spec)
+    ref.pop("device_assembly")
+    u = g.nodal_solution()
+    ug = O.global_solve(ref)
+    err = np.abs(u - ug).max() / np.abs(ug).max()
+    print("device assembly mixed err", err, flush=True)
+    assert err < 1e-8
+    g.close()
