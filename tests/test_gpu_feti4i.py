@@ -75,3 +75,20 @@ def test_feti4i_hybrid_heat():
     assert abs(u.min() - 100) < 1e-7 and abs(u.max() - 300) < 1e-7 and abs(u.mean() - 200) < 1e-7
     assert f4.info()["n_coarse"] == 1  # one corner fixes the constant mode of a pair
     f4.close()
+
+
+@pytest.mark.parametrize("method", ["total", "hybrid"])
+def test_feti4i_one_subdomain(method):
+    """FETI4I with a single subdomain (HYBRID_FETI: one cluster without corner
+    rows) against the direct solve."""
+    spec = PB.scaled(PB.GRAVITY, clusters=(1, 1, 1), domains=(3, 2, 2), elements=(1, 1, 1), traction_seed=4,
+                     traction_scale=1e5)
+    print("case: FETI4I one subdomain", method, flush=True)
+    elements, size, rhs, dirichlet, o = element_problem(spec)
+    f4 = core.FETI4I(elements, size, rhs, dirichlet, subdomains=1, preconditioner=3, precision=1e-10, method=method)
+    u = f4.solve()
+    ug = O.global_solve(spec).ravel()
+    err = np.abs(u - ug).max() / np.abs(ug).max()
+    print("FETI4I one subdomain", method, "err", err, flush=True)
+    assert err < 1e-8
+    f4.close()
