@@ -243,6 +243,9 @@ SINGLE = {
     "elasticity": PB.scaled(PB.GRAVITY, clusters=(1, 1, 1), domains=(1, 1, 1), elements=(4, 4, 4), traction_seed=5,
                             traction_scale=1e5),
     "heat": PB.scaled(PB.HEAT_SIMPLE, clusters=(1, 1, 1), domains=(1, 1, 1), elements=(5, 4, 3)),
+    # subdomains of one element: every DOF on the interface, no interior DOFs
+    "one_element_subdomains": PB.scaled(PB.GRAVITY, clusters=(1, 1, 1), domains=(3, 2, 2), elements=(1, 1, 1),
+                                        traction_seed=7, traction_scale=1e5),
     # eight clusters of one subdomain each: no corner rows anywhere, clusters glued by B1 only
     "clusters_of_one": PB.scaled(PB.GRAVITY, clusters=(2, 2, 2), domains=(1, 1, 1), elements=(3, 3, 3),
                                  traction_seed=6, traction_scale=1e5),
@@ -252,9 +255,9 @@ SINGLE = {
 @pytest.mark.parametrize("method", ["total", "hybrid"])
 @pytest.mark.parametrize("name", list(SINGLE))
 def test_single_subdomain(method, name):
-    """One subdomain per cluster: the multipliers of a lone subdomain are only
-    the Dirichlet rows of B1 and, with HYBRID_FETI, a cluster of one subdomain
-    has no corner rows (n0 = 0)."""
+    """Extreme decompositions: the multipliers of a lone subdomain are only the
+    Dirichlet rows of B1; with HYBRID_FETI a cluster of one subdomain has no
+    corner rows (n0 = 0); a one-element subdomain has no interior DOFs."""
     spec = SINGLE[name]
     say("case: single subdomain,", name, method)
     g = core.GridFETI(spec, preconditioner="dirichlet", precision=1e-10, method=method)
