@@ -267,3 +267,21 @@ def test_single_subdomain(method, name):
     say("single subdomain", name, method, "iterations", st["iterations"], "err", err)
     assert err < 1e-8
     g.close()
+
+
+@pytest.mark.parametrize("sc", ["fp64", "fp32"])
+def test_large_subdomains(sc):
+    """Subdomains of 16^3 elements (about 4600 interface DOFs each): the SYMV's
+    LDS window (2 m + R values) exceeds 64 KB.  Checked through the dual
+    residual and the primal residual in the oracle's FP64 global assembly."""
+    spec = PB.scaled(PB.GRAVITY, clusters=(1, 1, 1), domains=(2, 2, 2), elements=(16, 16, 16), traction_seed=8,
+                     traction_scale=1e5)
+    say("case: 8 subdomains of 16^3,", sc)
+    g = core.GridFETI(spec, preconditioner="dirichlet", precision=1e-10, sc_precision=sc, max_iterations=2000)
+    g.setup()
+    st = g.solve()
+    res = O.global_residual(O.undecomposed(spec), g.nodal_solution())
+    say("large subdomains", sc, "iterations", st["iterations"], "residual", res)
+    assert st["final_norm"] < st["tol"]
+    assert res < 1e-7
+    g.close()
