@@ -150,10 +150,6 @@ struct espreso_feti_s {
     double *d_x = nullptr, *d_r = nullptr, *d_w = nullptr, *d_y = nullptr, *d_z = nullptr, *d_p = nullptr,
            *d_Ap = nullptr, *d_b = nullptr, *d_tmp = nullptr, *d_r2 = nullptr, *d_partial = nullptr,
            *d_scal = nullptr;
-    // PCG stop test overlapped with the next iteration: x and r before it, the
-    // pinned host copies of ||w||^2 and their events
-    double *d_xs = nullptr, *d_rs = nullptr, *h_ww = nullptr;
-    hipEvent_t ev_ww[2] = {nullptr, nullptr};
     double *d_u = nullptr;    // per-subdomain primal (local order), sum n
     double *d_f = nullptr, *d_Qm = nullptr, *d_GGt = nullptr, *d_c = nullptr;  // resident inputs
     std::vector<int64_t> h_poff[3], h_boff;
@@ -173,9 +169,6 @@ struct espreso_feti_s {
         if (ev_main) (void)hipEventDestroy(ev_main);
         for (hipEvent_t e : asm_ev)
             if (e) (void)hipEventDestroy(e);
-        for (hipEvent_t e : ev_ww)
-            if (e) (void)hipEventDestroy(e);
-        if (h_ww) (void)hipHostFree(h_ww);
         if (asm_st) (void)hipStreamDestroy(asm_st);
         for (void *p : allocs) (void)hipFree(p);
         if (st) (void)hipStreamDestroy(st);
@@ -662,11 +655,6 @@ void espreso_feti_s::load() {
     d_z = alloc<double>(nlam); d_p = alloc<double>(nlam); d_Ap = alloc<double>(nlam); d_b = alloc<double>(nlam);
     d_tmp = alloc<double>(nlam);
     d_r2 = alloc<double>(nlam);
-    d_xs = alloc<double>(nlam);
-    d_rs = alloc<double>(nlam);
-    if (!h_ww) ESP_HIP(hipHostMalloc((void **)&h_ww, 2 * sizeof(double), hipHostMallocDefault));
-    for (hipEvent_t &e : ev_ww)
-        if (!e) ESP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     d_partial = alloc<double>(1024);
     d_scal = alloc<double>(16);
     d_c = upload(c);
@@ -1234,38 +1222,16 @@ void espreso_feti_s::pcg(const double *b, double *x, double *r, int prec, int &i
     last.norm_b = std::sqrt(hv[1]);
     iters = 0;
     norm = 0;
-    // The stop test of step `it` is read while step it+1 already runs: x and r
-    // are copied before step it+1, and restored if step `it` had converged, so
-    // iterations, iterate and residual are those of a synchronous test.
-    const size_t vb = nlam * sizeof(double);
-    auto step = [&](int it) {
+    for (int it = 0; it < opt.max_iterations; it++) {
         if (it == 0 || !use_graphs) pcg_step(x, r, prec, it == 0);
         else pcg_step_graph(x, r, prec);
-        ESP_HIP(hipMemcpyAsync(h_ww + (it & 1), s_ww, sizeof(double), hipMemcpyDeviceToHost, st));
-        ESP_HIP(hipEventRecord(ev_ww[it & 1], st));
-    };
-    if (opt.max_iterations > 0) step(0);
-    for (int it = 0; it < opt.max_iterations; it++) {
-        const bool ahead = it + 1 < opt.max_iterations;
-        if (ahead) {
-            ESP_HIP(hipMemcpyAsync(d_xs, x, vb, hipMemcpyDeviceToDevice, st));
-            ESP_HIP(hipMemcpyAsync(d_rs, r, vb, hipMemcpyDeviceToDevice, st));
-            step(it + 1);
-        }
-        ESP_HIP(hipEventSynchronize(ev_ww[it & 1]));
-        const double ww = h_ww[it & 1];
+        double ww = 0;
+        ESP_HIP(hipMemcpyAsync(&ww, s_ww, sizeof(double), hipMemcpyDeviceToHost, st));
+        sync();
+        ESP_CHECK(std::isfinite(ww), ESPRESO_ERR_NUMERIC, "PCG produced a non-finite residual");
         norm = std::sqrt(ww);
         iters = it + 1;
-        const bool done = !std::isfinite(ww) || norm < tol;
-        if (done || !ahead) {
-            sync();
-            if (ahead) {
-                ESP_HIP(hipMemcpyAsync(x, d_xs, vb, hipMemcpyDeviceToDevice, st));
-                ESP_HIP(hipMemcpyAsync(r, d_rs, vb, hipMemcpyDeviceToDevice, st));
-            }
-            ESP_CHECK(std::isfinite(ww), ESPRESO_ERR_NUMERIC, "PCG produced a non-finite residual");
-            break;
-        }
+        if (norm < tol) break;
     }
 }
 
