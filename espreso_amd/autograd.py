@@ -49,6 +49,17 @@ class FETICompliance(torch.autograd.Function):
 
 def compliance(young, spec, **options):
     """c = f^T u for per-element Young's moduli `young` (tensor, one per element,
-    index ex + NEx*(ey + NEy*ez)); differentiable with respect to `young`."""
+    index ex + NEx*(ey + NEy*ez)); differentiable with respect to `young`.
+
+    The adjoint K w = f reuses the forward problem's constraints, which is the
+    adjoint of the compliance only when the prescribed values are zero; specs
+    with non-zero Dirichlet values are refused."""
+    fixed = spec.get("fixed", (0,) * 6)
+    values = spec.get("fixed_value", (0.0,) * 6)
+    if any(f and v != 0.0 for f, v in zip(fixed, values)):
+        raise ValueError("compliance gradient: non-zero prescribed values (fixed_value) are not supported; "
+                         "the adjoint solve assumes homogeneous Dirichlet conditions")
+    if spec.get("physics", 0) != 0:
+        raise ValueError("compliance gradient: elasticity only (per-element Young's modulus)")
     return FETICompliance.apply(young, spec, options)
 

@@ -11,13 +11,15 @@ No CPU fallback exists: a missing library raises ``LibraryNotBuilt``.
 """
 import ctypes as C
 import os
+import warnings
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libespreso_amd.so")
 
-PREC = {"none": 0, "lumped": 1, "weight": 2, "dirichlet": 3}
+PREC = {"none": 0, "lumped": 1, "weight": 2, "dirichlet": 3, "super_dirichlet": 4}
+NOT_CONVERGED = 1  # ESPRESO_NOT_CONVERGED: solved, stop test not met (stats filled)
 SC = {"fp64": 0, "fp32": 1, "bf16": 2}
 
 
@@ -63,7 +65,7 @@ class Options(C.Structure):
         ("precision", C.c_double),
         ("memory_budget_gb", C.c_double),
         ("sd_precision", C.c_int32),
-        ("pad_", C.c_int32),
+        ("truncate_output", C.c_int32),
     ]
 
 
@@ -89,10 +91,15 @@ class Stats(C.Structure):
         ("time_load", C.c_double),
         ("symv_bytes", C.c_double),
         ("symv_bytes_sd", C.c_double),
+        ("converged", C.c_int32),
+        ("pad2", C.c_int32),
+        ("kkt_primal", C.c_double),
+        ("kkt_dual", C.c_double),
+        ("time_build", C.c_double),
     ]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pad", "pad2")}
 
 
 # every entry point of include/espreso_amd.h with its ctypes signature
@@ -112,6 +119,8 @@ SIGNATURES = {
     "espreso_grid_unit_element": (C.c_int, [C.POINTER(GridProblem), _D]),
     "espreso_grid_set_nodal_rhs": (C.c_int, [_H, _D]),
     "espreso_grid_element_sensitivity": (C.c_int, [_H, _D, _D, _D]),
+    "espreso_grid_update_material": (C.c_int, [_H, _D]),
+    "espreso_feti_update_values": (C.c_int, [_H, C.c_int32, _D]),
     "espreso_feti_set_rhs": (C.c_int, [_H, C.c_int32, _D]),
     "espreso_feti_load_work": (C.c_int, [_H, _D]),
     "espreso_grid_nodal_load": (C.c_int, [_H, _D]),
@@ -207,7 +216,14 @@ def lib():
     return _lib
 
 
-def _check(rc):
+def _check(rc, solve=False):
+    """Raise on a failed call.  For solve-like calls ESPRESO_NOT_CONVERGED
+    returns normally (the stats say converged = 0) with a warning, as the
+    reference's solver reports an unconverged PCG and still returns its
+    iterate."""
+    if rc == NOT_CONVERGED and solve:
+        warnings.warn("FETI: stop test not met within the iteration limit", RuntimeWarning, stacklevel=3)
+        return
     if rc != 0:
         raise RuntimeError(f"espreso_amd error {rc}: {lib().espreso_last_error().decode()}")
 
@@ -292,7 +308,7 @@ METHOD = {"total": 0, "TOTAL_FETI": 0, "hybrid": 1, "HYBRID_FETI": 1}
 
 
 def make_options(preconditioner="dirichlet", precision=1e-8, max_iterations=500, scaling=1, sc_precision="fp64",
-                 device=0, memory_budget_gb=0.0, method="total", sd_precision="fp64"):
+                 device=0, memory_budget_gb=0.0, method="total", sd_precision="fp64", truncate_output=False):
     o = Options()
     o.method = METHOD[method] if isinstance(method, str) else int(method)
     o.preconditioner = PREC[preconditioner] if isinstance(preconditioner, str) else int(preconditioner)
@@ -303,6 +319,7 @@ def make_options(preconditioner="dirichlet", precision=1e-8, max_iterations=500,
     o.precision = precision
     o.memory_budget_gb = memory_budget_gb
     o.sd_precision = SC[sd_precision] if isinstance(sd_precision, str) else int(sd_precision)
+    o.truncate_output = int(bool(truncate_output))
     return o
 
 
@@ -403,9 +420,15 @@ class FETISolver:
     # -- LinearSolver::solve --
     def solve(self):
         st = Stats()
-        _check(lib().espreso_feti_solve(self.h, C.byref(st)))
+        _check(lib().espreso_feti_solve(self.h, C.byref(st)), solve=True)
         self.stats = st.as_dict()
         return self.stats
+
+    def update_values(self, i, values):
+        """New K_i values on the structure given at set_subdomain; the next
+        setup() re-assembles F_i and S_D (FETISolver::update with K)."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        _check(lib().espreso_feti_update_values(self.h, i, _dp(v)))
 
     def set_rhs(self, i, f):
         f = np.ascontiguousarray(f, dtype=np.float64)
@@ -420,7 +443,7 @@ class FETISolver:
     def resolve(self):
         """Solve with the loads given by set_rhs, reusing the assembled operators."""
         st = Stats()
-        _check(lib().espreso_feti_resolve(self.h, C.byref(st)))
+        _check(lib().espreso_feti_resolve(self.h, C.byref(st)), solve=True)
         self.stats = st.as_dict()
         return self.stats
 
@@ -434,7 +457,8 @@ class FETISolver:
         it = np.zeros(nrhs, np.int32)
         nr = np.zeros(nrhs)
         st = Stats()
-        _check(lib().espreso_feti_resolve_block(self.h, nrhs, _dp(f), _dp(u), _ip(it), _dp(nr), C.byref(st)))
+        _check(lib().espreso_feti_resolve_block(self.h, nrhs, _dp(f), _dp(u), _ip(it), _dp(nr), C.byref(st)),
+               solve=True)
         self.block_stats = st.as_dict()
         return u, it, nr
 
@@ -535,6 +559,14 @@ class GridFETI(FETISolver):
         assert g.size == self.n_nodes * self.ndof
         _check(lib().espreso_grid_set_nodal_rhs(self.h, _dp(g)))
 
+    def update_material(self, material_elem):
+        """New per-element Young's modulus / conductivity (element index
+        ex + NEx*(ey + NEy*ez)); K_i is recomputed on the GPU and the next
+        setup() re-assembles the dual operators."""
+        m = np.ascontiguousarray(material_elem, dtype=np.float64).reshape(-1)
+        assert m.size == n_elements(self.spec)
+        _check(lib().espreso_grid_update_material(self.h, _dp(m)))
+
     def element_sensitivity(self, u, w=None):
         """-w_e^T K_e0 u_e for every element (GPU); w defaults to u."""
         u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
@@ -561,7 +593,8 @@ class GridFETI(FETISolver):
         it = np.zeros(nrhs, np.int32)
         nr = np.zeros(nrhs)
         st = Stats()
-        _check(lib().espreso_grid_resolve_block(self.h, nrhs, _dp(loads), _dp(un), _ip(it), _dp(nr), C.byref(st)))
+        _check(lib().espreso_grid_resolve_block(self.h, nrhs, _dp(loads), _dp(un), _ip(it), _dp(nr), C.byref(st)),
+               solve=True)
         self.block_stats = st.as_dict()
         return un, it, nr
 
@@ -601,8 +634,11 @@ def f4lib():
     return L
 
 
-def _f4check():
+def _f4check(solve=False):
     st = f4lib().FETI4IGetLastStatus()
+    if st == NOT_CONVERGED and solve:
+        warnings.warn("FETI4I: stop test not met within FETI4I_MAX_ITERATIONS", RuntimeWarning, stacklevel=3)
+        return
     if st != 0:
         raise RuntimeError(f"FETI4I error {st}: {f4lib().FETI4IGetLastError().decode()}")
 
@@ -622,26 +658,41 @@ class FETI4I:
         io[2] = METHOD[method] if isinstance(method, str) else method
         io[3] = preconditioner
         ro[0] = precision
-        m = C.c_void_p()
-        L.FETI4ICreateStiffnessMatrix(C.byref(m), 0, index_base)
+        self.index_base = index_base
+        self.m = self._matrix(elements)
+        rhs = np.ascontiguousarray(rhs, np.float64)
+        l2g = np.arange(size, dtype=np.int32) + index_base
+        di = np.ascontiguousarray([d for d, _ in dirichlet], np.int32)
+        dv = np.ascontiguousarray([v for _, v in dirichlet], np.float64)
+        inst = C.c_void_p()
+        L.FETI4ICreateInstance(C.byref(inst), self.m, size, _dp(rhs), _ip(l2g), 0, None, len(di), _ip(di), _dp(dv),
+                               _ip(io), _dp(ro))
         _f4check()
-        self.m = m
+        self.inst = inst
+        self.size = size
+
+    def _matrix(self, elements):
+        L = f4lib()
+        m = C.c_void_p()
+        L.FETI4ICreateStiffnessMatrix(C.byref(m), 0, self.index_base)
+        _f4check()
         for nodes, dofs, K in elements:
             nodes = np.ascontiguousarray(nodes, np.int32)
             dofs = np.ascontiguousarray(dofs, np.int32)
             K = np.ascontiguousarray(K, np.float64)
             L.FETI4IAddElement(m, 3, len(nodes), _ip(nodes), len(dofs), _ip(dofs), _dp(K))
         _f4check()
-        rhs = np.ascontiguousarray(rhs, np.float64)
-        l2g = np.arange(size, dtype=np.int32) + index_base
-        di = np.ascontiguousarray([d for d, _ in dirichlet], np.int32)
-        dv = np.ascontiguousarray([v for _, v in dirichlet], np.float64)
-        inst = C.c_void_p()
-        L.FETI4ICreateInstance(C.byref(inst), m, size, _dp(rhs), _ip(l2g), 0, None, len(di), _ip(di), _dp(dv),
-                               _ip(io), _dp(ro))
-        _f4check()
-        self.inst = inst
-        self.size = size
+        return m
+
+    def update_stiffness(self, elements):
+        """FETI4IUpdateStiffnessMatrix with a new element list (same structure:
+        values only are re-assembled and the decomposition is kept)."""
+        m = self._matrix(elements)
+        try:
+            f4lib().FETI4IUpdateStiffnessMatrix(self.inst, m)
+            _f4check()
+        finally:
+            f4lib().FETI4IDestroy(m)
 
     def info(self):
         v = np.zeros(8, np.int32)
@@ -689,7 +740,7 @@ class FETI4I:
     def solve(self):
         u = np.zeros(self.size)
         f4lib().FETI4ISolve(self.inst, self.size, _dp(u))
-        _f4check()
+        _f4check(solve=True)
         return u
 
     def stats(self):

@@ -190,4 +190,18 @@ void launch_copy(const double *src, double *dst, int64_t n, hipStream_t st);
 // read-only stream over n doubles (n even) with `wgs` workgroups (out: wgs doubles)
 void launch_stream_read(const double *src, int64_t n, int wgs, double *out, hipStream_t st);
 
+// SUPER_DIRICHLET: yb = K_bb xb - K_bn diag(K_nn)^-1 K_nb xb (n: interior DOFs
+// next to the boundary; t holds n_near values of scratch)
+void launch_superdir(const DualDev &D, int64_t n_near, const int64_t *sib_ptr, const int64_t *sib_col,
+                     const double *sib_val, const double *sdiag, const int64_t *sbi_ptr, const int64_t *sbi_col,
+                     const double *sbi_val, double *t, const double *xb, double *yb, hipStream_t st);
+// KKT sums per subdomain (out: 2 per subdomain) and the boundary values of u (ub)
+void launch_kkt_primal(const PrimalDev &P, int nsub, const double *u, const double *f, const double *bt1,
+                       const double *bt0, double *ub, double *out, hipStream_t st);
+// bt0 += B0^T lambda0 (C.lambda0) on the boundary (bt0 cleared by the caller)
+void launch_b0t(const DualDev &D, const ClusterDev &C, const int *qj, const double *qsign, double *bt0,
+                hipStream_t st);
+// u <- trunc(u D) / D (FETISolver::solve output truncation); partial: 256 doubles
+void launch_truncate(double *u, int64_t n, double precision, double *partial, hipStream_t st);
+
 }  // namespace esp

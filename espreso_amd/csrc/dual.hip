@@ -1370,4 +1370,128 @@ void launch_primal_multi(const PrimalDev &P, int nsub, int nr, const double *ub,
     }
 }
 
+// ---- SUPER_DIRICHLET preconditioner (clustercpu.cpp:243-314) ----
+// S = K_bb - K_bi diag(K_ii)^-1 K_ib applied from the sparse blocks: first
+// t = diag(K_nn)^-1 K_nb x_b for the interior DOFs next to the boundary,
+// then y_b = K_bb x_b - K_bn t.
+__global__ void k_superdir_near(int64_t n_near, const int64_t *ptr, const int64_t *col, const double *val,
+                                const double *diag, const double *xb, double *t) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n_near; g += (int64_t)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int64_t q = ptr[g]; q < ptr[g + 1]; q++) v += val[q] * xb[col[q]];
+        t[g] = v / diag[g];
+    }
+}
+
+__global__ void k_superdir_bnd(int64_t nb, const int64_t *kptr, const int64_t *kcol, const double *kval,
+                               const int64_t *ptr, const int64_t *col, const double *val, const double *xb,
+                               const double *t, double *yb) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nb; j += (int64_t)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int64_t q = kptr[j]; q < kptr[j + 1]; q++) v += kval[q] * xb[kcol[q]];
+        double w = 0.0;
+        for (int64_t q = ptr[j]; q < ptr[j + 1]; q++) w += val[q] * t[col[q]];
+        yb[j] = v - w;
+    }
+}
+
+void launch_superdir(const DualDev &D, int64_t n_near, const int64_t *sib_ptr, const int64_t *sib_col,
+                     const double *sib_val, const double *sdiag, const int64_t *sbi_ptr, const int64_t *sbi_col,
+                     const double *sbi_val, double *t, const double *xb, double *yb, hipStream_t st) {
+    if (n_near > 0)
+        hipLaunchKernelGGL(k_superdir_near, dim3(vgrid(n_near)), dim3(256), 0, st, n_near, sib_ptr, sib_col, sib_val,
+                           sdiag, xb, t);
+    hipLaunchKernelGGL(k_superdir_bnd, dim3(vgrid(D.nbnd)), dim3(256), 0, st, D.nbnd, D.kbb_ptr, D.kbb_col, D.kbb_val,
+                       sbi_ptr, sbi_col, sbi_val, xb, t, yb);
+}
+
+// ---- KKT report (itersolver.cpp:199-251) ----
+// One workgroup per subdomain: r = K_i u_i - (f_i - B1_i^T lambda) + B0_i^T lambda0
+// over the local DOFs; out[2s] = sum r^2, out[2s+1] = sum (f_i - B1_i^T lambda)^2.
+// bt1 = B1^T lambda and bt0 = B0^T lambda0 (may be null) are boundary vectors;
+// the boundary values of u are copied to ub (for B1 u).
+__global__ void __launch_bounds__(256) k_kkt_primal(const PrimalDev P, const double *u, const double *f,
+                                                    const double *bt1, const double *bt0, double *ub, double *out) {
+    __shared__ double red[4];
+    const int s = blockIdx.x;
+    const int n = P.n[s], ni = P.ni[s];
+    const int64_t uo = P.perm_off[s], ro = P.rp_off[s], zo = P.nz_off[s], bo = P.boff[s];
+    double a1 = 0.0, a2 = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        double ku = 0.0;
+        for (int q = P.rowptr[ro + i]; q < P.rowptr[ro + i + 1]; q++) ku += P.vals[zo + q] * u[uo + P.colidx[zo + q]];
+        double t = f[uo + i], c0 = 0.0;
+        const int pi = P.iperm[uo + i];
+        if (pi >= ni) {
+            const int64_t j = bo + pi - ni;
+            t -= bt1[j];
+            if (bt0) c0 = bt0[j];
+            ub[j] = u[uo + i];
+        }
+        const double r = ku - t + c0;
+        a1 += r * r;
+        a2 += t * t;
+    }
+    a1 = block_sum(a1, red);
+    a2 = block_sum(a2, red);
+    if (threadIdx.x == 0) {
+        out[2 * s] = a1;
+        out[2 * s + 1] = a2;
+    }
+}
+
+// bt0 = B0^T lambda0 on the boundary: one thread per subdomain walks its B0
+// entries in order (a corner DOF may sit in several rows)
+__global__ void k_b0t(int nsub, const int64_t *boff, const int64_t *qoff, const int *qrow, const int *qj,
+                      const double *qsign, const double *lambda0, double *bt0) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsub) return;
+    for (int64_t q = qoff[s]; q < qoff[s + 1]; q++) bt0[boff[s] + qj[q]] += qsign[q] * lambda0[qrow[q]];
+}
+
+void launch_kkt_primal(const PrimalDev &P, int nsub, const double *u, const double *f, const double *bt1,
+                       const double *bt0, double *ub, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_kkt_primal, dim3(nsub), dim3(256), 0, st, P, u, f, bt1, bt0, ub, out);
+}
+
+void launch_b0t(const DualDev &D, const ClusterDev &C, const int *qj, const double *qsign, double *bt0,
+                hipStream_t st) {
+    hipLaunchKernelGGL(k_b0t, dim3((D.nsub + 255) / 256), dim3(256), 0, st, D.nsub, D.boff, C.qoff, C.qrow, qj, qsign,
+                       C.lambda0, bt0);
+}
+
+// ---- output truncation (FETISolver.cpp:131-148) ----
+__global__ void k_absmax_partial(const double *u, int64_t n, double *partial) {
+    __shared__ double red[4];
+    double m = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(u[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// D = 1 / precision / max(1, 10^ceil(log10(max|u|))); u <- trunc(D u) / D.
+// The maximum starts at the smallest normal double, as the reference's does.
+__global__ void k_truncate(double *u, int64_t n, const double *partial, int np, double precision) {
+    __shared__ double dpl;
+    if (threadIdx.x == 0) {
+        double m = 2.2250738585072014e-308;
+        for (int b = 0; b < np; b++) m = fmax(m, partial[b]);
+        dpl = 1.0 / precision / fmax(1.0, pow(10.0, ceil(log10(m))));
+    }
+    __syncthreads();
+    const double d = dpl;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        u[i] = trunc(d * u[i]) / d;
+}
+
+void launch_truncate(double *u, int64_t n, double precision, double *partial, hipStream_t st) {
+    const int np = 256;
+    hipLaunchKernelGGL(k_absmax_partial, dim3(np), dim3(256), 0, st, u, n, partial);
+    hipLaunchKernelGGL(k_truncate, dim3(vgrid(n)), dim3(256), 0, st, u, n, (const double *)partial, np, precision);
+}
+
 }  // namespace esp

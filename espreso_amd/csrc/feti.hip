@@ -159,6 +159,31 @@ struct espreso_feti_s {
     std::vector<double> u_h;
     espreso_feti_stats last{};
     double fsc_bytes = 0, sd_bytes = 0, asm_flops = 0;
+    // K values changed since the last set-up (espreso_feti_update_values): the
+    // preconditioner values gathered from them are refreshed at the next set-up
+    bool values_dirty = false;
+    int64_t *d_kbb_pos = nullptr;                   // positions of the K_bb values in P.vals
+    int64_t n_kbb = 0;
+    // SUPER_DIRICHLET: K_bb - K_bi diag(K_ii)^-1 K_ib applied from the sparse blocks
+    int64_t n_near = 0, n_sib = 0, n_sbi = 0;       // near-interior DOFs, K_ib and K_bi entries
+    int64_t *d_sib_ptr = nullptr, *d_sib_pos = nullptr, *d_sbi_ptr = nullptr, *d_sbi_pos = nullptr,
+            *d_sdiag_pos = nullptr, *d_sib_col = nullptr, *d_sbi_col = nullptr;
+    double *d_sib_val = nullptr, *d_sbi_val = nullptr, *d_sdiag = nullptr, *d_snear = nullptr;
+    // KKT report scratch (boundary vectors, lambda vector, per-subdomain sums)
+    double *d_kkt = nullptr;
+    int measure = 0;                                // ESPRESO_MEASURE: phase times on stderr
+    void note(const char *what, double t0) const {
+        if (measure) std::fprintf(stderr, "espreso: %-28s %8.4f s\n", what, wall() - t0);
+    }
+    void gather_prec_values();
+    void grid_stiffness();
+    void kkt(const double *lambda, espreso_feti_stats &s);
+    void truncate_primal();
+    void check_launches(const char *where) {
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            throw Error(ESPRESO_ERR_HIP, std::string(where) + ": kernel launch failed: " + hipGetErrorString(e));
+    }
 
     ~espreso_feti_s() {
         drop_graphs();
@@ -272,7 +297,7 @@ struct espreso_feti_s {
     };
     std::vector<BlockGraph> block_graphs;
     void step_block_graph(int K);
-    void resolve_block(int nrhs, const double *f, double *u, int32_t *iters, double *norms,
+    bool resolve_block(int nrhs, const double *f, double *u, int32_t *iters, double *norms,
                        espreso_feti_stats *stats);
     void resolve_group(int K, const double *f, double *u, int32_t *iters, double *norms, espreso_feti_stats &tot);
     std::vector<double> Ns_h;                        // hybrid: N_s per subdomain (kdim x kN)
@@ -307,7 +332,7 @@ struct espreso_feti_s {
     void apply_prec(const double *x, double *y);
     void solve(espreso_feti_stats *st_out);
     void pcg(const double *b, double *x, double *r, int prec, int &iters, double &norm, double &tol, bool quiet_tol,
-             double tol_abs);
+             double tol_abs, double *norm_b = nullptr);
 };
 
 // ----------------------------------------------------------------- setup ----
@@ -315,8 +340,11 @@ void espreso_feti_s::load() {
     ESP_CHECK(nlam >= 0, ESPRESO_ERR_STATE, "espreso_feti_set_dual not called");
     for (int s = 0; s < nsub; s++) ESP_CHECK(sub[s].set, ESPRESO_ERR_STATE, "subdomain " + std::to_string(s) + " not set");
     double t0 = wall();
+    if (const char *e = std::getenv("ESPRESO_MEASURE")) measure = std::atoi(e);
     const int prec = opt.sc_precision;
     ESP_CHECK(prec >= 0 && prec <= 2, ESPRESO_ERR_ARG, "sc_precision must be 0, 1 or 2");
+    ESP_CHECK(opt.preconditioner >= 0 && opt.preconditioner <= 4, ESPRESO_ERR_ARG,
+              "preconditioner must be NONE, LUMPED, WEIGHT_FUNCTION, DIRICHLET or SUPER_DIRICHLET (0..4)");
 
     // ---- host preprocessing: boundary sets, permutations, bands ----
     std::vector<int64_t> boff(nsub + 1, 0), coffv(nsub + 1, 0);
@@ -362,11 +390,27 @@ void espreso_feti_s::load() {
             h.colstart[J] = mn / TILE;
         }
     }
-    int max_m = 0;
+    int max_m = 0, max_m_sub = 0;
     for (int s = 0; s < nsub; s++) {
         boff[s + 1] = boff[s] + sub[s].m;
         coffv[s + 1] = coffv[s] + sub[s].kdim;
-        max_m = std::max(max_m, sub[s].m);
+        if (sub[s].m > max_m) { max_m = sub[s].m; max_m_sub = s; }
+    }
+    note("load: boundary sets, bands", t0);
+    {
+        // the SYMV keeps x and the row/column sums of a work item's columns in
+        // LDS: (2 max_m + R) values of the accumulation type per workgroup
+        int maxsh = 0;
+        ESP_HIP(hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, opt.device));
+        for (int p = 0; p < 3; p++) {
+            if (!prec_used(p)) continue;
+            const int64_t need = (2 * (int64_t)max_m + strip_rows(p)) * (p ? 4 : 8);
+            ESP_CHECK(need <= maxsh, ESPRESO_ERR_ARG,
+                      "subdomain " + std::to_string(max_m_sub) + " has " + std::to_string(max_m) +
+                          " interface DOFs: its dual-operator workgroup would need " + std::to_string(need) +
+                          " bytes of LDS (device limit " + std::to_string(maxsh) +
+                          "); decompose into more subdomains");
+        }
     }
     asubv = coffv;
     const int64_t nb = boff[nsub];
@@ -401,6 +445,7 @@ void espreso_feti_s::load() {
         }
     }
     for (int l = 0; l < nlam; l++) ESP_CHECK(le_cnt[l] > 0, ESPRESO_ERR_ARG, "lambda row without entries");
+    note("load: lambda maps", t0);
 
     // R_b, Q = orth(R_b), fb, K_bb (boundary CSR), e = -R^T f, GG^T
     std::vector<int64_t> rboff(nsub + 1, 0);
@@ -431,31 +476,104 @@ void espreso_feti_s::load() {
     for (int s = 0; s < nsub; s++)
         ESP_CHECK(rank_ok[s], ESPRESO_ERR_NUMERIC,
                   "kernel restricted to the boundary of subdomain " + std::to_string(s) + " is rank deficient");
+    note("load: R_b, Q, e", t0);
     // K_bb in boundary indices: columns and positions of the values in the
-    // concatenated CSR value array (gathered on the device once K is there)
-    std::vector<int64_t> kbb_ptr(nb + 1, 0), kbb_col, kbb_pos;
+    // concatenated CSR value array (gathered on the device once K is there).
+    // SUPER_DIRICHLET also needs K_ib for the interior DOFs next to the
+    // boundary ("near" DOFs, global index over the subdomains), their
+    // diagonal, and K_bi by boundary row.
     h_nzoff.assign(nsub + 1, 0);
     for (int s = 0; s < nsub; s++) h_nzoff[s + 1] = h_nzoff[s] + sub[s].rowptr[sub[s].n];
-    {
-        int64_t tot = 0;
-        for (int s = 0; s < nsub; s++) tot += kbb_cnt[s];
-        kbb_col.reserve(tot);
-        kbb_pos.reserve(tot);
+    const bool superdir = opt.preconditioner == ESPRESO_PREC_SUPER_DIRICHLET;
+    std::vector<int64_t> kbb_off(nsub + 1, 0);
+    for (int s = 0; s < nsub; s++) kbb_off[s + 1] = kbb_off[s] + kbb_cnt[s];
+    std::vector<int64_t> kbb_ptr(nb + 1, 0), kbb_col(kbb_off[nsub]), kbb_pos(kbb_off[nsub]);
+    std::vector<std::vector<int>> near_of(superdir ? nsub : 0);  // per subdomain: near DOFs (internal index)
+    std::vector<int64_t> near_off(nsub + 1, 0), sib_off(nsub + 1, 0), sbi_off(nsub + 1, 0);
+    if (superdir) {
+#pragma omp parallel for schedule(dynamic)
         for (int s = 0; s < nsub; s++) {
             const SubHost &h = sub[s];
+            int64_t cib = 0, cbi = 0;
+            for (int r = 0; r < h.ni; r++) {
+                const int lr = h.perm[r];
+                int nbc = 0;
+                for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) nbc += h.iperm[h.colidx[k]] >= h.ni;
+                if (nbc) { near_of[s].push_back(r); cib += nbc; }
+            }
             for (int j = 0; j < h.m; j++) {
                 const int lr = h.bdofs[j];
-                for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) {
-                    const int pc = h.iperm[h.colidx[k]];
-                    if (pc >= h.ni) {
-                        kbb_col.push_back(boff[s] + pc - h.ni);
-                        kbb_pos.push_back(h_nzoff[s] + k);
-                    }
-                }
-                kbb_ptr[boff[s] + j + 1] = (int64_t)kbb_col.size();
+                for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) cbi += h.iperm[h.colidx[k]] < h.ni;
             }
+            sib_off[s + 1] = cib;
+            sbi_off[s + 1] = cbi;
+        }
+        for (int s = 0; s < nsub; s++) {
+            near_off[s + 1] = near_off[s] + (int64_t)near_of[s].size();
+            sib_off[s + 1] += sib_off[s];
+            sbi_off[s + 1] += sbi_off[s];
         }
     }
+    std::vector<int64_t> sib_ptr(superdir ? near_off[nsub] + 1 : 0, 0), sib_col(sib_off[nsub]), sib_pos(sib_off[nsub]),
+        sdiag_pos(near_off[nsub]), sbi_ptr(superdir ? nb + 1 : 0, 0), sbi_col(sbi_off[nsub]), sbi_pos(sbi_off[nsub]);
+#pragma omp parallel for schedule(dynamic)
+    for (int s = 0; s < nsub; s++) {
+        const SubHost &h = sub[s];
+        int64_t q = kbb_off[s];
+        for (int j = 0; j < h.m; j++) {
+            const int lr = h.bdofs[j];
+            for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) {
+                const int pc = h.iperm[h.colidx[k]];
+                if (pc >= h.ni) {
+                    kbb_col[q] = boff[s] + pc - h.ni;
+                    kbb_pos[q] = h_nzoff[s] + k;
+                    q++;
+                }
+            }
+            kbb_ptr[boff[s] + j + 1] = q;
+        }
+        if (!superdir) continue;
+        std::vector<int> near_idx(h.ni, -1);
+        for (size_t a = 0; a < near_of[s].size(); a++) near_idx[near_of[s][a]] = (int)a;
+        int64_t qi = sib_off[s];
+        for (size_t a = 0; a < near_of[s].size(); a++) {
+            const int r = near_of[s][a], lr = h.perm[r];
+            for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) {
+                const int pc = h.iperm[h.colidx[k]];
+                if (pc >= h.ni) {
+                    sib_col[qi] = boff[s] + pc - h.ni;
+                    sib_pos[qi] = h_nzoff[s] + k;
+                    qi++;
+                } else if (pc == r) {
+                    sdiag_pos[near_off[s] + a] = h_nzoff[s] + k;
+                }
+            }
+            sib_ptr[near_off[s] + a + 1] = qi;
+        }
+        int64_t qb = sbi_off[s];
+        for (int j = 0; j < h.m; j++) {
+            const int lr = h.bdofs[j];
+            for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) {
+                const int pc = h.iperm[h.colidx[k]];
+                if (pc < h.ni) {
+                    sbi_col[qb] = near_off[s] + near_idx[pc];
+                    sbi_pos[qb] = h_nzoff[s] + k;
+                    qb++;
+                }
+            }
+            sbi_ptr[boff[s] + j + 1] = qb;
+        }
+    }
+    if (superdir) {
+        for (int s = 0; s < nsub; s++)
+            for (size_t a = 0; a < near_of[s].size(); a++)
+                ESP_CHECK(sdiag_pos[near_off[s] + a] >= h_nzoff[s], ESPRESO_ERR_ARG,
+                          "SUPER_DIRICHLET: K_" + std::to_string(s) + " lacks a diagonal entry");
+        n_near = near_off[nsub];
+        n_sib = sib_off[nsub];
+        n_sbi = sbi_off[nsub];
+    }
+    note("load: K_bb (and K_ib) maps", t0);
     // keep what the coarse layout and the cluster setup need
     Rb_h = Rb;
     rboff_h = rboff;
@@ -502,7 +620,21 @@ void espreso_feti_s::load() {
     D.kbb_ptr = upload(kbb_ptr);
     D.kbb_col = upload(kbb_col);
     D.kbb_val = alloc<double>(kbb_pos.size());
-    int64_t *d_kbb_pos = upload(kbb_pos);
+    d_kbb_pos = upload(kbb_pos);
+    n_kbb = (int64_t)kbb_pos.size();
+    if (superdir) {
+        d_sib_ptr = upload(sib_ptr);
+        d_sib_col = upload(sib_col);
+        d_sib_pos = upload(sib_pos);
+        d_sdiag_pos = upload(sdiag_pos);
+        d_sbi_ptr = upload(sbi_ptr);
+        d_sbi_col = upload(sbi_col);
+        d_sbi_pos = upload(sbi_pos);
+        d_sib_val = alloc<double>(n_sib);
+        d_sbi_val = alloc<double>(n_sbi);
+        d_sdiag = alloc<double>(n_near);
+        d_snear = alloc<double>(n_near);
+    }
     D.Rb = upload(Rb);
     D.yb = alloc<double>(nb);
     D.xb = alloc<double>(nb);
@@ -579,24 +711,17 @@ void espreso_feti_s::load() {
         }
         P.rowptr = upload(rp);
         P.colidx = upload(ci);
-        int max_nnz = 0;
-        for (int s = 0; s < nsub; s++) max_nnz = std::max<int64_t>(max_nnz, nz_off[s + 1] - nz_off[s]);
+        P.nz_off = upload(nz_off);
+        P.rp_off = upload(rp_off);
         if (grid_dev) {
             // structured grid: K_i values from the unit element matrix and the element materials
             P.vals = alloc<double>(nz_off[nsub]);
-            int *d_box = upload(grid_box);
-            double *d_Ke = upload(gridKe0), *d_esc = upload(grid_escale);
-            int64_t *d_rpo = upload(rp_off), *d_nzo = upload(nz_off);
-            launch_grid_stiffness(nsub, ndof, d_box, gridNE, d_Ke, d_esc, P.rowptr, d_rpo, d_nzo, max_nnz, P.vals, st);
-            sync();
-            for (void *q : {(void *)d_box, (void *)d_Ke, (void *)d_esc, (void *)d_rpo, (void *)d_nzo})
-                release(q);
+            grid_stiffness();
         } else {
             P.vals = upload(kv);
         }
-        launch_gather_values((int64_t)kbb_col.size(), d_kbb_pos, P.vals, D.kbb_val, st);
+        gather_prec_values();
         sync();
-        release(d_kbb_pos);
         P.perm = upload(pm);
         P.iperm = upload(ip);
         P.R = upload(RR);
@@ -607,8 +732,6 @@ void espreso_feti_s::load() {
         P.kdim = D.kdim;
         P.asub = D.asub;
         P.boff = D.boff;
-        P.nz_off = upload(nz_off);
-        P.rp_off = upload(rp_off);
         P.perm_off = upload(n_off);
         P.band_off = upload(band_off);
         P.yf_off = upload(yf_off);
@@ -626,8 +749,10 @@ void espreso_feti_s::load() {
     d_Qm = d_Q;
     for (int p = 0; p < 3; p++) h_poff[p] = poff[p];
     h_boff = boff;
+    note("load: upload", t0);
     if (hybrid) load_clusters();
     else coarse_layout_subdomains();
+    note("load: coarse space, GG^T", t0);
 
     // ---- SYMV work lists (the item size is tuned at the first set-up) ----
     {
@@ -658,6 +783,7 @@ void espreso_feti_s::load() {
     d_partial = alloc<double>(1024);
     d_scal = alloc<double>(16);
     d_c = upload(c);
+    d_kkt = alloc<double>(2 * nb + nlam + 2 * (int64_t)nsub + 8);
     {
         // algorithmic bytes of one SYMV: the packed triangle in the storage
         // precision, 20 B per B1 entry (index, coefficient, lambda value), y_b
@@ -672,9 +798,36 @@ void espreso_feti_s::load() {
         last.symv_bytes = esz[prec] * tri + gat + yb;
         last.symv_bytes_sd = esz[D.sd_prec] * tri + gat + yb;
     }
+    check_launches("load");
     sync();
     loaded = true;
     last.time_load = wall() - t0;
+    note("load: total", t0);
+}
+
+// K values on the device for the structured grid: every CSR entry sums the
+// element contributions s_e * Ke0 (grid_escale, the unit element matrix)
+void espreso_feti_s::grid_stiffness() {
+    int64_t max_nnz = 0;
+    for (int s = 0; s < nsub; s++) max_nnz = std::max<int64_t>(max_nnz, h_nzoff[s + 1] - h_nzoff[s]);
+    int *d_box = upload(grid_box);
+    double *d_Ke = upload(gridKe0), *d_esc = upload(grid_escale);
+    launch_grid_stiffness(nsub, ndof, d_box, gridNE, d_Ke, d_esc, P.rowptr, P.rp_off, P.nz_off, (int)max_nnz, P.vals,
+                          st);
+    sync();
+    for (void *q : {(void *)d_box, (void *)d_Ke, (void *)d_esc}) release(q);
+}
+
+// preconditioner values taken from K: K_bb (lumped) and, for SUPER_DIRICHLET,
+// K_ib, K_bi and diag(K_ii) of the near-interior DOFs
+void espreso_feti_s::gather_prec_values() {
+    launch_gather_values(n_kbb, d_kbb_pos, P.vals, D.kbb_val, st);
+    if (opt.preconditioner == ESPRESO_PREC_SUPER_DIRICHLET) {
+        launch_gather_values(n_sib, d_sib_pos, P.vals, d_sib_val, st);
+        launch_gather_values(n_sbi, d_sbi_pos, P.vals, d_sbi_val, st);
+        launch_gather_values(n_near, d_sdiag_pos, P.vals, d_sdiag, st);
+    }
+    values_dirty = false;
 }
 
 // GPU part of the setup, repeatable on the resident inputs: local dual
@@ -683,16 +836,23 @@ void espreso_feti_s::gpu_setup() {
     ESP_CHECK(loaded, ESPRESO_ERR_STATE, "inputs not loaded");
     drop_graphs();
     if (const char *e = std::getenv("ESPRESO_HIP_GRAPHS")) use_graphs = std::atoi(e) != 0;
+    ready = false;
     double ta = wall();
+    if (values_dirty) gather_prec_values();
     assemble(d_f, d_Qm, h_poff, h_boff);
+    check_launches("assembly");
     last.time_assembly = wall() - ta;
+    note("setup: assembly", ta);
     double tc = wall();
     if (hybrid) cluster_setup();
     coarse();
     compute_b();
+    check_launches("coarse set-up");
     sync();
     last.time_coarse = wall() - tc;
+    note("setup: cluster, coarse, b", tc);
     tune_symv();  // first set-up only (outside the timed phases)
+    check_launches("autotuner");
     ready = true;
 }
 
@@ -1144,6 +1304,12 @@ void espreso_feti_s::apply_prec(const double *x, double *y) {
         launch_symv(D, true, 0, x, 1, true, st);
         launch_lambda_gather(D, D.yb, true, y, st);
         break;
+    case ESPRESO_PREC_SUPER_DIRICHLET:
+        launch_boundary_gather(D, x, true, D.xb, st);
+        launch_superdir(D, n_near, d_sib_ptr, d_sib_col, d_sib_val, d_sdiag, d_sbi_ptr, d_sbi_col, d_sbi_val, d_snear,
+                        D.xb, D.yb, st);
+        launch_lambda_gather(D, D.yb, true, y, st);
+        break;
     default:
         throw Error(ESPRESO_ERR_ARG, "unknown preconditioner");
     }
@@ -1209,7 +1375,7 @@ void espreso_feti_s::drop_graphs() {
 }
 
 void espreso_feti_s::pcg(const double *b, double *x, double *r, int prec, int &iters, double &norm, double &tol,
-                         bool use_abs, double tol_abs) {
+                         bool use_abs, double tol_abs, double *norm_b) {
     double *s_ww = d_scal + 3, *s_tmp = d_scal + 4;
     // tol = precision * min(||P r0||, ||b||)
     launch_project(D, r, d_w, 0, st);
@@ -1219,7 +1385,7 @@ void espreso_feti_s::pcg(const double *b, double *x, double *r, int prec, int &i
     ESP_HIP(hipMemcpyAsync(hv, s_tmp, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
     sync();
     tol = use_abs ? tol_abs : opt.precision * std::min(std::sqrt(hv[0]), std::sqrt(hv[1]));
-    last.norm_b = std::sqrt(hv[1]);
+    if (norm_b) *norm_b = std::sqrt(hv[1]);
     iters = 0;
     norm = 0;
     for (int it = 0; it < opt.max_iterations; it++) {
@@ -1246,8 +1412,10 @@ void espreso_feti_s::solve(espreso_feti_stats *out) {
     double norm = 0, tol = 0;
     const int prec = opt.sc_precision;
     int refine = 0;
+    bool converged = true;
     if (prec == 0) {
-        pcg(d_b, d_x, d_r, 0, iters, norm, tol, false, 0);
+        pcg(d_b, d_x, d_r, 0, iters, norm, tol, false, 0, &last.norm_b);
+        converged = norm < tol;
     } else {
         // mixed precision: inner PCG on the low-precision F_i, outer FP64
         // residual r = b - F lambda with the FP64 F_i (iterative refinement)
@@ -1290,7 +1458,9 @@ void espreso_feti_s::solve(espreso_feti_stats *out) {
         iters = total;
         norm = std::sqrt(fin);
         tol = tol64;
+        converged = norm < tol64;
     }
+    check_launches("PCG");
     double tp = wall();
     last.time_pcg = tp - t0;
     // amplitudes alpha = (GG^T)^-1 G r ; primal u = K~^+ (f - B^T lambda) + R alpha
@@ -1306,8 +1476,13 @@ void espreso_feti_s::solve(espreso_feti_stats *out) {
     }
     launch_alpha_combine(D, CL, hybrid, d_alpha_c, d_alpha_tot, st);
     launch_primal(P, nsub, D.xb, d_u, st);
+    // KKT report of the solution (GetSolution_Primal_singular_parallel, itersolver.cpp:171-370)
+    kkt(d_x, last);
+    if (opt.truncate_output) truncate_primal();
+    check_launches("primal recovery");
     sync();
     last.time_primal = wall() - tp;
+    last.converged = converged ? 1 : 0;
     last.iterations = iters;
     last.final_norm = norm;
     last.tol = tol;
@@ -1322,6 +1497,46 @@ void espreso_feti_s::solve(espreso_feti_stats *out) {
     solved = true;
     if (out) *out = last;
 }
+
+// KKT residuals of the last primal u (d_u) and dual lambda:
+//   kkt_primal = ||K u - f + B1^T lambda (+ B0^T lambda0)|| / ||f - B1^T lambda||
+//   kkt_dual   = ||B1 u - c|| / (||c|| + (||c|| < 1 ? 1 : 0))
+// (itersolver.cpp:199-251 and :286-353 with only equality constraints).  In
+// HYBRID_FETI the corner forces B0^T lambda0 of the cluster problems enter
+// the first residual, so that it measures K u = f - B^T lambda for the
+// constraints the solution satisfies.
+void espreso_feti_s::kkt(const double *lambda, espreso_feti_stats &s) {
+    const int64_t nb = D.nbnd;
+    double *bt1 = d_kkt, *bt0 = d_kkt + nb, *yl = d_kkt + 2 * nb, *sums = d_kkt + 2 * nb + nlam;
+    double *ub = D.yb;  // boundary values of u (scratch)
+    launch_boundary_gather(D, lambda, false, bt1, st);
+    if (hybrid) {
+        ESP_HIP(hipMemsetAsync(bt0, 0, nb * sizeof(double), st));
+        launch_b0t(D, CL, d_qjint, d_qsign, bt0, st);
+    }
+    launch_kkt_primal(P, nsub, d_u, d_f, bt1, hybrid ? bt0 : nullptr, ub, sums, st);
+    launch_lambda_gather(D, ub, false, yl, st);
+    launch_axpby(yl, yl, d_c, nlam, 1.0, -1.0, st);
+    launch_dot(yl, yl, nlam, d_partial, sums + 2 * nsub, st);
+    launch_dot(d_c, d_c, nlam, d_partial, sums + 2 * nsub + 1, st);
+    std::vector<double> h(2 * (size_t)nsub + 2);
+    ESP_HIP(hipMemcpyAsync(h.data(), sums, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    sync();
+    double num = 0, den = 0;
+    for (int q = 0; q < nsub; q++) {
+        num += h[2 * q];
+        den += h[2 * q + 1];
+    }
+    num = std::sqrt(num);
+    den = std::sqrt(den);
+    s.kkt_primal = den > 0 ? num / den : num;
+    double nc = std::sqrt(h[2 * nsub + 1]);
+    if (std::fabs(nc) < 1) nc += 1.0;
+    s.kkt_dual = std::sqrt(h[2 * nsub]) / nc;
+}
+
+// output truncation of FETISolver::solve (FETISolver.cpp:131-148), opt-in
+void espreso_feti_s::truncate_primal() { launch_truncate(d_u, uoff[nsub], opt.precision, d_partial, st); }
 
 // ------------------------------------------------------ block re-solve ----
 // Several load cases on the assembled operators, solved together: one
@@ -1613,13 +1828,15 @@ void espreso_feti_s::resolve_group(int K, const double *f, double *u, int32_t *i
         if (norms) norms[v] = nrm[v];
         tot.iterations = std::max(tot.iterations, it_v[v]);
         tot.final_norm = std::max(tot.final_norm, nrm[v]);
+        if (!(nrm[v] < tol[v])) tot.converged = 0;
     }
+    check_launches("block re-solve");
     tot.time_coarse += t1 - t0;
     tot.time_pcg += t2 - t1;
     tot.time_primal += wall() - t2;
 }
 
-void espreso_feti_s::resolve_block(int nrhs, const double *f, double *u, int32_t *iters, double *norms,
+bool espreso_feti_s::resolve_block(int nrhs, const double *f, double *u, int32_t *iters, double *norms,
                                    espreso_feti_stats *stats) {
     ESP_CHECK(ready, ESPRESO_ERR_STATE, "espreso_feti_setup not called");
     const int64_t nt = uoff[nsub];
@@ -1627,10 +1844,18 @@ void espreso_feti_s::resolve_block(int nrhs, const double *f, double *u, int32_t
     tot.n_lambda = nlam;
     tot.n_coarse = ncoarse;
     tot.n_subdomains = nsub;
+    tot.kkt_primal = tot.kkt_dual = -1.0;  // not computed for load cases
+    tot.converged = 1;
     if (opt.sc_precision != 0 || D.sd_prec != 0) {
-        // low-precision storage: the cases one after another (refinement per case)
+        // low-precision storage: the cases one after another (refinement per
+        // case); the instance's own solution (lambda, u, stats) is restored after
         std::vector<std::vector<double>> keep(nsub);
         for (int s = 0; s < nsub; s++) keep[s] = sub[s].f;
+        double *save_x = alloc<double>(nlam), *save_u = alloc<double>(nt);
+        ESP_HIP(hipMemcpyAsync(save_x, d_x, nlam * sizeof(double), hipMemcpyDeviceToDevice, st));
+        ESP_HIP(hipMemcpyAsync(save_u, d_u, nt * sizeof(double), hipMemcpyDeviceToDevice, st));
+        const espreso_feti_stats save_last = last;
+        const bool save_solved = solved;
         for (int v = 0; v < nrhs; v++) {
             for (int s = 0; s < nsub; s++)
                 std::copy(f + v * nt + uoff[s], f + v * nt + uoff[s] + sub[s].n, sub[s].f.begin());
@@ -1644,12 +1869,20 @@ void espreso_feti_s::resolve_block(int nrhs, const double *f, double *u, int32_t
             tot.final_norm = std::max(tot.final_norm, sv.final_norm);
             tot.time_pcg += sv.time_pcg;
             tot.time_primal += sv.time_primal;
+            tot.converged &= sv.converged;
         }
         for (int s = 0; s < nsub; s++) sub[s].f = keep[s];
         resolve_rhs();
+        ESP_HIP(hipMemcpyAsync(d_x, save_x, nlam * sizeof(double), hipMemcpyDeviceToDevice, st));
+        ESP_HIP(hipMemcpyAsync(d_u, save_u, nt * sizeof(double), hipMemcpyDeviceToDevice, st));
+        sync();
+        release(save_x);
+        release(save_u);
+        last = save_last;
+        solved = save_solved;
         tot.time_total = tot.time_pcg + tot.time_primal;
         if (stats) *stats = tot;
-        return;
+        return tot.converged != 0;
     }
     for (int done = 0; done < nrhs;) {
         const int rem = nrhs - done;
@@ -1660,6 +1893,7 @@ void espreso_feti_s::resolve_block(int nrhs, const double *f, double *u, int32_t
     }
     tot.time_total = tot.time_coarse + tot.time_pcg + tot.time_primal;
     if (stats) *stats = tot;
+    return tot.converged != 0;
 }
 
 // --------------------------------------------------- coarse space layout ----
@@ -2078,7 +2312,8 @@ int espreso_feti_set_subdomain(espreso_feti *h, int32_t i, int32_t n, const int3
     ESP_CHECK(h && i >= 0 && i < h->nsub && n > 0, ESPRESO_ERR_ARG, "invalid subdomain");
     ESP_CHECK(kdim >= 1 && kdim <= 6, ESPRESO_ERR_ARG, "kernel dimension must be 1..6");
     ESP_CHECK(nlam > 0, ESPRESO_ERR_ARG, "subdomain without multipliers (not floating-glued)");
-    ESP_CHECK(!h->ready, ESPRESO_ERR_STATE, "setup already done");
+    ESP_CHECK(!h->loaded, ESPRESO_ERR_STATE,
+              "setup already done (new K values: espreso_feti_update_values; new loads: espreso_feti_set_rhs)");
     SubHost &s = h->sub[i];
     s.n = n;
     s.rowptr.assign(rowptr, rowptr + n + 1);
@@ -2130,8 +2365,42 @@ int espreso_feti_set_cluster(espreso_feti *h, int32_t i, int32_t cluster, int32_
 int espreso_feti_set_dual(espreso_feti *h, int32_t n_lambda, const double *c) {
     ESP_API_BEGIN
     ESP_CHECK(h && n_lambda > 0 && c, ESPRESO_ERR_ARG, "invalid dual size");
+    for (int32_t l = 0; l < n_lambda; l++) ESP_CHECK(std::isfinite(c[l]), ESPRESO_ERR_NUMERIC, "non-finite value in c");
+    if (h->loaded) {
+        // after the first set-up the multiplier layout is fixed: new Dirichlet
+        // values of the same rows only (b is recomputed on the resident operators)
+        ESP_CHECK(n_lambda == h->nlam, ESPRESO_ERR_STATE,
+                  "espreso_feti_set_dual after setup: n_lambda differs from the loaded " + std::to_string(h->nlam));
+        ESP_HIP(hipSetDevice(h->opt.device));
+        h->c.assign(c, c + n_lambda);
+        ESP_HIP(hipMemcpyAsync(h->d_c, h->c.data(), n_lambda * sizeof(double), hipMemcpyHostToDevice, h->st));
+        if (h->ready) h->compute_b();
+        h->sync();
+        return ESPRESO_OK;
+    }
     h->nlam = n_lambda;
     h->c.assign(c, c + n_lambda);
+    ESP_API_END
+}
+
+int espreso_feti_update_values(espreso_feti *h, int32_t i, const double *values) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && values && i >= 0 && i < h->nsub, ESPRESO_ERR_ARG, "invalid argument");
+    SubHost &s = h->sub[i];
+    ESP_CHECK(s.set, ESPRESO_ERR_STATE, "subdomain not set");
+    const int nnz = s.rowptr[s.n];
+    for (int q = 0; q < nnz; q++) ESP_CHECK(std::isfinite(values[q]), ESPRESO_ERR_NUMERIC, "non-finite value in K");
+    if (!h->loaded) {
+        s.vals.assign(values, values + nnz);
+        return ESPRESO_OK;
+    }
+    ESP_HIP(hipSetDevice(h->opt.device));
+    ESP_HIP(hipMemcpyAsync(h->P.vals + h->h_nzoff[i], values, nnz * sizeof(double), hipMemcpyHostToDevice, h->st));
+    h->sync();
+    if (!s.vals.empty()) s.vals.assign(values, values + nnz);
+    h->values_dirty = true;
+    h->ready = false;   // F_i, S_D and the coarse data follow at the next espreso_feti_setup
+    h->solved = false;
     ESP_API_END
 }
 
@@ -2163,6 +2432,7 @@ int espreso_feti_resolve(espreso_feti *h, espreso_feti_stats *st) {
     const double trhs = wall() - t0;
     h->solve(st);
     if (st) st->time_coarse = trhs;
+    if (!h->last.converged) return ESPRESO_NOT_CONVERGED;
     ESP_API_END
 }
 
@@ -2194,7 +2464,7 @@ int espreso_feti_resolve_block(espreso_feti *h, int32_t nrhs, const double *f, d
 #pragma omp parallel for reduction(+ : bad)
     for (int64_t q = 0; q < nf; q++) bad += !std::isfinite(f[q]);
     ESP_CHECK(bad == 0, ESPRESO_ERR_NUMERIC, "non-finite load");
-    h->resolve_block(nrhs, f, u, iterations, final_norms, stats);
+    if (!h->resolve_block(nrhs, f, u, iterations, final_norms, stats)) return ESPRESO_NOT_CONVERGED;
     ESP_API_END
 }
 
@@ -2203,6 +2473,7 @@ int espreso_feti_solve(espreso_feti *h, espreso_feti_stats *st) {
     ESP_CHECK(h, ESPRESO_ERR_ARG, "null handle");
     ESP_HIP(hipSetDevice(h->opt.device));
     h->solve(st);
+    if (!h->last.converged) return ESPRESO_NOT_CONVERGED;
     ESP_API_END
 }
 
@@ -2478,14 +2749,16 @@ int espreso_grid_load(espreso_feti *h, const espreso_grid_problem *p) {
     ESP_CHECK(grid_subdomain_count(*p) == h->nsub, ESPRESO_ERR_ARG, "subdomain count differs from the instance");
     GridProblem g;
     h->grid_dev = p->device_assembly != 0;
+    const double tb = wall();
     grid_build(*p, h->opt.scaling, g, !h->grid_dev);
+    h->last.time_build = wall() - tb;
     h->ndof = g.ndof;
     h->nnodes = g.nnodes;
     for (int d = 0; d < 3; d++) h->gridNE[d] = g.NN[d] - 1;
     h->gridKe0 = g.Ke;
     h->grid_box.resize(6 * (size_t)h->nsub);
     for (int s = 0; s < h->nsub; s++) std::copy(g.sub[s].box, g.sub[s].box + 6, h->grid_box.begin() + 6 * s);
-    if (h->grid_dev) h->grid_escale.swap(g.escale);
+    h->grid_escale.swap(g.escale);
     h->l2g.resize(h->nsub);
     for (int s = 0; s < h->nsub; s++) {
         GridSubdomain &sd = g.sub[s];
@@ -2623,7 +2896,7 @@ int espreso_grid_resolve_block(espreso_feti *h, int32_t nrhs, const double *g, d
 #pragma omp parallel for reduction(+ : bad)
     for (int64_t q = 0; q < (int64_t)f.size(); q++) bad += !std::isfinite(f[q]);
     ESP_CHECK(bad == 0, ESPRESO_ERR_NUMERIC, "non-finite load");
-    h->resolve_block(nrhs, f.data(), u.data(), iterations, final_norms, stats);
+    const bool conv = h->resolve_block(nrhs, f.data(), u.data(), iterations, final_norms, stats);
     // solutions averaged over the subdomain copies (as espreso_grid_nodal_solution)
 #pragma omp parallel for schedule(static)
     for (int v = 0; v < nrhs; v++) {
@@ -2639,6 +2912,32 @@ int espreso_grid_resolve_block(espreso_feti *h, int32_t nrhs, const double *g, d
         for (int node = 0; node < h->nnodes; node++)
             for (int c = 0; c < nd; c++) un[(size_t)node * nd + c] /= mult[node];
     }
+    if (!conv) return ESPRESO_NOT_CONVERGED;
+    ESP_API_END
+}
+
+int espreso_grid_update_material(espreso_feti *h, const double *material_elem) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && material_elem, ESPRESO_ERR_ARG, "null argument");
+    ESP_CHECK(h->gridNE[0] > 0 && !h->gridKe0.empty(), ESPRESO_ERR_STATE, "grid not loaded");
+    const int64_t nel = (int64_t)h->gridNE[0] * h->gridNE[1] * h->gridNE[2];
+    ESP_CHECK((int64_t)h->grid_escale.size() == nel, ESPRESO_ERR_STATE, "grid materials missing");
+    for (int64_t e = 0; e < nel; e++)
+        ESP_CHECK(std::isfinite(material_elem[e]) && material_elem[e] > 0, ESPRESO_ERR_ARG,
+                  "element material must be positive and finite");
+    std::copy(material_elem, material_elem + nel, h->grid_escale.begin());
+    if (!h->loaded) {
+        // the values are computed on the device at the first set-up
+        h->grid_dev = true;
+        for (SubHost &s : h->sub) std::vector<double>().swap(s.vals);
+        return ESPRESO_OK;
+    }
+    ESP_HIP(hipSetDevice(h->opt.device));
+    h->grid_stiffness();
+    for (SubHost &s : h->sub) std::vector<double>().swap(s.vals);
+    h->values_dirty = true;
+    h->ready = false;
+    h->solved = false;
     ESP_API_END
 }
 

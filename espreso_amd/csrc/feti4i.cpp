@@ -89,8 +89,10 @@ struct FETI4IStructInstance {
     std::vector<Subdomain> sub;
     int nlam = 0, ndof_node = 1, max_kdim = 0, n0 = 0;
     std::vector<double> c;
+    std::vector<std::vector<int>> elems;  // elements of every subdomain (decomposition)
     espreso_feti *h = nullptr;
     bool gpu_ready = false;
+    bool needs_setup = false;             // K values updated on a set-up core
     espreso_feti_stats st{};
     int ncoarse = 0;
     ~FETI4IStructInstance() {
@@ -488,6 +490,42 @@ void corners(FETI4IStructInstance &I, const std::vector<int> &part) {
     I.n0 = row;
 }
 
+// K_i values of subdomain s from its elements on the existing CSR pattern
+// (the element structure is unchanged); false when the kernel R_i no longer
+// annihilates the new K_i.
+bool reassemble_values(FETI4IStructInstance &I, int s) {
+    const FETI4IStructMatrix &K = I.K;
+    Subdomain &S = I.sub[s];
+    std::fill(S.vals.begin(), S.vals.end(), 0.0);
+    auto loc = [&](int g) { return (int)(std::lower_bound(S.gdofs.begin(), S.gdofs.end(), g) - S.gdofs.begin()); };
+    std::vector<int> l;
+    for (int e : I.elems[s]) {
+        const int k = (int)(K.dptr[e + 1] - K.dptr[e]);
+        const int *ed = &K.dofs[K.dptr[e]];
+        const double *ev = &K.vals[K.vptr[e]];
+        l.resize(k);
+        for (int a = 0; a < k; a++) l[a] = loc(ed[a]);
+        for (int a = 0; a < k; a++) {
+            const int *c0 = &S.colidx[S.rowptr[l[a]]], *c1 = &S.colidx[S.rowptr[l[a] + 1]];
+            for (int b = 0; b < k; b++) {
+                const int q = (int)(std::lower_bound(c0, c1, l[b]) - &S.colidx[0]);
+                S.vals[q] += ev[(size_t)a * k + b];
+            }
+        }
+    }
+    const int n = (int)S.gdofs.size();
+    double kr = 0, kn = 0;
+    for (int i = 0; i < n; i++)
+        for (int q = S.rowptr[i]; q < S.rowptr[i + 1]; q++) kn = std::max(kn, std::fabs(S.vals[q]));
+    for (int a = 0; a < S.kdim; a++)
+        for (int i = 0; i < n; i++) {
+            double acc = 0;
+            for (int q = S.rowptr[i]; q < S.rowptr[i + 1]; q++) acc += S.vals[q] * S.R[(size_t)a * n + S.colidx[q]];
+            kr = std::max(kr, std::fabs(acc));
+        }
+    return kr <= 1e-7 * kn;
+}
+
 // --------------------------------------------------------- decomposition ----
 void decompose(FETI4IStructInstance &I) {
     const FETI4IStructMatrix &K = I.K;
@@ -524,7 +562,8 @@ void decompose(FETI4IStructInstance &I) {
     }
     for (int d = 0; d < n; d++) F4_CHECK(mult[d] > 0, ESPRESO_ERR_ARG, "DOF not covered by any element");
     // element lists per subdomain
-    std::vector<std::vector<int>> elems(ns);
+    std::vector<std::vector<int>> &elems = I.elems;
+    elems.assign(ns, {});
     for (int e = 0; e < ne; e++) elems[part[e]].push_back(e);
     std::string err;
 #pragma omp parallel for schedule(dynamic)
@@ -640,6 +679,7 @@ void decompose(FETI4IStructInstance &I) {
 void ensure_gpu(FETI4IStructInstance &I) {
     if (!I.built) decompose(I);
     if (I.gpu_ready) return;
+    I.needs_setup = false;
     if (I.h) {
         espreso_feti_destroy(I.h);
         I.h = nullptr;
@@ -682,8 +722,8 @@ void set_options(FETI4IStructInstance &I, const FETI4IInt *io, const FETI4IReal 
         F4_CHECK(io[FETI4I_MAX_ITERATIONS] >= 1, ESPRESO_ERR_ARG, "FETI4I_MAX_ITERATIONS must be positive");
         F4_CHECK(io[FETI4I_FETI_METHOD] == 0 || io[FETI4I_FETI_METHOD] == 1, ESPRESO_ERR_ARG,
                  "FETI4I_FETI_METHOD: TOTAL_FETI (0) or HYBRID_FETI (1)");
-        F4_CHECK(io[FETI4I_PRECONDITIONER] >= 0 && io[FETI4I_PRECONDITIONER] <= 3, ESPRESO_ERR_ARG,
-                 "FETI4I_PRECONDITIONER: NONE, LUMPED, WEIGHT_FUNCTION or DIRICHLET (0..3)");
+        F4_CHECK(io[FETI4I_PRECONDITIONER] >= 0 && io[FETI4I_PRECONDITIONER] <= 4, ESPRESO_ERR_ARG,
+                 "FETI4I_PRECONDITIONER: NONE, LUMPED, WEIGHT_FUNCTION, DIRICHLET or SUPER_DIRICHLET (0..4)");
         F4_CHECK(io[FETI4I_CGSOLVER] == 0, ESPRESO_ERR_ARG, "FETI4I_CGSOLVER: only PCG (0)");
         I.nparts = (int)io[FETI4I_SUBDOMAINS];
         I.max_it = (int)io[FETI4I_MAX_ITERATIONS];
@@ -805,8 +845,15 @@ void FETI4ISolve(FETI4IInstance instance, FETI4IInt solution_size, FETI4IReal *s
         F4_CHECK(solution && solution_size == instance->size, ESPRESO_ERR_ARG, "solution_size must equal the instance size");
         FETI4IStructInstance &I = *instance;
         ensure_gpu(I);
-        int rc = espreso_feti_solve(I.h, &I.st);
-        F4_CHECK(rc == 0, rc, espreso_last_error());
+        int rc = 0;
+        if (I.needs_setup) {
+            rc = espreso_feti_setup(I.h);
+            F4_CHECK(rc == 0, rc, espreso_last_error());
+            I.needs_setup = false;
+        }
+        rc = espreso_feti_solve(I.h, &I.st);
+        F4_CHECK(rc == 0 || rc == ESPRESO_NOT_CONVERGED, rc, espreso_last_error());
+        const bool converged = rc == 0;
         std::vector<double> acc(I.size, 0.0), u;
         std::vector<int> cnt(I.size, 0);
         for (size_t s = 0; s < I.sub.size(); s++) {
@@ -820,14 +867,39 @@ void FETI4ISolve(FETI4IInstance instance, FETI4IInt solution_size, FETI4IReal *s
             }
         }
         for (int d = 0; d < I.size; d++) solution[d] = acc[d] / std::max(cnt[d], 1);
+        // the solution is returned either way; the status says whether the stop test was met
+        F4_CHECK(converged, ESPRESO_NOT_CONVERGED, "FETI: stop test not met within FETI4I_MAX_ITERATIONS");
     });
 }
 
+// New element values on the same element structure keep the decomposition,
+// B1 and its scaling, the kernels and the GPU instance: only K_i values are
+// re-assembled and handed to espreso_feti_update_values, and the next solve
+// re-assembles the dual operators.  A changed structure, or values whose
+// kernel differs, take the full decomposition again.
 void FETI4IUpdateStiffnessMatrix(FETI4IInstance instance, FETI4IMatrix stiffnessMatrix) {
     guarded([&] {
         F4_CHECK(is_instance(instance) && is_matrix(stiffnessMatrix), ESPRESO_ERR_ARG, "invalid handle");
-        instance->K = *stiffnessMatrix;
-        decompose(*instance);
+        FETI4IStructInstance &I = *instance;
+        const FETI4IStructMatrix &M = *stiffnessMatrix;
+        const bool same = I.built && M.nelem() == I.K.nelem() && M.dptr == I.K.dptr && M.vptr == I.K.vptr &&
+                          M.dofs == I.K.dofs && M.base == I.K.base;
+        I.K = M;
+        if (same) {
+            bool ok = true;
+            for (size_t s = 0; s < I.sub.size() && ok; s++) ok = reassemble_values(I, (int)s);
+            if (ok) {
+                if (I.gpu_ready && I.h) {
+                    for (size_t s = 0; s < I.sub.size(); s++) {
+                        const int rc = espreso_feti_update_values(I.h, (int)s, I.sub[s].vals.data());
+                        F4_CHECK(rc == 0, rc, espreso_last_error());
+                    }
+                    I.needs_setup = true;
+                }
+                return;
+            }
+        }
+        decompose(I);
     });
 }
 

@@ -68,6 +68,17 @@ HEAT_UNEVEN_2000 = dict(
     fixed=(0, 0, 0, 0, 1, 0), heat_source=1.0,
 )
 
+# the elasticity variant of the uneven workload: 2000 subdomains of 4..14
+# elements per direction (seeded slabs), gravity, z=0 clamped; its load cases
+# are seeded U(-1,1) nodal forces (nodal_load_cases)
+ELASTICITY_UNEVEN_2000 = dict(
+    name="elasticity_uneven_2000sub",
+    physics=0, clusters=(1, 1, 1), domains=(10, 10, 20), elements=(9, 9, 9),
+    slab_elements=uneven_slabs((10, 10, 20), 20261208),
+    length=(1.0, 1.0, 2.0), young=E_STEEL, poisson=0.3, density=7850.0,
+    accel=(0.0, 0.0, -9.81), fixed=(0, 0, 0, 0, 7, 0),
+)
+
 ELASTICITY_UNEVEN_512 = dict(
     name="elasticity_uneven_512sub",
     physics=0, clusters=(1, 1, 1), domains=(8, 8, 8), elements=(9, 9, 9),
@@ -136,7 +147,7 @@ HEAT_SIMPLE = dict(
 )
 
 ALL = {p["name"]: p for p in (TFETI_64x5, HTFETI_512x10, MIXED_1000x12, ADJOINT_64x10, HEAT_UNEVEN_2000,
-                              ELASTICITY_UNEVEN_512, PRESSURE_PATCH, GRAVITY, HEAT_SIMPLE)}
+                              ELASTICITY_UNEVEN_2000, ELASTICITY_UNEVEN_512, PRESSURE_PATCH, GRAVITY, HEAT_SIMPLE)}
 
 
 def scaled(spec, **over):

@@ -26,6 +26,9 @@
  *
  * Conventions: every function returns 0 on success or a negative status; the
  * message of the last failure on the calling thread is espreso_last_error().
+ * espreso_feti_solve / _resolve return ESPRESO_NOT_CONVERGED (positive) when
+ * the stop test was not met within max_iterations (or the mixed-precision
+ * refinement limit); the stats and the last iterate are filled in as usual.
  * All pointers are caller-owned host memory unless the name says "device".
  * The library never calls exit().
  */
@@ -44,7 +47,8 @@ enum {
     ESPRESO_ERR_HIP = -2,        /* HIP runtime failure / no device      */
     ESPRESO_ERR_NUMERIC = -3,    /* factorisation breakdown, NaN/Inf     */
     ESPRESO_ERR_STATE = -4,      /* call out of order                    */
-    ESPRESO_ERR_MEMORY = -5      /* device memory budget exceeded        */
+    ESPRESO_ERR_MEMORY = -5,     /* device memory budget exceeded        */
+    ESPRESO_NOT_CONVERGED = 1    /* solved, stop test not met (stats filled) */
 };
 
 /* Structured box problem (same field meaning as the oracle's). */
@@ -79,7 +83,14 @@ typedef struct {
 } espreso_grid_problem;
 
 /* Values follow the reference enums (src/config/ecf/solver/feti.h). */
-enum { ESPRESO_PREC_NONE = 0, ESPRESO_PREC_LUMPED = 1, ESPRESO_PREC_WEIGHT = 2, ESPRESO_PREC_DIRICHLET = 3 };
+enum {
+    ESPRESO_PREC_NONE = 0,
+    ESPRESO_PREC_LUMPED = 1,          /* K_bb                                       */
+    ESPRESO_PREC_WEIGHT = 2,          /* B B^T                                      */
+    ESPRESO_PREC_DIRICHLET = 3,       /* K_bb - K_bi K_ii^-1 K_ib (dense S_D)      */
+    ESPRESO_PREC_SUPER_DIRICHLET = 4  /* K_bb - K_bi diag(K_ii)^-1 K_ib (sparse,
+                                         clustercpu.cpp:243-314, itersolvercpu.cpp:547-550) */
+};
 enum { ESPRESO_SC_FP64 = 0, ESPRESO_SC_FP32 = 1, ESPRESO_SC_BF16 = 2 };
 
 typedef struct {
@@ -95,7 +106,11 @@ typedef struct {
                                    of the three whatever sc_precision is (S_D is the
                                    preconditioner: it changes the iteration count,
                                    not the converged solution)                    */
-    int32_t pad_;
+    int32_t truncate_output;    /* 1: truncate the primal solution to the digits the
+                                   stop test guarantees, u <- trunc(u D)/D with
+                                   D = 1/precision/max(1, 10^ceil(log10 max|u|)), as
+                                   FETISolver::solve does (FETISolver.cpp:131-148);
+                                   0 (default): the solution as computed          */
 } espreso_feti_options;
 
 typedef struct {
@@ -120,6 +135,18 @@ typedef struct {
     double  symv_bytes;         /* algorithmic bytes of one F SYMV launch: packed F_i in storage
                                    precision + lambda gather (8+4+8 B per B1 entry) + y_b (8 B) */
     double  symv_bytes_sd;      /* the same for one S_D SYMV launch (FP64)                    */
+    int32_t converged;          /* 1: final_norm < tol; 0: stopped at the iteration limit  */
+    int32_t pad2;
+    /* KKT report of the primal/dual solution (itersolver.cpp:171-370), computed on the
+     * device after the primal recovery:
+     *   kkt_primal = ||K u - f + B1^T lambda (+ B0^T lambda0)|| / ||f - B1^T lambda||
+     *                over all subdomain DOFs (K the unregularised K_i),
+     *   kkt_dual   = ||B1 u - c|| / norm_c with norm_c = ||c||, plus 1 when ||c|| < 1
+     *                (the reference's normalisation of the equality constraints).   */
+    double  kkt_primal;
+    double  kkt_dual;
+    double  time_build;         /* espreso_grid_load only: the structured-grid builder on the
+                                   host (the reference's assembler), before set_subdomain  */
 } espreso_feti_stats;
 
 typedef struct espreso_feti_s espreso_feti;
@@ -168,6 +195,12 @@ int espreso_grid_set_nodal_rhs(espreso_feti *h, const double *g_nodal);
  * espreso_feti_resolve_block (iterations, final_norms, stats may be NULL).  */
 int espreso_grid_resolve_block(espreso_feti *h, int32_t nrhs, const double *g, double *u_nodal, int32_t *iterations,
                                double *final_norms, espreso_feti_stats *stats);
+/* New per-element material (Young's modulus or conductivity, index
+ * ex + NEx*(ey + NEy*ez)) on a loaded grid instance: the K_i values are
+ * recomputed on the GPU from the unit element matrix and the next
+ * espreso_feti_setup / espreso_feti_solve use them
+ * (espreso_feti_update_values for every subdomain).                          */
+int espreso_grid_update_material(espreso_feti *h, const double *material_elem);
 /* Element sensitivities on the GPU: sens[e] = -w_e^T K_e0 u_e for nodal
  * fields u, w (n_nodes x ndof); with w = u (the compliance adjoint) this is
  * d(f^T u)/dE_e.  No counterpart in the reference (adjoint extension).       */
@@ -188,7 +221,19 @@ int espreso_feti_set_subdomain(espreso_feti *h, int32_t i, int32_t n,
                                const double *f, int32_t kdim, const double *R,
                                int32_t nlam, const int32_t *lambda, const int32_t *dof,
                                const double *sign, const double *scale);
+/* c (n_lambda values).  Before the first espreso_feti_setup any size; after
+ * it only the loaded size (new Dirichlet values: b is recomputed and the next
+ * espreso_feti_resolve uses them).                                           */
 int espreso_feti_set_dual(espreso_feti *h, int32_t n_lambda, const double *c);
+/* New values of K_i on the structure of espreso_feti_set_subdomain (same
+ * rowptr/colidx, nnz values), for nonlinear and time-stepping callers.  The
+ * next espreso_feti_setup re-assembles F_i and S_D from them and keeps the
+ * boundary sets, lambda maps, coarse space G, (G G^T)^-1 and the workspace;
+ * the kernel R_i and the B1 scaling given at set_subdomain stay in force, so
+ * R_i must still span ker K_i (it does when only material values change).
+ * The reference rebuilds its whole SuperCluster on a K update
+ * (FETISolver::update with Matrices::K, FETISolver.cpp:59-76).               */
+int espreso_feti_update_values(espreso_feti *h, int32_t i, const double *values);
 /* Replace the load vector of subdomain i (local DOF order) after setup; the
  * next espreso_feti_resolve solves with it, reusing F_i, S_D, the interior
  * factors and the coarse space.  FETISolver::update with Matrices::f only

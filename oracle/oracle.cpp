@@ -48,6 +48,9 @@ double now() {
 // Subdomains are numbered cluster-major (cluster id, then domain inside the
 // cluster, x fastest), as the reference generator walks clusters
 // (gridgenerator.cpp:44-69).
+// Subdomain slabs may differ in element count (slab_elements; the reference's
+// generator takes one count per direction, gridgenerator.cpp:44-69): slab S of
+// direction d holds global elements soff[d][S] .. soff[d][S+1]-1.
 struct Grid {
     oracle_grid_problem p;
     int ndof;
@@ -56,16 +59,32 @@ struct Grid {
     int NN[3];      // nodes per direction (global)
     double h[3];
     int nsub;
+    std::vector<int> soff[3];
 
     explicit Grid(const oracle_grid_problem &pp) : p(pp) {
         ndof = p.physics == 0 ? 3 : 1;
         nsub = 1;
+        int q = 0;
         for (int d = 0; d < 3; d++) {
             NS[d] = p.clusters[d] * p.domains[d];
-            NE[d] = NS[d] * p.elements[d];
+            soff[d].assign(NS[d] + 1, 0);
+            for (int S = 0; S < NS[d]; S++) soff[d][S + 1] = soff[d][S] + (p.slab_elements ? p.slab_elements[q++] : p.elements[d]);
+            NE[d] = soff[d][NS[d]];
             NN[d] = NE[d] + 1;
             h[d] = p.length[d] / NE[d];
             nsub *= NS[d];
+        }
+        p.slab_elements = nullptr;  // not owned: the offsets hold the layout
+    }
+    // slab of global element e along d
+    int slab_of(int d, int e) const {
+        return (int)(std::upper_bound(soff[d].begin(), soff[d].end(), e) - soff[d].begin()) - 1;
+    }
+    // elements per direction and first global element of the subdomain at slab position S
+    void sub_box(const int *S, int *ex, int *org) const {
+        for (int d = 0; d < 3; d++) {
+            org[d] = soff[d][S[d]];
+            ex[d] = soff[d][S[d] + 1] - org[d];
         }
     }
     int nnodes() const { return NN[0] * NN[1] * NN[2]; }
@@ -91,9 +110,8 @@ struct Grid {
         int g[3] = {i, j, k};
         int lo[3], hi[3];
         for (int q = 0; q < 3; q++) {
-            int e = p.elements[q];
-            lo[q] = std::max(0, (g[q] + e - 1) / e - 1);
-            hi[q] = std::min(NS[q] - 1, g[q] / e);
+            hi[q] = std::min(NS[q] - 1, slab_of(q, g[q]));
+            lo[q] = (hi[q] > 0 && soff[q][hi[q]] == g[q]) ? hi[q] - 1 : hi[q];
         }
         int S[3];
         for (S[2] = lo[2]; S[2] <= hi[2]; S[2]++)
@@ -188,7 +206,7 @@ Loads make_loads(const Grid &G) {
         for (int k = 0; k < G.NE[2]; k++)
             for (int j = 0; j < G.NE[1]; j++)
                 for (int i = 0; i < G.NE[0]; i++) {
-                    int S[3] = {i / G.p.elements[0], j / G.p.elements[1], k / G.p.elements[2]};
+                    int S[3] = {G.slab_of(0, i), G.slab_of(1, j), G.slab_of(2, k)};
                     L.elem_scale[i + G.NE[0] * (j + G.NE[1] * k)] = ks[G.subid(S)];
                 }
     }
@@ -323,31 +341,94 @@ struct Profile {
 
 // ------------------------------------------------------- dense helpers ----
 // Column-major n x n.
+// Blocked right-looking Cholesky (lower triangle, column-major), OpenMP over
+// the panel rows and the trailing columns: the coarse G G^T of a thousand
+// subdomains is 6000 x 6000.
 bool dense_cholesky(int n, double *A) {
-    for (int j = 0; j < n; j++) {
-        double d = A[j + (size_t)j * n];
-        for (int k = 0; k < j; k++) d -= A[j + (size_t)k * n] * A[j + (size_t)k * n];
-        if (!(d > 0)) return false;
-        d = std::sqrt(d);
-        A[j + (size_t)j * n] = d;
-        for (int i = j + 1; i < n; i++) {
-            double s = A[i + (size_t)j * n];
-            for (int k = 0; k < j; k++) s -= A[i + (size_t)k * n] * A[j + (size_t)k * n];
-            A[i + (size_t)j * n] = s / d;
+    const int NB = 96;
+    std::vector<double> P;
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int kb = std::min(NB, n - k0), r0 = k0 + kb, nr = n - r0;
+        // diagonal block
+        for (int j = k0; j < r0; j++) {
+            double d = A[j + (size_t)j * n];
+            for (int k = k0; k < j; k++) d -= A[j + (size_t)k * n] * A[j + (size_t)k * n];
+            if (!(d > 0)) return false;
+            d = std::sqrt(d);
+            A[j + (size_t)j * n] = d;
+            for (int i = j + 1; i < r0; i++) {
+                double s = A[i + (size_t)j * n];
+                for (int k = k0; k < j; k++) s -= A[i + (size_t)k * n] * A[j + (size_t)k * n];
+                A[i + (size_t)j * n] = s / d;
+            }
+        }
+        if (nr <= 0) break;
+        // panel rows below the block, row-major copy P (nr x kb): P = A_rk L_kk^-T
+        P.assign((size_t)nr * kb, 0.0);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < nr; i++) {
+            double *pi = &P[(size_t)i * kb];
+            for (int t = 0; t < kb; t++) pi[t] = A[(r0 + i) + (size_t)(k0 + t) * n];
+            for (int j = 0; j < kb; j++) {
+                double s = pi[j];
+                for (int t = 0; t < j; t++) s -= pi[t] * A[(k0 + j) + (size_t)(k0 + t) * n];
+                pi[j] = s / A[(k0 + j) + (size_t)(k0 + j) * n];
+            }
+            for (int t = 0; t < kb; t++) A[(r0 + i) + (size_t)(k0 + t) * n] = pi[t];
+        }
+        // trailing update of the lower triangle: A_ij -= P_i . P_j
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int j = 0; j < nr; j++) {
+            const double *pj = &P[(size_t)j * kb];
+            double *col = A + (size_t)(r0 + j) * n + r0;
+            for (int i = j; i < nr; i++) {
+                const double *pi = &P[(size_t)i * kb];
+                double s = 0;
+#pragma omp simd reduction(+ : s)
+                for (int t = 0; t < kb; t++) s += pi[t] * pj[t];
+                col[i] -= s;
+            }
         }
     }
     return true;
 }
+// L L^T x = b in place (L from dense_cholesky), column-oriented sweeps with
+// the row ranges split over threads
 void dense_chol_solve(int n, const double *L, double *x) {
-    for (int i = 0; i < n; i++) {
-        double s = x[i];
-        for (int k = 0; k < i; k++) s -= L[i + (size_t)k * n] * x[k];
-        x[i] = s / L[i + (size_t)i * n];
+    const int NB = 64;
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int kb = std::min(NB, n - k0), r0 = k0 + kb;
+        for (int k = k0; k < r0; k++) {
+            x[k] /= L[k + (size_t)k * n];
+            for (int i = k + 1; i < r0; i++) x[i] -= L[i + (size_t)k * n] * x[k];
+        }
+        if (r0 >= n) break;
+#pragma omp parallel for schedule(static) if (n - r0 > 4096)
+        for (int c = r0; c < n; c += 512) {
+            const int ce = std::min(n, c + 512);
+            for (int k = k0; k < r0; k++) {
+                const double xk = x[k];
+                const double *lk = L + (size_t)k * n;
+                for (int i = c; i < ce; i++) x[i] -= lk[i] * xk;
+            }
+        }
     }
-    for (int i = n - 1; i >= 0; i--) {
-        double s = x[i];
-        for (int k = i + 1; k < n; k++) s -= L[k + (size_t)i * n] * x[k];
-        x[i] = s / L[i + (size_t)i * n];
+    for (int k1 = n; k1 > 0; k1 -= NB) {
+        const int k0 = std::max(0, k1 - NB);
+        if (k1 < n) {
+#pragma omp parallel for schedule(static) if (n - k1 > 4096)
+            for (int i = k0; i < k1; i++) {
+                const double *li = L + (size_t)i * n;
+                double s = 0;
+                for (int k = k1; k < n; k++) s += li[k] * x[k];
+                x[i] -= s;
+            }
+        }
+        for (int i = k1 - 1; i >= k0; i--) {
+            double s = x[i];
+            for (int k = i + 1; k < k1; k++) s -= L[k + (size_t)i * n] * x[k];
+            x[i] = s / L[i + (size_t)i * n];
+        }
     }
 }
 // SPD inverse via Cholesky: A <- A^-1 (full symmetric)
@@ -413,6 +494,7 @@ struct Subdomain {
     std::vector<int> ipos;
     Profile Kii;
     std::vector<double> SD;       // m x m Schur of K on the boundary
+    std::vector<double> SSD;      // m x m K_bb - K_bi diag(K_ii)^-1 K_ib (SUPER_DIRICHLET)
     std::vector<double> Fb;       // m x m (K^+)_bb for the active mode
     std::vector<double> u;        // primal of last solve
 
@@ -449,7 +531,8 @@ struct Oracle {
     // ---- subdomain assembly (physics.cpp:147-228 element loop) ----
     void assemble(Subdomain &sd, int s) {
         G.subpos(s, sd.S);
-        const int *ex = G.p.elements;
+        int ex[3], org[3];
+        G.sub_box(sd.S, ex, org);
         int nd = G.ndof;
         for (int q = 0; q < 3; q++) sd.nn[q] = ex[q] + 1;
         int nnode = sd.nn[0] * sd.nn[1] * sd.nn[2];
@@ -458,8 +541,7 @@ struct Oracle {
         for (int c = 0; c < sd.nn[2]; c++)
             for (int b = 0; b < sd.nn[1]; b++)
                 for (int a = 0; a < sd.nn[0]; a++)
-                    sd.l2g[a + sd.nn[0] * (b + sd.nn[1] * c)] =
-                        G.node(sd.S[0] * ex[0] + a, sd.S[1] * ex[1] + b, sd.S[2] * ex[2] + c);
+                    sd.l2g[a + sd.nn[0] * (b + sd.nn[1] * c)] = G.node(org[0] + a, org[1] + b, org[2] + c);
         // CSR pattern: 27-point node stencil
         sd.rp.assign(sd.n + 1, 0);
         std::vector<std::vector<int>> cols(sd.n);
@@ -490,7 +572,7 @@ struct Oracle {
         for (int c = 0; c < ex[2]; c++)
             for (int b = 0; b < ex[1]; b++)
                 for (int a = 0; a < ex[0]; a++) {
-                    int gi = sd.S[0] * ex[0] + a, gj = sd.S[1] * ex[1] + b, gk = sd.S[2] * ex[2] + c;
+                    int gi = org[0] + a, gj = org[1] + b, gk = org[2] + c;
                     double sc = L.elem_scale[gi + G.NE[0] * (gj + G.NE[1] * gk)];
                     int dofs[24];
                     for (int v = 0; v < 8; v++) {
@@ -543,7 +625,8 @@ struct Oracle {
     // ---- fix-point regularisation (structuralmechanics3d.cpp:101-183) ----
     void regularise(Subdomain &sd) {
         int nd = G.ndof;
-        const int *ex = G.p.elements;
+        int ex[3], org[3];
+        G.sub_box(sd.S, ex, org);
         int pa[3][2];
         for (int q = 0; q < 3; q++) { pa[q][0] = ex[q] / 4; pa[q][1] = ex[q] - ex[q] / 4; }
         std::vector<int> fixnodes;
@@ -668,6 +751,28 @@ struct Oracle {
             }
     }
 
+    // SUPER_DIRICHLET block (clustercpu.cpp:243-314 with diagonalized_K_rr):
+    // K_bb - K_bi diag(K_ii)^-1 K_ib, K the unregularised stiffness
+    void super_dirichlet(Subdomain &sd) {
+        int n = sd.n, m = (int)sd.bdofs.size();
+        std::vector<int> bpos(n, -1);
+        for (int q = 0; q < m; q++) bpos[sd.bdofs[q]] = q;
+        sd.SSD.assign((size_t)m * m, 0.0);
+        for (int bj = 0; bj < m; bj++) {
+            int j = sd.bdofs[bj];
+            for (int q = sd.rp[j]; q < sd.rp[j + 1]; q++) {
+                int c = sd.ci[q];
+                if (bpos[c] >= 0) { sd.SSD[bpos[c] + (size_t)bj * m] += sd.kv[q]; continue; }
+                // interior neighbour c of boundary DOF j: K_bc K_cc^-1 K_cj for every boundary b next to c
+                double w = sd.kv[q] / sd.kdiag(c);
+                for (int q2 = sd.rp[c]; q2 < sd.rp[c + 1]; q2++) {
+                    int b = bpos[sd.ci[q2]];
+                    if (b >= 0) sd.SSD[b + (size_t)bj * m] -= sd.kv[q2] * w;
+                }
+            }
+        }
+    }
+
     // boundary-kernel generalised inverse: F_b = (S_D + rho Q Q^T)^-1,
     // Q = orth(R_b), rho = max diag S_D.
     void boundary_inverse(Subdomain &sd) {
@@ -730,10 +835,10 @@ struct Oracle {
         std::vector<int> subs;
         // subdomain-local dof of a global node inside subdomain s
         auto localdof = [&](int s, int gi, int gj, int gk, int c) {
-            int S[3];
+            int S[3], ex[3], org[3];
             G.subpos(s, S);
-            const int *ex = G.p.elements;
-            int a = gi - S[0] * ex[0], b = gj - S[1] * ex[1], cc = gk - S[2] * ex[2];
+            G.sub_box(S, ex, org);
+            int a = gi - org[0], b = gj - org[1], cc = gk - org[2];
             return (a + (ex[0] + 1) * (b + (ex[1] + 1) * cc)) * nd + c;
         };
         cvec.clear();
@@ -833,6 +938,7 @@ struct Oracle {
             try {
                 factor_kreg(sub[s]);
                 if (o.preconditioner == 3 || o.kplus == 1) schur(sub[s]);
+
                 if (o.kplus == 1) boundary_inverse(sub[s]);
             } catch (std::string &e) {
 #pragma omp critical
@@ -873,6 +979,27 @@ struct Oracle {
                 sd.matvec(v.data(), w.data());
             } else if (o.preconditioner == 2) {
                 w = v;
+            } else if (o.preconditioner == 4) {
+                // K_bb v_b - K_bi diag(K_ii)^-1 K_ib v_b straight from the sparse K
+                // (the same block super_dirichlet() forms densely)
+                std::vector<char> isb(sd.n, 0);
+                for (int b : sd.bdofs) isb[b] = 1;
+                std::vector<double> t(sd.n, 0.0);
+                for (int i = 0; i < sd.n; i++) {
+                    if (isb[i]) continue;
+                    double acc = 0;
+                    for (int q = sd.rp[i]; q < sd.rp[i + 1]; q++)
+                        if (isb[sd.ci[q]]) acc += sd.kv[q] * v[sd.ci[q]];
+                    t[i] = acc / sd.kdiag(i);
+                }
+                for (int b : sd.bdofs) {
+                    double acc = 0;
+                    for (int q = sd.rp[b]; q < sd.rp[b + 1]; q++) {
+                        const int c = sd.ci[q];
+                        acc += isb[c] ? sd.kv[q] * v[c] : -sd.kv[q] * t[c];
+                    }
+                    w[b] = acc;
+                }
             } else {
                 int m = (int)sd.bdofs.size();
                 for (int bi = 0; bi < m; bi++) {
@@ -1034,9 +1161,27 @@ struct Oracle {
             st->tol = tol;
             st->norm_b = std::sqrt(dot(b, b));
             st->kkt_primal = kp_den > 0 ? std::sqrt(kp_num / kp_den) : std::sqrt(kp_num);
-            st->kkt_dual = cc > 0 ? std::sqrt(bu / cc) : std::sqrt(bu);
+            double nc = std::sqrt(cc);
+            if (std::fabs(nc) < 1) nc += 1.0;   // itersolver.cpp:330-332
+            st->kkt_dual = std::sqrt(bu) / nc;
             st->time_setup = t_setup;
             st->time_solve = now() - t0;
+        }
+    }
+
+    // new loads: each node's load split evenly over the subdomains sharing it
+    void set_nodal_load(const double *f_nodal) {
+        int nd = G.ndof;
+        std::vector<int> mult(G.nnodes(), 0);
+        for (auto &sd : sub)
+            for (int g : sd.l2g) mult[g]++;
+#pragma omp parallel for schedule(dynamic)
+        for (size_t s = 0; s < sub.size(); s++) {
+            Subdomain &sd = sub[s];
+            for (size_t ln = 0; ln < sd.l2g.size(); ln++) {
+                int g = sd.l2g[ln];
+                for (int d = 0; d < nd; d++) sd.f[ln * nd + d] = f_nodal[(size_t)g * nd + d] / mult[g];
+            }
         }
     }
 
@@ -1263,6 +1408,11 @@ int oracle_subdomain_dense(void *hh, int32_t i, int32_t which, double *out) {
         }
         return 0;
     }
+    if (which == 5) {
+        if (sd.SSD.empty()) h->super_dirichlet(sd);
+        std::copy(sd.SSD.begin(), sd.SSD.end(), out);
+        return 0;
+    }
     return 1;
 }
 
@@ -1299,6 +1449,8 @@ int oracle_solve(void *hh, double *u_nodal, double *lambda, oracle_stats *st) {
     if (lambda) std::copy(h->lambda.begin(), h->lambda.end(), lambda);
     return 0;
 }
+
+void oracle_set_nodal_load(void *hh, const double *f_nodal) { ((Oracle *)hh)->set_nodal_load(f_nodal); }
 
 void oracle_subdomain_primal(void *hh, int32_t i, double *u) {
     Oracle *h = (Oracle *)hh;
@@ -1352,6 +1504,115 @@ int oracle_cpu_sample(const oracle_grid_problem *p, const oracle_options *o, int
         times[0] = (t1 - t0) / ns;
         times[1] = (t2 - t1) / (ns * reps);
         times[2] = (t3 - t2) / (ns * reps);
+    } catch (std::string &e) {
+        g_err = e;
+        return 1;
+    }
+    return 0;
+}
+
+// CPU baseline with explicit dense S_i (the reference's Create_SC_w_Mat
+// semantics: the dense Schur complement of each subdomain and its generalised
+// inverse F_i, plus the Dirichlet block S_D from the same Schur step):
+//   times[0]  seconds per subdomain of the explicit set-up (sample, threaded)
+//   times[1]  seconds per subdomain of one dense F_i apply (sample)
+//   times[2]  seconds per subdomain of one dense S_D apply (sample)
+//   times[3]  coarse set-up at full size: G G^T formed and factorised (s)
+//   times[4]  per PCG iteration at full size, everything but the subdomain
+//             matrices: two projections, the B gathers/scatters of the F and
+//             preconditioner applies, three dots and three vector updates (s)
+//   times[5]  subdomains in the sample
+int oracle_cpu_baseline(const oracle_grid_problem *p, const oracle_options *o, int32_t sample, double *times) {
+#ifdef _OPENMP
+    if (o->threads > 0) omp_set_num_threads(o->threads);
+#endif
+    try {
+        Oracle h(*p, *o);
+        h.L = make_loads(h.G);
+        element_matrices(p->physics, h.G.h, p->poisson, h.Ke1, h.Nint);
+        h.sub.assign(h.G.nsub, Subdomain());
+#pragma omp parallel for schedule(dynamic)
+        for (int s = 0; s < h.G.nsub; s++) {
+            h.assemble(h.sub[s], s);
+            h.kernels(h.sub[s]);
+        }
+        h.build_b1();
+        const int ns = std::max(1, std::min<int>(sample, h.G.nsub));
+        double t0 = now();
+#pragma omp parallel for schedule(dynamic)
+        for (int s = 0; s < ns; s++) {
+            h.schur(h.sub[s]);
+            h.boundary_inverse(h.sub[s]);
+        }
+        double t1 = now();
+        const int reps = 3;
+        auto gemv_time = [&](bool fi) {
+            double ta = now();
+#pragma omp parallel for schedule(dynamic)
+            for (int s = 0; s < ns; s++) {
+                const Subdomain &sd = h.sub[s];
+                const int m = (int)sd.bdofs.size();
+                const std::vector<double> &A = fi ? sd.Fb : sd.SD;
+                std::vector<double> v(m, 1.0), w(m, 0.0);
+                for (int r = 0; r < reps; r++) {
+                    std::fill(w.begin(), w.end(), 0.0);
+                    for (int j = 0; j < m; j++) {
+                        const double *col = &A[(size_t)j * m];
+                        const double vj = v[j];
+                        for (int i = 0; i < m; i++) w[i] += col[i] * vj;
+                    }
+                    v[0] = w[0] * 1e-30 + 1.0;
+                }
+            }
+            return (now() - ta) / (ns * reps);
+        };
+        times[0] = (t1 - t0) / ns;
+        times[1] = gemv_time(true);
+        times[2] = gemv_time(false);
+        for (int s = 0; s < ns; s++) {  // release the dense blocks before the full-size steps
+            std::vector<double>().swap(h.sub[s].SD);
+            std::vector<double>().swap(h.sub[s].Fb);
+            h.sub[s].Kii = Profile();
+        }
+        double t2 = now();
+        h.coarse();
+        double t3 = now();
+        // one iteration's work outside the subdomain matrices
+        const int n = h.nlam;
+        std::vector<double> x(n), y(n), z(n), r(n);
+        Rng rng(7);
+        for (int i = 0; i < n; i++) x[i] = 2.0 * rng.uniform() - 1.0;
+        auto gather_scatter = [&](const double *in, double *out, bool scaled) {
+            std::vector<std::vector<double>> part(h.sub.size());
+#pragma omp parallel for schedule(dynamic)
+            for (size_t s = 0; s < h.sub.size(); s++) {
+                const Subdomain &sd = h.sub[s];
+                std::vector<double> v(sd.n, 0.0);
+                for (auto &e : sd.lam) v[e.dof] += e.sign * (scaled ? e.scale : 1.0) * in[e.lam];
+                part[s].resize(sd.lam.size());
+                for (size_t q = 0; q < sd.lam.size(); q++)
+                    part[s][q] = sd.lam[q].sign * (scaled ? sd.lam[q].scale : 1.0) * v[sd.lam[q].dof];
+            }
+            std::fill(out, out + n, 0.0);
+            for (size_t s = 0; s < h.sub.size(); s++)
+                for (size_t q = 0; q < h.sub[s].lam.size(); q++) out[h.sub[s].lam[q].lam] += part[s][q];
+        };
+        double t4 = now();
+        h.project(x.data(), y.data(), 0);
+        h.project(y.data(), z.data(), 0);
+        gather_scatter(z.data(), r.data(), false);
+        gather_scatter(r.data(), y.data(), true);
+        double acc = 0;
+        for (int k = 0; k < 3; k++) {
+            double d = 0;
+            for (int i = 0; i < n; i++) d += x[i] * y[i];
+            acc += d;
+            for (int i = 0; i < n; i++) x[i] += 1e-30 * d * y[i];
+        }
+        double t5 = now();
+        times[3] = t3 - t2;
+        times[4] = (t5 - t4) + 0.0 * acc;
+        times[5] = ns;
     } catch (std::string &e) {
         g_err = e;
         return 1;

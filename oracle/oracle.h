@@ -58,11 +58,15 @@ typedef struct {
     uint64_t conductivity_seed; /* !=0: k_s = conductivity*10^U(-1,1) per subdomain */
     const double *material_elem; /* optional per element: Young's modulus (elasticity) or
                                     conductivity (heat); overrides the seeded fields      */
+    const int32_t *slab_elements; /* optional uneven subdomains: elements of every subdomain
+                                    slab, x slabs then y then z (NSx+NSy+NSz values);
+                                    overrides elements[] (one element size h = L/NE)     */
 } oracle_grid_problem;
 
 typedef struct {
     int32_t method;             /* 0 TOTAL_FETI (HYBRID_FETI not restated)     */
-    int32_t preconditioner;     /* 0 NONE, 1 LUMPED, 2 WEIGHT_FUNCTION, 3 DIRICHLET */
+    int32_t preconditioner;     /* 0 NONE, 1 LUMPED, 2 WEIGHT_FUNCTION, 3 DIRICHLET,
+                                   4 SUPER_DIRICHLET (K_bb - K_bi diag(K_ii)^-1 K_ib) */
     int32_t max_iterations;
     int32_t scaling;            /* 1: K-diagonal B1 scaling, 0: 1/multiplicity */
     int32_t kplus;              /* 0: fix-point regularised K (reference ANALYTIC)
@@ -80,7 +84,7 @@ typedef struct {
     double  tol;                /* precision * min(||P r0||, ||b||)             */
     double  norm_b;
     double  kkt_primal;         /* ||K u - f + B^T lambda|| / ||f - B^T lambda|| */
-    double  kkt_dual;           /* ||B u - c|| / ||c|| (or absolute if c = 0)   */
+    double  kkt_dual;           /* ||B u - c|| / (||c|| + 1 if ||c|| < 1)         */
     double  time_setup;
     double  time_solve;
 } oracle_stats;
@@ -114,7 +118,8 @@ void  oracle_subdomain_lambdas(void *h, int32_t i, int32_t *lam, int32_t *dof, d
 /* dense matrices of subdomain i, column-major:
  *   which 0: K (n x n)        1: K_reg (n x n, fix-point)
  *         2: R (n x k)        3: S_D Dirichlet Schur on boundary (m x m)
- *         4: F_i = (K^+)_bb on boundary (m x m) for the active kplus mode   */
+ *         4: F_i = (K^+)_bb on boundary (m x m) for the active kplus mode
+ *         5: SUPER_DIRICHLET K_bb - K_bi diag(K_ii)^-1 K_ib (m x m)          */
 int   oracle_subdomain_dense(void *h, int32_t i, int32_t which, double *out);
 void  oracle_subdomain_rhs(void *h, int32_t i, double *f);
 /* local node -> global node map of subdomain i */
@@ -128,6 +133,11 @@ void  oracle_project(void *h, const double *x, double *y);  /* P x */
 void  oracle_apply_G(void *h, const double *x, double *y);  /* y = G x (n_coarse) */
 /* full solve: nodal u (average of subdomain copies), lambda, stats */
 int   oracle_solve(void *h, double *u_nodal, double *lambda, oracle_stats *st);
+/* replace the loads by the nodal load f_nodal (n_nodes x ndof), each node's
+ * load split evenly over the subdomains sharing it; the next oracle_solve
+ * reuses the factorisations (the reference's FETISolver::update with
+ * Matrices::f only, FETISolver.cpp:78-92)                                      */
+void  oracle_set_nodal_load(void *h, const double *f_nodal);
 /* per-subdomain primal of the last solve */
 void  oracle_subdomain_primal(void *h, int32_t i, double *u);
 
@@ -138,6 +148,13 @@ void  oracle_subdomain_primal(void *h, int32_t i, double *u);
  * sample subdomains are the first `sample` subdomains of the problem.       */
 int   oracle_cpu_sample(const oracle_grid_problem *p, const oracle_options *o,
                         int32_t sample, double *times);
+
+/* CPU baseline with explicit dense S_i on a sample (times[0..2] per subdomain:
+ * explicit set-up, dense F_i apply, dense S_D apply) and the full-size coarse
+ * set-up (times[3]) and per-iteration work outside the subdomain matrices
+ * (times[4]: projections, B gathers/scatters, dots, updates); times[5] = sample */
+int   oracle_cpu_baseline(const oracle_grid_problem *p, const oracle_options *o,
+                          int32_t sample, double *times);
 
 const char *oracle_last_error(void);
 

@@ -1,0 +1,95 @@
+"""Writes the full-size golden fixtures: the oracle's TFETI solution of a
+benchmark workload (and of seeded nodal load cases), sampled.
+
+The oracle (oracle/liboracle.so, the CPU restatement of the reference's
+TFETI path) is solved to a relative dual residual of 1e-11 (1e-12 lies below
+what FP64 reaches on these problems: the residual stalls near 1e-17 absolute).  A GPU test
+then checks a solution of the same workload against the fixture, without
+running the oracle at that size itself (it takes minutes to an hour on the
+host).  The fixture is data only:
+
+  sample_nodes   seeded node ids (every node equally likely; 8192, or 2048 per case
+                 for the multi-case fixtures)
+  u_sample       the oracle's nodal solution at those nodes (cases x samples x ndof)
+  norm2, maxabs  2-norm and max |u| of the whole nodal solution, per case
+  checksum       sum over all DOFs of w * u with w ~ U(-1,1) (seed CHECK_SEED)
+  iterations, final_norm, tol, kkt_primal, kkt_dual, precision: the oracle's stop
+
+Usage: python tests/golden/make_fullsize.py WORKLOAD [--cases N] [--threads T]
+       [--preconditioner P] (writes tests/golden/fullsize_WORKLOAD.npz)
+Case 0 is the workload's own load; cases 1..N are espreso_amd.problems.
+nodal_load_cases(spec, N) applied as nodal loads on the factorised oracle.
+"""
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+import oracle_lib as O  # noqa: E402
+from espreso_amd import problems as PB  # noqa: E402
+
+SAMPLE_SEED = 20261209
+CHECK_SEED = 20261210
+PRECISION = 1e-11
+
+
+def digest(u, nodes, w):
+    return dict(u_sample=u[nodes], norm2=float(np.linalg.norm(u)), maxabs=float(np.abs(u).max()),
+                checksum=float(np.dot(w, u.ravel())))
+
+
+def checksum_weights(n_values):
+    return np.random.default_rng(CHECK_SEED).uniform(-1.0, 1.0, size=n_values)
+
+
+def sample_nodes(n_nodes, n):
+    return np.sort(np.random.default_rng(SAMPLE_SEED).choice(n_nodes, size=min(n, n_nodes), replace=False))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("workload")
+    ap.add_argument("--cases", type=int, default=0)
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--preconditioner", type=int, default=4, help="oracle preconditioner (4: SUPER_DIRICHLET)")
+    ap.add_argument("--max-iterations", type=int, default=8000)
+    a = ap.parse_args()
+    spec = PB.ALL[a.workload]
+    t0 = time.time()
+    o = O.Oracle(spec, preconditioner=a.preconditioner, precision=PRECISION, max_iterations=a.max_iterations,
+                 threads=a.threads)
+    print(f"{a.workload}: oracle set up in {time.time() - t0:.0f} s ({o.n_sub} subdomains, {o.n_lambda} multipliers)",
+          flush=True)
+    nodes = sample_nodes(o.n_nodes, 8192 if a.cases == 0 else 2048)
+    w = checksum_weights(o.n_nodes * o.ndof)
+    loads = PB.nodal_load_cases(spec, a.cases) if a.cases else []
+    out = {k: [] for k in ("u_sample", "norm2", "maxabs", "checksum", "iterations", "final_norm", "tol",
+                           "kkt_primal", "kkt_dual", "seconds")}
+    for case in range(1 + a.cases):
+        if case > 0:
+            o.set_nodal_load(loads[case - 1])
+        t1 = time.time()
+        u, _, st = o.solve()
+        if not st["final_norm"] < st["tol"]:
+            raise SystemExit(f"case {case}: the oracle did not reach {PRECISION} in {st['iterations']} iterations")
+        d = digest(u, nodes, w)
+        for k, v in d.items():
+            out[k].append(v)
+        for k in ("iterations", "final_norm", "tol", "kkt_primal", "kkt_dual"):
+            out[k].append(st[k])
+        out["seconds"].append(time.time() - t1)
+        print(f"  case {case}: {st['iterations']} iterations, {time.time() - t1:.0f} s, kkt {st['kkt_primal']:.1e} "
+              f"{st['kkt_dual']:.1e}", flush=True)
+    o.close()
+    np.savez_compressed(os.path.join(HERE, f"fullsize_{a.workload}.npz"), workload=a.workload, sample_nodes=nodes,
+                        precision=PRECISION, preconditioner=a.preconditioner, check_seed=CHECK_SEED,
+                        **{k: np.asarray(v) for k, v in out.items()})
+
+
+if __name__ == "__main__":
+    main()

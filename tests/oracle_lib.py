@@ -34,6 +34,7 @@ class OracleProblem(C.Structure):
         ("young_seed", C.c_uint64),
         ("conductivity_seed", C.c_uint64),
         ("material_elem", C.POINTER(C.c_double)),
+        ("slab_elements", C.POINTER(C.c_int32)),
     ]
 
 
@@ -92,6 +93,7 @@ def lib():
         for name in ("oracle_apply_F", "oracle_apply_prec", "oracle_project", "oracle_apply_G"):
             getattr(_lib, name).argtypes = [C.c_void_p, d, d]
         _lib.oracle_solve.argtypes = [C.c_void_p, d, d, C.POINTER(OracleStats)]
+        _lib.oracle_set_nodal_load.argtypes = [C.c_void_p, d]
         _lib.oracle_global_solve.argtypes = [C.POINTER(OracleProblem), d]
         _lib.oracle_unit_element.argtypes = [C.POINTER(OracleProblem), d]
         _lib.oracle_global_solve_load.argtypes = [C.POINTER(OracleProblem), d, d]
@@ -101,6 +103,7 @@ def lib():
         _lib.oracle_seeded_conductivity.argtypes = [C.c_uint64, C.c_int32, C.c_double, d]
         _lib.oracle_seeded_conductivity.restype = None
         _lib.oracle_cpu_sample.argtypes = [C.POINTER(OracleProblem), C.POINTER(OracleOptions), C.c_int32, d]
+        _lib.oracle_cpu_baseline.argtypes = [C.POINTER(OracleProblem), C.POINTER(OracleOptions), C.c_int32, d]
         _lib.oracle_last_error.restype = C.c_char_p
     return _lib
 
@@ -133,11 +136,19 @@ def make_problem(spec):
     p.traction_scale = spec.get("traction_scale", 1.0)
     p.young_seed = spec.get("young_seed", 0)
     p.conductivity_seed = spec.get("conductivity_seed", 0)
+    keep = []  # arrays the struct points to must outlive it
     ye = spec.get("material_elem")
     if ye is not None:
         ye = np.ascontiguousarray(ye, dtype=np.float64)
-        p._keep = ye  # the array must outlive the struct
+        keep.append(ye)
         p.material_elem = _dp(ye)
+    sl = spec.get("slab_elements")
+    if sl is not None:
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(v, dtype=np.int32) for v in sl]), dtype=np.int32)
+        assert flat.size == sum(spec["clusters"][d] * spec["domains"][d] for d in range(3))
+        keep.append(flat)
+        p.slab_elements = _ip(flat)
+    p._keep = keep
     return p
 
 
@@ -287,7 +298,7 @@ class Oracle:
     def dense(self, i, which):
         inf = self.info(i)
         n, m = inf["n"], inf["m"]
-        shape = {0: (n, n), 1: (n, n), 2: (self.kdim, n), 3: (m, m), 4: (m, m)}[which]
+        shape = {0: (n, n), 1: (n, n), 2: (self.kdim, n), 3: (m, m), 4: (m, m), 5: (m, m)}[which]
         out = np.zeros(shape[0] * shape[1])
         lib().oracle_subdomain_dense(self.h, i, which, _dp(out))
         return out.reshape(shape).T if which != 2 else out.reshape(shape).T
@@ -349,6 +360,13 @@ class Oracle:
         lib().oracle_subdomain_primal(self.h, i, _dp(u))
         return u
 
+    def set_nodal_load(self, f_nodal):
+        """Replace the loads by a nodal load (n_nodes x ndof); the next solve()
+        reuses the factorisations."""
+        f = np.ascontiguousarray(f_nodal, dtype=np.float64).reshape(-1)
+        assert f.size == self.n_nodes * self.ndof
+        lib().oracle_set_nodal_load(self.h, _dp(f))
+
 
 def cpu_sample(spec, sample, threads=0, preconditioner=3):
     p = make_problem(spec)
@@ -358,6 +376,19 @@ def cpu_sample(spec, sample, threads=0, preconditioner=3):
     if rc:
         raise RuntimeError(lib().oracle_last_error().decode())
     return dict(setup_per_sub=t[0], kplus_per_sub=t[1], sd_apply_per_sub=t[2])
+
+
+def cpu_baseline(spec, sample, threads=0):
+    """Explicit-S_i CPU restatement timed on a sample of subdomains plus the
+    full-size coarse set-up and per-iteration vector work (seconds)."""
+    p = make_problem(spec)
+    o = make_options(preconditioner=3, threads=threads, kplus=1)
+    t = np.zeros(6)
+    rc = lib().oracle_cpu_baseline(C.byref(p), C.byref(o), sample, _dp(t))
+    if rc:
+        raise RuntimeError(lib().oracle_last_error().decode())
+    return dict(setup_per_sub=t[0], f_apply_per_sub=t[1], sd_apply_per_sub=t[2], coarse_setup=t[3],
+                iteration_rest=t[4], sample=int(t[5]))
 
 
 def hybrid(spec, dirichlet=False):
