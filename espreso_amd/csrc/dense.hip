@@ -786,6 +786,49 @@ __global__ void k_absmax(const DenseDesc *D, float *scale) {
     if (threadIdx.x == 0) scale[blockIdx.x] = v > 0.0 ? (float)v : 1.0f;
 }
 
+// nrm[b] = max_c sum_k |S(c, k)|, the infinity norm of the symmetric S held in
+// its lower triangle (column-major, ld m): it bounds every partial sum of S x
+// by nrm max|x| (the dual-operator SYMV scales its fixed-point accumulators
+// with it).  Row c is its part left of the diagonal (S(c, k), k <= c: thread c
+// walks the row, consecutive threads take consecutive rows, so the loads
+// coalesce) plus the part below the diagonal (column c, read by one wave per
+// column).  Fixed summation order; the max over the rows is exact, nrm[b]
+// starts at zero.
+__global__ void __launch_bounds__(256) k_rownorm(const DenseDesc *D, double *nrm) {
+    const DenseDesc &d = D[blockIdx.y];
+    const int m = d.m;
+    const int c0 = blockIdx.x * 256;
+    if (c0 >= m) return;
+    __shared__ double below[256], wm[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const double *S = d.S;
+    for (int q = wave; q < 256; q += 4) {
+        const int c = c0 + q;
+        double v = 0.0;
+        if (c < m)
+            for (int i = c + 1 + lane; i < m; i += 64) v += fabs(S[i + (size_t)c * m]);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) below[q] = v;
+    }
+    __syncthreads();
+    const int c = c0 + tid;
+    double r = 0.0;
+    if (c < m) {
+        for (int k = 0; k <= c; k++) r += fabs(S[c + (size_t)k * m]);
+        r += below[tid];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) r = fmax(r, __shfl_xor(r, o, 64));
+    if (lane == 0) wm[wave] = r;
+    __syncthreads();
+    if (tid == 0) {
+        const double b = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+        // non-negative doubles order like their bit patterns
+        atomicMax((unsigned long long *)&nrm[blockIdx.y], (unsigned long long)__double_as_longlong(b));
+    }
+}
+
 // strip t, columns c = tR + blockIdx.y*256 + tid
 template <int R, typename T>
 __global__ void k_pack(const DenseDesc *D, T *pool, const int64_t *off, const float *scale) {
@@ -1101,6 +1144,11 @@ void launch_pack(const DenseDesc *desc, int nb, int prec, void *pool, const int6
         hipLaunchKernelGGL((k_pack<STRIP_B16, uint16_t>), grid, dim3(256), 0, st, desc, (uint16_t *)pool, off,
                            (const float *)scale);
     }
+}
+
+void launch_rownorm(const DenseDesc *desc, int nb, double *nrm, int max_m, hipStream_t st) {
+    launch_fill(nrm, nb, 0.0, st);
+    hipLaunchKernelGGL(k_rownorm, dim3(cdiv(max_m, 256), nb), dim3(256), 0, st, desc, nrm);
 }
 
 void launch_band_extract(const DenseDesc *desc, int nb, double *band, const int64_t *band_off, const int *bw,

@@ -39,6 +39,8 @@ struct DualDev {
     int64_t *foff = nullptr;         // offset of packed F_i in its storage type
     int64_t *sdoff = nullptr;        // offset of packed S_D (FP64)
     float *fscale = nullptr;         // BF16 storage scale
+    // infinity norms of F_i and S_D (FP64 values): the SYMV's fixed-point scale
+    double *fnrm = nullptr, *sdnrm = nullptr;
     // boundary -> lambda (CSR over all boundary DOFs)
     int64_t *bl_ptr = nullptr;
     int *bl_lam = nullptr;

@@ -10,9 +10,10 @@
 // contiguous range of strips of one subdomain; B_i^T x is gathered straight
 // from lambda space into LDS, row and column contributions accumulate in LDS,
 // and each workgroup writes one partial vector that a second pass sums and
-// scatters back to lambda space in a fixed order (no global atomics; the LDS
-// atomics of the waves inside a workgroup have no fixed order, so results can
-// differ by rounding from run to run).
+// scatters back to lambda space in a fixed order (no global atomics).  The
+// waves of a workgroup meet in 64-bit fixed-point accumulators (integer adds,
+// so the order in which the waves arrive does not change a bit of the result:
+// repeated applies are bitwise equal).
 #include <hip/hip_runtime.h>
 
 #include "dual.h"
@@ -57,22 +58,34 @@ __device__ __forceinline__ A wave_rows_reduce(A *v, int lane) {
     return s;
 }
 
+// Fixed-point accumulation of the SYMV: value v enters the accumulators as
+// rint(v 2^k).  With 2^e <= 2 ||F_s||_inf max|x| (the bound of every partial
+// sum, doubled against the rounding of the stored values) and k = 60 - e,
+// every accumulator stays below 2^60 in magnitude; one unit is 2^-59 of
+// ||F_s||_inf max|x| or less, below the rounding of an FP64 sum of the same
+// terms.
+__device__ __forceinline__ unsigned long long to_fixed(double v, int k) {
+    return (unsigned long long)(long long)__builtin_rint(ldexp(v, k));
+}
+
 template <typename T, int R>
 __global__ void __launch_bounds__(256) k_symv(const SymvItem *__restrict__ items, const int *__restrict__ M,
                                               const int64_t *__restrict__ boff, const T *__restrict__ pool,
                                               const int64_t *__restrict__ poff, const float *__restrict__ fscale,
-                                              const int64_t *__restrict__ bl_ptr, const int *__restrict__ bl_lam,
-                                              const double *__restrict__ coef, const double *__restrict__ x,
-                                              int gather, double *__restrict__ ypart) {
+                                              const double *__restrict__ nrm, const int64_t *__restrict__ bl_ptr,
+                                              const int *__restrict__ bl_lam, const double *__restrict__ coef,
+                                              const double *__restrict__ x, int gather, double *__restrict__ ypart) {
     typedef typename Acc<T>::type A;
     extern __shared__ unsigned char smem[];
+    __shared__ double wmax[4];
     const SymvItem it = items[blockIdx.x];
     const int s = it.sub, m = M[s];
     const int j0 = it.t0 * R, len = m - j0;
-    A *xs = (A *)smem;   // len + R entries: x of the columns, zero padding past m
-    A *ys = xs + len + R;
+    unsigned long long *ys = (unsigned long long *)smem;  // len fixed-point sums
+    A *xs = (A *)(ys + len);  // len + R entries: x of the columns, zero padding past m
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t b0 = boff[s] + j0;
+    double xm = 0.0;
     for (int j = tid; j < len; j += 256) {
         double v;
         if (gather) {
@@ -82,10 +95,20 @@ __global__ void __launch_bounds__(256) k_symv(const SymvItem *__restrict__ items
             v = x[b0 + j];
         }
         xs[j] = (A)v;
-        ys[j] = (A)0;
+        ys[j] = 0ull;
+        xm = fmax(xm, fabs(v));
     }
     if (tid < R) xs[len + tid] = (A)0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) xm = fmax(xm, __shfl_xor(xm, o, 64));
+    if (lane == 0) wmax[wave] = xm;
     __syncthreads();
+    xm = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+    const double sc = fscale ? (double)fscale[s] : 1.0;
+    const double bound = 2.0 * nrm[s] / sc * xm;
+    // a non-finite x or operator gives NaN outputs (the fixed point would hide it)
+    const bool finite = bound <= 1.7976931348623157e308;
+    const int k = (finite && bound > 0.0) ? 60 - ilogb(bound) : 0;
     const T *base = pool + poff[s];
     for (int t = it.t0 + wave; t < it.t1; t += 4) {
         const int r0 = t * R;
@@ -133,16 +156,16 @@ __global__ void __launch_bounds__(256) k_symv(const SymvItem *__restrict__ items
                     }
                     racc[r] = fma(av, xc, racc[r]);
                 }
-                atomicAdd(&ys[r0 + c - j0], (col[0] + col[1]) + (col[2] + col[3]));
+                atomicAdd(&ys[r0 + c - j0], to_fixed((double)((col[0] + col[1]) + (col[2] + col[3])), k));
             }
         }
         const A rs = wave_rows_reduce<R, A>(racc, lane);
         const int row = lane / (64 / R);
-        if ((lane & (64 / R - 1)) == 0 && r0 + row < m) atomicAdd(&ys[r0 + row - j0], rs);
+        if ((lane & (64 / R - 1)) == 0 && r0 + row < m) atomicAdd(&ys[r0 + row - j0], to_fixed((double)rs, k));
     }
     __syncthreads();
-    const double sc = fscale ? (double)fscale[s] : 1.0;
-    for (int j = tid; j < len; j += 256) ypart[it.part_off + j] = (double)ys[j] * sc;
+    for (int j = tid; j < len; j += 256)
+        ypart[it.part_off + j] = finite ? ldexp((double)(long long)ys[j], -k) * sc : __builtin_nan("");
 }
 
 // yb[boff+j] = sum of the partials of the items of subdomain s covering j
@@ -261,20 +284,26 @@ __global__ void __launch_bounds__(256) k_symv_mfma(const SymvItem *__restrict__ 
             const int c = c0 + q + 4 * i;
             if (c < m && l16 < nr) {
                 const int j = c - r0i;
-                if (j < nrow) atomicAdd(&yr[j * 16 + l16], dc[i]);
+                if (j < nrow) yr[j * 16 + l16] = dc[i];  // the only column sum of j
                 else ypart[(it.part_off + j) * nr + l16] = dc[i];
             }
         }
     }
-    // row sums: lane holds D[strip row q + 4i][l16] per strip
-#pragma unroll
-    for (int tt = 0; tt < S; tt++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int row = tt * R + q + 4 * i;
-            if (tt < ns && row < nrow && l16 < nr) atomicAdd(&yr[row * 16 + l16], dr[tt][i]);
-        }
+    // row sums: lane holds D[strip row q + 4i][l16] per strip; the waves add
+    // them in turn (a fixed order, so repeated applies are bitwise equal)
     __syncthreads();
+    for (int w = 0; w < 4; w++) {
+        if (wave == w) {
+#pragma unroll
+            for (int tt = 0; tt < S; tt++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int row = tt * R + q + 4 * i;
+                    if (tt < ns && row < nrow && l16 < nr) yr[row * 16 + l16] += dr[tt][i];
+                }
+        }
+        __syncthreads();
+    }
     for (int i = tid; i < nrow * 16; i += 256) {
         const int j = i >> 4, v = i & 15;
         if (v < nr) ypart[(it.part_off + j) * nr + v] = yr[i];
@@ -818,24 +847,24 @@ static inline unsigned vgrid(int64_t n) { return (unsigned)std::max<int64_t>(1, 
 void launch_symv(const DualDev &D, bool sd, int prec, const double *x, int gather, bool scaled, hipStream_t st,
                  hipEvent_t ev0, hipEvent_t ev1) {
     if (sd) prec = D.sd_prec;
-    const bool lp = prec != 0;
     const SymvItem *items = D.items[prec];
     const int n_items = D.n_items[prec];
     const double *coef = scaled ? D.bl_sscale : D.bl_sign;
-    const size_t shm = (2 * (size_t)D.max_m + strip_rows(prec)) * (lp ? 4 : 8);
+    const double *nrm = sd ? D.sdnrm : D.fnrm;
+    const size_t shm = symv_lds_bytes(D.max_m, prec);
     if (ev0) (void)hipEventRecord(ev0, st);
     if (prec == 0) {
         const double *pool = sd ? D.SD : D.Fd;
         const int64_t *off = sd ? D.sdoff : D.foff;
         hipLaunchKernelGGL((k_symv<double, STRIP_F64>), dim3(n_items), dim3(256), shm, st, items, D.m, D.boff, pool,
-                           off, (const float *)nullptr, D.bl_ptr, D.bl_lam, coef, x, gather, D.ypart);
+                           off, (const float *)nullptr, nrm, D.bl_ptr, D.bl_lam, coef, x, gather, D.ypart);
     } else if (prec == 1) {
         hipLaunchKernelGGL((k_symv<float, STRIP_F32>), dim3(n_items), dim3(256), shm, st, items, D.m, D.boff,
-                           sd ? D.SDf : D.Ff, D.poff[1], (const float *)nullptr, D.bl_ptr, D.bl_lam, coef, x, gather,
-                           D.ypart);
+                           sd ? D.SDf : D.Ff, D.poff[1], (const float *)nullptr, nrm, D.bl_ptr, D.bl_lam, coef, x,
+                           gather, D.ypart);
     } else {
         hipLaunchKernelGGL((k_symv<uint16_t, STRIP_B16>), dim3(n_items), dim3(256), shm, st, items, D.m, D.boff,
-                           sd ? D.SDh : D.Fh, D.poff[2], (const float *)(sd ? D.sdscale : D.fscale), D.bl_ptr,
+                           sd ? D.SDh : D.Fh, D.poff[2], (const float *)(sd ? D.sdscale : D.fscale), nrm, D.bl_ptr,
                            D.bl_lam, coef, x, gather, D.ypart);
     }
     if (ev1) (void)hipEventRecord(ev1, st);

@@ -399,12 +399,12 @@ void espreso_feti_s::load() {
     note("load: boundary sets, bands", t0);
     {
         // the SYMV keeps x and the row/column sums of a work item's columns in
-        // LDS: (2 max_m + R) values of the accumulation type per workgroup
+        // LDS (symv_lds_bytes, plus the workgroup's max|x| slots)
         int maxsh = 0;
         ESP_HIP(hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, opt.device));
         for (int p = 0; p < 3; p++) {
             if (!prec_used(p)) continue;
-            const int64_t need = (2 * (int64_t)max_m + strip_rows(p)) * (p ? 4 : 8);
+            const int64_t need = symv_lds_bytes(max_m, p) + 4 * 8;
             ESP_CHECK(need <= maxsh, ESPRESO_ERR_ARG,
                       "subdomain " + std::to_string(max_m_sub) + " has " + std::to_string(max_m) +
                           " interface DOFs: its dual-operator workgroup would need " + std::to_string(need) +
@@ -673,6 +673,8 @@ void espreso_feti_s::load() {
     if (D.sd_prec == 2) D.SDh = alloc<uint16_t>(poff[2][nsub]);
     D.fscale = alloc<float>(nsub);
     D.sdscale = alloc<float>(nsub);
+    D.fnrm = alloc<double>(nsub);
+    D.sdnrm = alloc<double>(nsub);
     {
         std::vector<float> ones(nsub, 1.0f);
         ESP_HIP(hipMemcpyAsync(D.fscale, ones.data(), nsub * sizeof(float), hipMemcpyHostToDevice, st));
@@ -1023,6 +1025,7 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
                            d_nloc, max_n, st);
         factor_solve(d_desc, nbt, 0, max_ntii, max_band, max_ntc);
         launch_syrk(d_desc, nbt, 0, max_ntm, st);
+        launch_rownorm(d_desc, nbt, D.sdnrm + s0, max_m, st);
         launch_hf(d_desc, nbt, d_fb, d_hf, d_b_off, max_m, st);
         launch_band_extract(d_desc, nbt, P.band, d_band_off, d_bw, P.yf, d_yf_off, max_ni, st);
         if (D.sd_prec == 0) launch_pack(d_desc, nbt, 0, D.SD, d_p_off[0], nullptr, max_m, st);
@@ -1033,6 +1036,7 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
             ESP_HIP(hipMemcpyAsync(CL.rho_s + s0, d_rho, nbt * sizeof(double), hipMemcpyDeviceToDevice, st));
         factor_solve(d_desc, nbt, 1, max_ntm, max_ntm, max_ntm);
         launch_syrk(d_desc, nbt, 1, max_ntm, st);
+        launch_rownorm(d_desc, nbt, D.fnrm + s0, max_m, st);
         if (hybrid) {
             std::vector<int> nq(nq_h.begin() + s0, nq_h.begin() + s1);
             std::vector<int64_t> qo(nbt), fo(nbt), co(nbt);

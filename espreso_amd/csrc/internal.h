@@ -39,6 +39,12 @@ constexpr int64_t SYMV_DEFAULT_TARGET = 320 * 1024;
 constexpr int64_t SYMV_TUNE_TARGETS[5] = {96 * 1024, 192 * 1024, 320 * 1024, 640 * 1024, 1280 * 1024};
 
 __host__ __device__ inline int strip_rows(int prec) { return prec == 0 ? STRIP_F64 : (prec == 1 ? STRIP_F32 : STRIP_B16); }
+// LDS of a SYMV workgroup (interface up to max_m DOFs): the 64-bit fixed-point
+// row/column sums and x of the columns (plus R padding) in the accumulation
+// type (FP64 for FP64 storage, FP32 otherwise)
+__host__ __device__ inline int64_t symv_lds_bytes(int64_t max_m, int prec) {
+    return 8 * max_m + (max_m + strip_rows(prec)) * (prec ? 4 : 8);
+}
 
 // Packed-strip layout of a symmetric m x m matrix with R rows per strip:
 // strip t holds rows [tR, tR+R) x columns [tR, m) (R*(m-tR) values), zero
@@ -95,6 +101,7 @@ void launch_kernel_reg(const DenseDesc *desc, int nb, const double *q, const int
 void launch_hf(const DenseDesc *desc, int nb, const double *fb, double *hf, const int64_t *off, int max_m,
                hipStream_t st);
 // pack the lower triangle of S into strips: prec 0 FP64, 1 FP32, 2 BF16 (scaled by 1/max|S|)
+void launch_rownorm(const DenseDesc *desc, int nb, double *nrm, int max_m, hipStream_t st);
 void launch_pack(const DenseDesc *desc, int nb, int prec, void *pool, const int64_t *off, float *scale,
                  int max_m, hipStream_t st);
 // copy band of L (ni x (bw+1), column c holds L[c..c+bw][c]) and y_f out of the workspace
