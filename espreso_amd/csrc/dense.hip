@@ -396,12 +396,14 @@ __device__ __forceinline__ void xcd_tile(int &bx, int &by) {
     by = L / gx;
 }
 
-__device__ __forceinline__ void pick(const DenseDesc &d, int which, double *&A, int &n, int &band, int &nt,
+// the factored matrix A (n x n, leading dimension lda) and the right-hand
+// sides B (n x ncols, leading dimension n) of a stage
+__device__ __forceinline__ void pick(const DenseDesc &d, int which, double *&A, int &lda, int &n, int &band, int &nt,
                                      double *&B, int &ncols, int &ntc) {
     if (which == 0) {
-        A = d.Kii; n = d.ni; band = d.band_ii; nt = d.ntii; B = d.Kib; ncols = d.m + 1; ntc = d.ntc;
+        A = d.Kii; lda = d.ldk; n = d.ni; band = d.band_ii; nt = d.ntii; B = d.Kib; ncols = d.m + 1; ntc = d.ntc;
     } else {
-        A = d.S; n = d.m; band = d.ntm; nt = d.ntm; B = d.W; ncols = d.m; ntc = d.ntm;
+        A = d.S; lda = d.m; n = d.m; band = d.ntm; nt = d.ntm; B = d.W; ncols = d.m; ntc = d.ntm;
     }
 }
 
@@ -411,16 +413,16 @@ __device__ __forceinline__ void pick(const DenseDesc &d, int which, double *&A, 
 __global__ void __launch_bounds__(256) k_potrf_diag(const DenseDesc *D, int which, int step) {
     const DenseDesc &d = D[blockIdx.x];
     double *A, *B;
-    int n, band, nt, ncols, ntc;
-    pick(d, which, A, n, band, nt, B, ncols, ntc);
+    int lda, n, band, nt, ncols, ntc;
+    pick(d, which, A, lda, n, band, nt, B, ncols, ntc);
     const int k0 = step * TILE;
     if (k0 >= n) return;
     const int nb = min(TILE, n - k0);
     const int j0 = max(0, step - band) * TILE;
-    double *Akk = A + k0 + (size_t)k0 * n;
+    double *Akk = A + k0 + (size_t)k0 * lda;
     if (k0 > j0) {
-        const double *Lk = A + k0 + (size_t)j0 * n;
-        tile_gemm<false, true>(Akk, n, Lk, n, Lk, n, nb, nb, k0 - j0, -1.0, 1.0);
+        const double *Lk = A + k0 + (size_t)j0 * lda;
+        tile_gemm<false, true>(Akk, lda, Lk, lda, Lk, lda, nb, nb, k0 - j0, -1.0, 1.0);
         __syncthreads();
     }
     __shared__ double T[TILE][TILE + 1];
@@ -432,7 +434,7 @@ __global__ void __launch_bounds__(256) k_potrf_diag(const DenseDesc *D, int whic
     for (int q = tid; q < TILE * TILE; q += 256) {
         int i = q & 63, j = q >> 6;
         double v;
-        if (i < nb && j < nb) v = (i >= j) ? Akk[i + (size_t)j * n] : 0.0;
+        if (i < nb && j < nb) v = (i >= j) ? Akk[i + (size_t)j * lda] : 0.0;
         else v = (i == j) ? 1.0 : 0.0;
         T[i][j] = v;
     }
@@ -563,7 +565,7 @@ __global__ void __launch_bounds__(256) k_potrf_diag(const DenseDesc *D, int whic
     for (int q = tid; q < TILE * TILE; q += 256) {
         int i = q & 63, j = q >> 6;
         Dinv[q] = i > j ? T[j][i] : (i == j ? xd[i] : 0.0);
-        if (i < nb && j < nb && i >= j) Akk[i + (size_t)j * n] = T[i][j];
+        if (i < nb && j < nb && i >= j) Akk[i + (size_t)j * lda] = T[i][j];
     }
 }
 
@@ -574,20 +576,21 @@ __global__ void __launch_bounds__(256) k_potrf_panel(const DenseDesc *D, int whi
     xcd_tile(bx, by);
     const DenseDesc &d = D[by];
     double *A, *B;
-    int n, band, nt, ncols, ntc;
-    pick(d, which, A, n, band, nt, B, ncols, ntc);
+    int lda, n, band, nt, ncols, ntc;
+    pick(d, which, A, lda, n, band, nt, B, ncols, ntc);
     const int i = step + 1 + bx;
     if (i >= nt || i > step + band) return;
     const int k0 = step * TILE, i0 = i * TILE;
     const int nb = min(TILE, n - k0), mi = min(TILE, n - i0);
     const int j0 = max(0, i - band) * TILE;
-    double *C = A + i0 + (size_t)k0 * n;
+    double *C = A + i0 + (size_t)k0 * lda;
     // transposed: T = L_kw L_iw^T (nb x mi), C^T = Dinv_k (C^T - T), i.e. C = (C - L_iw L_kw^T) Dinv_k^T
     __shared__ double lds[FUSED_LDS];
     d4 acc[4] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
     if (k0 > j0)
-        strip_gemm<false, true>(acc, lds, A + k0 + (size_t)j0 * n, n, A + i0 + (size_t)j0 * n, n, nb, mi, k0 - j0);
-    strip_finish<true>(acc, lds, C, n, nb, mi, d.Dinv + (size_t)step * TILE * TILE, 1.0);
+        strip_gemm<false, true>(acc, lds, A + k0 + (size_t)j0 * lda, lda, A + i0 + (size_t)j0 * lda, lda, nb, mi,
+                                k0 - j0);
+    strip_finish<true>(acc, lds, C, lda, nb, mi, d.Dinv + (size_t)step * TILE * TILE, 1.0);
 }
 
 // ---- left-looking forward substitution X = L^-1 B, block row i ----
@@ -598,8 +601,8 @@ __global__ void __launch_bounds__(256, 3) k_trsm_row(const DenseDesc *D, int whi
     xcd_tile(bx, by);
     const DenseDesc &d = D[by];
     double *A, *B;
-    int n, band, nt, ncols, ntc;
-    pick(d, which, A, n, band, nt, B, ncols, ntc);
+    int lda, n, band, nt, ncols, ntc;
+    pick(d, which, A, lda, n, band, nt, B, ncols, ntc);
     const int J0 = 2 * bx, i = step;
     if (J0 >= ntc || i >= nt) return;
     const int i0 = i * TILE, c0 = J0 * TILE;
@@ -636,7 +639,8 @@ __global__ void __launch_bounds__(256, 3) k_trsm_row(const DenseDesc *D, int whi
     double *C = B + i0 + (size_t)c0 * n;
     if (i0 > jmin) {
         const int kfirst[2] = {valid[0] ? jst[0] - jmin : i0 - jmin, valid[1] ? jst[1] - jmin : i0 - jmin};
-        strip2_gemm(acc, lds, A + i0 + (size_t)jmin * n, n, B + jmin + (size_t)c0 * n, n, mi, N, i0 - jmin, kfirst);
+        strip2_gemm(acc, lds, A + i0 + (size_t)jmin * lda, lda, B + jmin + (size_t)c0 * n, n, mi, N, i0 - jmin,
+                    kfirst);
     }
     // which 1: the identity has no entries off the diagonal tile (beta 0)
     strip2_finish(acc, lds, C, n, mi, N, Di, which == 1 ? 0.0 : 1.0, valid);
@@ -648,8 +652,8 @@ __global__ void __launch_bounds__(256, 4) k_trsm_row1(const DenseDesc *D, int wh
     xcd_tile(bx, by);
     const DenseDesc &d = D[by];
     double *A, *B;
-    int n, band, nt, ncols, ntc;
-    pick(d, which, A, n, band, nt, B, ncols, ntc);
+    int lda, n, band, nt, ncols, ntc;
+    pick(d, which, A, lda, n, band, nt, B, ncols, ntc);
     const int J = bx, i = step;
     if (J >= ntc || i >= nt) return;
     const int cs = which == 0 ? d.colstart[J] : J;
@@ -670,7 +674,8 @@ __global__ void __launch_bounds__(256, 4) k_trsm_row1(const DenseDesc *D, int wh
     __shared__ double lds[FUSED_LDS];
     d4 acc[4] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
     if (i0 > j0)
-        strip_gemm<false, false>(acc, lds, A + i0 + (size_t)j0 * n, n, B + j0 + (size_t)c0 * n, n, mi, nj, i0 - j0);
+        strip_gemm<false, false>(acc, lds, A + i0 + (size_t)j0 * lda, lda, B + j0 + (size_t)c0 * n, n, mi, nj,
+                                 i0 - j0);
     // which 1: the identity has no entries off the diagonal tile (beta 0)
     strip_finish<false>(acc, lds, C, n, mi, nj, d.Dinv + (size_t)i * TILE * TILE, which == 1 ? 0.0 : 1.0);
 }
@@ -866,7 +871,7 @@ __global__ void k_band_extract(const DenseDesc *D, double *band, const int64_t *
     double *dst = band + band_off[blockIdx.y];
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         int c = (int)(e / w), dd = (int)(e % w);
-        dst[e] = (c + dd < d.ni) ? d.Kii[(c + dd) + (size_t)c * d.ni] : 0.0;
+        dst[e] = (c + dd < d.ni) ? d.Kii[(c + dd) + (size_t)c * d.ldk] : 0.0;
     }
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < d.ni; r += (int64_t)gridDim.x * blockDim.x)
         yf[yf_off[blockIdx.y] + r] = d.Kib[r + (size_t)d.m * d.ni];
@@ -950,7 +955,7 @@ __global__ void __launch_bounds__(256) k_materialize(const DenseDesc *D, const i
         }
         wave_lds_sync();
         if (c < ni) {
-            double *dst = d.Kii + (size_t)c * ni;
+            double *dst = d.Kii + (size_t)c * d.ldk;
             for (int r = w0 + lane; r < w1; r += 64) dst[r] = cb[r - w0];
         } else {
             const int j = c - ni;

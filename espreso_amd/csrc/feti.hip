@@ -114,6 +114,102 @@ bool mgs(int n, int k, double *V) {
 
 }  // namespace
 
+// The device's default memory pool keeps freed memory for the process
+// (release threshold: no limit) instead of returning it to the driver at each
+// synchronisation.
+void keep_pooled_memory(int device) {
+    static std::mutex mu;
+    static std::vector<char> done;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)done.size() <= device) done.resize(device + 1, 0);
+    if (done[device]) return;
+    hipMemPool_t pool = nullptr;
+    ESP_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+    uint64_t keep = UINT64_MAX;
+    ESP_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+    done[device] = 1;
+}
+
+// The assembly workspaces (tens to hundreds of GB) are kept for the process
+// once allocated and lent to the instances, largest first fit: memory that
+// was in use before (by an earlier instance, or a block the stream-ordered
+// pool gave back) is cleared by the driver at its next allocation, which
+// takes seconds at this size.  ESPRESO_CACHE_WORKSPACE=0 frees them with
+// their instance instead.
+struct WorkspaceCache {
+    struct Block {
+        void *ptr;
+        size_t bytes;
+        bool busy;
+    };
+    std::mutex mu;
+    std::map<int, std::vector<Block>> blocks;  // per device
+
+    static bool keep() {
+        const char *e = std::getenv("ESPRESO_CACHE_WORKSPACE");
+        return !e || std::atoi(e) != 0;
+    }
+    // bytes of this device's blocks no instance uses
+    size_t idle(int device) {
+        std::lock_guard<std::mutex> lock(mu);
+        size_t n = 0;
+        for (const Block &b : blocks[device])
+            if (!b.busy) n += b.bytes;
+        return n;
+    }
+    // nullptr when the device has no room
+    void *acquire(int device, size_t bytes) {
+        std::lock_guard<std::mutex> lock(mu);
+        std::vector<Block> &v = blocks[device];
+        Block *best = nullptr;
+        for (Block &b : v)
+            if (!b.busy && b.bytes >= bytes && (!best || b.bytes < best->bytes)) best = &b;
+        if (best) {
+            best->busy = true;
+            return best->ptr;
+        }
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipErrorOutOfMemory) {
+            // make room: give the idle blocks of the device back and retry
+            (void)hipGetLastError();
+            for (auto it = v.begin(); it != v.end();) {
+                if (!it->busy) {
+                    (void)hipFree(it->ptr);
+                    it = v.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            e = hipMalloc(&p, bytes);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        v.push_back(Block{p, bytes, true});
+        return p;
+    }
+    void give_back(int device, void *p) {
+        std::lock_guard<std::mutex> lock(mu);
+        std::vector<Block> &v = blocks[device];
+        for (auto it = v.begin(); it != v.end(); ++it)
+            if (it->ptr == p) {
+                if (keep()) {
+                    it->busy = false;
+                } else {
+                    (void)hipFree(p);
+                    v.erase(it);
+                }
+                return;
+            }
+    }
+};
+WorkspaceCache &workspace_cache() {
+    static WorkspaceCache c;
+    return c;
+}
+
 struct espreso_feti_s {
     espreso_feti_options opt{};
     int nsub = 0, nlam = -1, ncoarse = 0;
@@ -121,6 +217,8 @@ struct espreso_feti_s {
     std::vector<double> c;
     hipStream_t st = nullptr;
     std::vector<void *> allocs;
+    std::map<void *, size_t> alloc_size;  // bytes of each allocation
+    size_t held_bytes = 0;                  // device memory the instance holds
     bool loaded = false, ready = false, solved = false;
     // grid bookkeeping
     int ndof = 0, nnodes = 0;
@@ -186,6 +284,13 @@ struct espreso_feti_s {
     }
 
     ~espreso_feti_s() {
+        staging_release();
+        if (asm_pool) {
+            (void)hipStreamSynchronize(st);
+            if (asm_st) (void)hipStreamSynchronize(asm_st);
+            workspace_cache().give_back(opt.device, asm_pool);
+            asm_pool = nullptr;
+        }
         drop_graphs();
         for (Lane &l : lanes) {
             if (l.ev) (void)hipEventDestroy(l.ev);
@@ -195,16 +300,90 @@ struct espreso_feti_s {
         for (hipEvent_t e : asm_ev)
             if (e) (void)hipEventDestroy(e);
         if (asm_st) (void)hipStreamDestroy(asm_st);
-        for (void *p : allocs) (void)hipFree(p);
-        if (st) (void)hipStreamDestroy(st);
+        for (void *p : allocs) (void)hipFreeAsync(p, st);
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
+    }
+    // Device memory comes from the device's stream-ordered pool, which keeps
+    // what instances free for the next ones of the process (like a caching
+    // allocator): giving back and taking again the tens to hundreds of GB of
+    // an instance costs seconds in the driver, reusing pooled memory nothing.
+    // Allocations and frees are ordered on st; every other stream of the
+    // instance starts behind an event of st.
+    // nullptr when the device has no room (must_fit false), else an error
+    void *alloc_bytes(size_t bytes, bool must_fit) {
+        void *p = nullptr;
+        if (bytes == 0) bytes = 8;
+        auto get = [&]() { return hipMallocAsync(&p, bytes, st); };
+        hipError_t e = get();
+        if (e == hipErrorOutOfMemory) {
+            // the pools' unused memory may sit in blocks that do not fit the
+            // request: give it back to the driver and try once more
+            (void)hipGetLastError();
+            ESP_HIP(hipStreamSynchronize(st));
+            hipMemPool_t pool = nullptr;
+            ESP_HIP(hipDeviceGetDefaultMemPool(&pool, opt.device));
+            ESP_HIP(hipMemPoolTrimTo(pool, 0));
+            e = get();
+        }
+        if (e == hipErrorOutOfMemory && !must_fit) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        ESP_HIP(e);
+        allocs.push_back(p);
+        alloc_size[p] = bytes;
+        held_bytes += bytes;
+        return p;
     }
     template <class T>
-    T *alloc(size_t n) {
-        void *p = nullptr;
-        if (n == 0) n = 1;
-        ESP_HIP(hipMalloc(&p, n * sizeof(T)));
-        allocs.push_back(p);
-        return (T *)p;
+    T *alloc(size_t n) { return (T *)alloc_bytes(n * sizeof(T), true); }
+    // Uploads the concatenation of per-subdomain host pieces (piece(s) points
+    // at off[s+1] - off[s] values) through two pinned staging buffers: the
+    // host threads copy the pieces of one chunk into a buffer while the copy
+    // engine drains the other.  Pageable copies of the same bytes run at less
+    // than half the link rate, and a concatenated host copy first would add
+    // two passes over them.
+    static constexpr size_t STAGE_BYTES = (size_t)128 << 20;
+    void *stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    void staging_release() {
+        for (int b = 0; b < 2; b++) {
+            if (stage_ev[b]) (void)hipEventSynchronize(stage_ev[b]);
+            if (stage[b]) (void)hipHostFree(stage[b]);
+            if (stage_ev[b]) (void)hipEventDestroy(stage_ev[b]);
+            stage[b] = nullptr;
+            stage_ev[b] = nullptr;
+        }
+    }
+    template <class T, class Piece>
+    T *upload_pieces(const std::vector<int64_t> &off, Piece piece) {
+        const int64_t total = off.back();
+        T *dst = alloc<T>(total);
+        if (total == 0) return dst;
+        for (int b = 0; b < 2; b++) {
+            if (!stage[b]) ESP_HIP(hipHostMalloc(&stage[b], STAGE_BYTES, hipHostMallocDefault));
+            if (!stage_ev[b]) ESP_HIP(hipEventCreateWithFlags(&stage_ev[b], hipEventDisableTiming));
+        }
+        const int64_t chunk = (int64_t)(STAGE_BYTES / sizeof(T));
+        int buf = 0;
+        for (int64_t c0 = 0; c0 < total; c0 += chunk, buf ^= 1) {
+            const int64_t c1 = std::min(total, c0 + chunk);
+            ESP_HIP(hipEventSynchronize(stage_ev[buf]));  // its previous chunk has left
+            T *sb = (T *)stage[buf];
+            const int s0 = (int)(std::upper_bound(off.begin(), off.end(), c0) - off.begin()) - 1;
+            const int s1 = (int)(std::lower_bound(off.begin(), off.end(), c1) - off.begin());
+#pragma omp parallel for schedule(dynamic)
+            for (int q = s0; q < s1; q++) {
+                const int64_t a = std::max(off[q], c0), b = std::min(off[q + 1], c1);
+                if (a < b) std::memcpy(sb + (a - c0), piece(q) + (a - off[q]), (size_t)(b - a) * sizeof(T));
+            }
+            ESP_HIP(hipMemcpyAsync(dst + c0, sb, (size_t)(c1 - c0) * sizeof(T), hipMemcpyHostToDevice, st));
+            ESP_HIP(hipEventRecord(stage_ev[buf], st));
+        }
+        return dst;
     }
     template <class T>
     T *upload(const std::vector<T> &v) {
@@ -215,8 +394,10 @@ struct espreso_feti_s {
     void release(void *p) {
         auto it = std::find(allocs.begin(), allocs.end(), p);
         if (it != allocs.end()) {
-            (void)hipFree(p);
+            (void)hipFreeAsync(p, st);
             allocs.erase(it);
+            held_bytes -= alloc_size[p];
+            alloc_size.erase(p);
         }
     }
     void sync() { ESP_HIP(hipStreamSynchronize(st)); }
@@ -485,8 +666,10 @@ void espreso_feti_s::load() {
     h_nzoff.assign(nsub + 1, 0);
     for (int s = 0; s < nsub; s++) h_nzoff[s + 1] = h_nzoff[s] + sub[s].rowptr[sub[s].n];
     const bool superdir = opt.preconditioner == ESPRESO_PREC_SUPER_DIRICHLET;
+    // K_bb is applied by the lumped and super-Dirichlet preconditioners only
+    const bool need_kbb = superdir || opt.preconditioner == ESPRESO_PREC_LUMPED;
     std::vector<int64_t> kbb_off(nsub + 1, 0);
-    for (int s = 0; s < nsub; s++) kbb_off[s + 1] = kbb_off[s] + kbb_cnt[s];
+    for (int s = 0; s < nsub; s++) kbb_off[s + 1] = kbb_off[s] + (need_kbb ? kbb_cnt[s] : 0);
     std::vector<int64_t> kbb_ptr(nb + 1, 0), kbb_col(kbb_off[nsub]), kbb_pos(kbb_off[nsub]);
     std::vector<std::vector<int>> near_of(superdir ? nsub : 0);  // per subdomain: near DOFs (internal index)
     std::vector<int64_t> near_off(nsub + 1, 0), sib_off(nsub + 1, 0), sbi_off(nsub + 1, 0);
@@ -519,6 +702,7 @@ void espreso_feti_s::load() {
 #pragma omp parallel for schedule(dynamic)
     for (int s = 0; s < nsub; s++) {
         const SubHost &h = sub[s];
+        if (!need_kbb) continue;
         int64_t q = kbb_off[s];
         for (int j = 0; j < h.m; j++) {
             const int lr = h.bdofs[j];
@@ -694,25 +878,14 @@ void espreso_feti_s::load() {
         r_off[s + 1] = r_off[s] + (int64_t)h.n * h.kdim;
     }
     {
-        std::vector<int> rp(rp_off[nsub]), ci(nz_off[nsub]), pm(n_off[nsub]), ip(n_off[nsub]);
-        std::vector<double> kv(nz_off[nsub]), ff(n_off[nsub]), RR(r_off[nsub]);
         std::vector<int> niv(nsub), nv(nsub), bwv(nsub);
-#pragma omp parallel for schedule(dynamic)
         for (int s = 0; s < nsub; s++) {
-            const SubHost &h = sub[s];
-            std::copy(h.rowptr.begin(), h.rowptr.end(), rp.begin() + rp_off[s]);
-            std::copy(h.colidx.begin(), h.colidx.end(), ci.begin() + nz_off[s]);
-            if (!grid_dev) std::copy(h.vals.begin(), h.vals.end(), kv.begin() + nz_off[s]);
-            std::copy(h.perm.begin(), h.perm.end(), pm.begin() + n_off[s]);
-            std::copy(h.iperm.begin(), h.iperm.end(), ip.begin() + n_off[s]);
-            std::copy(h.f.begin(), h.f.end(), ff.begin() + n_off[s]);
-            std::copy(h.R.begin(), h.R.end(), RR.begin() + r_off[s]);
-            niv[s] = h.ni;
-            nv[s] = h.n;
-            bwv[s] = h.bw;
+            niv[s] = sub[s].ni;
+            nv[s] = sub[s].n;
+            bwv[s] = sub[s].bw;
         }
-        P.rowptr = upload(rp);
-        P.colidx = upload(ci);
+        P.rowptr = upload_pieces<int>(rp_off, [&](int s) { return sub[s].rowptr.data(); });
+        P.colidx = upload_pieces<int>(nz_off, [&](int s) { return sub[s].colidx.data(); });
         P.nz_off = upload(nz_off);
         P.rp_off = upload(rp_off);
         if (grid_dev) {
@@ -720,13 +893,12 @@ void espreso_feti_s::load() {
             P.vals = alloc<double>(nz_off[nsub]);
             grid_stiffness();
         } else {
-            P.vals = upload(kv);
+            P.vals = upload_pieces<double>(nz_off, [&](int s) { return sub[s].vals.data(); });
         }
         gather_prec_values();
-        sync();
-        P.perm = upload(pm);
-        P.iperm = upload(ip);
-        P.R = upload(RR);
+        P.perm = upload_pieces<int>(n_off, [&](int s) { return sub[s].perm.data(); });
+        P.iperm = upload_pieces<int>(n_off, [&](int s) { return sub[s].iperm.data(); });
+        P.R = upload_pieces<double>(r_off, [&](int s) { return sub[s].R.data(); });
         P.ni = upload(niv);
         P.n = upload(nv);
         P.bw = upload(bwv);
@@ -746,7 +918,9 @@ void espreso_feti_s::load() {
         P.alpha = d_alpha_tot;
         d_u = alloc<double>(n_off[nsub]);
         uoff = n_off;
-        d_f = upload(ff);
+        d_f = upload_pieces<double>(n_off, [&](int s) { return sub[s].f.data(); });
+        sync();
+        staging_release();
     }
     d_Qm = d_Q;
     for (int p = 0; p < 3; p++) h_poff[p] = poff[p];
@@ -915,32 +1089,67 @@ void espreso_feti_s::resolve_rhs() {
 void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::vector<int64_t> *poff,
                               const std::vector<int64_t> &boff) {
     const int prec = opt.sc_precision;
-    // per-subdomain workspace (doubles)
+    // per-subdomain workspace (doubles): K_ii in band storage, [K_ib | f], S,
+    // W (in [K_ib | f]'s storage when it fits there), diagonal inverses
+    auto kii_rows = [](const SubHost &h) { return std::min<int64_t>((int64_t)(h.band_tiles + 2) * TILE, h.ni + TILE); };
+    auto w_apart = [](const SubHost &h) { return (int64_t)h.m * h.m > (int64_t)h.ni * (h.m + 1); };
     auto ws = [&](const SubHost &h) {
         const int64_t nt = std::max((h.ni + TILE - 1) / TILE, (h.m + TILE - 1) / TILE);
-        return (int64_t)h.ni * h.ni + (int64_t)h.ni * (h.m + 1) + 2 * (int64_t)h.m * h.m + nt * TILE * TILE;
+        return (int64_t)h.ni * kii_rows(h) + (int64_t)h.ni * (h.m + 1) + (int64_t)h.m * h.m +
+               (w_apart(h) ? (int64_t)h.m * h.m : 0) + nt * TILE * TILE;
     };
-    // wave plan and workspace: planned once from the free memory at the first
-    // set-up and kept (re-planning against a changed free size, and reallocating
-    // a multi-GB workspace, cost more than the assembly on some runs)
+    // wave plan and workspace: planned once at the first set-up and kept
+    // (re-planning and reallocating a multi-GB workspace costs more than the
+    // assembly).  The momentary free memory can lag far behind what an earlier
+    // instance gave back (the driver reclaims large frees lazily, and the pool
+    // keeps what it holds), so the plan also counts on the device memory this
+    // instance does not hold; when that is not there the workspace allocation
+    // fails and the plan is redone with half the budget.
     if (wave_plan.empty()) {
         size_t freeb = 0, totalb = 0;
         ESP_HIP(hipMemGetInfo(&freeb, &totalb));
-        double budget = opt.memory_budget_gb > 0 ? opt.memory_budget_gb * 1e9 : 0.85 * (double)freeb;
-        budget = std::min(budget, 0.85 * (double)freeb);
-        size_t most = 0;
-        for (int a = 0; a < nsub;) {
-            int b = a;
-            double bytes = 0;
-            while (b < nsub && (b == a || bytes + 8.0 * ws(sub[b]) <= budget)) bytes += 8.0 * ws(sub[b++]);
-            ESP_CHECK(bytes <= budget, ESPRESO_ERR_MEMORY,
-                      "one subdomain's assembly workspace exceeds the device budget");
-            wave_plan.push_back(b);
-            most = std::max(most, (size_t)(bytes / 8) + 1);
-            a = b;
+        double pool_free = (double)workspace_cache().idle(opt.device);
+        {
+            hipMemPool_t pool = nullptr;
+            uint64_t reserved = 0, used = 0;
+            ESP_HIP(hipDeviceGetDefaultMemPool(&pool, opt.device));
+            ESP_HIP(hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &reserved));
+            ESP_HIP(hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &used));
+            if (reserved > used) pool_free += (double)(reserved - used);
         }
-        asm_pool = alloc<double>(most);
-        asm_pool_n = most;
+        double budget = std::max(0.85 * ((double)freeb + pool_free), 0.85 * (double)totalb - (double)held_bytes);
+        if (opt.memory_budget_gb > 0) budget = std::min(budget, opt.memory_budget_gb * 1e9);
+        double need = 0, largest = 0;
+        for (int s = 0; s < nsub; s++) {
+            need += 8.0 * ws(sub[s]);
+            largest = std::max(largest, 8.0 * ws(sub[s]));
+        }
+        budget = std::min(budget, need);
+        for (;;) {
+            ESP_CHECK(largest <= budget, ESPRESO_ERR_MEMORY,
+                      "one subdomain's assembly workspace exceeds the device memory left");
+            wave_plan.clear();
+            size_t most = 0;
+            for (int a = 0; a < nsub;) {
+                int b = a;
+                double bytes = 0;
+                while (b < nsub && (b == a || bytes + 8.0 * ws(sub[b]) <= budget)) bytes += 8.0 * ws(sub[b++]);
+                wave_plan.push_back(b);
+                most = std::max(most, (size_t)(bytes / 8) + 1);
+                a = b;
+            }
+            ESP_HIP(hipStreamSynchronize(st));  // a lent block may have been the last instance's
+            asm_pool = (double *)workspace_cache().acquire(opt.device, most * sizeof(double));
+            if (asm_pool) {
+                asm_pool_n = most;
+                break;
+            }
+            budget *= 0.5;
+        }
+        if (measure)
+            std::fprintf(stderr, "espreso: assembly workspace %.1f GB in %zu wave(s) (free %.1f GB, pool %.1f GB, "
+                                 "held %.1f GB)\n", 8e-9 * asm_pool_n, wave_plan.size(), 1e-9 * freeb,
+                         1e-9 * pool_free, 1e-9 * held_bytes);
     }
     int s0 = 0;
     asm_flops = 0;
@@ -961,10 +1170,12 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
             d.ntm = (h.m + TILE - 1) / TILE;
             d.ntc = (h.m + 1 + TILE - 1) / TILE;
             d.band_ii = h.band_tiles;
-            d.Kii = cur; cur += (int64_t)h.ni * h.ni;
+            d.ldk = (int)kii_rows(h) - 1;
+            d.Kii = cur + (TILE - 1); cur += (int64_t)h.ni * kii_rows(h);
             d.Kib = cur; cur += (int64_t)h.ni * (h.m + 1);
             d.S = cur; cur += (int64_t)h.m * h.m;
-            d.W = cur; cur += (int64_t)h.m * h.m;
+            if (w_apart(h)) { d.W = cur; cur += (int64_t)h.m * h.m; }
+            else d.W = d.Kib;
             d.Dinv = cur; cur += (int64_t)std::max(d.ntii, d.ntm) * TILE * TILE;
             cs_off[b] = (int64_t)cs_all.size();
             cs_all.insert(cs_all.end(), h.colstart.begin(), h.colstart.end());
@@ -1021,9 +1232,16 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         int *d_nloc = upload(nloc), *d_kd = upload(kd), *d_bw = upload(bw);
         double *d_rho = alloc<double>(nbt);
 
+        const double tw = wall();
+        if (measure) sync();
+        note("assembly: wave inputs", tw);
         launch_materialize(d_desc, nbt, P.rowptr, d_rp_off, P.colidx, P.vals, d_nz_off, P.iperm, P.perm, d_n_off, d_f,
                            d_nloc, max_n, st);
+        if (measure) sync();
+        note("assembly: materialize", tw);
         factor_solve(d_desc, nbt, 0, max_ntii, max_band, max_ntc);
+        if (measure) sync();
+        note("assembly: K_ii factor, solve", tw);
         launch_syrk(d_desc, nbt, 0, max_ntm, st);
         launch_rownorm(d_desc, nbt, D.sdnrm + s0, max_m, st);
         launch_hf(d_desc, nbt, d_fb, d_hf, d_b_off, max_m, st);
@@ -1034,8 +1252,14 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         launch_kernel_reg(d_desc, nbt, d_Q, d_q_off, d_kd, d_rho, max_m, st);
         if (hybrid)
             ESP_HIP(hipMemcpyAsync(CL.rho_s + s0, d_rho, nbt * sizeof(double), hipMemcpyDeviceToDevice, st));
+        if (measure) sync();
+        note("assembly: S_D, band", tw);
         factor_solve(d_desc, nbt, 1, max_ntm, max_ntm, max_ntm);
+        if (measure) sync();
+        note("assembly: S factor, inverse", tw);
         launch_syrk(d_desc, nbt, 1, max_ntm, st);
+        if (measure) sync();
+        note("assembly: F = W^T W", tw);
         launch_rownorm(d_desc, nbt, D.fnrm + s0, max_m, st);
         if (hybrid) {
             std::vector<int> nq(nq_h.begin() + s0, nq_h.begin() + s1);
@@ -1061,9 +1285,12 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         launch_pack(d_desc, nbt, 0, D.Fd, d_p_off[0], nullptr, max_m, st);
         if (prec == 1) launch_pack(d_desc, nbt, 1, D.Ff, d_p_off[1], nullptr, max_m, st);
         if (prec == 2) launch_pack(d_desc, nbt, 2, D.Fh, d_p_off[2], D.fscale + s0, max_m, st);
+        if (measure) sync();
+        note("assembly: pack", tw);
         std::vector<int> info(nbt);
         ESP_HIP(hipMemcpyAsync(info.data(), d_info, nbt * sizeof(int), hipMemcpyDeviceToHost, st));
         sync();
+        note("assembly: status", tw);
         for (int b = 0; b < nbt; b++)
             ESP_CHECK(info[b] == 0, ESPRESO_ERR_NUMERIC,
                       "dense factorisation breakdown in subdomain " + std::to_string(s0 + b) + " at column " +
@@ -1074,6 +1301,7 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
             release(p);
         for (int64_t *q : d_p_off)
             if (q) release(q);
+        note("assembly: wave released", tw);
         s0 = s1;
     }
     if (hybrid && CL.FB0f) launch_fb0_round(D, CL, st);
@@ -1906,22 +2134,40 @@ void espreso_feti_s::build_ggt(const std::vector<double> &Rt, const std::vector<
                                const std::vector<int> &ck, const std::vector<int> &cblk, const std::vector<int> &coff,
                                std::vector<double> &GGt) {
     const int nc = ncoarse;
-    GGt.assign((size_t)nc * nc, 0.0);
     const std::vector<int64_t> &boff = h_boff;
-    for (int l = 0; l < nlam; l++)
-        for (int a = 0; a < le_cnt_h[l]; a++)
-            for (int b2 = 0; b2 < le_cnt_h[l]; b2++) {
-                const int s1 = le_sub_h[2 * l + a], s2 = le_sub_h[2 * l + b2];
-                const int64_t j1 = le_b_h[2 * l + a] - boff[s1], j2 = le_b_h[2 * l + b2] - boff[s2];
-                const double g = le_sign_h[2 * l + a] * le_sign_h[2 * l + b2];
-                const int m1 = sub[s1].m, m2 = sub[s2].m;
-                const int o1 = coff[cblk[s1]], o2 = coff[cblk[s2]];
-                for (int i = 0; i < ck[s1]; i++) {
-                    const double r1 = g * Rt[rtoff[s1] + j1 + (int64_t)i * m1];
-                    for (int j = 0; j < ck[s2]; j++)
-                        GGt[(o1 + i) + (size_t)(o2 + j) * nc] += r1 * Rt[rtoff[s2] + j2 + (int64_t)j * m2];
+    int nblk = 0;
+    for (int s = 0; s < nsub; s++) nblk = std::max(nblk, cblk[s] + 1);
+    std::vector<std::vector<int>> subs_of(nblk);
+    for (int s = 0; s < nsub; s++) subs_of[cblk[s]].push_back(s);
+    // each thread owns the columns of its coarse blocks (zeroes and fills
+    // them): no shared writes, and a fixed order of the sums
+    GGt.resize((size_t)nc * nc);
+#pragma omp parallel for schedule(dynamic)
+    for (int blk = 0; blk < nblk; blk++) {
+        if (subs_of[blk].empty()) continue;
+        const int o2 = coff[blk], k2 = ck[subs_of[blk][0]];
+        std::fill(GGt.begin() + (size_t)o2 * nc, GGt.begin() + (size_t)(o2 + k2) * nc, 0.0);
+        for (const int s2 : subs_of[blk]) {
+            const SubHost &h2 = sub[s2];
+            for (size_t q = 0; q < h2.lam.size(); q++) {
+                const int l = h2.lam[q];
+                const int e2 = le_sub_h[2 * l] == s2 ? 0 : 1;
+                const int64_t j2 = le_b_h[2 * l + e2] - boff[s2];
+                for (int a = 0; a < le_cnt_h[l]; a++) {
+                    const int s1 = le_sub_h[2 * l + a];
+                    const int64_t j1 = le_b_h[2 * l + a] - boff[s1];
+                    const double g = le_sign_h[2 * l + a] * le_sign_h[2 * l + e2];
+                    const int m1 = sub[s1].m, m2 = h2.m;
+                    const int o1 = coff[cblk[s1]];
+                    for (int j = 0; j < ck[s2]; j++) {
+                        const double r2 = g * Rt[rtoff[s2] + j2 + (int64_t)j * m2];
+                        double *col = GGt.data() + (size_t)(o2 + j) * nc + o1;
+                        for (int i = 0; i < ck[s1]; i++) col[i] += Rt[rtoff[s1] + j1 + (int64_t)i * m1] * r2;
+                    }
                 }
             }
+        }
+    }
 }
 
 // Total FETI: one coarse block per subdomain, basis R_b, N_s = I
@@ -2274,6 +2520,7 @@ extern "C" {
 const char *espreso_last_error(void) { return g_last_error.c_str(); }
 int espreso_version(void) { return 1; }
 
+
 int espreso_feti_create(espreso_feti **h, int32_t n_subdomains, const espreso_feti_options *o) {
     ESP_API_BEGIN
     ESP_CHECK(h && o && n_subdomains > 0, ESPRESO_ERR_ARG, "invalid arguments to espreso_feti_create");
@@ -2283,6 +2530,7 @@ int espreso_feti_create(espreso_feti **h, int32_t n_subdomains, const espreso_fe
     ESP_CHECK(ndev > 0, ESPRESO_ERR_HIP, "no HIP device");
     ESP_CHECK(o->device >= 0 && o->device < ndev, ESPRESO_ERR_ARG, "device ordinal out of range");
     ESP_HIP(hipSetDevice(o->device));
+    keep_pooled_memory(o->device);
     auto f = std::make_unique<espreso_feti_s>();
     f->opt = *o;
     if (f->opt.max_iterations <= 0) f->opt.max_iterations = 200;
@@ -2296,6 +2544,7 @@ int espreso_feti_create(espreso_feti **h, int32_t n_subdomains, const espreso_fe
 
 int espreso_feti_destroy(espreso_feti *h) {
     ESP_API_BEGIN
+    if (h) ESP_HIP(hipSetDevice(h->opt.device));
     delete h;
     ESP_API_END
 }
@@ -2764,23 +3013,32 @@ int espreso_grid_load(espreso_feti *h, const espreso_grid_problem *p) {
     for (int s = 0; s < h->nsub; s++) std::copy(g.sub[s].box, g.sub[s].box + 6, h->grid_box.begin() + 6 * s);
     h->grid_escale.swap(g.escale);
     h->l2g.resize(h->nsub);
+    // the subdomains are handed over through the public entry points, in
+    // parallel (each call touches its own subdomain only)
+    std::vector<int> rcs(h->nsub, 0);
+    std::vector<std::string> errs(h->nsub);
+#pragma omp parallel for schedule(dynamic)
     for (int s = 0; s < h->nsub; s++) {
         GridSubdomain &sd = g.sub[s];
         int rc = espreso_feti_set_subdomain(h, s, sd.n, sd.rowptr.data(), sd.colidx.data(),
                                             h->grid_dev ? nullptr : sd.vals.data(),
                                             sd.f.data(), sd.kdim, sd.R.data(), (int)sd.lam.size(), sd.lam.data(),
                                             sd.dof.data(), sd.sign.data(), sd.scale.data());
-        if (rc) return rc;
-        if (h->opt.method == 1) {
+        if (!rc && h->opt.method == 1)
             rc = espreso_feti_set_cluster(h, s, sd.cluster, (int)sd.b0row.size(), sd.b0row.data(), sd.b0dof.data(),
                                           sd.b0sign.data());
-            if (rc) return rc;
+        if (rc) {
+            rcs[s] = rc;
+            errs[s] = espreso_last_error();
+            continue;
         }
         h->l2g[s].swap(sd.l2g);
         GridSubdomain().rowptr.swap(sd.rowptr);
         std::vector<int>().swap(sd.colidx);
         std::vector<double>().swap(sd.vals);
     }
+    for (int s = 0; s < h->nsub; s++)
+        if (rcs[s]) throw Error(rcs[s], errs[s]);
     return espreso_feti_set_dual(h, g.nlam, g.c.data());
     ESP_API_END
 }

@@ -66,14 +66,22 @@ __host__ __device__ inline int64_t strip_index(int64_t m, int64_t R, int64_t VEC
 // Per-subdomain descriptor of the dense assembly workspace (device memory).
 // Interior DOFs first (ascending local DOF), boundary DOFs after them.
 struct DenseDesc {
-    double *Kii;          // ni x ni, ld ni (lower used); becomes L (K_ii = L L^T)
+    // K_ii (lower band, becomes L: K_ii = L L^T) in diagonal-band storage:
+    // element (i, j) at Kii[i + j ldk] for -63 <= i - j <= ldk - 63, with
+    // ldk + 1 = min((band_ii + 2) 64, ni + 64) stored rows per column and Kii
+    // 63 elements past the start of the storage.  The tiles of the band, and every
+    // window the kernels read, address like a column-major matrix of leading
+    // dimension ldk; the storage is ni (ldk + 1) values instead of ni^2.
+    double *Kii;
     double *Kib;          // ni x (m+1), ld ni: [K_ib | f_i]  ->  [Y | y_f] = L^-1 [K_ib | f_i]
     double *S;            // m x m, ld m (lower): K_bb - Y^T Y -> + rho Q Q^T -> L_s -> F
-    double *W;            // m x m, ld m: L_s^-1 (lower tiles)
+    double *W;            // m x m, ld m: L_s^-1 (lower tiles); may reuse Kib's storage
+                          // (Y and y_f are no longer read once S_D and the band are out)
     double *Dinv;         // max(ntii, ntm) x 64 x 64: inverses of diagonal blocks
     const int *colstart;  // ntc: first tile row of a nonzero in each column tile of K_ib
     int *info;            // factorisation status (0 = ok, else 1-based failing column)
     int ni, m;
+    int ldk;              // leading dimension of Kii (band storage, see above)
     int band_ii;          // block bandwidth of K_ii in tiles
     int ntii, ntm, ntc;   // tile counts: ceil(ni/64), ceil(m/64), ceil((m+1)/64)
 };
