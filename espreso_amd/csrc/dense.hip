@@ -206,9 +206,9 @@ __device__ __forceinline__ void strip_gemm(d4 (&acc)[4], double *lds, const doub
 // then C = Dinv(:, :M) R.  TR: the tile's element (r, c) is C[c + r ldc] (the
 // transposed step), else C[r + c ldc].  Dinv is 64 x 64, column-major.  The
 // operand buffers must be free (no wave still reads them).
-template <bool TR>
+template <bool TR, bool KEEP = false>
 __device__ __forceinline__ void strip_finish(d4 (&acc)[4], double *lds, double *C, int ldc, int M, int N,
-                                             const double *Dinv, double beta) {
+                                             const double *Dinv, double beta, d4 *keep = nullptr) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c = 16 * wave + (lane & 15);
     auto at = [&](int r) { return TR ? (size_t)c + (size_t)r * ldc : (size_t)r + (size_t)c * ldc; };
@@ -260,7 +260,42 @@ __device__ __forceinline__ void strip_finish(d4 (&acc)[4], double *lds, double *
             const int r = 16 * ob + (lane >> 4) + 4 * v;
             if (r < M && c < N) C[at(r)] = out[v];
         }
+        if constexpr (KEEP) keep[ob] = out;
     }
+}
+
+// A_ii -= L L^T for the panel tile L (mi x 64, zero past its valid rows and
+// columns) that strip_finish<true, true> just produced: lane value v of o[ob]
+// is L(16 w + l%16, 16 ob + l/16 + 4v).  L goes through LDS (no re-read of
+// the stored tile) and the 64 x 64 x 64 product runs on MFMA, wave w owning
+// columns 16w..16w+15 of A_ii.  A_ii is written by this workgroup only.
+__device__ __forceinline__ void diag_update(const d4 (&o)[4], double *lds, double *Aii, int lda, int mi) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __syncthreads();  // every wave has read the staged diagonal inverse
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            lds[(16 * wave + (lane & 15)) * (TILE + 1) + 16 * ob + (lane >> 4) + 4 * v] = o[ob][v];
+    __syncthreads();
+    d4 acc[4] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+    for (int ks = 0; ks < TILE; ks += 4) {
+        const double b = lds[(16 * wave + (lane & 15)) * (TILE + 1) + ks + (lane >> 4)];
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++) {
+            const double a = lds[(16 * rb + (lane & 15)) * (TILE + 1) + ks + (lane >> 4)];
+            acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[rb], 0, 0, 0);
+        }
+    }
+    const int q = 16 * wave + (lane & 15);
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int p = 16 * rb + (lane >> 4) + 4 * v;
+            if (p < mi && q < mi) Aii[p + (size_t)q * lda] -= acc[rb][v];
+        }
 }
 
 // ---- two column tiles per workgroup (forward substitution) ----
@@ -407,9 +442,11 @@ __device__ __forceinline__ void pick(const DenseDesc &d, int which, double *&A, 
     }
 }
 
-// ---- left-looking block Cholesky, step k ----
-// diagonal tile: A_kk -= L_kw L_kw^T over the band window w, then factor the
-// tile and invert its factor (one workgroup per subdomain)
+// ---- block Cholesky, step k ----
+// diagonal tile: factor the tile and invert its factor (one workgroup per
+// subdomain).  Its updates by the block columns to its left, A_kk -= L_kj
+// L_kj^T, were applied by the panel steps j that produced L_kj, so this
+// latency-bound chain of launches carries no GEMM of its own.
 __global__ void __launch_bounds__(256) k_potrf_diag(const DenseDesc *D, int which, int step) {
     const DenseDesc &d = D[blockIdx.x];
     double *A, *B;
@@ -418,13 +455,7 @@ __global__ void __launch_bounds__(256) k_potrf_diag(const DenseDesc *D, int whic
     const int k0 = step * TILE;
     if (k0 >= n) return;
     const int nb = min(TILE, n - k0);
-    const int j0 = max(0, step - band) * TILE;
     double *Akk = A + k0 + (size_t)k0 * lda;
-    if (k0 > j0) {
-        const double *Lk = A + k0 + (size_t)j0 * lda;
-        tile_gemm<false, true>(Akk, lda, Lk, lda, Lk, lda, nb, nb, k0 - j0, -1.0, 1.0);
-        __syncthreads();
-    }
     __shared__ double T[TILE][TILE + 1];
     __shared__ double xd[TILE];
     __shared__ double tmp[3][16][17];
@@ -570,7 +601,8 @@ __global__ void __launch_bounds__(256) k_potrf_diag(const DenseDesc *D, int whic
 }
 
 // column below the diagonal: A_ik = (A_ik - L_iw L_kw^T) inv(L_kk)^T for
-// step < i <= step + band, window w = [max(0, i - band), step)
+// step < i <= step + band, window w = [max(0, i - band), step) (left-looking),
+// then A_ii -= L_ik L_ik^T (right-looking for the diagonal tiles)
 __global__ void __launch_bounds__(256) k_potrf_panel(const DenseDesc *D, int which, int step) {
     int bx, by;
     xcd_tile(bx, by);
@@ -590,7 +622,10 @@ __global__ void __launch_bounds__(256) k_potrf_panel(const DenseDesc *D, int whi
     if (k0 > j0)
         strip_gemm<false, true>(acc, lds, A + k0 + (size_t)j0 * lda, lda, A + i0 + (size_t)j0 * lda, lda, nb, mi,
                                 k0 - j0);
-    strip_finish<true>(acc, lds, C, lda, nb, mi, d.Dinv + (size_t)step * TILE * TILE, 1.0);
+    d4 o[4];
+    strip_finish<true, true>(acc, lds, C, lda, nb, mi, d.Dinv + (size_t)step * TILE * TILE, 1.0, o);
+    // right-looking update of the diagonal tile of row block i by this column
+    diag_update(o, lds, A + i0 + (size_t)i0 * lda, lda, mi);
 }
 
 // ---- left-looking forward substitution X = L^-1 B, block row i ----
