@@ -172,8 +172,11 @@ def main():
         return
 
     log(f"bench: workload {args.workload}, method {method}, F_i {args.sc}, S_D {args.sd}; cold steps")
+    first = None
     for w in range(args.warmup):
         g, st, t = cold(args.sc, args.sd)
+        if first is None:
+            first = t
         log(f"bench: warmup {w}: {t:.3f} s (build {st['time_build']:.2f} s excluded; load {st['time_load']:.3f}, "
             f"assembly {st['time_assembly']:.3f}, coarse {st['time_coarse']:.3f}, pcg {st['time_pcg']:.3f} / "
             f"{st['iterations']} it, primal {st['time_primal']:.3f})")
@@ -239,15 +242,17 @@ def main():
     symv_items = {k: v for k, v in zip(("fp64", "fp32", "bf16"), g.symv_items()) if v}
     g.close()
 
+    # HBM bytes per F SYMV launch from the FETCH_SIZE / WRITE_SIZE counter runs
+    # of an earlier profile (counters cannot be read inside this run)
     traffic, traffic_src = None, None
     tfile = os.path.join(ROOT, "profiles", "symv_traffic.json")
     if os.path.exists(tfile):
         try:
-            tr = json.load(open(tfile))
-            if tr.get("workload") == args.workload and tr.get("sc") == args.sc:
-                traffic = tr.get("bytes_per_launch")
-                traffic_src = tr.get("source")
-        except (OSError, ValueError):
+            for tr in json.load(open(tfile)).get("entries", []):
+                if tr.get("workload") == args.workload and tr.get("sc") == args.sc:
+                    traffic = tr.get("bytes_per_launch")
+                    traffic_src = "rocprofv3 FETCH_SIZE + WRITE_SIZE runs, " + tr.get("source", "")
+        except (OSError, ValueError, AttributeError):
             traffic = None
 
     asm_tflops = st["assembly_flops"] / st["time_assembly"] / 1e12
@@ -311,9 +316,13 @@ def main():
             "precision": 1e-8,
             "symv_item_elements": symv_items,
         },
-        "timed": ("cold: espreso_feti_set_subdomain data to the primal solution on the device (input "
-                  "preprocessing and upload, assembly, coarse problem, PCG, primal recovery); the structured-grid "
-                  "builder that produces the inputs is measured apart (inputs_s.host_build)"),
+        "timed": ("cold: a new instance from the espreso_feti_set_subdomain data to the primal solution on the "
+                  "device (handoff of the inputs, their preprocessing and upload, assembly, coarse problem, PCG, "
+                  "primal recovery); the structured-grid builder that produces the inputs is measured apart "
+                  "(inputs_s.host_build). The process keeps the assembly workspace once allocated and lends it "
+                  "to later instances; the first solve of the process also pays that allocation and the loading "
+                  "of the kernels (first_solve_in_process_s)"),
+        "first_solve_in_process_s": None if first is None else round(first, 6),
         "iterations": st["iterations"],
         "phases_s": phases(st),
         "inputs_s": {"host_build": round(st["time_build"], 4),
