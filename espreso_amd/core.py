@@ -143,6 +143,7 @@ SIGNATURES = {
     "espreso_feti_apply_F": (C.c_int, [_H, _D, _D]),
     "espreso_feti_apply_prec": (C.c_int, [_H, _D, _D]),
     "espreso_feti_project": (C.c_int, [_H, _D, _D]),
+    "espreso_feti_project_device": (C.c_int, [_H, C.c_void_p, C.c_void_p, C.c_void_p]),
     "espreso_feti_vec_b": (C.c_int, [_H, _D]),
     "espreso_feti_subdomain_dense": (C.c_int, [_H, C.c_int32, C.c_int32, _D]),
     "espreso_feti_subdomain_boundary": (C.c_int, [_H, C.c_int32, _I, _I]),
@@ -486,6 +487,13 @@ class FETISolver:
 
     def project(self, x):
         return self._op("espreso_feti_project", x)
+
+    def project_device(self, x_ptr, y_ptr, stream=None):
+        """y = P x on device vectors (addresses of n_lambda doubles), launched
+        on `stream` (hipStream_t address, None: the instance's stream) and not
+        waited for; allocates nothing (espreso_feti_project_device)."""
+        _check(lib().espreso_feti_project_device(self.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr),
+                                                 C.c_void_p(stream) if stream else None))
 
     def vec_b(self):
         b = np.zeros(self.n_lambda)

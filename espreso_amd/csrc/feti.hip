@@ -2772,6 +2772,48 @@ int espreso_feti_apply_F(espreso_feti *h, const double *x, double *y) { return d
 int espreso_feti_apply_prec(espreso_feti *h, const double *x, double *y) { return dual_op(h, x, y, 1); }
 int espreso_feti_project(espreso_feti *h, const double *x, double *y) { return dual_op(h, x, y, 2); }
 
+// ---- allocation-free projector ----
+// Scratch of espreso_feti_project_device: the coarse vectors of one
+// projection (G x per subdomain, G x per coarse block, (G G^T)^-1 G x).  Each
+// GPU the library runs on holds its own copy for the life of the process;
+// kernels get its device address (hipGetSymbolAddress, once per device),
+// never the name.  Calls that overlap on one GPU share it and race.
+constexpr int64_t PROJECT_SCRATCH_DOUBLES = (int64_t)1 << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[PROJECT_SCRATCH_DOUBLES];
+
+static double *scratch_address(int device) {
+    static std::mutex mu;
+    static std::map<int, double *> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto f = addr.find(device);
+    if (f != addr.end()) return f->second;
+    void *p = nullptr;
+    ESP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ESP_CHECK(p != nullptr, ESPRESO_ERR_HIP, "no device address for the projector scratch");
+    addr[device] = (double *)p;
+    return (double *)p;
+}
+
+int espreso_feti_project_device(espreso_feti *h, const double *x, double *y, void *stream) {
+    ESP_API_BEGIN
+    ESP_CHECK(h && h->ready, ESPRESO_ERR_STATE, "setup not done");
+    ESP_CHECK(x && y, ESPRESO_ERR_ARG, "null vector");
+    ESP_HIP(hipSetDevice(h->opt.device));
+    const int64_t need = h->n_dsub + 2 * (int64_t)h->ncoarse;
+    ESP_CHECK(need <= PROJECT_SCRATCH_DOUBLES, ESPRESO_ERR_ARG,
+              "coarse space too large for the allocation-free projector: it needs " + std::to_string(need) +
+                  " doubles of scratch, the limit is " + std::to_string(PROJECT_SCRATCH_DOUBLES) +
+                  " (use espreso_feti_project)");
+    double *sp = scratch_address(h->opt.device);
+    DualDev E = h->D;  // the instance's operators with the scratch in place of its coarse vectors
+    E.dsub = sp;
+    E.cd = sp + h->n_dsub;
+    E.ce = sp + h->n_dsub + h->ncoarse;
+    launch_project(E, x, y, 0, stream ? (hipStream_t)stream : h->st);
+    h->check_launches("projector");
+    ESP_API_END
+}
+
 int espreso_feti_vec_b(espreso_feti *h, double *b) {
     ESP_API_BEGIN
     ESP_CHECK(h && h->ready, ESPRESO_ERR_STATE, "setup not done");

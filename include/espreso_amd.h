@@ -285,6 +285,16 @@ int espreso_feti_dual(espreso_feti *h, double *lambda);
 int espreso_feti_apply_F(espreso_feti *h, const double *x, double *y);
 int espreso_feti_apply_prec(espreso_feti *h, const double *x, double *y);
 int espreso_feti_project(espreso_feti *h, const double *x, double *y);
+/* y = P x (as espreso_feti_project) on device vectors x, y of length n_lambda,
+ * launched on `stream` (a hipStream_t; NULL: the instance's stream) without
+ * waiting for it.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * It allocates nothing: its scratch (G x per subdomain and per coarse block,
+ * (G G^T)^-1 G x: sum of the kernel dimensions + 2 x the coarse dimension
+ * doubles) is a static device array of 2^20 doubles (8 MiB) per GPU, shared by
+ * every call on that GPU, so calls that overlap on one GPU race.  A larger
+ * coarse space is refused with ESPRESO_ERR_ARG. */
+int espreso_feti_project_device(espreso_feti *h, const double *x, double *y, void *stream);
 int espreso_feti_vec_b(espreso_feti *h, double *b);
 /* SYMV work-item sizes in matrix elements per workgroup, targets[3]: FP64,
  * FP32 and BF16 storage (0 where neither F_i nor S_D is stored in that
