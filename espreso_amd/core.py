@@ -50,7 +50,7 @@ class GridProblem(C.Structure):
         ("material_elem", C.POINTER(C.c_double)),
         ("slab_elements", C.POINTER(C.c_int32)),
         ("device_assembly", C.c_int32),
-        ("pad_", C.c_int32),
+        ("b0_type", C.c_int32),
     ]
 
 
@@ -257,6 +257,7 @@ def grid_problem(spec):
     p.young_seed = spec.get("young_seed", 0)
     p.conductivity_seed = spec.get("conductivity_seed", 0)
     p.device_assembly = int(bool(spec.get("device_assembly", False)))
+    p.b0_type = {"corners": 0, "kernels": 1}[spec.get("b0", "corners")]
     keep = []  # arrays the struct points to must outlive it
     ye = spec.get("material_elem")
     if ye is not None:

@@ -1004,41 +1004,46 @@ __global__ void __launch_bounds__(256) k_materialize(const DenseDesc *D, const i
     }
 }
 
-// ---- hybrid FETI: corner columns of F_i (lower triangle of S) ----
-// FB0[j + q m] = sign_q F(j, j_q);  Fcc[q1 + q2 nq] = sign_q1 FB0[j_q1 + q2 m]
-__global__ void k_extract_corners(const DenseDesc *D, const int *nq, const int64_t *qoff, const int *qj,
-                                  const double *qsign, const int64_t *fb0off, const int64_t *fccoff, double *FB0,
-                                  double *Fcc) {
+// ---- hybrid FETI: F_i B0_i^T and B0_i F_i B0_i^T (S holds F_i) ----
+// Slot q of a subdomain is one B0 row touching it, with entries (boundary
+// index j_e, coefficient c_e): FB0[j + q m] = sum_e c_e F(j, j_e).  With
+// `sym` S holds F in both triangles (column j_e read contiguously), else in
+// its lower triangle only (one entry per slot: corners).
+__global__ void k_b0_columns(const DenseDesc *D, const int *nq, const int64_t *qoff, const int64_t *qe_ptr,
+                             const int *qe_j, const double *qe_c, const int64_t *fb0off, double *FB0, int sym) {
     const DenseDesc &d = D[blockIdx.y];
     const int b = blockIdx.y, m = d.m, n = nq[b];
-    const int *jq = qj + qoff[b];
-    const double *sg = qsign + qoff[b];
     double *fb = FB0 + fb0off[b];
     const int64_t tot = (int64_t)m * n;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
         const int j = (int)(e % m), q = (int)(e / m);
-        const int c = jq[q];
-        const double v = j >= c ? d.S[j + (size_t)c * m] : d.S[c + (size_t)j * m];
-        fb[e] = sg[q] * v;
-    }
-    if (blockIdx.x == 0) {
-        double *fc = Fcc + fccoff[b];
-        for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-            const int q1 = e % n, q2 = e / n;
-            const int c1 = jq[q1], c2 = jq[q2];
-            const double v = c1 >= c2 ? d.S[c1 + (size_t)c2 * m] : d.S[c2 + (size_t)c1 * m];
-            fc[e] = sg[q1] * sg[q2] * v;
+        const int64_t slot = qoff[b] + q;
+        double v = 0.0;
+        for (int64_t k = qe_ptr[slot]; k < qe_ptr[slot + 1]; k++) {
+            const int c = qe_j[k];
+            v += qe_c[k] * ((sym || j >= c) ? d.S[j + (size_t)c * m] : d.S[c + (size_t)j * m]);
         }
+        fb[e] = v;
     }
 }
 
-void launch_extract_corners(const DenseDesc *desc, int nb, const int *nq, const int64_t *qoff, const int *qj,
-                            const double *qsign, const int64_t *fb0off, const int64_t *fccoff, double *FB0,
-                            double *Fcc, int max_m, int max_nq, hipStream_t st) {
-    unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(((int64_t)max_m * max_nq + 255) / 256, 1024));
-    hipLaunchKernelGGL(k_extract_corners, dim3(g, nb), dim3(256), 0, st, desc, nq, qoff, qj, qsign, fb0off, fccoff,
-                       FB0, Fcc);
+// Fcc[q1 + q2 nq] = sum over the entries e of slot q1 of c_e FB0[j_e + q2 m]
+__global__ void k_b0_block(const DenseDesc *D, const int *nq, const int64_t *qoff, const int64_t *qe_ptr,
+                           const int *qe_j, const double *qe_c, const int64_t *fb0off, const int64_t *fccoff,
+                           const double *FB0, double *Fcc) {
+    const DenseDesc &d = D[blockIdx.x];
+    const int b = blockIdx.x, m = d.m, n = nq[b];
+    const double *fb = FB0 + fb0off[b];
+    double *fc = Fcc + fccoff[b];
+    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+        const int q1 = e % n, q2 = e / n;
+        const int64_t slot = qoff[b] + q1;
+        double v = 0.0;
+        for (int64_t k = qe_ptr[slot]; k < qe_ptr[slot + 1]; k++) v += qe_c[k] * fb[qe_j[k] + (int64_t)q2 * m];
+        fc[e] = v;
+    }
 }
+
 
 // S(upper) = S(lower)^T for every descriptor
 __global__ void k_symmetrize_desc(const DenseDesc *D) {
@@ -1217,5 +1222,16 @@ void launch_materialize(const DenseDesc *desc, int nb, const int *rowptr, const 
                        vals, nz_off, iperm, perm, perm_off, f, n);
 }
 
+
+void launch_extract_b0(const DenseDesc *desc, int nb, const int *nq, const int64_t *qoff, const int64_t *qe_ptr,
+                       const int *qe_j, const double *qe_c, const int64_t *fb0off, const int64_t *fccoff, double *FB0,
+                       double *Fcc, int max_m, int max_nq, bool sym, hipStream_t st) {
+    if (sym) launch_symmetrize_desc(desc, nb, max_m, st);
+    const unsigned g = std::max<unsigned>(1, std::min<unsigned>(cdiv((int64_t)max_m * max_nq, 256), 4096));
+    hipLaunchKernelGGL(k_b0_columns, dim3(g, nb), dim3(256), 0, st, desc, nq, qoff, qe_ptr, qe_j, qe_c, fb0off, FB0,
+                       sym ? 1 : 0);
+    hipLaunchKernelGGL(k_b0_block, dim3(nb), dim3(256), 0, st, desc, nq, qoff, qe_ptr, qe_j, qe_c, fb0off, fccoff, FB0,
+                       Fcc);
+}
 
 }  // namespace esp

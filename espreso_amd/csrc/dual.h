@@ -93,18 +93,26 @@ struct ClusterDev {
     int *sfirst = nullptr, *scount = nullptr;        // consecutive subdomains of each cluster
     int64_t *rowoff = nullptr, *f0off = nullptr, *g0off = nullptr, *saoff = nullptr, *eoff = nullptr,
             *ncoff = nullptr;
-    int64_t *row_b1 = nullptr, *row_b2 = nullptr;   // global boundary index of the +1 / -1 entry of each row
+    // entries of each cluster row (global rows rowoff[c] + r): global boundary
+    // index and coefficient, CSR by row (g0 = B0 y_b)
+    int64_t *re_ptr = nullptr, *re_b = nullptr;
+    double *re_c = nullptr;
     double *F0inv = nullptr;   // n0 x n0 per cluster (full, symmetric)
     double *G0T = nullptr;     // n0 x kt per cluster: -(B0 R), i.e. G0^T
     double *H = nullptr;       // n0 x kt per cluster: F0^-1 G0^T
     double *Sainv = nullptr;   // kt x kt per cluster: (G0 F0^-1 G0^T + rho N N^T)^-1
     double *Nc = nullptr;      // kt x kn per cluster: orthonormal basis of ker G0^T (cluster rigid modes)
-    // per subdomain: B0 entries q (cluster-local row, FB0 column)
+    // per subdomain: slots q = the B0 rows touching it (global row qrow, FB0
+    // column q); the slot's entries (boundary index, coefficient) as a CSR over
+    // all slots: corner rows have one entry per slot, kernel rows a face's worth
     int *nq = nullptr, *qrow = nullptr;
     int64_t *qoff = nullptr, *fb0off = nullptr, *fccoff = nullptr, *asub = nullptr;
-    double *FB0 = nullptr;     // m x nq per subdomain: sign_q * F_i[:, j_q]
+    int64_t *qe_ptr = nullptr;
+    int *qe_j = nullptr;
+    double *qe_c = nullptr;
+    double *FB0 = nullptr;     // m x nq per subdomain: F_i B0_i^T, column q = sum_e c_e F_i[:, j_e]
     float *FB0f = nullptr;     // FB0 in FP32 (FP32/BF16 storage of F_i only)
-    double *Fcc = nullptr;     // nq x nq per subdomain: sign_q1 sign_q2 F_i[j_q1, j_q2]
+    double *Fcc = nullptr;     // nq x nq per subdomain: B0_i F_i B0_i^T
     double *lambda0 = nullptr, *alphaK = nullptr;
     double *rho_s = nullptr;   // per subdomain: max diag S_i (the scale of the kernel regularisation)
 };
@@ -201,8 +209,7 @@ void launch_superdir(const DualDev &D, int64_t n_near, const int64_t *sib_ptr, c
 void launch_kkt_primal(const PrimalDev &P, int nsub, const double *u, const double *f, const double *bt1,
                        const double *bt0, double *ub, double *out, hipStream_t st);
 // bt0 += B0^T lambda0 (C.lambda0) on the boundary (bt0 cleared by the caller)
-void launch_b0t(const DualDev &D, const ClusterDev &C, const int *qj, const double *qsign, double *bt0,
-                hipStream_t st);
+void launch_b0t(const DualDev &D, const ClusterDev &C, double *bt0, hipStream_t st);
 // u <- trunc(u D) / D (FETISolver::solve output truncation); partial: 256 doubles
 void launch_truncate(double *u, int64_t n, double precision, double *partial, hipStream_t st);
 

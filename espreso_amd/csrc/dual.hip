@@ -494,7 +494,11 @@ __global__ void __launch_bounds__(256) k_cluster_t(const ClusterDev C, const dou
     const int rb = blockIdx.x * 32;
     if (rb >= n0) return;  // uniform over the workgroup
     const int64_t ro = C.rowoff[c];
-    for (int r = threadIdx.x; r < n0; r += 256) g0[r] = yb[C.row_b1[ro + r]] - yb[C.row_b2[ro + r]];
+    for (int r = threadIdx.x; r < n0; r += 256) {
+        double v = 0.0;
+        for (int64_t e = C.re_ptr[ro + r]; e < C.re_ptr[ro + r + 1]; e++) v += C.re_c[e] * yb[C.re_b[e]];
+        g0[r] = v;
+    }
     __syncthreads();
     const double *F = C.F0inv + C.f0off[c];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -702,7 +706,9 @@ __global__ void __launch_bounds__(256) k_cluster_t_multi(const ClusterDev C, con
     const int64_t ro = C.rowoff[c];
     for (int e = threadIdx.x; e < n0 * kc; e += 256) {
         const int r = e / kc, v = e % kc;
-        g0[e] = yb[v * ystr + C.row_b1[ro + r]] - yb[v * ystr + C.row_b2[ro + r]];
+        double g = 0.0;
+        for (int64_t q = C.re_ptr[ro + r]; q < C.re_ptr[ro + r + 1]; q++) g += C.re_c[q] * yb[v * ystr + C.re_b[q]];
+        g0[e] = g;
     }
     __syncthreads();
     const double *F = C.F0inv + C.f0off[c];
@@ -1470,12 +1476,14 @@ __global__ void __launch_bounds__(256) k_kkt_primal(const PrimalDev P, const dou
 }
 
 // bt0 = B0^T lambda0 on the boundary: one thread per subdomain walks its B0
-// entries in order (a corner DOF may sit in several rows)
-__global__ void k_b0t(int nsub, const int64_t *boff, const int64_t *qoff, const int *qrow, const int *qj,
-                      const double *qsign, const double *lambda0, double *bt0) {
+// slots and their entries in order (a DOF may sit in several rows)
+__global__ void k_b0t(int nsub, const int64_t *boff, const ClusterDev C, double *bt0) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nsub) return;
-    for (int64_t q = qoff[s]; q < qoff[s + 1]; q++) bt0[boff[s] + qj[q]] += qsign[q] * lambda0[qrow[q]];
+    for (int64_t q = C.qoff[s]; q < C.qoff[s + 1]; q++) {
+        const double l0 = C.lambda0[C.qrow[q]];
+        for (int64_t e = C.qe_ptr[q]; e < C.qe_ptr[q + 1]; e++) bt0[boff[s] + C.qe_j[e]] += C.qe_c[e] * l0;
+    }
 }
 
 void launch_kkt_primal(const PrimalDev &P, int nsub, const double *u, const double *f, const double *bt1,
@@ -1483,10 +1491,8 @@ void launch_kkt_primal(const PrimalDev &P, int nsub, const double *u, const doub
     hipLaunchKernelGGL(k_kkt_primal, dim3(nsub), dim3(256), 0, st, P, u, f, bt1, bt0, ub, out);
 }
 
-void launch_b0t(const DualDev &D, const ClusterDev &C, const int *qj, const double *qsign, double *bt0,
-                hipStream_t st) {
-    hipLaunchKernelGGL(k_b0t, dim3((D.nsub + 255) / 256), dim3(256), 0, st, D.nsub, D.boff, C.qoff, C.qrow, qj, qsign,
-                       C.lambda0, bt0);
+void launch_b0t(const DualDev &D, const ClusterDev &C, double *bt0, hipStream_t st) {
+    hipLaunchKernelGGL(k_b0t, dim3((D.nsub + 255) / 256), dim3(256), 0, st, D.nsub, D.boff, C, bt0);
 }
 
 // ---- output truncation (FETISolver.cpp:131-148) ----

@@ -238,11 +238,9 @@ struct espreso_feti_s {
     std::vector<double> le_sign_h;
     std::vector<int> nq_h;
     std::vector<int64_t> qoff_h, fccoff_h;
-    std::vector<int> qrow_h, qj_h;                  // per B0 entry: cluster row, boundary index
-    std::vector<double> qsign_h;
+    std::vector<int> qrow_h;                        // per B0 slot: global cluster row
+    int64_t max_qe_h = 0;                           // entries in one B0 slot (1: corners)
     double *d_qj = nullptr, *d_Fcc = nullptr;       // (device) boundary index as double, corner blocks
-    int *d_qjint = nullptr;
-    double *d_qsign = nullptr;
     double *d_e0f = nullptr, *d_e0s = nullptr, *d_alpha_tot = nullptr;
     double *d_hf = nullptr, *d_ubf = nullptr, *d_fb = nullptr;  // boundary arrays
     double *d_x = nullptr, *d_r = nullptr, *d_w = nullptr, *d_y = nullptr, *d_z = nullptr, *d_p = nullptr,
@@ -1277,8 +1275,8 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
             int *d_nq = upload(nq);
             int64_t *d_qo = upload(qo), *d_fo = upload(fo), *d_co = upload(co);
             if (max_nq > 0)
-                launch_extract_corners(d_desc, nbt, d_nq, d_qo, d_qjint, d_qsign, d_fo, d_co, CL.FB0, d_Fcc, max_m,
-                                       max_nq, st);
+                launch_extract_b0(d_desc, nbt, d_nq, d_qo, CL.qe_ptr, CL.qe_j, CL.qe_c, d_fo, d_co, CL.FB0, d_Fcc, max_m,
+                                  max_nq, max_qe_h > 1, st);
             sync();
             for (void *p : {(void *)d_nq, (void *)d_qo, (void *)d_fo, (void *)d_co}) release(p);
         }
@@ -1744,7 +1742,7 @@ void espreso_feti_s::kkt(const double *lambda, espreso_feti_stats &s) {
     launch_boundary_gather(D, lambda, false, bt1, st);
     if (hybrid) {
         ESP_HIP(hipMemsetAsync(bt0, 0, nb * sizeof(double), st));
-        launch_b0t(D, CL, d_qjint, d_qsign, bt0, st);
+        launch_b0t(D, CL, bt0, st);
     }
     launch_kkt_primal(P, nsub, d_u, d_f, bt1, hybrid ? bt0 : nullptr, ub, sums, st);
     launch_lambda_gather(D, ub, false, yl, st);
@@ -2235,44 +2233,55 @@ void espreso_feti_s::load_clusters() {
         sfirst[c] = L[0];
         scount[c] = (int)L.size();
     }
-    std::vector<int64_t> rb1(rowoff[ncl], -1), rb2(rowoff[ncl], -1);
+    // slots: the B0 rows touching a subdomain, ascending, each with its
+    // entries (boundary index, coefficient) in input order; and the entries of
+    // every cluster row over all its subdomains (g0 = B0 y_b)
     nq_h.assign(nsub, 0);
     qoff_h.assign(nsub + 1, 0);
     std::vector<int64_t> fb0off(nsub + 1, 0);
     fccoff_h.assign(nsub + 1, 0);
     qrow_h.clear();
-    qj_h.clear();
-    qsign_h.clear();
+    std::vector<int64_t> qe_ptr(1, 0);
+    std::vector<int> qe_j;
+    std::vector<double> qe_c;
+    std::vector<std::vector<std::pair<int64_t, double>>> rowent(rowoff[ncl]);
+    max_qe_h = 0;
     for (int s = 0; s < nsub; s++) {
         const SubHost &h = sub[s];
         const int c = h.cluster;
-        nq_h[s] = (int)h.b0row.size();
-        ESP_CHECK(nq_h[s] <= 256, ESPRESO_ERR_ARG, "HYBRID_FETI: more than 256 B0 entries in one subdomain");
+        std::map<int, std::vector<std::pair<int, double>>> slots;
+        for (size_t q = 0; q < h.b0row.size(); q++) {
+            const int r = h.b0row[q];
+            ESP_CHECK(r >= 0 && r < n0_h[c], ESPRESO_ERR_ARG, "B0 row out of range");
+            slots[r].push_back({h.iperm[h.b0dof[q]] - h.ni, h.b0sign[q]});
+        }
+        nq_h[s] = (int)slots.size();
+        ESP_CHECK(nq_h[s] <= 256, ESPRESO_ERR_ARG, "HYBRID_FETI: more than 256 B0 rows touch one subdomain");
         qoff_h[s + 1] = qoff_h[s] + nq_h[s];
         fb0off[s + 1] = fb0off[s] + (int64_t)h.m * nq_h[s];
         fccoff_h[s + 1] = fccoff_h[s] + (int64_t)nq_h[s] * nq_h[s];
-        std::vector<char> seen(n0_h[c], 0);
-        for (int q = 0; q < nq_h[s]; q++) {
-            const int r = h.b0row[q];
-            ESP_CHECK(r >= 0 && r < n0_h[c], ESPRESO_ERR_ARG, "B0 row out of range");
-            ESP_CHECK(!seen[r], ESPRESO_ERR_ARG, "a B0 row may touch a subdomain only once");
-            seen[r] = 1;
-            const int j = h.iperm[h.b0dof[q]] - h.ni;
-            const int64_t row = rowoff[c] + r;
+        for (const auto &kv : slots) {
+            const int64_t row = rowoff[c] + kv.first;
             qrow_h.push_back((int)row);
-            qj_h.push_back(j);
-            qsign_h.push_back(h.b0sign[q]);
-            if (h.b0sign[q] > 0) {
-                ESP_CHECK(rb1[row] < 0, ESPRESO_ERR_ARG, "B0 row with two +1 entries");
-                rb1[row] = boff[s] + j;
-            } else {
-                ESP_CHECK(rb2[row] < 0, ESPRESO_ERR_ARG, "B0 row with two -1 entries");
-                rb2[row] = boff[s] + j;
+            for (const auto &e : kv.second) {
+                qe_j.push_back(e.first);
+                qe_c.push_back(e.second);
+                rowent[row].push_back({boff[s] + e.first, e.second});
             }
+            qe_ptr.push_back((int64_t)qe_j.size());
+            max_qe_h = std::max<int64_t>(max_qe_h, (int64_t)kv.second.size());
         }
     }
-    for (int64_t r = 0; r < rowoff[ncl]; r++)
-        ESP_CHECK(rb1[r] >= 0 && rb2[r] >= 0, ESPRESO_ERR_ARG, "each B0 row needs one +1 and one -1 entry");
+    std::vector<int64_t> re_ptr(rowoff[ncl] + 1, 0), re_b;
+    std::vector<double> re_c;
+    for (int64_t r = 0; r < rowoff[ncl]; r++) {
+        ESP_CHECK(!rowent[r].empty(), ESPRESO_ERR_ARG, "HYBRID_FETI: a B0 row without entries");
+        for (const auto &e : rowent[r]) {
+            re_b.push_back(e.first);
+            re_c.push_back(e.second);
+        }
+        re_ptr[r + 1] = (int64_t)re_b.size();
+    }
 
     // G0^T (n0 x kt per cluster) and the cluster kernel
     std::vector<int64_t> g0off(ncl + 1, 0), ncoff(ncl + 1, 0);
@@ -2295,8 +2304,9 @@ void espreso_feti_s::load_clusters() {
             const SubHost &h = sub[s];
             for (int64_t q = qoff_h[s]; q < qoff_h[s + 1]; q++) {
                 const int r = qrow_h[q] - (int)rowoff[c];
-                for (int a = 0; a < h.kdim; a++)
-                    G[r + (int64_t)(aoff + a) * n] -= qsign_h[q] * Rb_h[rboff_h[s] + qj_h[q] + (int64_t)a * h.m];
+                for (int64_t e = qe_ptr[q]; e < qe_ptr[q + 1]; e++)
+                    for (int a = 0; a < h.kdim; a++)
+                        G[r + (int64_t)(aoff + a) * n] -= qe_c[e] * Rb_h[rboff_h[s] + qe_j[e] + (int64_t)a * h.m];
             }
             aoff += h.kdim;
         }
@@ -2346,8 +2356,12 @@ void espreso_feti_s::load_clusters() {
     CL.g0off = upload(g0off);
     CL.saoff = upload(saoff_h);
     CL.ncoff = upload(ncoff);
-    CL.row_b1 = upload(rb1);
-    CL.row_b2 = upload(rb2);
+    CL.re_ptr = upload(re_ptr);
+    CL.re_b = upload(re_b);
+    CL.re_c = upload(re_c);
+    CL.qe_ptr = upload(qe_ptr);
+    CL.qe_j = upload(qe_j);
+    CL.qe_c = upload(qe_c);
     CL.G0T = upload(G0T);
     CL.H = alloc<double>(std::max<int64_t>(1, g0off[ncl]));
     CL.Nc = upload(Nall);
@@ -2366,8 +2380,6 @@ void espreso_feti_s::load_clusters() {
     CL.lambda0 = alloc<double>(n_lambda0);
     CL.alphaK = alloc<double>(asubv[nsub]);
     CL.rho_s = alloc<double>(nsub);
-    d_qjint = upload(qj_h);
-    d_qsign = upload(qsign_h);
     d_Fcc = CL.Fcc;
 
     // coarse space: one block per cluster, basis Rt_s = R_b,s N_s
@@ -2609,7 +2621,7 @@ int espreso_feti_set_cluster(espreso_feti *h, int32_t i, int32_t cluster, int32_
     s.b0sign.assign(sign, sign + n0);
     for (int q = 0; q < n0; q++) {
         ESP_CHECK(s.b0dof[q] >= 0 && s.b0dof[q] < s.n, ESPRESO_ERR_ARG, "B0 DOF out of range");
-        ESP_CHECK(s.b0sign[q] == 1.0 || s.b0sign[q] == -1.0, ESPRESO_ERR_ARG, "B0 values must be +1 or -1");
+        ESP_CHECK(std::isfinite(s.b0sign[q]), ESPRESO_ERR_NUMERIC, "non-finite B0 value");
         ESP_CHECK(s.b0row[q] >= 0, ESPRESO_ERR_ARG, "B0 row must be non-negative");
     }
     ESP_API_END

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <tuple>
 
 #include "grid.h"
 
@@ -453,6 +454,69 @@ void grid_build(const espreso_grid_problem &p, int scaling, GridProblem &out, bo
     // consecutive pairs of the cluster's owners, one row per DOF.
     const int ncl = p.clusters[0] * p.clusters[1] * p.clusters[2];
     for (int s = 0; s < ns; s++) out.sub[s].cluster = s / B.dpc;
+    if (p.b0_type == 1) {
+        // ---- B0 KERNELS (equalityconstraints.cpp:425-502; face pairs as fetidata.cpp:40-140) ----
+        // Every pair (d, nd), d < nd, of subdomains of one cluster that share a
+        // face gives one row per kernel column c of the master (the pair's
+        // subdomain with more kernel columns, d on a tie): sum over the face
+        // nodes n (ascending global id) and components k of
+        // R_master(n, k, c) (u_d(n, k) - u_nd(n, k)).  Rows per cluster in the
+        // order of the pairs.
+#pragma omp parallel for schedule(dynamic)
+        for (int cl = 0; cl < ncl; cl++) {
+            std::vector<std::pair<int, int>> pairs;  // (d, axis of the face towards nd)
+            for (int d = cl * B.dpc; d < (cl + 1) * B.dpc && d < ns; d++) {
+                int S[3];
+                B.coord_of(d, S);
+                for (int a = 0; a < 3; a++) {
+                    int T[3] = {S[0], S[1], S[2]};
+                    T[a]++;
+                    if (T[a] >= B.NS[a] || T[a] / p.domains[a] != S[a] / p.domains[a]) continue;  // other cluster
+                    pairs.push_back({d, a});
+                }
+            }
+            std::vector<std::tuple<int, int, int>> pl;  // (d, nd, axis), sorted by (d, nd)
+            for (auto &pr : pairs) {
+                int S[3];
+                B.coord_of(pr.first, S);
+                S[pr.second]++;
+                pl.emplace_back(pr.first, B.id_of(S[0], S[1], S[2]), pr.second);
+            }
+            std::sort(pl.begin(), pl.end());
+            int row = 0;
+            for (auto &t : pl) {
+                const int d = std::get<0>(t), nd_ = std::get<1>(t), a = std::get<2>(t);
+                const int master = out.sub[nd_].kdim > out.sub[d].kdim ? nd_ : d;
+                const GridSubdomain &M = out.sub[master];
+                int S[3];
+                B.coord_of(d, S);
+                int lo[3], hi[3];
+                for (int q = 0; q < 3; q++) {
+                    lo[q] = B.off[q][S[q]];
+                    hi[q] = B.off[q][S[q] + 1];
+                }
+                lo[a] = hi[a];  // the face between d and nd
+                for (int c = 0; c < M.kdim; c++) {
+                    for (int side = 0; side < 2; side++) {
+                        const int s2 = side ? nd_ : d;
+                        const double sg = side ? -1.0 : 1.0;
+                        GridSubdomain &sd = out.sub[s2];
+                        int g3[3];
+                        for (g3[2] = lo[2]; g3[2] <= hi[2]; g3[2]++)
+                            for (g3[1] = lo[1]; g3[1] <= hi[1]; g3[1]++)
+                                for (g3[0] = lo[0]; g3[0] <= hi[0]; g3[0]++)
+                                    for (int k = 0; k < nd; k++) {
+                                        sd.b0row.push_back(row + c);
+                                        sd.b0dof.push_back(local_dof(s2, g3, k));
+                                        sd.b0sign.push_back(sg * M.R[local_dof(master, g3, k) + (size_t)c * M.n]);
+                                    }
+                    }
+                }
+                row += M.kdim;
+            }
+        }
+        return;
+    }
 #pragma omp parallel for schedule(dynamic)
     for (int cl = 0; cl < ncl; cl++) {
         const int cc[3] = {cl % p.clusters[0], (cl / p.clusters[0]) % p.clusters[1], cl / (p.clusters[0] * p.clusters[1])};

@@ -119,9 +119,9 @@ void launch_band_extract(const DenseDesc *desc, int nb, double *band, const int6
 void launch_unpack(const void *pool, int prec, int64_t off, int m, float scale, double *out, hipStream_t st);
 // hybrid FETI: FB0 = F_i B0_i^T (m x nq, signed corner columns) and the
 // corner block B0_i F_i B0_i^T (nq x nq), from F_i in the lower triangle of S
-void launch_extract_corners(const DenseDesc *desc, int nb, const int *nq, const int64_t *qoff, const int *qj,
-                            const double *qsign, const int64_t *fb0off, const int64_t *fccoff, double *FB0,
-                            double *Fcc, int max_m, int max_nq, hipStream_t st);
+void launch_extract_b0(const DenseDesc *desc, int nb, const int *nq, const int64_t *qoff, const int64_t *qe_ptr,
+                       const int *qe_j, const double *qe_c, const int64_t *fb0off, const int64_t *fccoff, double *FB0,
+                       double *Fcc, int max_m, int max_nq, bool sym, hipStream_t st);
 
 // diagnostics (dense.hip)
 void launch_probe_mfma(const double *a, const double *b, double *out, hipStream_t st);

@@ -79,7 +79,10 @@ typedef struct {
                                    the GPU from the unit element matrix and the element
                                    materials (the host builds structure, loads, kernels
                                    and B1/B0 only); 0 = host-assembled values         */
-    int32_t pad_;
+    int32_t b0_type;            /* HYBRID_FETI B0: 0 = CORNERS (corner nodes, one row per
+                                   DOF), 1 = KERNELS (per face pair of a cluster, one row
+                                   per kernel column of the master subdomain; the
+                                   reference's default, src/config/ecf/solver/feti.cpp:90) */
 } espreso_grid_problem;
 
 /* Values follow the reference enums (src/config/ecf/solver/feti.h). */
@@ -260,12 +263,14 @@ int espreso_feti_resolve_block(espreso_feti *h, int32_t nrhs, const double *f, d
 /* sum over subdomains of f_i^T u_i for the last solution (the compliance
  * f^T u of the assembled problem)                                            */
 int espreso_feti_load_work(espreso_feti *h, double *c);
-/* Hybrid FETI (options.method = 1): the cluster of subdomain i and the B0
- * rows touching it (cluster-local row id, local DOF, +1/-1); every B0 row
- * holds one +1 and one -1 entry of two subdomains of the same cluster, and
- * the subdomains of a cluster are numbered consecutively.  Replaces
- * Constraints::B0Corners (equalityconstraints.cpp:504-552) and
- * ClusterBase::CompressB0 (cluster.cpp:1186-1224) inputs.                    */
+/* Hybrid FETI (options.method = 1): the cluster of subdomain i and its B0
+ * entries (cluster-local row id, local DOF, value), any number per row; the
+ * subdomains of a cluster are numbered consecutively, at most 256 B0 rows
+ * touch one subdomain, and every row has entries.  Corner rows (one +1 and
+ * one -1 entry: Constraints::B0Corners, equalityconstraints.cpp:504-552) and
+ * kernel rows (R_master-weighted face DOFs of two subdomains:
+ * Constraints::B0Kernels, equalityconstraints.cpp:425-502) both fit.
+ * Replaces the ClusterBase::CompressB0 (cluster.cpp:1186-1224) inputs.       */
 int espreso_feti_set_cluster(espreso_feti *h, int32_t i, int32_t cluster, int32_t n0, const int32_t *row,
                              const int32_t *dof, const double *sign);
 

@@ -8,6 +8,8 @@ restates the reference's hybrid path:
 
   corners()        corner nodes   src/mesh/preprocessing/fetidata.cpp:146-187
                    B0 rows        src/assembler/constraints/equalityconstraints.cpp:504-552
+  kernel_rows()    face pairs     src/mesh/preprocessing/fetidata.cpp:40-140
+                   B0 KERNELS     src/assembler/constraints/equalityconstraints.cpp:425-502
   HybridFETI       G0             src/solver/specific/cluster.cpp:1226-1291
                    F0             cluster.cpp:1293-1478
                    Sa             cluster.cpp:1480-2016 (regularised on the cluster kernel)
@@ -72,6 +74,60 @@ def corners(l2g, ndof, dpc):
                             out[s][2].append(pos[s][g] * ndof + c)
                             out[s][3].append(sg)
                         row += 1
+    return [(c, np.array(r, np.int64), np.array(d, np.int64), np.array(v)) for c, r, d, v in out]
+
+
+def face_pairs(l2g, dpc, NN):
+    """Pairs (d, nd), d < nd, of subdomains of one cluster that share a face,
+    with the shared face nodes ascending (fetidata.cpp:40-140: the nodes of
+    the element faces of d whose nodes are all shared with nd).  On the
+    structured grid (NN nodes per direction) the nodes two boxes share form a
+    full rectangle of nodes when they share a face, a line or a point when
+    they share only an edge or a corner; a rectangle spans two directions."""
+    sets = [set(int(g) for g in L) for L in l2g]
+    out = []
+    for d in range(len(l2g)):
+        for nd in range(d + 1, len(l2g)):
+            if d // dpc != nd // dpc:
+                continue
+            shared = sorted(sets[d] & sets[nd])
+            if not shared:
+                continue
+            g = np.array(shared)
+            xyz = np.stack([g % NN[0], (g // NN[0]) % NN[1], g // (NN[0] * NN[1])], axis=1)
+            if int((xyz.max(axis=0) > xyz.min(axis=0)).sum()) == 2:
+                out.append((d, nd, shared))
+    return out
+
+
+def kernel_rows(l2g, R, ndof, dpc, NN):
+    """B0 entries of the KERNELS variant (equalityconstraints.cpp:425-502),
+    the reference's default B0 type (src/config/ecf/solver/feti.cpp:90).
+
+    For every face pair (d, nd) of a cluster (face_pairs), the master is the
+    subdomain of the pair with more kernel columns (d on a tie); each kernel
+    column c of the master gives one row: sum over the face nodes n and
+    components k of R_master(n, k, c) (u_d(n, k) - u_nd(n, k)), i.e. +R on d's
+    copies and -R on nd's.  Rows are numbered per cluster in the order of
+    the pairs (d ascending, then nd).  R[s] is n_s x k_s, local DOF = node *
+    ndof + component.  Returns a list of (cluster, rows, dofs, values) per
+    subdomain, in the layout of corners()."""
+    nsub = len(l2g)
+    pos = [{int(g): i for i, g in enumerate(L)} for L in l2g]
+    out = [(s // dpc, [], [], []) for s in range(nsub)]
+    nrow = {}
+    for d, nd, shared in face_pairs(l2g, dpc, NN):
+        cl = d // dpc
+        master = nd if R[nd].shape[1] > R[d].shape[1] else d
+        r0 = nrow.get(cl, 0)
+        for c in range(R[master].shape[1]):
+            for s, sg in ((d, 1.0), (nd, -1.0)):
+                for g in shared:
+                    for k in range(ndof):
+                        out[s][1].append(r0 + c)
+                        out[s][2].append(pos[s][g] * ndof + k)
+                        out[s][3].append(sg * R[master][pos[master][g] * ndof + k, c])
+        nrow[cl] = r0 + R[master].shape[1]
     return [(c, np.array(r, np.int64), np.array(d, np.int64), np.array(v)) for c, r, d, v in out]
 
 

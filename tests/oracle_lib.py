@@ -391,10 +391,11 @@ def cpu_baseline(spec, sample, threads=0):
                 iteration_rest=t[4], sample=int(t[5]))
 
 
-def hybrid(spec, dirichlet=False):
+def hybrid(spec, dirichlet=False, b0="corners"):
     """Numpy HTFETI restatement (oracle/htfeti.py) fed by the C oracle's
     decomposition: K^+ = K_reg^-1 (fix-point regularisation), analytic R, f,
-    B1, c, corner B0 from the subdomain node maps."""
+    B1, c, and B0 from the subdomain node maps: corner rows (b0 "corners") or
+    the kernel rows of the face pairs ("kernels")."""
     from oracle import htfeti
     o = Oracle(spec, preconditioner=3 if dirichlet else 0, kplus=0)
     try:
@@ -412,7 +413,12 @@ def hybrid(spec, dirichlet=False):
     finally:
         o.close()
     dpc = spec["domains"][0] * spec["domains"][1] * spec["domains"][2]
-    B0 = htfeti.corners(l2g, ndof, dpc)
+    if b0 == "kernels":
+        from espreso_amd.core import slabs
+        NN = [sum(v) + 1 for v in slabs(spec)]
+        B0 = htfeti.kernel_rows(l2g, R, ndof, dpc, NN)
+    else:
+        B0 = htfeti.corners(l2g, ndof, dpc)
     H = htfeti.HybridFETI(Kp, R, f, B1, c, B0, SD if dirichlet else None)
     H.l2g = l2g
     H.ndof = ndof

@@ -9,7 +9,8 @@ solves the same workload here and each of those four differences must stay
 within 1e-8 relative (the primal-error bar of the FP64 path; the mixed
 storages reach it through the FP64 iterative refinement):
 
-  * 512 subdomains of 10^3 elements, HYBRID_FETI, Dirichlet, FP64
+  * 512 subdomains of 10^3 elements, HYBRID_FETI, Dirichlet, FP64, with B0
+    corner rows and with B0 kernel rows
   * 1000 subdomains of 12^3 elements, F_i FP32 and S_D BF16, TOTAL_FETI and
     HYBRID_FETI; and the FP64 TOTAL_FETI path
   * 2000 uneven subdomains (4..14 elements per direction), heat with seeded
@@ -67,6 +68,15 @@ def test_elasticity_512sub_10e3_htfeti_fp64():
     g, st = solve(PB.HTFETI_512x10, method="hybrid", preconditioner="dirichlet", precision=1e-10,
                   max_iterations=2000)
     check(fx, g.nodal_solution(), what="512sub hybrid fp64")
+    g.close()
+
+
+def test_elasticity_512sub_10e3_htfeti_kernel_rows():
+    say("case: elasticity_512sub_10e3 HYBRID_FETI with B0 kernel rows, dirichlet fp64, precision 1e-10")
+    fx = fixture("elasticity_512sub_10e3")
+    g, st = solve(dict(PB.HTFETI_512x10, b0="kernels"), method="hybrid", preconditioner="dirichlet",
+                  precision=1e-10, max_iterations=2000)
+    check(fx, g.nodal_solution(), what="512sub hybrid kernels fp64")
     g.close()
 
 

@@ -1,5 +1,6 @@
 """GPU parity of the hybrid FETI path (options.method = 1) through the C ABI.
 
+B0 of both types: corner rows and kernel rows (the reference's default).
 Against the numpy HTFETI restatement (oracle/htfeti.py): the projected dual
 operator P F~ P and the initial projected residual, which do not depend on
 the choice of generalised inverse (the GPU uses the boundary-kernel F_i, the
@@ -33,14 +34,16 @@ def rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.fixture(scope="module", params=sorted(SPECS))
+@pytest.fixture(scope="module", params=[(n, b0) for n in sorted(SPECS) for b0 in ("corners", "kernels")],
+                ids=lambda p: f"{p[0]}-{p[1]}")
 def pair(request):
-    spec = SPECS[request.param]
-    say("case: hybrid", request.param)
+    name, b0 = request.param
+    spec = dict(SPECS[name], b0=b0)
+    say("case: hybrid", name, "B0", b0)
     g = core.GridFETI(spec, method="hybrid", preconditioner="dirichlet", precision=1e-10)
     g.setup()
-    H = O.hybrid(spec, dirichlet=True)
-    yield request.param, spec, g, H
+    H = O.hybrid(spec, dirichlet=True, b0=b0)
+    yield name, spec, g, H
     g.close()
 
 
@@ -104,6 +107,24 @@ def test_clusters_of_eight_vs_global(sc, sd):
     # FP64 takes 50 iterations; a corner correction that does not cancel the
     # corner directions (BF16-rounded on both sides) took 1602
     assert st["iterations"] < 300
+    g.close()
+
+
+@pytest.mark.parametrize("sc,sd", [("fp64", "fp64"), ("fp32", "bf16")])
+def test_clusters_of_eight_kernel_rows(sc, sd):
+    """B0 KERNELS on 8 clusters of 2x2x2 subdomains: 12 face pairs x 6 rows per
+    cluster, each row a face's worth of weighted DOFs."""
+    spec = PB.scaled(PB.HTFETI_512x10, clusters=(2, 2, 2), elements=(5, 5, 5), b0="kernels")
+    say("case: hybrid 8 clusters, B0 kernels", sc, sd)
+    g = core.GridFETI(spec, method="hybrid", preconditioner="dirichlet", precision=1e-10, sc_precision=sc,
+                      sd_precision=sd)
+    g.setup()
+    st = g.solve()
+    err = rel(g.nodal_solution(), O.global_solve(spec))
+    say("hybrid kernels", sc, sd, "err", err, "iterations", st["iterations"], "coarse", st["n_coarse"])
+    assert err < 1e-8
+    assert st["n_coarse"] == 8 * 6
+    assert st["kkt_primal"] < 1e-7
     g.close()
 
 

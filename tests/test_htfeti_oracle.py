@@ -105,3 +105,86 @@ def test_hybrid_solve_vs_global(hyb, prec):
         T = np.concatenate([u[s] for s in range(H.nsub)])
         # golden HEXA8.HYBRID_FETI.emr: T in [100, 300]
         assert abs(T.min() - 100) < 1e-6 and abs(T.max() - 300) < 1e-6
+
+
+# ---- B0 KERNELS (equalityconstraints.cpp:425-502) ----
+# Pins: each row holds entries of exactly the two subdomains of one face
+# pair of a cluster, on the same nodes (the shared face) with opposite signs;
+# B0 u = 0 for every continuous nodal field and the rows are independent
+# (the face restriction of a kernel is full rank); the glued cluster keeps
+# exactly the rigid-body kernel; the hybrid primal equals the direct solve.
+@pytest.fixture(scope="module", params=sorted(SPECS))
+def hybk(request):
+    spec = SPECS[request.param]
+    return spec, O.hybrid(spec, dirichlet=True, b0="kernels")
+
+
+def test_kernel_rows_structure(hybk):
+    spec, H = hybk
+    rows = {}
+    for s, (cl, r, d, v) in enumerate(H.B0data):
+        for q in range(len(r)):
+            node = int(H.l2g[s][d[q] // H.ndof])
+            rows.setdefault((cl, int(r[q])), {}).setdefault(s, {})[(node, int(d[q] % H.ndof))] = v[q]
+    assert rows
+    kfull = 6 if H.ndof == 3 else 1
+    per_pair = {}
+    for (cl, r), by_sub in rows.items():
+        assert len(by_sub) == 2
+        (s1, e1), (s2, e2) = sorted(by_sub.items())
+        assert H.cluster[s1] == H.cluster[s2] == cl
+        assert set(e1) == set(e2)  # the same face nodes and components
+        assert all(e1[k] == -e2[k] for k in e1)
+        per_pair[(s1, s2)] = per_pair.get((s1, s2), 0) + 1
+    assert set(per_pair.values()) == {kfull}
+
+
+def test_kernel_rows_continuity(hybk):
+    spec, H = hybk
+    rng = np.random.default_rng(3)
+    nn = 1 + max(int(L.max()) for L in H.l2g)
+    for cl, subs in enumerate(H.subs):
+        B0 = np.hstack([H.cl[cl]["B0"][s] for s in subs])
+        g = rng.standard_normal(nn * H.ndof).reshape(nn, H.ndof)
+        u = np.concatenate([g[H.l2g[s]].ravel() for s in subs])
+        assert np.abs(B0 @ u).max() <= 1e-12 * np.abs(B0).max() * np.abs(u).max() * B0.shape[1]
+        assert np.linalg.matrix_rank(B0) == B0.shape[0]
+
+
+def test_kernel_rows_cluster_kernel(hybk):
+    spec, H = hybk
+    kfull = 6 if H.ndof == 3 else 1
+    for C in H.cl:
+        assert C["N"].shape[1] == kfull
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_kernel_rows_solve_vs_global(hybk, prec):
+    spec, H = hybk
+    u, lam, it = H.solve(precision=1e-10, preconditioner=prec)
+    ug = O.global_solve(spec)
+    ref = np.abs(ug).max()
+    for s in range(H.nsub):
+        assert np.abs(u[s].reshape(-1, H.ndof) - ug[H.l2g[s]]).max() < 1e-8 * ref, s
+
+
+def test_product_kernel_rows_match(hybk):
+    """The grid builder's kernel rows against the restatement's: the same
+    rows (pair by pair, in the same order) on the same DOFs, spanning the
+    same space (both builders may take different bases of the kernel)."""
+    spec, H = hybk
+    kspec = dict(spec, b0="kernels")
+    for s in range(H.nsub):
+        e = core.grid_export_cluster(kspec, s)
+        cl, r, d, v = H.B0data[s]
+        assert e["cluster"] == cl
+        assert np.array_equal(np.unique(e["row"]), np.unique(r))
+        for rows in np.split(np.unique(r), np.unique(r).size // (6 if H.ndof == 3 else 1)):
+            P = np.zeros((rows.size, H.n[s]))
+            Q = np.zeros((rows.size, H.n[s]))
+            for i, row in enumerate(rows):
+                mp, mq = e["row"] == row, r == row
+                P[i, e["dof"][mp]] = e["sign"][mp]
+                Q[i, d[mq]] = v[mq]
+            assert np.array_equal(np.nonzero(P.any(axis=0))[0], np.nonzero(Q.any(axis=0))[0])
+            assert np.linalg.matrix_rank(np.vstack([P, Q]), tol=1e-10 * np.abs(Q).max()) == rows.size
