@@ -151,7 +151,7 @@ SIGNATURES = {
     "espreso_debug_mfma": (C.c_int, [_D, _D, _D]),
     "espreso_debug_copy": (C.c_int, [C.c_int64, C.c_int32, _D]),
     "espreso_debug_stream": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, _D]),
-    "espreso_debug_mfma_rate": (C.c_int, [C.c_int32, C.c_int32, _D]),
+    "espreso_debug_mfma_rate": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _D]),
     "espreso_debug_tile_gemm": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _D, C.c_int32, _D,
                                           C.c_int32, _D, C.c_int32, C.c_double, C.c_double]),
 }
@@ -160,9 +160,19 @@ SIGNATURES = {
 def debug_mfma(a, b):
     a = np.ascontiguousarray(a, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
-    out = np.zeros(256)
+    out = np.zeros(320)
     _check(lib().espreso_debug_mfma(_dp(a), _dp(b), _dp(out)))
-    return out.reshape(64, 4)
+    return out[:256].reshape(64, 4)
+
+
+def debug_mfma4(a, b):
+    """One v_mfma_f64_4x4x4f64 (4 blocks) with lane l holding a[l], b[l]: the
+    result of each lane."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    out = np.zeros(320)
+    _check(lib().espreso_debug_mfma(_dp(a), _dp(b), _dp(out)))
+    return out[256:].copy()
 
 
 def debug_copy(nbytes, reps=5):
@@ -178,10 +188,11 @@ def debug_stream(nbytes, reps=5, workgroups=4096):
     return ms.value
 
 
-def debug_mfma_rate(shape=0, chains=8):
-    """FP64 MFMA issue rate in TFLOP/s (shape 0: 16x16x4, 1: 4x4x4 with 4 blocks)."""
+def debug_mfma_rate(shape=0, chains=8, workgroups_per_cu=0):
+    """FP64 MFMA issue rate in TFLOP/s (shape 0: 16x16x4, 1: 4x4x4 with 4 blocks);
+    workgroups_per_cu > 0 caps the occupancy (waves per SIMD)."""
     t = C.c_double()
-    _check(lib().espreso_debug_mfma_rate(shape, chains, C.byref(t)))
+    _check(lib().espreso_debug_mfma_rate(shape, chains, workgroups_per_cu, C.byref(t)))
     return t.value
 
 

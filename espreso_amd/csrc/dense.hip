@@ -18,6 +18,7 @@
 // chunks of 16.
 #include <hip/hip_runtime.h>
 
+
 #include "internal.h"
 
 namespace esp {
@@ -1069,23 +1070,66 @@ __global__ void k_probe_mfma(const double *a, const double *b, double *out) {
     for (int v = 0; v < 4; v++) out[lane * 4 + v] = acc[v];
 }
 
-__global__ void __launch_bounds__(256) k_probe_gemm(int ta, int tb, int M, int N, int K, const double *A, int lda,
-                                                    const double *B, int ldb, double *C, int ldc, double alpha,
-                                                    double beta) {
-    if (!ta && !tb) tile_gemm<false, false>(C, ldc, A, lda, B, ldb, M, N, K, alpha, beta);
-    else if (!ta && tb) tile_gemm<false, true>(C, ldc, A, lda, B, ldb, M, N, K, alpha, beta);
-    else if (ta && !tb) tile_gemm<true, false>(C, ldc, A, lda, B, ldb, M, N, K, alpha, beta);
-    else tile_gemm<true, true>(C, ldc, A, lda, B, ldb, M, N, K, alpha, beta);
+__global__ void k_probe_mfma4(const double *a, const double *b, double *out) {
+    const int lane = threadIdx.x;
+    out[lane] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[lane], b[lane], 0.0, 0, 0, 0);
+}
+
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) k_probe_gemm(int M, int N, int K, const double *A, int lda, const double *B,
+                                                    int ldb, double *C, int ldc, double alpha, double beta) {
+    tile_gemm<TA, TB>(C, ldc, A, lda, B, ldb, M, N, K, alpha, beta);
 }
 
 // FP64 MFMA issue rate: CH independent accumulator chains per wave, `iter`
 // steps each, register operands only (one store per lane at the end).  SHAPE 0:
-// v_mfma_f64_16x16x4f64 (2048 flops), 1: v_mfma_f64_4x4x4f64 with 4 blocks (512).
+// v_mfma_f64_16x16x4f64 (2048 flops), 1: v_mfma_f64_4x4x4f64 with 4 blocks (512),
+// both on one pair of operand registers.  SHAPE 2 (4x4x4) and 3 (16x16x4): the
+// operand grid of a register-blocked tile, CH A and CH B registers and CH x CH
+// accumulators, every MFMA on another operand pair.
 template <int SHAPE, int CH>
 __global__ void __launch_bounds__(256) k_mfma_rate(double *out, int iter) {
     const double a = 1.0 + threadIdx.x * 1e-9, b = 1.0 - threadIdx.x * 1e-9;
     double s = 0.0;
-    if constexpr (SHAPE == 0) {
+    if constexpr (SHAPE == 2) {
+        double av[CH], bv[CH], acc[CH][CH];
+#pragma unroll
+        for (int r = 0; r < CH; r++) {
+            av[r] = a + r * 1e-7;
+            bv[r] = b - r * 1e-7;
+#pragma unroll
+            for (int c = 0; c < CH; c++) acc[r][c] = 0.0;
+        }
+        for (int i = 0; i < iter; i++)
+#pragma unroll
+            for (int r = 0; r < CH; r++)
+#pragma unroll
+                for (int c = 0; c < CH; c++) acc[r][c] = __builtin_amdgcn_mfma_f64_4x4x4f64(av[r], bv[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < CH; r++)
+#pragma unroll
+            for (int c = 0; c < CH; c++) s += acc[r][c];
+    } else if constexpr (SHAPE == 3) {
+        double av[CH], bv[CH];
+        d4 acc[CH][CH];
+#pragma unroll
+        for (int r = 0; r < CH; r++) {
+            av[r] = a + r * 1e-7;
+            bv[r] = b - r * 1e-7;
+#pragma unroll
+            for (int c = 0; c < CH; c++) acc[r][c] = d4{0.0, 0.0, 0.0, 0.0};
+        }
+        for (int i = 0; i < iter; i++)
+#pragma unroll
+            for (int r = 0; r < CH; r++)
+#pragma unroll
+                for (int c = 0; c < CH; c++)
+                    acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[r], bv[c], acc[r][c], 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < CH; r++)
+#pragma unroll
+            for (int c = 0; c < CH; c++) s += (acc[r][c][0] + acc[r][c][1]) + (acc[r][c][2] + acc[r][c][3]);
+    } else if constexpr (SHAPE == 0) {
         d4 acc[CH];
 #pragma unroll
         for (int c = 0; c < CH; c++) acc[c] = d4{0.0, 0.0, 0.0, 0.0};
@@ -1107,21 +1151,42 @@ __global__ void __launch_bounds__(256) k_mfma_rate(double *out, int iter) {
     out[blockIdx.x * 256 + threadIdx.x] = s;
 }
 
-double mfma_rate_flops(int shape) { return shape == 0 ? 2048.0 : 512.0; }
+double mfma_rate_flops(int shape) { return (shape == 0 || shape == 3) ? 2048.0 : 512.0; }
 
-void launch_mfma_rate(int shape, int chains, int blocks, int iter, double *out, hipStream_t st) {
-    if (shape == 0 && chains == 4) hipLaunchKernelGGL((k_mfma_rate<0, 4>), dim3(blocks), dim3(256), 0, st, out, iter);
-    else if (shape == 0) hipLaunchKernelGGL((k_mfma_rate<0, 8>), dim3(blocks), dim3(256), 0, st, out, iter);
-    else if (chains == 4) hipLaunchKernelGGL((k_mfma_rate<1, 4>), dim3(blocks), dim3(256), 0, st, out, iter);
-    else hipLaunchKernelGGL((k_mfma_rate<1, 8>), dim3(blocks), dim3(256), 0, st, out, iter);
+void launch_mfma_rate(int shape, int chains, int blocks, int iter, int lds, double *out, hipStream_t st) {
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), lds, st, out, iter); };
+    if (shape == 2) {
+        if (chains == 2) go(k_mfma_rate<2, 2>);
+        else if (chains == 4) go(k_mfma_rate<2, 4>);
+        else go(k_mfma_rate<2, 8>);
+    } else if (shape == 3) {
+        if (chains == 2) go(k_mfma_rate<3, 2>);
+        else go(k_mfma_rate<3, 4>);
+    } else if (shape == 0) {
+        if (chains == 4) go(k_mfma_rate<0, 4>);
+        else if (chains == 8) go(k_mfma_rate<0, 8>);
+        else if (chains == 16) go(k_mfma_rate<0, 16>);
+        else go(k_mfma_rate<0, 32>);
+    } else {
+        if (chains == 4) go(k_mfma_rate<1, 4>);
+        else if (chains == 8) go(k_mfma_rate<1, 8>);
+        else if (chains == 16) go(k_mfma_rate<1, 16>);
+        else go(k_mfma_rate<1, 32>);
+    }
 }
 
 void launch_probe_mfma(const double *a, const double *b, double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_probe_mfma, dim3(1), dim3(64), 0, st, a, b, out);
+    hipLaunchKernelGGL(k_probe_mfma4, dim3(1), dim3(64), 0, st, a, b, out + 256);
 }
 void launch_probe_gemm(int ta, int tb, int M, int N, int K, const double *A, int lda, const double *B, int ldb,
                        double *C, int ldc, double alpha, double beta, hipStream_t st) {
-    hipLaunchKernelGGL(k_probe_gemm, dim3(1), dim3(256), 0, st, ta, tb, M, N, K, A, lda, B, ldb, C, ldc, alpha, beta);
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(1), dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc,
+                                                  alpha, beta); };
+    if (!ta && !tb) go(k_probe_gemm<false, false>);
+    else if (!ta && tb) go(k_probe_gemm<false, true>);
+    else if (ta && !tb) go(k_probe_gemm<true, false>);
+    else go(k_probe_gemm<true, true>);
 }
 
 // ------------------------------------------------------------ launchers ----

@@ -10,6 +10,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
+#include <thread>
+#include <omp.h>
 #include <map>
 #include <mutex>
 #include <cstdlib>
@@ -33,6 +37,30 @@ thread_local std::string g_last_error;
 double wall() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
+
+// std::vector storage without value-initialisation: the large host arrays of
+// the set-up (R_b, GG^T, the lambda maps) are written in full by parallel
+// loops, and zeroing them first would touch every page on one thread
+template <class T>
+struct NoInit : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInit<U>;
+    };
+    NoInit() = default;
+    template <class U>
+    NoInit(const NoInit<U> &) noexcept {}
+    template <class U>
+    void construct(U *p) noexcept {
+        ::new ((void *)p) U;
+    }
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+};
+template <class T>
+using hvec = std::vector<T, NoInit<T>>;
 
 struct SubHost {
     int n = 0, ni = 0, m = 0, kdim = 0;
@@ -231,11 +259,11 @@ struct espreso_feti_s {
     int ncl = 0;
     std::vector<std::vector<int>> cl_subs;          // subdomains of each cluster
     std::vector<int64_t> asubv;                     // kernel amplitude offsets per subdomain
-    std::vector<double> Rb_h, e0f_h;                // host copies: R_b, -R_s^T f_s
+    hvec<double> Rb_h, e0f_h;                       // host copies: R_b, -R_s^T f_s
     std::vector<int64_t> rboff_h;
-    std::vector<int> le_cnt_h, le_sub_h;
-    std::vector<int64_t> le_b_h;
-    std::vector<double> le_sign_h;
+    hvec<int> le_cnt_h, le_sub_h;
+    hvec<int64_t> le_b_h;
+    hvec<double> le_sign_h;
     std::vector<int> nq_h;
     std::vector<int64_t> qoff_h, fccoff_h;
     std::vector<int> qrow_h;                        // per B0 slot: global cluster row
@@ -281,7 +309,26 @@ struct espreso_feti_s {
             throw Error(ESPRESO_ERR_HIP, std::string(where) + ": kernel launch failed: " + hipGetErrorString(e));
     }
 
+    // TOTAL_FETI: GG^T is built and uploaded by a host thread while the GPU
+    // assembles the local operators (it needs only R_b and the lambda maps);
+    // coarse() joins it before reading d_GGt
+    std::thread ggt_thread;
+    std::exception_ptr ggt_err;
+    hipStream_t ggt_st = nullptr;
+    hipEvent_t ggt_ev = nullptr;
+    void join_ggt() {
+        if (ggt_thread.joinable()) ggt_thread.join();
+        if (ggt_err) {
+            std::exception_ptr e = ggt_err;
+            ggt_err = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
+
     ~espreso_feti_s() {
+        if (ggt_thread.joinable()) ggt_thread.join();
+        if (ggt_ev) (void)hipEventDestroy(ggt_ev);
+        if (ggt_st) (void)hipStreamDestroy(ggt_st);
         staging_release();
         if (asm_pool) {
             (void)hipStreamSynchronize(st);
@@ -356,35 +403,54 @@ struct espreso_feti_s {
             stage_ev[b] = nullptr;
         }
     }
+    // Copies the concatenation of the pieces into dst on stream s; threads
+    // (0: the OpenMP default) fill the staging buffer, each with pieces of at
+    // most 4 MB
     template <class T, class Piece>
-    T *upload_pieces(const std::vector<int64_t> &off, Piece piece) {
+    void stage_pieces(T *dst, const std::vector<int64_t> &off, Piece piece, hipStream_t s, int threads = 0) {
         const int64_t total = off.back();
-        T *dst = alloc<T>(total);
-        if (total == 0) return dst;
+        if (total == 0) return;
         for (int b = 0; b < 2; b++) {
             if (!stage[b]) ESP_HIP(hipHostMalloc(&stage[b], STAGE_BYTES, hipHostMallocDefault));
             if (!stage_ev[b]) ESP_HIP(hipEventCreateWithFlags(&stage_ev[b], hipEventDisableTiming));
         }
-        const int64_t chunk = (int64_t)(STAGE_BYTES / sizeof(T));
+        const int64_t chunk = (int64_t)(STAGE_BYTES / sizeof(T)), seg = (int64_t)(4 << 20) / (int64_t)sizeof(T);
+        if (threads <= 0) threads = omp_get_max_threads();
+        std::vector<std::array<int64_t, 3>> segs;  // piece, first value, end
         int buf = 0;
         for (int64_t c0 = 0; c0 < total; c0 += chunk, buf ^= 1) {
             const int64_t c1 = std::min(total, c0 + chunk);
-            ESP_HIP(hipEventSynchronize(stage_ev[buf]));  // its previous chunk has left
-            T *sb = (T *)stage[buf];
             const int s0 = (int)(std::upper_bound(off.begin(), off.end(), c0) - off.begin()) - 1;
             const int s1 = (int)(std::lower_bound(off.begin(), off.end(), c1) - off.begin());
-#pragma omp parallel for schedule(dynamic)
-            for (int q = s0; q < s1; q++) {
-                const int64_t a = std::max(off[q], c0), b = std::min(off[q + 1], c1);
-                if (a < b) std::memcpy(sb + (a - c0), piece(q) + (a - off[q]), (size_t)(b - a) * sizeof(T));
+            segs.clear();
+            for (int q = s0; q < s1; q++)
+                for (int64_t a = std::max(off[q], c0), b = std::min(off[q + 1], c1); a < b; a += seg)
+                    segs.push_back({q, a, std::min(b, a + seg)});
+            ESP_HIP(hipEventSynchronize(stage_ev[buf]));  // its previous chunk has left
+            T *sb = (T *)stage[buf];
+#pragma omp parallel for schedule(dynamic) num_threads(threads)
+            for (size_t g = 0; g < segs.size(); g++) {
+                const int q = (int)segs[g][0];
+                const int64_t a = segs[g][1], b = segs[g][2];
+                std::memcpy(sb + (a - c0), piece(q) + (a - off[q]), (size_t)(b - a) * sizeof(T));
             }
-            ESP_HIP(hipMemcpyAsync(dst + c0, sb, (size_t)(c1 - c0) * sizeof(T), hipMemcpyHostToDevice, st));
-            ESP_HIP(hipEventRecord(stage_ev[buf], st));
+            ESP_HIP(hipMemcpyAsync(dst + c0, sb, (size_t)(c1 - c0) * sizeof(T), hipMemcpyHostToDevice, s));
+            ESP_HIP(hipEventRecord(stage_ev[buf], s));
         }
+    }
+    template <class T, class Piece>
+    T *upload_pieces(const std::vector<int64_t> &off, Piece piece) {
+        T *dst = alloc<T>(off.back());
+        stage_pieces(dst, off, piece, st);
         return dst;
     }
-    template <class T>
-    T *upload(const std::vector<T> &v) {
+    // vectors of more than 8 MB go through the pinned staging buffers
+    template <class T, class A>
+    T *upload(const std::vector<T, A> &v) {
+        if (v.size() * sizeof(T) > ((size_t)8 << 20)) {
+            const std::vector<int64_t> off = {0, (int64_t)v.size()};
+            return upload_pieces<T>(off, [&](int) { return v.data(); });
+        }
         T *p = alloc<T>(v.size());
         if (!v.empty()) ESP_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
         return p;
@@ -429,7 +495,7 @@ struct espreso_feti_s {
     bool use_graphs = true;
     void compute_b();
     void compute_b_into(const double *hf, double *ubf, const double *e0f, double *b);
-    void rhs_host(const double *f, std::vector<double> &e0f, std::vector<double> &e) const;
+    void rhs_host(const double *f, hvec<double> &e0f, std::vector<double> &e) const;
     void resolve_rhs();
     // block re-solve of several load cases (groups of up to 16)
     struct BlockBuffers {
@@ -492,8 +558,8 @@ struct espreso_feti_s {
                             const char *what);
     std::vector<int> n0_h, kt_h;
     std::vector<int64_t> f0off_h, saoff_h;
-    void build_ggt(const std::vector<double> &Rt, const std::vector<int64_t> &rtoff, const std::vector<int> &ck,
-                   const std::vector<int> &cblk, const std::vector<int> &coff, std::vector<double> &GGt);
+    void build_ggt(const double *Rt, const std::vector<int64_t> &rtoff, const std::vector<int> &ck,
+                   const std::vector<int> &cblk, const std::vector<int> &coff, hvec<double> &GGt);
     void build_items(int prec, int64_t target, std::vector<SymvItem> &items, std::vector<int> &item0,
                      std::vector<int> &nit, int64_t &plen);
     double tile_flops(const SubHost &h) const;
@@ -524,6 +590,36 @@ void espreso_feti_s::load() {
     ESP_CHECK(prec >= 0 && prec <= 2, ESPRESO_ERR_ARG, "sc_precision must be 0, 1 or 2");
     ESP_CHECK(opt.preconditioner >= 0 && opt.preconditioner <= 4, ESPRESO_ERR_ARG,
               "preconditioner must be NONE, LUMPED, WEIGHT_FUNCTION, DIRICHLET or SUPER_DIRICHLET (0..4)");
+
+    // The K_i CSR arrays need no preprocessing: their staged upload runs on a
+    // host thread (eight copy threads) beside the host work below, which puts
+    // nothing on the stream before the thread has been joined
+    std::vector<int64_t> nz_off(nsub + 1, 0), rp_off(nsub + 1, 0);
+    for (int s = 0; s < nsub; s++) {
+        nz_off[s + 1] = nz_off[s] + sub[s].rowptr[sub[s].n];
+        rp_off[s + 1] = rp_off[s] + sub[s].n + 1;
+    }
+    P.rowptr = alloc<int>(rp_off[nsub]);
+    P.colidx = alloc<int>(nz_off[nsub]);
+    P.vals = alloc<double>(nz_off[nsub]);
+    std::exception_ptr csr_err;
+    struct Joined {
+        std::thread t;
+        ~Joined() {
+            if (t.joinable()) t.join();
+        }
+    } csr_up;
+    csr_up.t = std::thread([&] {
+        try {
+            ESP_HIP(hipSetDevice(opt.device));
+            stage_pieces(P.rowptr, rp_off, [&](int s) { return sub[s].rowptr.data(); }, st, 8);
+            stage_pieces(P.colidx, nz_off, [&](int s) { return sub[s].colidx.data(); }, st, 8);
+            // the structured grid assembles the values on the device
+            if (!grid_dev) stage_pieces(P.vals, nz_off, [&](int s) { return sub[s].vals.data(); }, st, 8);
+        } catch (...) {
+            csr_err = std::current_exception();
+        }
+    });
 
     // ---- host preprocessing: boundary sets, permutations, bands ----
     std::vector<int64_t> boff(nsub + 1, 0), coffv(nsub + 1, 0);
@@ -594,42 +690,86 @@ void espreso_feti_s::load() {
     asubv = coffv;
     const int64_t nb = boff[nsub];
 
-    // boundary -> lambda CSR and lambda -> entries
-    std::vector<int64_t> bl_ptr(nb + 1, 0);
-    for (int s = 0; s < nsub; s++)
-        for (size_t q = 0; q < sub[s].lam.size(); q++) bl_ptr[boff[s] + sub[s].iperm[sub[s].dof[q]] - sub[s].ni + 1]++;
-    for (int64_t j = 0; j < nb; j++) bl_ptr[j + 1] += bl_ptr[j];
-    std::vector<int> bl_lam(bl_ptr[nb]);
-    std::vector<double> bl_sign(bl_ptr[nb]), bl_ss(bl_ptr[nb]);
-    std::vector<int64_t> fill(bl_ptr.begin(), bl_ptr.end() - 1);
-    std::vector<int> le_cnt(nlam, 0), le_sub(2 * (size_t)nlam, 0);
-    std::vector<int64_t> le_b(2 * (size_t)nlam, 0);
-    std::vector<double> le_sign(2 * (size_t)nlam, 0.0), le_ss(2 * (size_t)nlam, 0.0);
+    // boundary -> lambda CSR and lambda -> entries.  Every boundary index
+    // belongs to one subdomain, so the subdomains fill their ranges of the
+    // CSR in parallel.  A lambda's entries keep the order of (subdomain,
+    // position in its lambda list): the entry with the smaller key is
+    // entry 0, found with an atomic minimum over the keys.
+    std::vector<int64_t> lam_base(nsub + 1, 0);
+    for (int s = 0; s < nsub; s++) lam_base[s + 1] = lam_base[s] + (int64_t)sub[s].lam.size();
+    const int64_t nle = lam_base[nsub];
+    hvec<int64_t> bl_ptr(nb + 1);
+    hvec<int> bl_lam(nle);
+    hvec<double> bl_sign(nle), bl_ss(nle);
+    hvec<int> le_cnt(nlam), le_sub(2 * (size_t)nlam);
+    hvec<int64_t> le_b(2 * (size_t)nlam), le_key(nlam);
+    hvec<double> le_sign(2 * (size_t)nlam), le_ss(2 * (size_t)nlam);
+    bl_ptr[0] = 0;
+#pragma omp parallel for schedule(static)
+    for (int l = 0; l < nlam; l++) {
+        le_cnt[l] = 0;
+        le_key[l] = INT64_MAX;
+        le_sub[2 * (size_t)l] = le_sub[2 * (size_t)l + 1] = 0;
+        le_b[2 * (size_t)l] = le_b[2 * (size_t)l + 1] = 0;
+        le_sign[2 * (size_t)l] = le_sign[2 * (size_t)l + 1] = 0.0;
+        le_ss[2 * (size_t)l] = le_ss[2 * (size_t)l + 1] = 0.0;
+    }
+    std::atomic<bool> lam_range_ok{true};
+#pragma omp parallel for schedule(dynamic)
     for (int s = 0; s < nsub; s++) {
         const SubHost &h = sub[s];
+        int64_t *cnt = bl_ptr.data() + boff[s] + 1;  // counts, then ends of the rows
+        std::fill(cnt, cnt + h.m, 0);
+        for (size_t q = 0; q < h.lam.size(); q++) {
+            cnt[h.iperm[h.dof[q]] - h.ni]++;
+            const int l = h.lam[q];
+            if (l < 0 || l >= nlam) {
+                lam_range_ok = false;
+                continue;
+            }
+            __atomic_fetch_add(&le_cnt[l], 1, __ATOMIC_RELAXED);
+            const int64_t key = ((int64_t)s << 32) | (int64_t)q;
+            int64_t cur = __atomic_load_n(&le_key[l], __ATOMIC_RELAXED);
+            while (key < cur && !__atomic_compare_exchange_n(&le_key[l], &cur, key, true, __ATOMIC_RELAXED,
+                                                              __ATOMIC_RELAXED)) {
+            }
+        }
+        int64_t acc = lam_base[s];
+        for (int j = 0; j < h.m; j++) {
+            const int64_t c = cnt[j];
+            cnt[j] = acc;  // start of row j for now
+            acc += c;
+        }
+    }
+    ESP_CHECK(lam_range_ok, ESPRESO_ERR_ARG, "lambda id out of range");
+    for (int l = 0; l < nlam; l++) {
+        ESP_CHECK(le_cnt[l] > 0, ESPRESO_ERR_ARG, "lambda row without entries");
+        ESP_CHECK(le_cnt[l] <= 2, ESPRESO_ERR_ARG, "a lambda row may touch at most two subdomain DOFs");
+    }
+#pragma omp parallel for schedule(dynamic)
+    for (int s = 0; s < nsub; s++) {
+        const SubHost &h = sub[s];
+        int64_t *cur = bl_ptr.data() + boff[s] + 1;  // row starts; each ends as the row's end
         for (size_t q = 0; q < h.lam.size(); q++) {
             const int l = h.lam[q];
-            ESP_CHECK(l >= 0 && l < nlam, ESPRESO_ERR_ARG, "lambda id out of range");
             const int64_t bj = boff[s] + h.iperm[h.dof[q]] - h.ni;
-            const int64_t k = fill[bj]++;
+            const int64_t k = cur[bj - boff[s]]++;
             bl_lam[k] = l;
             bl_sign[k] = h.sign[q];
             bl_ss[k] = h.sign[q] * h.scale[q];
-            ESP_CHECK(le_cnt[l] < 2, ESPRESO_ERR_ARG, "a lambda row may touch at most two subdomain DOFs");
-            const int e = le_cnt[l]++;
+            const int e = le_key[l] == (((int64_t)s << 32) | (int64_t)q) ? 0 : 1;
             le_b[2 * (size_t)l + e] = bj;
             le_sub[2 * (size_t)l + e] = s;
             le_sign[2 * (size_t)l + e] = h.sign[q];
             le_ss[2 * (size_t)l + e] = h.sign[q] * h.scale[q];
         }
     }
-    for (int l = 0; l < nlam; l++) ESP_CHECK(le_cnt[l] > 0, ESPRESO_ERR_ARG, "lambda row without entries");
     note("load: lambda maps", t0);
 
     // R_b, Q = orth(R_b), fb, K_bb (boundary CSR), e = -R^T f, GG^T
     std::vector<int64_t> rboff(nsub + 1, 0);
     for (int s = 0; s < nsub; s++) rboff[s + 1] = rboff[s] + (int64_t)sub[s].m * sub[s].kdim;
-    std::vector<double> Rb(rboff[nsub]), Q(rboff[nsub]), fb(nb), e(coffv[nsub]);
+    hvec<double> Rb(rboff[nsub]), Q(rboff[nsub]), fb(nb), e(coffv[nsub]);
     std::vector<int64_t> kbb_cnt(nsub);
     std::vector<char> rank_ok(nsub, 1);
 #pragma omp parallel for schedule(dynamic)
@@ -756,14 +896,7 @@ void espreso_feti_s::load() {
         n_sbi = sbi_off[nsub];
     }
     note("load: K_bb (and K_ib) maps", t0);
-    // keep what the coarse layout and the cluster setup need
-    Rb_h = Rb;
     rboff_h = rboff;
-    e0f_h = e;
-    le_cnt_h = le_cnt;
-    le_sub_h = le_sub;
-    le_b_h = le_b;
-    le_sign_h = le_sign;
     hybrid = opt.method == 1;
     if (hybrid) {
         ncl = 0;
@@ -778,6 +911,11 @@ void espreso_feti_s::load() {
     }
 
     // ---- device: problem arrays ----
+    note("load: host preprocessing", t0);
+    csr_up.t.join();
+    if (csr_err) std::rethrow_exception(csr_err);
+    if (measure) sync();
+    note("load: K CSR upload joined", t0);
     D.nsub = nsub;
     D.nlam = nlam;
     D.ncoarse = ncoarse;
@@ -829,6 +967,13 @@ void espreso_feti_s::load() {
     d_e0s = alloc<double>(n_e0s);
     d_alpha_tot = alloc<double>(coffv[nsub]);
     double *d_Q = upload(Q);
+    // keep what the coarse layout and the cluster setup need
+    Rb_h = std::move(Rb);
+    e0f_h = std::move(e);
+    le_cnt_h = std::move(le_cnt);
+    le_sub_h = std::move(le_sub);
+    le_b_h = std::move(le_b);
+    le_sign_h = std::move(le_sign);
 
     // packed pools: one strip geometry per storage precision in use (F_i and
     // S_D may be stored in different precisions)
@@ -864,12 +1009,9 @@ void espreso_feti_s::load() {
     }
 
     // primal support
-    std::vector<int64_t> nz_off(nsub + 1, 0), rp_off(nsub + 1, 0), n_off(nsub + 1, 0), band_off(nsub + 1, 0),
-        yf_off(nsub + 1, 0), r_off(nsub + 1, 0);
+    std::vector<int64_t> n_off(nsub + 1, 0), band_off(nsub + 1, 0), yf_off(nsub + 1, 0), r_off(nsub + 1, 0);
     for (int s = 0; s < nsub; s++) {
         const SubHost &h = sub[s];
-        nz_off[s + 1] = nz_off[s] + h.rowptr[h.n];
-        rp_off[s + 1] = rp_off[s] + h.n + 1;
         n_off[s + 1] = n_off[s] + h.n;
         band_off[s + 1] = band_off[s] + (int64_t)h.ni * (h.bw + 1);
         yf_off[s + 1] = yf_off[s] + h.ni;
@@ -882,17 +1024,11 @@ void espreso_feti_s::load() {
             nv[s] = sub[s].n;
             bwv[s] = sub[s].bw;
         }
-        P.rowptr = upload_pieces<int>(rp_off, [&](int s) { return sub[s].rowptr.data(); });
-        P.colidx = upload_pieces<int>(nz_off, [&](int s) { return sub[s].colidx.data(); });
+        note("load: small uploads", t0);
         P.nz_off = upload(nz_off);
         P.rp_off = upload(rp_off);
-        if (grid_dev) {
-            // structured grid: K_i values from the unit element matrix and the element materials
-            P.vals = alloc<double>(nz_off[nsub]);
-            grid_stiffness();
-        } else {
-            P.vals = upload_pieces<double>(nz_off, [&](int s) { return sub[s].vals.data(); });
-        }
+        // structured grid: K_i values from the unit element matrix and the element materials
+        if (grid_dev) grid_stiffness();
         gather_prec_values();
         P.perm = upload_pieces<int>(n_off, [&](int s) { return sub[s].perm.data(); });
         P.iperm = upload_pieces<int>(n_off, [&](int s) { return sub[s].iperm.data(); });
@@ -1019,6 +1155,7 @@ void espreso_feti_s::gpu_setup() {
     note("setup: assembly", ta);
     double tc = wall();
     if (hybrid) cluster_setup();
+    join_ggt();
     coarse();
     compute_b();
     check_launches("coarse set-up");
@@ -1044,7 +1181,7 @@ void espreso_feti_s::compute_b_into(const double *hf, double *ubf, const double 
 
 // host part of new loads f (all subdomains, local order): e0f = -R_s^T f_s per
 // subdomain and the coarse right-hand side e (hybrid: per cluster through N_s)
-void espreso_feti_s::rhs_host(const double *f, std::vector<double> &e0f, std::vector<double> &e) const {
+void espreso_feti_s::rhs_host(const double *f, hvec<double> &e0f, std::vector<double> &e) const {
     e0f.assign(asubv[nsub], 0.0);
 #pragma omp parallel for schedule(dynamic, 16)
     for (int s = 0; s < nsub; s++) {
@@ -1065,7 +1202,7 @@ void espreso_feti_s::rhs_host(const double *f, std::vector<double> &e0f, std::ve
                     e[ccoff_h[c] + b] += Ns_h[noff_h[s] + a + (int64_t)b * sub[s].kdim] * e0f[asubv[s] + a];
         }
     } else {
-        e = e0f;
+        e.assign(e0f.begin(), e0f.end());
     }
 }
 
@@ -1988,7 +2125,8 @@ void espreso_feti_s::resolve_group(int K, const double *f, double *u, int32_t *i
     const int64_t nl = nlam, nb = D.nbnd, nt = uoff[nsub], na = std::max<int64_t>(asubv[nsub], 1);
     const int64_t nc = std::max(ncoarse, 1);
     // loads: e0f, e on the host; f, y_f, hf (one band-solve launch) and b on the device
-    std::vector<double> e0f, e;
+    hvec<double> e0f;
+    std::vector<double> e;
     ESP_HIP(hipMemcpyAsync(BB.f, f, K * nt * sizeof(double), hipMemcpyHostToDevice, st));
     for (int v = 0; v < K; v++) {
         rhs_host(f + v * nt, e0f, e);
@@ -2128,9 +2266,8 @@ bool espreso_feti_s::resolve_block(int nrhs, const double *f, double *u, int32_t
 
 // --------------------------------------------------- coarse space layout ----
 // GG^T (dense): sum over lambda of the outer products of G columns, G = -(B Rt)^T
-void espreso_feti_s::build_ggt(const std::vector<double> &Rt, const std::vector<int64_t> &rtoff,
-                               const std::vector<int> &ck, const std::vector<int> &cblk, const std::vector<int> &coff,
-                               std::vector<double> &GGt) {
+void espreso_feti_s::build_ggt(const double *Rt, const std::vector<int64_t> &rtoff, const std::vector<int> &ck,
+                               const std::vector<int> &cblk, const std::vector<int> &coff, hvec<double> &GGt) {
     const int nc = ncoarse;
     const std::vector<int64_t> &boff = h_boff;
     int nblk = 0;
@@ -2203,10 +2340,23 @@ void espreso_feti_s::coarse_layout_subdomains() {
     D.cd = alloc<double>(ncoarse);
     D.ce = alloc<double>(ncoarse);
     d_alpha_c = alloc<double>(ncoarse);
-    std::vector<double> GGt;
-    build_ggt(Rb_h, rboff_h, ck, cblk, coff, GGt);
-    d_GGt = upload(GGt);
     d_e = upload(e0f_h);
+    d_GGt = alloc<double>((size_t)ncoarse * ncoarse);
+    if (!ggt_st) ESP_HIP(hipStreamCreate(&ggt_st));
+    if (!ggt_ev) ESP_HIP(hipEventCreateWithFlags(&ggt_ev, hipEventDisableTiming));
+    ESP_HIP(hipEventRecord(ggt_ev, st));  // behind the allocation
+    ESP_HIP(hipStreamWaitEvent(ggt_st, ggt_ev, 0));
+    ggt_thread = std::thread([this, ck, cblk, coff] {
+        try {
+            ESP_HIP(hipSetDevice(opt.device));
+            hvec<double> GGt;
+            build_ggt(Rb_h.data(), rboff_h, ck, cblk, coff, GGt);
+            ESP_HIP(hipMemcpyAsync(d_GGt, GGt.data(), GGt.size() * sizeof(double), hipMemcpyHostToDevice, ggt_st));
+            ESP_HIP(hipStreamSynchronize(ggt_st));
+        } catch (...) {
+            ggt_err = std::current_exception();
+        }
+    });
 }
 
 // Hybrid FETI, host part (once per load; cluster.cpp:1186-1291, 2018-2159):
@@ -2441,8 +2591,8 @@ void espreso_feti_s::load_clusters() {
     D.cd = alloc<double>(ncoarse);
     D.ce = alloc<double>(ncoarse);
     d_alpha_c = alloc<double>(ncoarse);
-    std::vector<double> GGt;
-    build_ggt(Rt, rtoff, ck, cblk, coff, GGt);
+    hvec<double> GGt;
+    build_ggt(Rt.data(), rtoff, ck, cblk, coff, GGt);
     d_GGt = upload(GGt);
     d_e = upload(e);
 }
@@ -2673,6 +2823,13 @@ int espreso_feti_setup(espreso_feti *h) {
     ESP_API_BEGIN
     ESP_CHECK(h, ESPRESO_ERR_ARG, "null handle");
     ESP_HIP(hipSetDevice(h->opt.device));
+    // a set-up that fails leaves no host thread reading the instance
+    struct JoinGGt {
+        espreso_feti *h;
+        ~JoinGGt() {
+            if (h->ggt_thread.joinable()) h->ggt_thread.join();
+        }
+    } join_ggt{h};
     if (!h->loaded) h->load();
     h->gpu_setup();
     ESP_API_END
@@ -2975,11 +3132,26 @@ int espreso_debug_stream(int64_t bytes, int32_t reps, int32_t workgroups, double
     ESP_API_END
 }
 
-int espreso_debug_mfma_rate(int32_t shape, int32_t chains, double *tflops) {
+int espreso_debug_mfma_rate(int32_t shape, int32_t chains, int32_t workgroups_per_cu, double *tflops) {
     ESP_API_BEGIN
-    ESP_CHECK((shape == 0 || shape == 1) && (chains == 4 || chains == 8) && tflops, ESPRESO_ERR_ARG,
-              "shape 0 or 1, chains 4 or 8");
+    const bool chains_ok = shape <= 1   ? (chains == 4 || chains == 8 || chains == 16 || chains == 32)
+                           : shape == 2 ? (chains == 2 || chains == 4 || chains == 8)
+                                        : (chains == 2 || chains == 4);
+    ESP_CHECK(shape >= 0 && shape <= 3 && chains_ok && tflops && workgroups_per_cu >= 0 && workgroups_per_cu <= 8,
+              ESPRESO_ERR_ARG,
+              "shape 0..3; chains 4, 8, 16 or 32 (shapes 0, 1), 2, 4 or 8 (shape 2), 2 or 4 (shape 3); "
+              "0..8 workgroups per CU");
     const int blocks = 4096, iter = 2000;
+    // unused dynamic LDS that lets only workgroups_per_cu workgroups share a CU
+    int lds = 0;
+    if (workgroups_per_cu > 0) {
+        int cu_lds = 0;
+        ESP_HIP(hipDeviceGetAttribute(&cu_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, 0));
+        lds = cu_lds / workgroups_per_cu - 1024;
+        int blk_lds = 0;
+        ESP_HIP(hipDeviceGetAttribute(&blk_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, 0));
+        ESP_CHECK(lds > 0 && lds <= blk_lds, ESPRESO_ERR_ARG, "workgroups per CU not reachable by the LDS size");
+    }
     double *o = nullptr;
     ESP_HIP(hipMalloc(&o, (size_t)blocks * 256 * sizeof(double)));
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2987,9 +3159,9 @@ int espreso_debug_mfma_rate(int32_t shape, int32_t chains, double *tflops) {
     if (e == hipSuccess) e = hipEventCreate(&e1);
     float t = 0;
     if (e == hipSuccess) {
-        launch_mfma_rate(shape, chains, blocks, 16, o, nullptr);  // warm
+        launch_mfma_rate(shape, chains, blocks, 16, lds, o, nullptr);  // warm
         (void)hipEventRecord(e0, nullptr);
-        launch_mfma_rate(shape, chains, blocks, iter, o, nullptr);
+        launch_mfma_rate(shape, chains, blocks, iter, lds, o, nullptr);
         (void)hipEventRecord(e1, nullptr);
         e = hipEventSynchronize(e1);
         if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
@@ -2999,7 +3171,8 @@ int espreso_debug_mfma_rate(int32_t shape, int32_t chains, double *tflops) {
     (void)hipFree(o);
     ESP_HIP(e);
     ESP_CHECK(t > 0, ESPRESO_ERR_STATE, "MFMA rate loop not timed");
-    *tflops = (double)blocks * 4 * iter * chains * mfma_rate_flops(shape) / (t * 1e-3) / 1e12;
+    const double per_iter = shape >= 2 ? (double)chains * chains : (double)chains;  // MFMAs per wave and step
+    *tflops = (double)blocks * 4 * iter * per_iter * mfma_rate_flops(shape) / (t * 1e-3) / 1e12;
     ESP_API_END
 }
 
@@ -3007,12 +3180,12 @@ int espreso_debug_mfma_rate(int32_t shape, int32_t chains, double *tflops) {
 int espreso_debug_mfma(const double *a, const double *b, double *out) {
     ESP_API_BEGIN
     double *d = nullptr;
-    ESP_HIP(hipMalloc(&d, (64 + 64 + 256) * sizeof(double)));
+    ESP_HIP(hipMalloc(&d, (64 + 64 + 320) * sizeof(double)));
     hipError_t e = hipMemcpy(d, a, 64 * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d + 64, b, 64 * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         launch_probe_mfma(d, d + 64, d + 128, nullptr);
-        e = hipMemcpy(out, d + 128, 256 * sizeof(double), hipMemcpyDeviceToHost);
+        e = hipMemcpy(out, d + 128, 320 * sizeof(double), hipMemcpyDeviceToHost);
     }
     (void)hipFree(d);
     ESP_HIP(e);

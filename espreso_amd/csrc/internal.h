@@ -127,7 +127,7 @@ void launch_extract_b0(const DenseDesc *desc, int nb, const int *nq, const int64
 void launch_probe_mfma(const double *a, const double *b, double *out, hipStream_t st);
 // FP64 MFMA issue-rate loop (shape 0: 16x16x4, 1: 4x4x4 with 4 blocks; chains 4 or 8);
 // out holds blocks x 256 doubles
-void launch_mfma_rate(int shape, int chains, int blocks, int iter, double *out, hipStream_t st);
+void launch_mfma_rate(int shape, int chains, int blocks, int iter, int lds, double *out, hipStream_t st);
 double mfma_rate_flops(int shape);
 void launch_probe_gemm(int ta, int tb, int M, int N, int K, const double *A, int lda, const double *B, int ldb,
                        double *C, int ldc, double alpha, double beta, hipStream_t st);

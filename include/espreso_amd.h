@@ -326,7 +326,8 @@ int espreso_feti_time_apply(espreso_feti *h, int32_t op, int32_t reps,
 
 /* Diagnostics of the FP64 MFMA tile routine used by the assembly.
  * espreso_debug_mfma: one v_mfma_f64_16x16x4f64 with lane l holding a[l], b[l];
- * out[4*l + v] = accumulator value v of lane l.
+ * out[4*l + v] = accumulator value v of lane l; then one v_mfma_f64_4x4x4f64
+ * (4 blocks) on the same operands, out[256 + l] = its result in lane l.
  * espreso_debug_tile_gemm: C = alpha op(A) op(B) + beta C for one tile
  * (M, N <= 64), column-major host arrays.                                     */
 int espreso_debug_mfma(const double *a, const double *b, double *out);
@@ -338,10 +339,14 @@ int espreso_debug_copy(int64_t bytes, int32_t reps, double *ms);
  * bandwidth-bound kernels are compared with).                               */
 int espreso_debug_stream(int64_t bytes, int32_t reps, int32_t workgroups, double *ms);
 /* FP64 MFMA issue rate with register operands only (4096 workgroups of 4 waves,
- * `chains` = 4 or 8 independent accumulators per wave): shape 0
- * v_mfma_f64_16x16x4f64, 1 v_mfma_f64_4x4x4f64 (4 blocks); TFLOP/s.  The
- * ceiling that the assembly kernels' MFMA rate is compared with.            */
-int espreso_debug_mfma_rate(int32_t shape, int32_t chains, double *tflops);
+ * `chains` = 4, 8, 16 or 32 independent accumulators per wave on one operand
+ * pair): shape 0 v_mfma_f64_16x16x4f64, 1 v_mfma_f64_4x4x4f64 (4 blocks);
+ * shapes 2 (4x4x4) and 3 (16x16x4): `chains` A and B operand registers and
+ * chains x chains accumulators, as in a register-blocked tile; TFLOP/s.  The
+ * ceiling that the assembly kernels' MFMA rate is compared with.
+ * workgroups_per_cu > 0 caps the workgroups sharing a CU (through unused LDS),
+ * i.e. the waves per SIMD; 0 leaves the occupancy to the register use.      */
+int espreso_debug_mfma_rate(int32_t shape, int32_t chains, int32_t workgroups_per_cu, double *tflops);
 int espreso_debug_tile_gemm(int32_t ta, int32_t tb, int32_t M, int32_t N, int32_t K, const double *A, int32_t lda,
                             const double *B, int32_t ldb, double *C, int32_t ldc, double alpha, double beta);
 
