@@ -748,6 +748,19 @@ __global__ void k_fill(double *p, int64_t n, double v) {
         p[i] = v;
 }
 
+// d[i] = s[i] (16-bit indices widened to int), eight per thread and step
+__global__ void k_widen_u16(const uint16_t *s, int *d, int64_t n) {
+    const int64_t n8 = n / 8;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n8; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint4 v = reinterpret_cast<const uint4 *>(s)[i];
+        reinterpret_cast<int4 *>(d)[2 * i] = make_int4(v.x & 0xffff, v.x >> 16, v.y & 0xffff, v.y >> 16);
+        reinterpret_cast<int4 *>(d)[2 * i + 1] = make_int4(v.z & 0xffff, v.z >> 16, v.w & 0xffff, v.w >> 16);
+    }
+    for (int64_t i = 8 * n8 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = s[i];
+}
+
 
 __device__ __forceinline__ double block_max(double v) {
     __shared__ double red[256];
@@ -1191,6 +1204,12 @@ void launch_probe_gemm(int ta, int tb, int M, int N, int K, const double *A, int
 
 // ------------------------------------------------------------ launchers ----
 static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+void launch_widen_u16(const uint16_t *s, int *d, int64_t n, hipStream_t st) {
+    if (n <= 0) return;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n / 8 + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_widen_u16, dim3(g), dim3(256), 0, st, s, d, n);
+}
 
 void launch_fill(double *p, int64_t n, double v, hipStream_t st) {
     if (n <= 0) return;

@@ -13,6 +13,7 @@
 #include <array>
 #include <atomic>
 #include <thread>
+#include <type_traits>
 #include <omp.h>
 #include <map>
 #include <mutex>
@@ -342,6 +343,8 @@ struct espreso_feti_s {
             if (l.s) (void)hipStreamDestroy(l.s);
         }
         if (ev_main) (void)hipEventDestroy(ev_main);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
         for (hipEvent_t e : asm_ev)
             if (e) (void)hipEventDestroy(e);
         if (asm_st) (void)hipStreamDestroy(asm_st);
@@ -394,6 +397,13 @@ struct espreso_feti_s {
     static constexpr size_t STAGE_BYTES = (size_t)128 << 20;
     void *stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    double stage_wait_s = 0, stage_copy_s = 0;  // staged uploads: waits for the copy engine, host copies
+    void staging_ready() {
+        for (int b = 0; b < 2; b++) {
+            if (!stage[b]) ESP_HIP(hipHostMalloc(&stage[b], STAGE_BYTES, hipHostMallocDefault));
+            if (!stage_ev[b]) ESP_HIP(hipEventCreateWithFlags(&stage_ev[b], hipEventDisableTiming));
+        }
+    }
     void staging_release() {
         for (int b = 0; b < 2; b++) {
             if (stage_ev[b]) (void)hipEventSynchronize(stage_ev[b]);
@@ -405,15 +415,12 @@ struct espreso_feti_s {
     }
     // Copies the concatenation of the pieces into dst on stream s; threads
     // (0: the OpenMP default) fill the staging buffer, each with pieces of at
-    // most 4 MB
+    // most 4 MB.  Pieces of another element type are converted to T on the way.
     template <class T, class Piece>
     void stage_pieces(T *dst, const std::vector<int64_t> &off, Piece piece, hipStream_t s, int threads = 0) {
         const int64_t total = off.back();
         if (total == 0) return;
-        for (int b = 0; b < 2; b++) {
-            if (!stage[b]) ESP_HIP(hipHostMalloc(&stage[b], STAGE_BYTES, hipHostMallocDefault));
-            if (!stage_ev[b]) ESP_HIP(hipEventCreateWithFlags(&stage_ev[b], hipEventDisableTiming));
-        }
+        staging_ready();
         const int64_t chunk = (int64_t)(STAGE_BYTES / sizeof(T)), seg = (int64_t)(4 << 20) / (int64_t)sizeof(T);
         if (threads <= 0) threads = omp_get_max_threads();
         std::vector<std::array<int64_t, 3>> segs;  // piece, first value, end
@@ -426,14 +433,24 @@ struct espreso_feti_s {
             for (int q = s0; q < s1; q++)
                 for (int64_t a = std::max(off[q], c0), b = std::min(off[q + 1], c1); a < b; a += seg)
                     segs.push_back({q, a, std::min(b, a + seg)});
+            const double w0 = wall();
             ESP_HIP(hipEventSynchronize(stage_ev[buf]));  // its previous chunk has left
+            const double w1 = wall();
             T *sb = (T *)stage[buf];
 #pragma omp parallel for schedule(dynamic) num_threads(threads)
             for (size_t g = 0; g < segs.size(); g++) {
                 const int q = (int)segs[g][0];
                 const int64_t a = segs[g][1], b = segs[g][2];
-                std::memcpy(sb + (a - c0), piece(q) + (a - off[q]), (size_t)(b - a) * sizeof(T));
+                const auto *src = piece(q) + (a - off[q]);
+                if constexpr (std::is_same_v<std::remove_cv_t<std::remove_pointer_t<decltype(src)>>, T>) {
+                    std::memcpy(sb + (a - c0), src, (size_t)(b - a) * sizeof(T));
+                } else {
+                    T *o = sb + (a - c0);  // narrowing copy (the caller checked the range)
+                    for (int64_t e = 0; e < b - a; e++) o[e] = (T)src[e];
+                }
             }
+            stage_wait_s += w1 - w0;
+            stage_copy_s += wall() - w1;
             ESP_HIP(hipMemcpyAsync(dst + c0, sb, (size_t)(c1 - c0) * sizeof(T), hipMemcpyHostToDevice, s));
             ESP_HIP(hipEventRecord(stage_ev[buf], s));
         }
@@ -521,6 +538,21 @@ struct espreso_feti_s {
         ClusterDev C;
     } lanes[NLANE];
     hipEvent_t ev_main = nullptr;
+    // assembly kernels that only read what an earlier step wrote run on asm_st
+    // beside the main stream's work: fork (asm_st behind st) and join (st
+    // behind asm_st)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    void asm_fork() {
+        if (!asm_st) ESP_HIP(hipStreamCreate(&asm_st));
+        if (!ev_fork) ESP_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        ESP_HIP(hipEventRecord(ev_fork, st));
+        ESP_HIP(hipStreamWaitEvent(asm_st, ev_fork, 0));
+    }
+    void asm_join() {
+        if (!ev_join) ESP_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        ESP_HIP(hipEventRecord(ev_join, asm_st));
+        ESP_HIP(hipStreamWaitEvent(st, ev_join, 0));
+    }
     // assembly: forward substitution on a second stream (default priority), one
     // event per Cholesky step plus the join
     hipStream_t asm_st = nullptr;
@@ -602,7 +634,14 @@ void espreso_feti_s::load() {
     P.rowptr = alloc<int>(rp_off[nsub]);
     P.colidx = alloc<int>(nz_off[nsub]);
     P.vals = alloc<double>(nz_off[nsub]);
+    // column indices of subdomains with at most 65536 DOFs cross the link as
+    // 16-bit values (half the bytes of the largest input) and are widened on
+    // the device
+    int max_n_in = 0;
+    for (int s = 0; s < nsub; s++) max_n_in = std::max(max_n_in, sub[s].n);
+    uint16_t *colidx16 = max_n_in <= 65536 ? alloc<uint16_t>(nz_off[nsub]) : nullptr;
     std::exception_ptr csr_err;
+    double csr_t[2] = {0, 0};  // staging buffers ready, last chunk queued (measure mode)
     struct Joined {
         std::thread t;
         ~Joined() {
@@ -612,10 +651,14 @@ void espreso_feti_s::load() {
     csr_up.t = std::thread([&] {
         try {
             ESP_HIP(hipSetDevice(opt.device));
+            staging_ready();
+            csr_t[0] = wall() - t0;
             stage_pieces(P.rowptr, rp_off, [&](int s) { return sub[s].rowptr.data(); }, st, 8);
-            stage_pieces(P.colidx, nz_off, [&](int s) { return sub[s].colidx.data(); }, st, 8);
+            if (colidx16) stage_pieces(colidx16, nz_off, [&](int s) { return sub[s].colidx.data(); }, st, 8);
+            else stage_pieces(P.colidx, nz_off, [&](int s) { return sub[s].colidx.data(); }, st, 8);
             // the structured grid assembles the values on the device
             if (!grid_dev) stage_pieces(P.vals, nz_off, [&](int s) { return sub[s].vals.data(); }, st, 8);
+            csr_t[1] = wall() - t0;
         } catch (...) {
             csr_err = std::current_exception();
         }
@@ -914,7 +957,17 @@ void espreso_feti_s::load() {
     note("load: host preprocessing", t0);
     csr_up.t.join();
     if (csr_err) std::rethrow_exception(csr_err);
-    if (measure) sync();
+    if (colidx16) {
+        launch_widen_u16(colidx16, P.colidx, nz_off[nsub], st);
+        release(colidx16);
+    }
+    if (measure) {
+        sync();
+        std::fprintf(stderr,
+                     "espreso: load: K CSR staging buffers at %.4f s, last chunk queued at %.4f s (host copies %.4f s, "
+                     "waits for the copy engine %.4f s)\n",
+                     csr_t[0], csr_t[1], stage_copy_s, stage_wait_s);
+    }
     note("load: K CSR upload joined", t0);
     D.nsub = nsub;
     D.nlam = nlam;
@@ -1377,13 +1430,21 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         factor_solve(d_desc, nbt, 0, max_ntii, max_band, max_ntc);
         if (measure) sync();
         note("assembly: K_ii factor, solve", tw);
+        // h_f and the band copy of the K_ii factor read only [K_ib f] and K_ii:
+        // on asm_st beside S = K_bb - Y^T Y (the S-stage substitution that
+        // later writes W over [K_ib f] runs behind them on asm_st)
+        asm_fork();
+        launch_hf(d_desc, nbt, d_fb, d_hf, d_b_off, max_m, asm_st);
+        launch_band_extract(d_desc, nbt, P.band, d_band_off, d_bw, P.yf, d_yf_off, max_ni, asm_st);
         launch_syrk(d_desc, nbt, 0, max_ntm, st);
-        launch_rownorm(d_desc, nbt, D.sdnrm + s0, max_m, st);
-        launch_hf(d_desc, nbt, d_fb, d_hf, d_b_off, max_m, st);
-        launch_band_extract(d_desc, nbt, P.band, d_band_off, d_bw, P.yf, d_yf_off, max_ni, st);
+        // S_D = S: its row norms (asm_st) and its packed copy (st) both only
+        // read S; the kernel regularisation then changes S
+        asm_fork();
+        launch_rownorm(d_desc, nbt, D.sdnrm + s0, max_m, asm_st);
         if (D.sd_prec == 0) launch_pack(d_desc, nbt, 0, D.SD, d_p_off[0], nullptr, max_m, st);
         if (D.sd_prec == 1) launch_pack(d_desc, nbt, 1, D.SDf, d_p_off[1], nullptr, max_m, st);
         if (D.sd_prec == 2) launch_pack(d_desc, nbt, 2, D.SDh, d_p_off[2], D.sdscale + s0, max_m, st);
+        asm_join();
         launch_kernel_reg(d_desc, nbt, d_Q, d_q_off, d_kd, d_rho, max_m, st);
         if (hybrid)
             ESP_HIP(hipMemcpyAsync(CL.rho_s + s0, d_rho, nbt * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -1395,7 +1456,9 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         launch_syrk(d_desc, nbt, 1, max_ntm, st);
         if (measure) sync();
         note("assembly: F = W^T W", tw);
-        launch_rownorm(d_desc, nbt, D.fnrm + s0, max_m, st);
+        // F's row norms beside its packed copies (both only read F); joined below
+        asm_fork();
+        launch_rownorm(d_desc, nbt, D.fnrm + s0, max_m, asm_st);
         if (hybrid) {
             std::vector<int> nq(nq_h.begin() + s0, nq_h.begin() + s1);
             std::vector<int64_t> qo(nbt), fo(nbt), co(nbt);
@@ -1420,6 +1483,7 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         launch_pack(d_desc, nbt, 0, D.Fd, d_p_off[0], nullptr, max_m, st);
         if (prec == 1) launch_pack(d_desc, nbt, 1, D.Ff, d_p_off[1], nullptr, max_m, st);
         if (prec == 2) launch_pack(d_desc, nbt, 2, D.Fh, d_p_off[2], D.fscale + s0, max_m, st);
+        asm_join();
         if (measure) sync();
         note("assembly: pack", tw);
         std::vector<int> info(nbt);

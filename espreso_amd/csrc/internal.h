@@ -89,6 +89,8 @@ struct DenseDesc {
 // Launchers (dense.hip).  `desc` is a device array of nb descriptors.
 // which = 0: operate on K_ii / [K_ib|f];  which = 1: on S / W.
 void launch_fill(double *p, int64_t n, double v, hipStream_t st);
+// d[i] = s[i] for i < n; s and d 16-byte aligned
+void launch_widen_u16(const uint16_t *s, int *d, int64_t n, hipStream_t st);
 void launch_potrf_step(const DenseDesc *desc, int nb, int which, int step, int max_band, hipStream_t st);
 // the two halves of a step: factor and invert the diagonal tile, then the panel below it
 void launch_potrf_diag(const DenseDesc *desc, int nb, int which, int step, hipStream_t st);
