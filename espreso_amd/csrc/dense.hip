@@ -743,6 +743,54 @@ __global__ void __launch_bounds__(256) k_syrk(const DenseDesc *D, int which) {
     }
 }
 
+// ---- W = L^-1 of the S factor (which 1) by recursive doubling over tiles ----
+// L is in the lower triangle of S and the inverses of its diagonal tiles in
+// Dinv (lower triangular, zeros above their diagonal).  The diagonal tiles of
+// W are those inverses; level t (1, 2, 4, ... tiles) joins the pairs of
+// t-tile blocks [b1, b1 + t), [b1 + t, b1 + 2t), b1 = 2tp:
+//   [W11 0; W21 W22] = [L11 0; L21 L22]^-1,   W21 = -W22 (L21 W11),
+// phase 0 forming T = L21 W11 (tile (i, j) over the K range where W11's
+// column tile j is nonzero), stored transposed in the upper triangle of S,
+// which nothing reads after the factorisation; phase 1 writing
+// W21 = -W22 T (K range where W22's row tile i is nonzero).  Every tile of a
+// level is independent, so each level is two launches over all subdomains
+// instead of one forward-substitution step per tile row.
+__global__ void __launch_bounds__(256) k_trtri_diag(const DenseDesc *D) {
+    int bx, by;
+    xcd_tile(bx, by);
+    const DenseDesc &d = D[by];
+    const int i = bx;
+    if (i >= d.ntm) return;
+    const int r0 = i * TILE, mi = min(TILE, d.m - r0);
+    const double *Di = d.Dinv + (size_t)i * TILE * TILE;
+    double *W = d.W + r0 + (size_t)r0 * d.m;
+    for (int q = threadIdx.x; q < TILE * TILE; q += 256) {
+        const int r = q & 63, c = q >> 6;
+        if (r < mi && c < mi) W[r + (size_t)c * d.m] = Di[q];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_trtri_level(const DenseDesc *D, int t, int phase) {
+    int bx, by;
+    xcd_tile(bx, by);
+    const DenseDesc &d = D[by];
+    const int nt = d.ntm, m = d.m, tt = t * t;
+    const int p = bx / tt, i = (bx % tt) / t, j = bx % t;  // pair, tile row in block 2, tile column in block 1
+    const int b1 = 2 * t * p, b2 = b1 + t;
+    if (b2 + i >= nt) return;
+    const int r2 = (b2 + i) * TILE, c1 = (b1 + j) * TILE, k2 = b2 * TILE;
+    const int mi = min(TILE, m - r2);
+    if (phase == 0) {
+        // T(r2.., c1..) = L(r2.., c1..k2) W(c1..k2, c1..), stored as S[c1 + x, r2 + y] = T(y, x)
+        tile_gemm<true, true>(d.S + c1 + (size_t)r2 * m, m, d.W + c1 + (size_t)c1 * m, m, d.S + r2 + (size_t)c1 * m, m,
+                              TILE, mi, k2 - c1, 1.0, 0.0);
+    } else {
+        // W(r2.., c1..) = -W(r2.., k2..r2+mi) T(k2..r2+mi, c1..)
+        tile_gemm<false, true>(d.W + r2 + (size_t)c1 * m, m, d.W + r2 + (size_t)k2 * m, m, d.S + c1 + (size_t)k2 * m, m,
+                               mi, TILE, r2 + mi - k2, -1.0, 0.0);
+    }
+}
+
 __global__ void k_fill(double *p, int64_t n, double v) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -1239,6 +1287,15 @@ void launch_trsm_step(const DenseDesc *desc, int nb, int which, int step, int ma
         hipLaunchKernelGGL(k_trsm_row, dim3((max_ntc + 1) / 2, nb), dim3(256), 0, st, desc, which, step);
     else
         hipLaunchKernelGGL(k_trsm_row1, dim3(max_ntc, nb), dim3(256), 0, st, desc, which, step);
+}
+
+void launch_trtri(const DenseDesc *desc, int nb, int max_ntm, hipStream_t st) {
+    hipLaunchKernelGGL(k_trtri_diag, dim3(max_ntm, nb), dim3(256), 0, st, desc);
+    for (int t = 1; t < max_ntm; t *= 2) {
+        const unsigned g = (unsigned)(cdiv(max_ntm, 2 * t) * t * t);
+        hipLaunchKernelGGL(k_trtri_level, dim3(g, nb), dim3(256), 0, st, desc, t, 0);
+        hipLaunchKernelGGL(k_trtri_level, dim3(g, nb), dim3(256), 0, st, desc, t, 1);
+    }
 }
 
 void launch_syrk(const DenseDesc *desc, int nb, int which, int max_ntm, hipStream_t st) {

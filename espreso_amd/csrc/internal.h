@@ -98,6 +98,9 @@ void launch_potrf_panel(const DenseDesc *desc, int nb, int which, int step, int 
 void launch_trsm_step(const DenseDesc *desc, int nb, int which, int step, int max_band, int max_ntc,
                       hipStream_t st);
 void launch_syrk(const DenseDesc *desc, int nb, int which, int max_ntm, hipStream_t st);
+// W = L^-1 for the S factor (which 1: L in S's lower triangle, diagonal tile
+// inverses in Dinv) by recursive doubling; uses S's upper triangle as scratch
+void launch_trtri(const DenseDesc *desc, int nb, int max_ntm, hipStream_t st);
 void launch_symmetrize_desc(const DenseDesc *desc, int nb, int max_m, hipStream_t st);
 // CSR K_i (local DOFs) into the dense K_ii band, [K_ib|f] and K_bb, zeros included;
 // iperm maps local -> internal index, perm internal -> local
