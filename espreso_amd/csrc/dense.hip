@@ -831,28 +831,39 @@ __global__ void k_rho(const DenseDesc *D, double *rho) {
     if (threadIdx.x == 0) rho[blockIdx.x] = v;
 }
 
-// S += rho Q Q^T on the lower triangle: columns j over workgroups, rows i >= j
-// over threads (coalesced, no index division); Q(j, :) in registers for k <= 8
+// S += rho Q Q^T on the lower triangle, one 64 x 64 tile per workgroup (the
+// lower tiles in the order of k_syrk): the tile's rows and columns of Q (up to
+// 8 kernel vectors) staged in LDS, so S is read and written once, coalesced
+// along columns; kernel vectors beyond 8 are read from Q directly
 __global__ void __launch_bounds__(256) k_kernel_reg(const DenseDesc *D, const double *Q, const int64_t *q_off,
                                                     const int *kdim, const double *rho) {
     constexpr int KR = 8;
-    const DenseDesc &d = D[blockIdx.y];
-    const int m = d.m, k = kdim[blockIdx.y];
-    const double r = rho[blockIdx.y];
-    const double *q = Q + q_off[blockIdx.y];
-    for (int j = blockIdx.x; j < m; j += gridDim.x) {
-        double qj[KR];
+    __shared__ double qr[KR][TILE], qc[KR][TILE];
+    int bx, by;
+    xcd_tile(bx, by);
+    const DenseDesc &d = D[by];
+    int I = (int)((sqrtf(8.0f * bx + 1.0f) - 1.0f) * 0.5f);
+    while ((I + 1) * (I + 2) / 2 <= bx) I++;
+    while (I * (I + 1) / 2 > bx) I--;
+    const int J = bx - I * (I + 1) / 2;
+    if (I >= d.ntm) return;
+    const int m = d.m, k = kdim[by], i0 = I * TILE, j0 = J * TILE;
+    const double r = rho[by];
+    const double *q = Q + q_off[by];
+    for (int t = threadIdx.x; t < KR * TILE; t += 256) {
+        const int a = t / TILE, x = t % TILE;
+        qr[a][x] = (a < k && i0 + x < m) ? q[i0 + x + (size_t)a * m] : 0.0;
+        qc[a][x] = (a < k && j0 + x < m) ? q[j0 + x + (size_t)a * m] : 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < TILE * TILE; e += 256) {
+        const int x = e & (TILE - 1), y = e / TILE, i = i0 + x, j = j0 + y;
+        if (i >= m || j >= m || i < j) continue;
+        double s = 0.0;
 #pragma unroll
-        for (int a = 0; a < KR; a++) qj[a] = a < k ? q[j + (size_t)a * m] : 0.0;
-        double *col = d.S + (size_t)j * m;
-        for (int i = j + threadIdx.x; i < m; i += 256) {
-            double s = 0.0;
-#pragma unroll
-            for (int a = 0; a < KR; a++)
-                if (a < k) s += q[i + (size_t)a * m] * qj[a];
-            for (int a = KR; a < k; a++) s += q[i + (size_t)a * m] * q[j + (size_t)a * m];
-            col[i] += r * s;
-        }
+        for (int a = 0; a < KR; a++) s += qr[a][x] * qc[a][y];
+        for (int a = KR; a < k; a++) s += q[i + (size_t)a * m] * q[j + (size_t)a * m];
+        d.S[i + (size_t)j * m] += r * s;
     }
 }
 
@@ -865,7 +876,8 @@ __global__ void k_hf(const DenseDesc *D, const double *fb, double *hf, const int
     const double *y = d.Kib + (size_t)j * d.ni;
     const double *yf = d.Kib + (size_t)d.m * d.ni;
     double s = 0.0;
-    for (int r = lane; r < d.ni; r += 64) s += y[r] * yf[r];
+    // Y(:, j) is zero above its column tile's first coupled row (not stored)
+    for (int r = d.colstart[j / TILE] * TILE + lane; r < d.ni; r += 64) s += y[r] * yf[r];
     for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
     if (lane == 0) hf[off[blockIdx.y] + j] = fb[off[blockIdx.y] + j] - s;
 }
@@ -1036,8 +1048,12 @@ __global__ void __launch_bounds__(256) k_materialize(const DenseDesc *D, const i
         lo = (c / TILE) * TILE;
         hi = min(ni, lo + (d.band_ii + 1) * TILE);
     } else {
-        lo = 0;
-        hi = n;  // [K_ib(:, j) ; S(:, j)] as one permuted column
+        // [K_ib(:, j) ; S(:, j)] as one permuted column, from the column tile's
+        // first coupled interior row: the rows above it are zero in K_ib and in
+        // Y = L^-1 K_ib, and no kernel reads them (the substitution, the SYRK
+        // and h_f start there)
+        lo = d.colstart[(c - ni) / TILE] * TILE;
+        hi = n;
     }
     const int l = pm[c];
     const int q0 = rp[l], q1 = rp[l + 1];
@@ -1306,8 +1322,8 @@ void launch_syrk(const DenseDesc *desc, int nb, int which, int max_ntm, hipStrea
 void launch_kernel_reg(const DenseDesc *desc, int nb, const double *q, const int64_t *q_off, const int *kdim,
                        double *rho, int max_m, hipStream_t st) {
     hipLaunchKernelGGL(k_rho, dim3(nb), dim3(256), 0, st, desc, rho);
-    const unsigned g = (unsigned)std::max(1, std::min(max_m, 4096));
-    hipLaunchKernelGGL(k_kernel_reg, dim3(g, nb), dim3(256), 0, st, desc, q, q_off, kdim, rho);
+    const int nt = (max_m + TILE - 1) / TILE;
+    hipLaunchKernelGGL(k_kernel_reg, dim3(nt * (nt + 1) / 2, nb), dim3(256), 0, st, desc, q, q_off, kdim, rho);
 }
 
 void launch_hf(const DenseDesc *desc, int nb, const double *fb, double *hf, const int64_t *off, int max_m,
