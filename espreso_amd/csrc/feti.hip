@@ -186,13 +186,16 @@ struct WorkspaceCache {
             if (!b.busy) n += b.bytes;
         return n;
     }
-    // nullptr when the device has no room
-    void *acquire(int device, size_t bytes) {
+    // nullptr when the device has no room; an idle block is lent only if it
+    // holds at most max_ratio times the request (0: any size)
+    void *acquire(int device, size_t bytes, double max_ratio = 0.0) {
         std::lock_guard<std::mutex> lock(mu);
         std::vector<Block> &v = blocks[device];
         Block *best = nullptr;
         for (Block &b : v)
-            if (!b.busy && b.bytes >= bytes && (!best || b.bytes < best->bytes)) best = &b;
+            if (!b.busy && b.bytes >= bytes && (max_ratio <= 0.0 || (double)b.bytes <= max_ratio * (double)bytes) &&
+                (!best || b.bytes < best->bytes))
+                best = &b;
         if (best) {
             best->busy = true;
             return best->ptr;
@@ -218,6 +221,19 @@ struct WorkspaceCache {
         }
         v.push_back(Block{p, bytes, true});
         return p;
+    }
+    // frees the idle blocks of a device (room for allocations elsewhere)
+    void release_idle(int device) {
+        std::lock_guard<std::mutex> lock(mu);
+        std::vector<Block> &v = blocks[device];
+        for (auto it = v.begin(); it != v.end();) {
+            if (!it->busy) {
+                (void)hipFree(it->ptr);
+                it = v.erase(it);
+            } else {
+                ++it;
+            }
+        }
     }
     void give_back(int device, void *p) {
         std::lock_guard<std::mutex> lock(mu);
@@ -331,12 +347,18 @@ struct espreso_feti_s {
         if (ggt_ev) (void)hipEventDestroy(ggt_ev);
         if (ggt_st) (void)hipStreamDestroy(ggt_st);
         staging_release();
-        if (asm_pool) {
-            (void)hipStreamSynchronize(st);
+        if (asm_pool || !cached_allocs.empty()) {
+            if (st) (void)hipStreamSynchronize(st);
             if (asm_st) (void)hipStreamSynchronize(asm_st);
+            for (Lane &l : lanes)
+                if (l.s) (void)hipStreamSynchronize(l.s);
+        }
+        if (asm_pool) {
             workspace_cache().give_back(opt.device, asm_pool);
             asm_pool = nullptr;
         }
+        for (void *p : cached_allocs) workspace_cache().give_back(opt.device, p);
+        cached_allocs.clear();
         drop_graphs();
         for (Lane &l : lanes) {
             if (l.ev) (void)hipEventDestroy(l.ev);
@@ -359,21 +381,47 @@ struct espreso_feti_s {
     // allocator): giving back and taking again the tens to hundreds of GB of
     // an instance costs seconds in the driver, reusing pooled memory nothing.
     // Allocations and frees are ordered on st; every other stream of the
-    // instance starts behind an event of st.
+    // instance starts behind an event of st.  Arrays of 256 MB and more come
+    // from the process's block cache instead (a block holding at most twice
+    // the request): the pool does not always find a free block of that size
+    // again, and then the driver's fresh allocation costs a second or more.
     // nullptr when the device has no room (must_fit false), else an error
+    static constexpr size_t CACHED_ALLOC_BYTES = (size_t)256 << 20;
+    std::vector<void *> cached_allocs;
     void *alloc_bytes(size_t bytes, bool must_fit) {
         void *p = nullptr;
         if (bytes == 0) bytes = 8;
+        if (bytes >= CACHED_ALLOC_BYTES && WorkspaceCache::keep()) {
+            // blocks are used on st (and the streams joined to it) only after
+            // this point; the cache holds only blocks whose users have finished
+            const double ta = measure ? wall() : 0.0;
+            p = workspace_cache().acquire(opt.device, bytes, 2.0);
+            if (measure)
+                std::fprintf(stderr, "espreso: cached block %.3f GB in %.4f s%s\n", 1e-9 * bytes, wall() - ta,
+                             p ? "" : " (none: from the pool)");
+            if (p) {
+                cached_allocs.push_back(p);
+                alloc_size[p] = bytes;
+                held_bytes += bytes;
+                return p;
+            }
+        }
         auto get = [&]() { return hipMallocAsync(&p, bytes, st); };
+        const double ta = measure ? wall() : 0.0;
         hipError_t e = get();
+        if (measure && bytes >= ((size_t)256 << 20))
+            std::fprintf(stderr, "espreso: alloc %.3f GB in %.4f s%s\n", 1e-9 * bytes, wall() - ta,
+                         e == hipErrorOutOfMemory ? " (out of memory, trimming the pool)" : "");
         if (e == hipErrorOutOfMemory) {
             // the pools' unused memory may sit in blocks that do not fit the
-            // request: give it back to the driver and try once more
+            // request, the block cache may hold idle blocks: give both back
+            // to the driver and try once more
             (void)hipGetLastError();
             ESP_HIP(hipStreamSynchronize(st));
             hipMemPool_t pool = nullptr;
             ESP_HIP(hipDeviceGetDefaultMemPool(&pool, opt.device));
             ESP_HIP(hipMemPoolTrimTo(pool, 0));
+            workspace_cache().release_idle(opt.device);
             e = get();
         }
         if (e == hipErrorOutOfMemory && !must_fit) {
@@ -477,6 +525,16 @@ struct espreso_feti_s {
         if (it != allocs.end()) {
             (void)hipFreeAsync(p, st);
             allocs.erase(it);
+            held_bytes -= alloc_size[p];
+            alloc_size.erase(p);
+            return;
+        }
+        auto ic = std::find(cached_allocs.begin(), cached_allocs.end(), p);
+        if (ic != cached_allocs.end()) {
+            // its users run on st: they finish before the block is lent again
+            ESP_HIP(hipStreamSynchronize(st));
+            workspace_cache().give_back(opt.device, p);
+            cached_allocs.erase(ic);
             held_bytes -= alloc_size[p];
             alloc_size.erase(p);
         }
