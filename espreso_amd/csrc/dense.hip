@@ -604,6 +604,45 @@ __global__ void __launch_bounds__(256) k_potrf_diag(const DenseDesc *D, int whic
 // column below the diagonal: A_ik = (A_ik - L_iw L_kw^T) inv(L_kk)^T for
 // step < i <= step + band, window w = [max(0, i - band), step) (left-looking),
 // then A_ii -= L_ik L_ik^T (right-looking for the diagonal tiles)
+// Right-looking Cholesky steps for a few large matrices (which 1): the panel
+// only scales the already updated column tiles, L_ik = A_ik Dinv_k^T, and a
+// separate launch updates the whole trailing matrix, A_ij -= L_ik L_jk^T for
+// step < j <= i, with every tile in parallel.  One matrix gets work spread
+// over all tiles at every step instead of one long left-looking update per
+// panel tile (the batched per-subdomain factorisations keep the left-looking
+// steps, which have workgroups enough).
+__global__ void __launch_bounds__(256) k_potrf_panel_rl(const DenseDesc *D, int step) {
+    int bx, by;
+    xcd_tile(bx, by);
+    const DenseDesc &d = D[by];
+    const int nt = d.ntm, n = d.m, lda = d.m;
+    const int i = step + 1 + bx;
+    if (i >= nt) return;
+    const int k0 = step * TILE, i0 = i * TILE;
+    const int nb = min(TILE, n - k0), mi = min(TILE, n - i0);
+    __shared__ double lds[FUSED_LDS];
+    d4 acc[4] = {d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}, d4{0.0, 0.0, 0.0, 0.0}};
+    // C^T = Dinv_k C^T on the transposed tile, i.e. C = C Dinv_k^T
+    strip_finish<true>(acc, lds, d.S + i0 + (size_t)k0 * lda, lda, nb, mi, d.Dinv + (size_t)step * TILE * TILE, 1.0);
+}
+
+__global__ void __launch_bounds__(256) k_potrf_trail(const DenseDesc *D, int step) {
+    int bx, by;
+    xcd_tile(bx, by);
+    const DenseDesc &d = D[by];
+    const int q = bx;
+    int I = (int)((sqrtf(8.0f * q + 1.0f) - 1.0f) * 0.5f);
+    while ((I + 1) * (I + 2) / 2 <= q) I++;
+    while (I * (I + 1) / 2 > q) I--;
+    const int J = q - I * (I + 1) / 2;
+    const int i = step + 1 + I, j = step + 1 + J;
+    if (i >= d.ntm) return;
+    const int n = d.m, lda = d.m, k0 = step * TILE, i0 = i * TILE, j0 = j * TILE;
+    const int nb = min(TILE, n - k0), mi = min(TILE, n - i0), mj = min(TILE, n - j0);
+    tile_gemm<false, true>(d.S + i0 + (size_t)j0 * lda, lda, d.S + i0 + (size_t)k0 * lda, lda,
+                           d.S + j0 + (size_t)k0 * lda, lda, mi, mj, nb, -1.0, 1.0);
+}
+
 __global__ void __launch_bounds__(256) k_potrf_panel(const DenseDesc *D, int which, int step) {
     int bx, by;
     xcd_tile(bx, by);
@@ -1288,6 +1327,16 @@ void launch_potrf_diag(const DenseDesc *desc, int nb, int which, int step, hipSt
 void launch_potrf_panel(const DenseDesc *desc, int nb, int which, int step, int max_band, hipStream_t st) {
     if (max_band <= 0) return;
     hipLaunchKernelGGL(k_potrf_panel, dim3(max_band, nb), dim3(256), 0, st, desc, which, step);
+}
+
+void launch_potrf_rl(const DenseDesc *desc, int nb, int max_ntm, hipStream_t st) {
+    for (int k = 0; k < max_ntm; k++) {
+        launch_potrf_diag(desc, nb, 1, k, st);
+        const int T = max_ntm - k - 1;
+        if (T <= 0) break;
+        hipLaunchKernelGGL(k_potrf_panel_rl, dim3(T, nb), dim3(256), 0, st, desc, k);
+        hipLaunchKernelGGL(k_potrf_trail, dim3(T * (T + 1) / 2, nb), dim3(256), 0, st, desc, k);
+    }
 }
 
 void launch_potrf_step(const DenseDesc *desc, int nb, int which, int step, int max_band, hipStream_t st) {

@@ -1585,9 +1585,10 @@ void espreso_feti_s::coarse() {
     d.ntm = nt;
     d.ntc = nt;
     DenseDesc *dd = upload(std::vector<DenseDesc>{d});
-    // one matrix: W = L^-1 by recursive doubling (two launches per level over
-    // all tiles) instead of one forward-substitution launch per tile row
-    for (int k = 0; k < nt; k++) launch_potrf_step(dd, 1, 1, k, nt, st);
+    // one matrix: right-looking Cholesky steps (the trailing update over all
+    // tiles) and W = L^-1 by recursive doubling (two launches per level over
+    // all tiles) instead of left-looking panels and one substitution per row
+    launch_potrf_rl(dd, 1, nt, st);
     launch_trtri(dd, 1, nt, st);
     launch_syrk(dd, 1, 1, nt, st);
     launch_symmetrize(S, nc, st);
@@ -2762,7 +2763,7 @@ void espreso_feti_s::invert_spd_batched(double *pool, const std::vector<int64_t>
     DenseDesc *dd = upload(desc);
     // few matrices: W = L^-1 by recursive doubling (the forward substitution
     // would be one short launch per tile row)
-    for (int k = 0; k < max_nt; k++) launch_potrf_step(dd, nb, 1, k, max_nt, st);
+    launch_potrf_rl(dd, nb, max_nt, st);
     launch_trtri(dd, nb, max_nt, st);
     launch_syrk(dd, nb, 1, max_nt, st);
     launch_symmetrize_desc(dd, nb, max_n, st);

@@ -95,6 +95,9 @@ void launch_potrf_step(const DenseDesc *desc, int nb, int which, int step, int m
 // the two halves of a step: factor and invert the diagonal tile, then the panel below it
 void launch_potrf_diag(const DenseDesc *desc, int nb, int which, int step, hipStream_t st);
 void launch_potrf_panel(const DenseDesc *desc, int nb, int which, int step, int max_band, hipStream_t st);
+// whole right-looking Cholesky of S (which 1) for a few large matrices: per
+// step the diagonal tile, the scaled column below it and the trailing update
+void launch_potrf_rl(const DenseDesc *desc, int nb, int max_ntm, hipStream_t st);
 void launch_trsm_step(const DenseDesc *desc, int nb, int which, int step, int max_band, int max_ntc,
                       hipStream_t st);
 void launch_syrk(const DenseDesc *desc, int nb, int which, int max_ntm, hipStream_t st);
