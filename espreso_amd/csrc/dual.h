@@ -124,6 +124,7 @@ struct PrimalDev {
     double *band = nullptr, *yf = nullptr, *vals = nullptr, *R = nullptr, *alpha = nullptr, *work = nullptr;
     int64_t *asub = nullptr;
     int *perm = nullptr, *iperm = nullptr, *rowptr = nullptr, *colidx = nullptr;
+    int max_ni = 0;  // largest interior DOF count (host value: launch grids)
 };
 
 // y_b (D.yb) = A x_b for every subdomain; A = S_D (sd) or F_i in storage
@@ -205,9 +206,12 @@ void launch_stream_read(const double *src, int64_t n, int wgs, double *out, hipS
 void launch_superdir(const DualDev &D, int64_t n_near, const int64_t *sib_ptr, const int64_t *sib_col,
                      const double *sib_val, const double *sdiag, const int64_t *sbi_ptr, const int64_t *sbi_col,
                      const double *sbi_val, double *t, const double *xb, double *yb, hipStream_t st);
-// KKT sums per subdomain (out: 2 per subdomain) and the boundary values of u (ub)
-void launch_kkt_primal(const PrimalDev &P, int nsub, const double *u, const double *f, const double *bt1,
-                       const double *bt0, double *ub, double *out, hipStream_t st);
+// KKT sums per subdomain (out: 2 per subdomain) and the boundary values of u
+// (ub); part holds 2 nsub kkt_groups(max_n) partial sums (max_n: the largest
+// subdomain's DOF count)
+int kkt_groups(int max_n);
+void launch_kkt_primal(const PrimalDev &P, int nsub, int max_n, const double *u, const double *f, const double *bt1,
+                       const double *bt0, double *ub, double *part, double *out, hipStream_t st);
 // bt0 += B0^T lambda0 (C.lambda0) on the boundary (bt0 cleared by the caller)
 void launch_b0t(const DualDev &D, const ClusterDev &C, double *bt0, hipStream_t st);
 // u <- trunc(u D) / D (FETISolver::solve output truncation); partial: 256 doubles

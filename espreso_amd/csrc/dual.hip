@@ -1203,7 +1203,53 @@ __global__ void __launch_bounds__(256) k_rhs_prepare(const PrimalDev P, const do
 }
 
 // ---- primal recovery (K at once): u_int = K_ii^-1 (f_int - K_ib u_b) via the
-// banded factor, u = [u_int; u_b] + R alpha in local DOF order
+// banded factor, u = [u_int; u_b] + R alpha in local DOF order.
+// First z = K_ib u_b on the interior rows (z[r K + t], r in factor order):
+// workgroup (g, s) takes interior rows 64g..64g+63 of subdomain s, 16 lanes
+// per CSR row reading consecutive entries (the same product in one thread
+// per row would chain three dependent loads per entry).
+constexpr int PRIMAL_ROWS = 64;
+template <int K>
+__global__ void __launch_bounds__(256) k_primal_kib(const PrimalDev P, const double *ub, int64_t bstr, double *work) {
+    const int s = blockIdx.y, g = blockIdx.x;
+    const int ni = P.ni[s];
+    const int *perm = P.perm + P.perm_off[s];
+    const int *iperm = P.iperm + P.perm_off[s];
+    const int *rp = P.rowptr + P.rp_off[s];
+    const int *ci = P.colidx + P.nz_off[s];
+    const double *kv = P.vals + P.nz_off[s];
+    const double *ubs = ub + P.boff[s];
+    double *z = work + P.work_off[s] * K;
+    const int grp = threadIdx.x >> 4, l = threadIdx.x & 15;
+#pragma unroll
+    for (int k = 0; k < PRIMAL_ROWS / 16; k++) {
+        const int r = g * PRIMAL_ROWS + 16 * k + grp;
+        if (r >= ni) break;  // uniform over the group
+        const int lr = perm[r];
+        double v[K];
+#pragma unroll
+        for (int t = 0; t < K; t++) v[t] = 0.0;
+        for (int q = rp[lr] + l; q < rp[lr + 1]; q += 16) {
+            const int pc = iperm[ci[q]];
+            if (pc >= ni)
+#pragma unroll
+                for (int t = 0; t < K; t++) v[t] += kv[q] * ubs[t * bstr + pc - ni];
+        }
+#pragma unroll
+        for (int t = 0; t < K; t++)
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) v[t] += __shfl_xor(v[t], o, 16);
+        if (l < K) {
+            double x = v[0];
+#pragma unroll
+            for (int t = 1; t < K; t++)
+                if (l == t) x = v[t];
+            z[(int64_t)r * K + l] = x;
+        }
+    }
+}
+
+// then, one workgroup per subdomain, the band solves and the assembly of u
 template <int K>
 __global__ void __launch_bounds__(256) k_primal_interior(const PrimalDev P, const double *ub, int64_t bstr,
                                                          const double *yf, int64_t ystr, const double *alpha,
@@ -1220,23 +1266,8 @@ __global__ void __launch_bounds__(256) k_primal_interior(const PrimalDev P, cons
     const double *ubs = ub + P.boff[s];
     double *z = work + P.work_off[s] * K;
     __shared__ double Ds[64 * BAND_DL];
-    // z = K_ib u_b (interior rows), then z <- y_f - L^-1 z = L^-1 (f_int - K_ib u_b)
+    // z = K_ib u_b (k_primal_kib), then z <- y_f - L^-1 z = L^-1 (f_int - K_ib u_b)
     // and z <- L^-T z
-    for (int r = threadIdx.x; r < ni; r += blockDim.x) {
-        const int lr = perm[r];
-        double v[K];
-#pragma unroll
-        for (int q = 0; q < K; q++) v[q] = 0.0;
-        for (int q = rp[lr]; q < rp[lr + 1]; q++) {
-            const int pc = iperm[ci[q]];
-            if (pc >= ni)
-#pragma unroll
-                for (int t = 0; t < K; t++) v[t] += kv[q] * ubs[t * bstr + pc - ni];
-        }
-#pragma unroll
-        for (int q = 0; q < K; q++) z[(int64_t)r * K + q] = v[q];
-    }
-    __syncthreads();
     band_lower_solve<K>(L, w, bw, ni, z, Ds);
     for (int i = threadIdx.x; i < ni * K; i += blockDim.x) z[i] = yf[(i % K) * ystr + P.yf_off[s] + i / K] - z[i];
     __syncthreads();
@@ -1382,6 +1413,8 @@ void launch_gather_values(int64_t n, const int64_t *pos, const double *src, doub
 }
 
 void launch_primal(const PrimalDev &P, int nsub, const double *ub, double *u, hipStream_t st) {
+    hipLaunchKernelGGL(k_primal_kib<1>, dim3((P.max_ni + PRIMAL_ROWS - 1) / PRIMAL_ROWS, nsub), dim3(256), 0, st, P,
+                       ub, (int64_t)0, P.work);
     hipLaunchKernelGGL(k_primal_interior<1>, dim3(nsub), dim3(256), 0, st, P, ub, (int64_t)0, P.yf, (int64_t)0,
                        P.alpha, (int64_t)0, u, (int64_t)0, P.work);
 }
@@ -1396,6 +1429,8 @@ void launch_primal_multi(const PrimalDev &P, int nsub, int nr, const double *ub,
         double *uv = u + v0 * ustr;
 #define ESP_PRIMAL_CASE(KK)                                                                                        \
     case KK:                                                                                                       \
+        hipLaunchKernelGGL(k_primal_kib<KK>, dim3((P.max_ni + PRIMAL_ROWS - 1) / PRIMAL_ROWS, nsub), dim3(256), 0,  \
+                           st, P, bv, bstr, work);                                                                 \
         hipLaunchKernelGGL(k_primal_interior<KK>, dim3(nsub), dim3(256), 0, st, P, bv, bstr, yv, ystr, av, astr, uv, \
                            ustr, work);                                                                            \
         break;
@@ -1441,38 +1476,64 @@ void launch_superdir(const DualDev &D, int64_t n_near, const int64_t *sib_ptr, c
 }
 
 // ---- KKT report (itersolver.cpp:199-251) ----
-// One workgroup per subdomain: r = K_i u_i - (f_i - B1_i^T lambda) + B0_i^T lambda0
-// over the local DOFs; out[2s] = sum r^2, out[2s+1] = sum (f_i - B1_i^T lambda)^2.
+// r = K_i u_i - (f_i - B1_i^T lambda) + B0_i^T lambda0 over the local DOFs of
+// every subdomain; out[2s] = sum r^2, out[2s+1] = sum (f_i - B1_i^T lambda)^2.
 // bt1 = B1^T lambda and bt0 = B0^T lambda0 (may be null) are boundary vectors;
-// the boundary values of u are copied to ub (for B1 u).
+// the boundary values of u are copied to ub (for B1 u).  Workgroup (g, s)
+// takes rows 64g..64g+63 of subdomain s, 16 lanes per CSR row (the lanes of
+// a group read consecutive entries of the row, 16 groups read 16 consecutive
+// rows), and writes its two sums to part[2 (s G + g)]; k_kkt_sums adds the G
+// partials of a subdomain in order (deterministic).
+constexpr int KKT_ROWS = 64;
 __global__ void __launch_bounds__(256) k_kkt_primal(const PrimalDev P, const double *u, const double *f,
-                                                    const double *bt1, const double *bt0, double *ub, double *out) {
+                                                    const double *bt1, const double *bt0, double *ub, double *part) {
     __shared__ double red[4];
-    const int s = blockIdx.x;
+    const int s = blockIdx.y, g = blockIdx.x, G = gridDim.x;
     const int n = P.n[s], ni = P.ni[s];
     const int64_t uo = P.perm_off[s], ro = P.rp_off[s], zo = P.nz_off[s], bo = P.boff[s];
+    const int grp = threadIdx.x >> 4, l = threadIdx.x & 15;
     double a1 = 0.0, a2 = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) {
+#pragma unroll
+    for (int k = 0; k < KKT_ROWS / 16; k++) {
+        const int i = g * KKT_ROWS + 16 * k + grp;
+        if (i >= n) break;  // uniform over the group
         double ku = 0.0;
-        for (int q = P.rowptr[ro + i]; q < P.rowptr[ro + i + 1]; q++) ku += P.vals[zo + q] * u[uo + P.colidx[zo + q]];
-        double t = f[uo + i], c0 = 0.0;
-        const int pi = P.iperm[uo + i];
-        if (pi >= ni) {
-            const int64_t j = bo + pi - ni;
-            t -= bt1[j];
-            if (bt0) c0 = bt0[j];
-            ub[j] = u[uo + i];
+        const int q1 = P.rowptr[ro + i + 1];
+        for (int q = P.rowptr[ro + i] + l; q < q1; q += 16) ku += P.vals[zo + q] * u[uo + P.colidx[zo + q]];
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) ku += __shfl_xor(ku, o, 16);
+        if (l == 0) {
+            double t = f[uo + i], c0 = 0.0;
+            const int pi = P.iperm[uo + i];
+            if (pi >= ni) {
+                const int64_t j = bo + pi - ni;
+                t -= bt1[j];
+                if (bt0) c0 = bt0[j];
+                ub[j] = u[uo + i];
+            }
+            const double r = ku - t + c0;
+            a1 += r * r;
+            a2 += t * t;
         }
-        const double r = ku - t + c0;
-        a1 += r * r;
-        a2 += t * t;
     }
     a1 = block_sum(a1, red);
     a2 = block_sum(a2, red);
     if (threadIdx.x == 0) {
-        out[2 * s] = a1;
-        out[2 * s + 1] = a2;
+        part[2 * ((int64_t)s * G + g)] = a1;
+        part[2 * ((int64_t)s * G + g) + 1] = a2;
     }
+}
+
+__global__ void k_kkt_sums(int nsub, int G, const double *part, double *out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsub) return;
+    double a1 = 0.0, a2 = 0.0;
+    for (int g = 0; g < G; g++) {
+        a1 += part[2 * ((int64_t)s * G + g)];
+        a2 += part[2 * ((int64_t)s * G + g) + 1];
+    }
+    out[2 * s] = a1;
+    out[2 * s + 1] = a2;
 }
 
 // bt0 = B0^T lambda0 on the boundary: one thread per subdomain walks its B0
@@ -1486,9 +1547,13 @@ __global__ void k_b0t(int nsub, const int64_t *boff, const ClusterDev C, double 
     }
 }
 
-void launch_kkt_primal(const PrimalDev &P, int nsub, const double *u, const double *f, const double *bt1,
-                       const double *bt0, double *ub, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_kkt_primal, dim3(nsub), dim3(256), 0, st, P, u, f, bt1, bt0, ub, out);
+int kkt_groups(int max_n) { return std::max(1, (max_n + KKT_ROWS - 1) / KKT_ROWS); }
+
+void launch_kkt_primal(const PrimalDev &P, int nsub, int max_n, const double *u, const double *f, const double *bt1,
+                       const double *bt0, double *ub, double *part, double *out, hipStream_t st) {
+    const int G = kkt_groups(max_n);
+    hipLaunchKernelGGL(k_kkt_primal, dim3(G, nsub), dim3(256), 0, st, P, u, f, bt1, bt0, ub, part);
+    hipLaunchKernelGGL(k_kkt_sums, dim3((nsub + 255) / 256), dim3(256), 0, st, nsub, G, part, out);
 }
 
 void launch_b0t(const DualDev &D, const ClusterDev &C, double *bt0, hipStream_t st) {

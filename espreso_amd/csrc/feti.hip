@@ -310,8 +310,10 @@ struct espreso_feti_s {
     int64_t *d_sib_ptr = nullptr, *d_sib_pos = nullptr, *d_sbi_ptr = nullptr, *d_sbi_pos = nullptr,
             *d_sdiag_pos = nullptr, *d_sib_col = nullptr, *d_sbi_col = nullptr;
     double *d_sib_val = nullptr, *d_sbi_val = nullptr, *d_sdiag = nullptr, *d_snear = nullptr;
-    // KKT report scratch (boundary vectors, lambda vector, per-subdomain sums)
+    // KKT report scratch (boundary vectors, lambda vector, per-subdomain sums,
+    // their partial sums over row groups)
     double *d_kkt = nullptr;
+    int kkt_max_n = 0;
     int measure = 0;                                // ESPRESO_MEASURE: phase times on stderr
     void note(const char *what, double t0) const {
         if (measure) std::fprintf(stderr, "espreso: %-28s %8.4f s\n", what, wall() - t0);
@@ -1145,6 +1147,7 @@ void espreso_feti_s::load() {
         P.iperm = upload_pieces<int>(n_off, [&](int s) { return sub[s].iperm.data(); });
         P.R = upload_pieces<double>(r_off, [&](int s) { return sub[s].R.data(); });
         P.ni = upload(niv);
+        P.max_ni = nsub ? *std::max_element(niv.begin(), niv.end()) : 0;
         P.n = upload(nv);
         P.bw = upload(bwv);
         P.m = D.m;
@@ -1204,7 +1207,8 @@ void espreso_feti_s::load() {
     d_partial = alloc<double>(1024);
     d_scal = alloc<double>(16);
     d_c = upload(c);
-    d_kkt = alloc<double>(2 * nb + nlam + 2 * (int64_t)nsub + 8);
+    for (int s = 0; s < nsub; s++) kkt_max_n = std::max(kkt_max_n, sub[s].n);
+    d_kkt = alloc<double>(2 * nb + nlam + 2 * (int64_t)nsub + 8 + 2 * (int64_t)nsub * kkt_groups(kkt_max_n));
     {
         // algorithmic bytes of one SYMV: the packed triangle in the storage
         // precision, 20 B per B1 entry (index, coefficient, lambda value), y_b
@@ -2006,7 +2010,8 @@ void espreso_feti_s::kkt(const double *lambda, espreso_feti_stats &s) {
         ESP_HIP(hipMemsetAsync(bt0, 0, nb * sizeof(double), st));
         launch_b0t(D, CL, bt0, st);
     }
-    launch_kkt_primal(P, nsub, d_u, d_f, bt1, hybrid ? bt0 : nullptr, ub, sums, st);
+    double *part = d_kkt + 2 * nb + nlam + 2 * (int64_t)nsub + 8;
+    launch_kkt_primal(P, nsub, kkt_max_n, d_u, d_f, bt1, hybrid ? bt0 : nullptr, ub, part, sums, st);
     launch_lambda_gather(D, ub, false, yl, st);
     launch_axpby(yl, yl, d_c, nlam, 1.0, -1.0, st);
     launch_dot(yl, yl, nlam, d_partial, sums + 2 * nsub, st);
