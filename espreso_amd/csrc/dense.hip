@@ -5,7 +5,8 @@
 //   K_ii = L L^T                       (block-banded right-looking Cholesky)
 //   [Y | y_f] = L^-1 [K_ib | f_i]      (blocked forward substitution, zero
 //                                       leading tiles of each column skipped)
-//   S = K_bb - Y^T Y                   (SYRK; this is the Dirichlet S_D)
+//   S = K_bb - Y^T Y                   (SYRK writing -Y^T Y, then the CSR
+//                                       entries of K_bb added; the Dirichlet S_D)
 //   S + rho Q Q^T = L_s L_s^T          (Q = orth(R_b), rho = max diag S)
 //   W = L_s^-1,  F = W^T W             (F = (K^+)_bb, the local dual operator)
 // which is the reference's S_i = B_i K_i^+ B_i^T (clustercpu.cpp:11-40 via
@@ -755,7 +756,7 @@ __global__ void __launch_bounds__(256, 4) k_trsm_row1(const DenseDesc *D, int wh
     strip_finish<false>(acc, lds, C, n, mi, nj, d.Dinv + (size_t)i * TILE * TILE, which == 1 ? 0.0 : 1.0);
 }
 
-// lower tiles of  S = S - Y^T Y  (which 0)  or  S = W^T W  (which 1)
+// lower tiles of  S = -Y^T Y  (which 0)  or  S = W^T W  (which 1)
 __global__ void __launch_bounds__(256) k_syrk(const DenseDesc *D, int which) {
     int bx, by;
     xcd_tile(bx, by);
@@ -770,11 +771,12 @@ __global__ void __launch_bounds__(256) k_syrk(const DenseDesc *D, int which) {
     const int mi = min(TILE, m - i0), mj = min(TILE, m - j0);
     double *C = d.S + i0 + (size_t)j0 * m;
     if (which == 0) {
-        int ks = max(d.colstart[I], d.colstart[J]) * TILE;
-        if (ks >= d.ni) return;
+        // S = -Y^T Y (K_bb is added by k_add_kbb); a tile with no coupled
+        // interior rows (K = 0) is written as zeros
+        const int ks = min(max(d.colstart[I], d.colstart[J]) * TILE, d.ni);
         const double *Y = d.Kib;
         tile_gemm<true, false>(C, m, Y + ks + (size_t)i0 * d.ni, d.ni, Y + ks + (size_t)j0 * d.ni, d.ni, mi, mj,
-                               d.ni - ks, -1.0, 1.0);
+                               d.ni - ks, -1.0, 0.0);
     } else {
         int ks = i0;  // W is lower triangular: rows < max(I,J)*64 of these columns vanish
         tile_gemm<true, false>(C, m, d.W + ks + (size_t)i0 * m, m, d.W + ks + (size_t)j0 * m, m, mi, mj, m - ks,
@@ -1047,9 +1049,9 @@ __global__ void k_unpack(const T *pool, int64_t off, int m, float scale, double 
 // Dense inputs of the assembly straight from the CSR rows, zeros included, in
 // one coalesced pass (K is symmetric, so row perm[c] of K holds permuted
 // column c): for interior column c the lower band tiles of K_ii, for boundary
-// column j all of K_ib(:, j) and the rows of S(:, j) from its diagonal tile
-// down, and the permuted f as column m of [K_ib | f].  Every entry a later
-// kernel reads is written here, so the workspace needs no zero fill.
+// column j K_ib(:, j) from its first coupled row, and the permuted f as column
+// m of [K_ib | f].  Every entry a later kernel reads is written here or by the
+// SYRK (S), so the workspace needs no zero fill.
 // One wave per column (4 columns per workgroup), each wave assembling its column
 // in its own LDS window of MAT_W rows; only wave-level synchronisation, so the
 // waves of a workgroup proceed independently.
@@ -1087,12 +1089,14 @@ __global__ void __launch_bounds__(256) k_materialize(const DenseDesc *D, const i
         lo = (c / TILE) * TILE;
         hi = min(ni, lo + (d.band_ii + 1) * TILE);
     } else {
-        // [K_ib(:, j) ; S(:, j)] as one permuted column, from the column tile's
-        // first coupled interior row: the rows above it are zero in K_ib and in
-        // Y = L^-1 K_ib, and no kernel reads them (the substitution, the SYRK
-        // and h_f start there)
+        // K_ib(:, j) from the column tile's first coupled interior row: the
+        // rows above it are zero in K_ib and in Y = L^-1 K_ib, and no kernel
+        // reads them (the substitution, the SYRK and h_f start there).  K_bb is
+        // not written here: S = -Y^T Y is written whole by the SYRK and K_bb's
+        // entries are added to it afterwards (k_add_kbb), so the zeros of S
+        // cross HBM once instead of three times.
         lo = d.colstart[(c - ni) / TILE] * TILE;
-        hi = n;
+        hi = ni;
     }
     const int l = pm[c];
     const int q0 = rp[l], q1 = rp[l + 1];
@@ -1106,18 +1110,33 @@ __global__ void __launch_bounds__(256) k_materialize(const DenseDesc *D, const i
             if (pr >= w0 && pr < w1) cb[pr - w0] = kv[q];
         }
         wave_lds_sync();
-        if (c < ni) {
-            double *dst = d.Kii + (size_t)c * d.ldk;
-            for (int r = w0 + lane; r < w1; r += 64) dst[r] = cb[r - w0];
-        } else {
-            const int j = c - ni;
-            const int s0 = ni + (j / TILE) * TILE;  // first S row kept, in the permuted column
-            for (int r = w0 + lane; r < w1; r += 64) {
-                if (r < ni) d.Kib[(size_t)j * ni + r] = cb[r - w0];
-                else if (r >= s0) d.S[(size_t)j * m + (r - ni)] = cb[r - w0];
-            }
-        }
+        double *dst = c < ni ? d.Kii + (size_t)c * d.ldk : d.Kib + (size_t)(c - ni) * ni;
+        for (int r = w0 + lane; r < w1; r += 64) dst[r] = cb[r - w0];
         wave_lds_sync();  // the window is read before the next one clears it
+    }
+}
+
+// S += K_bb on the lower tiles (rows from the diagonal tile of each column
+// down, the region the SYRK writes), after S = -Y^T Y: one wave per boundary
+// column walks its CSR row; the entries of a column are distinct, so no two
+// lanes add to one element.
+__global__ void __launch_bounds__(256) k_add_kbb(const DenseDesc *D, const int *rowptr, const int64_t *rp_off,
+                                                 const int *colidx, const double *vals, const int64_t *nz_off,
+                                                 const int *iperm, const int *perm, const int64_t *perm_off) {
+    const DenseDesc &d = D[blockIdx.y];
+    const int b = blockIdx.y, ni = d.ni, m = d.m;
+    const int lane = threadIdx.x & 63, j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= m) return;
+    const int *rp = rowptr + rp_off[b];
+    const int *ci = colidx + nz_off[b];
+    const double *kv = vals + nz_off[b];
+    const int *ip = iperm + perm_off[b];
+    const int l = perm[perm_off[b] + ni + j];
+    const int s0 = ni + (j / TILE) * TILE;  // first S row of the column's lower tiles, permuted
+    double *col = d.S + (size_t)j * m - ni;
+    for (int q = rp[l] + lane; q < rp[l + 1]; q += 64) {
+        const int pr = ip[ci[q]];
+        if (pr >= s0) col[pr] += kv[q];
     }
 }
 
@@ -1419,6 +1438,13 @@ void launch_unpack(const void *pool, int prec, int64_t off, int m, float scale, 
     else
         hipLaunchKernelGGL((k_unpack<STRIP_B16, uint16_t>), dim3(g), dim3(256), 0, st, (const uint16_t *)pool, off,
                            m, scale, out);
+}
+
+void launch_add_kbb(const DenseDesc *desc, int nb, const int *rowptr, const int64_t *rp_off, const int *colidx,
+                    const double *vals, const int64_t *nz_off, const int *iperm, const int *perm,
+                    const int64_t *perm_off, int max_m, hipStream_t st) {
+    hipLaunchKernelGGL(k_add_kbb, dim3(cdiv(max_m, 4), nb), dim3(256), 0, st, desc, rowptr, rp_off, colidx, vals,
+                       nz_off, iperm, perm, perm_off);
 }
 
 void launch_materialize(const DenseDesc *desc, int nb, const int *rowptr, const int64_t *rp_off, const int *colidx,

@@ -1499,7 +1499,9 @@ void espreso_feti_s::assemble(const double *d_f, const double *d_Q, const std::v
         launch_hf(d_desc, nbt, d_fb, d_hf, d_b_off, max_m, asm_st);
         launch_band_extract(d_desc, nbt, P.band, d_band_off, d_bw, P.yf, d_yf_off, max_ni, asm_st);
         launch_syrk(d_desc, nbt, 0, max_ntm, st);
-        // S_D = S: its row norms (asm_st) and its packed copy (st) both only
+        launch_add_kbb(d_desc, nbt, P.rowptr, d_rp_off, P.colidx, P.vals, d_nz_off, P.iperm, P.perm, d_n_off, max_m,
+                       st);
+        // S_D = S = K_bb - Y^T Y: its row norms (asm_st) and its packed copy (st) both only
         // read S; the kernel regularisation then changes S
         asm_fork();
         launch_rownorm(d_desc, nbt, D.sdnrm + s0, max_m, asm_st);

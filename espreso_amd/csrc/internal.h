@@ -105,8 +105,11 @@ void launch_syrk(const DenseDesc *desc, int nb, int which, int max_ntm, hipStrea
 // inverses in Dinv) by recursive doubling; uses S's upper triangle as scratch
 void launch_trtri(const DenseDesc *desc, int nb, int max_ntm, hipStream_t st);
 void launch_symmetrize_desc(const DenseDesc *desc, int nb, int max_m, hipStream_t st);
-// CSR K_i (local DOFs) into the dense K_ii band, [K_ib|f] and K_bb, zeros included;
+// CSR K_i (local DOFs) into the dense K_ii band and [K_ib|f], zeros included;
 // iperm maps local -> internal index, perm internal -> local
+void launch_add_kbb(const DenseDesc *desc, int nb, const int *rowptr, const int64_t *rp_off, const int *colidx,
+                    const double *vals, const int64_t *nz_off, const int *iperm, const int *perm,
+                    const int64_t *perm_off, int max_m, hipStream_t st);
 void launch_materialize(const DenseDesc *desc, int nb, const int *rowptr, const int64_t *rp_off, const int *colidx,
                         const double *vals, const int64_t *nz_off, const int *iperm, const int *perm,
                         const int64_t *perm_off, const double *f, const int *n, int max_n, hipStream_t st);
