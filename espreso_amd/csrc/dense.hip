@@ -851,6 +851,18 @@ __global__ void k_widen_u16(const uint16_t *s, int *d, int64_t n) {
 }
 
 
+// d[off[s] + k] = img[pos[s] + k] for k < off[s + 1] - off[s]: the 16-bit
+// column indices of subdomain s, at pos[s] in the image of the pinned blocks,
+// widened into their place in the 32-bit CSR array
+__global__ void k_widen_runs(const uint16_t *img, const int64_t *pos, const int64_t *off, int *d) {
+    const int s = blockIdx.y;
+    const int64_t n = off[s + 1] - off[s];
+    const uint16_t *src = img + pos[s];
+    int *dst = d + off[s];
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+        dst[k] = src[k];
+}
+
 __device__ __forceinline__ double block_max(double v) {
     __shared__ double red[256];
     red[threadIdx.x] = v;
@@ -1326,6 +1338,13 @@ void launch_probe_gemm(int ta, int tb, int M, int N, int K, const double *A, int
 
 // ------------------------------------------------------------ launchers ----
 static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+void launch_widen_runs(const uint16_t *img, const int64_t *pos, const int64_t *off, int nsub, int64_t max_n, int *d,
+                       hipStream_t st) {
+    if (nsub <= 0 || max_n <= 0) return;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((max_n + 1023) / 1024, 256));
+    hipLaunchKernelGGL(k_widen_runs, dim3(g, nsub), dim3(256), 0, st, img, pos, off, d);
+}
 
 void launch_widen_u16(const uint16_t *s, int *d, int64_t n, hipStream_t st) {
     if (n <= 0) return;

@@ -1259,10 +1259,6 @@ __global__ void __launch_bounds__(256) k_primal_interior(const PrimalDev P, cons
     const int w = bw + 1;
     const double *L = P.band + P.band_off[s];
     const int *perm = P.perm + P.perm_off[s];
-    const int *iperm = P.iperm + P.perm_off[s];
-    const int *rp = P.rowptr + P.rp_off[s];
-    const int *ci = P.colidx + P.nz_off[s];
-    const double *kv = P.vals + P.nz_off[s];
     const double *ubs = ub + P.boff[s];
     double *z = work + P.work_off[s] * K;
     __shared__ double Ds[64 * BAND_DL];

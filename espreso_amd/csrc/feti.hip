@@ -66,6 +66,12 @@ using hvec = std::vector<T, NoInit<T>>;
 struct SubHost {
     int n = 0, ni = 0, m = 0, kdim = 0;
     std::vector<int> rowptr, colidx;
+    // column indices of a subdomain of at most 65536 DOFs, as 16-bit values
+    // in pinned host memory (then colidx stays empty unless a 32-bit upload
+    // needs it); col(k) reads whichever copy the subdomain has
+    const uint16_t *col16 = nullptr;
+    int64_t pin_pos = 0;  // position of col16 in the instance's pinned blocks laid end to end (16-bit units)
+    int col(size_t k) const { return col16 ? (int)col16[k] : colidx[k]; }
     std::vector<double> vals, f, R;
     std::vector<int> lam, dof;
     std::vector<double> sign, scale;
@@ -255,6 +261,40 @@ WorkspaceCache &workspace_cache() {
     return c;
 }
 
+// Pinned host blocks that receive the 16-bit column indices of the K_i CSR
+// at the handoff, so that they cross the link straight from there (no staging
+// copy), kept for the process once allocated and lent to the instances:
+// pinning a gigabyte costs tens of milliseconds.
+struct PinnedPool {
+    static constexpr size_t BLOCK = (size_t)64 << 20;
+    std::mutex mu;
+    std::vector<void *> idle;
+    void *acquire() {
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            if (!idle.empty()) {
+                void *p = idle.back();
+                idle.pop_back();
+                return p;
+            }
+        }
+        void *p = nullptr;
+        if (hipHostMalloc(&p, BLOCK, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        return p;
+    }
+    void give_back(void *p) {
+        std::lock_guard<std::mutex> lock(mu);
+        idle.push_back(p);
+    }
+};
+PinnedPool &pinned_pool() {
+    static PinnedPool p;
+    return p;
+}
+
 struct espreso_feti_s {
     espreso_feti_options opt{};
     int nsub = 0, nlam = -1, ncoarse = 0;
@@ -314,6 +354,28 @@ struct espreso_feti_s {
     // their partial sums over row groups)
     double *d_kkt = nullptr;
     int kkt_max_n = 0;
+    // pinned blocks holding the subdomains' 16-bit column indices (bump
+    // allocation; espreso_feti_set_subdomain may run in parallel)
+    std::mutex pin_mu;
+    std::vector<void *> pin_blocks;
+    size_t pin_used = PinnedPool::BLOCK;
+    // n 16-bit values in pinned memory; nullptr when none is left or the
+    // request exceeds a block (the caller keeps 32-bit indices then)
+    uint16_t *pin_u16(size_t n, int64_t &pos) {
+        const size_t bytes = (n * sizeof(uint16_t) + 63) & ~(size_t)63;
+        if (bytes > PinnedPool::BLOCK) return nullptr;
+        std::lock_guard<std::mutex> lock(pin_mu);
+        if (pin_used + bytes > PinnedPool::BLOCK) {
+            void *b = pinned_pool().acquire();
+            if (!b) return nullptr;
+            pin_blocks.push_back(b);
+            pin_used = 0;
+        }
+        uint16_t *p = (uint16_t *)((char *)pin_blocks.back() + pin_used);
+        pos = (int64_t)((pin_blocks.size() - 1) * PinnedPool::BLOCK + pin_used) / (int64_t)sizeof(uint16_t);
+        pin_used += bytes;
+        return p;
+    }
     int measure = 0;                                // ESPRESO_MEASURE: phase times on stderr
     void note(const char *what, double t0) const {
         if (measure) std::fprintf(stderr, "espreso: %-28s %8.4f s\n", what, wall() - t0);
@@ -349,6 +411,11 @@ struct espreso_feti_s {
         if (ggt_ev) (void)hipEventDestroy(ggt_ev);
         if (ggt_st) (void)hipStreamDestroy(ggt_st);
         staging_release();
+        if (!pin_blocks.empty()) {
+            if (st) (void)hipStreamSynchronize(st);  // the uploads from the blocks have finished
+            for (void *b : pin_blocks) pinned_pool().give_back(b);
+            pin_blocks.clear();
+        }
         if (asm_pool || !cached_allocs.empty()) {
             if (st) (void)hipStreamSynchronize(st);
             if (asm_st) (void)hipStreamSynchronize(asm_st);
@@ -683,9 +750,11 @@ void espreso_feti_s::load() {
     ESP_CHECK(opt.preconditioner >= 0 && opt.preconditioner <= 4, ESPRESO_ERR_ARG,
               "preconditioner must be NONE, LUMPED, WEIGHT_FUNCTION, DIRICHLET or SUPER_DIRICHLET (0..4)");
 
-    // The K_i CSR arrays need no preprocessing: their staged upload runs on a
-    // host thread (eight copy threads) beside the host work below, which puts
-    // nothing on the stream before the thread has been joined
+    // The K_i CSR arrays need no preprocessing: their upload (the 16-bit
+    // column indices straight from the pinned blocks they were handed over
+    // into, the rest staged by eight copy threads) runs on a host thread beside
+    // the host work below, which puts nothing on the stream before the thread
+    // has been joined
     std::vector<int64_t> nz_off(nsub + 1, 0), rp_off(nsub + 1, 0);
     for (int s = 0; s < nsub; s++) {
         nz_off[s + 1] = nz_off[s] + sub[s].rowptr[sub[s].n];
@@ -699,23 +768,50 @@ void espreso_feti_s::load() {
     // the device
     int max_n_in = 0;
     for (int s = 0; s < nsub; s++) max_n_in = std::max(max_n_in, sub[s].n);
-    uint16_t *colidx16 = max_n_in <= 65536 ? alloc<uint16_t>(nz_off[nsub]) : nullptr;
+    int n_pinned = 0;
+    for (int s = 0; s < nsub; s++) n_pinned += sub[s].col16 != nullptr;
+    // every subdomain's indices pinned at the handoff: the pinned blocks go up
+    // whole (a few large copies; one copy per subdomain costs more in calls
+    // than in bytes) into an image of the blocks, and a gather widens each
+    // subdomain's run into place on the device
+    const bool all_pinned = max_n_in <= 65536 && n_pinned == nsub && nsub > 0;
+    if (measure)
+        std::fprintf(stderr, "espreso: load: 16-bit column indices pinned at the handoff: %d of %d subdomains\n",
+                     n_pinned, nsub);
+    uint16_t *pin_img = all_pinned ? alloc<uint16_t>(pin_blocks.size() * (PinnedPool::BLOCK / sizeof(uint16_t))) : nullptr;
+    uint16_t *colidx16 = max_n_in <= 65536 && !all_pinned ? alloc<uint16_t>(nz_off[nsub]) : nullptr;
     std::exception_ptr csr_err;
-    double csr_t[2] = {0, 0};  // staging buffers ready, last chunk queued (measure mode)
+    double csr_t[3] = {0, 0, 0};  // staging buffers ready, row pointers queued, last chunk queued (measure mode)
     struct Joined {
         std::thread t;
         ~Joined() {
             if (t.joinable()) t.join();
         }
     } csr_up;
+    // 16-bit indices already pinned for every subdomain go up as they are
     csr_up.t = std::thread([&] {
         try {
             ESP_HIP(hipSetDevice(opt.device));
             staging_ready();
             csr_t[0] = wall() - t0;
             stage_pieces(P.rowptr, rp_off, [&](int s) { return sub[s].rowptr.data(); }, st, 8);
-            if (colidx16) stage_pieces(colidx16, nz_off, [&](int s) { return sub[s].colidx.data(); }, st, 8);
-            else stage_pieces(P.colidx, nz_off, [&](int s) { return sub[s].colidx.data(); }, st, 8);
+            csr_t[2] = wall() - t0;
+            if (all_pinned) {
+                for (size_t b = 0; b < pin_blocks.size(); b++)
+                    ESP_HIP(hipMemcpyAsync(pin_img + b * (PinnedPool::BLOCK / sizeof(uint16_t)), pin_blocks[b],
+                                           b + 1 < pin_blocks.size() ? PinnedPool::BLOCK : pin_used,
+                                           hipMemcpyHostToDevice, st));
+            } else {
+                // mixed: the staged path reads 32-bit indices of every subdomain
+#pragma omp parallel for schedule(dynamic) num_threads(8)
+                for (int s = 0; s < nsub; s++) {
+                    SubHost &h = sub[s];
+                    if (h.col16 && h.colidx.empty())
+                        h.colidx.assign(h.col16, h.col16 + h.rowptr[h.n]);
+                }
+                if (colidx16) stage_pieces(colidx16, nz_off, [&](int s) { return sub[s].colidx.data(); }, st, 8);
+                else stage_pieces(P.colidx, nz_off, [&](int s) { return sub[s].colidx.data(); }, st, 8);
+            }
             // the structured grid assembles the values on the device
             if (!grid_dev) stage_pieces(P.vals, nz_off, [&](int s) { return sub[s].vals.data(); }, st, 8);
             csr_t[1] = wall() - t0;
@@ -751,7 +847,7 @@ void espreso_feti_s::load() {
             const int lr = h.perm[r];
             int fr = r;
             for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) {
-                const int pc = h.iperm[h.colidx[k]];
+                const int pc = h.iperm[h.col(k)];
                 if (pc < h.ni) fr = std::min(fr, pc);
                 else first_int[pc - h.ni] = std::min(first_int[pc - h.ni], r);
             }
@@ -891,7 +987,7 @@ void espreso_feti_s::load() {
         int64_t cnt = 0;
         for (int j = 0; j < h.m; j++) {
             const int lr = h.bdofs[j];
-            for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) cnt += h.iperm[h.colidx[k]] >= h.ni;
+            for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) cnt += h.iperm[h.col(k)] >= h.ni;
         }
         kbb_cnt[s] = cnt;
     }
@@ -922,12 +1018,12 @@ void espreso_feti_s::load() {
             for (int r = 0; r < h.ni; r++) {
                 const int lr = h.perm[r];
                 int nbc = 0;
-                for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) nbc += h.iperm[h.colidx[k]] >= h.ni;
+                for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) nbc += h.iperm[h.col(k)] >= h.ni;
                 if (nbc) { near_of[s].push_back(r); cib += nbc; }
             }
             for (int j = 0; j < h.m; j++) {
                 const int lr = h.bdofs[j];
-                for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) cbi += h.iperm[h.colidx[k]] < h.ni;
+                for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) cbi += h.iperm[h.col(k)] < h.ni;
             }
             sib_off[s + 1] = cib;
             sbi_off[s + 1] = cbi;
@@ -948,7 +1044,7 @@ void espreso_feti_s::load() {
         for (int j = 0; j < h.m; j++) {
             const int lr = h.bdofs[j];
             for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) {
-                const int pc = h.iperm[h.colidx[k]];
+                const int pc = h.iperm[h.col(k)];
                 if (pc >= h.ni) {
                     kbb_col[q] = boff[s] + pc - h.ni;
                     kbb_pos[q] = h_nzoff[s] + k;
@@ -964,7 +1060,7 @@ void espreso_feti_s::load() {
         for (size_t a = 0; a < near_of[s].size(); a++) {
             const int r = near_of[s][a], lr = h.perm[r];
             for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) {
-                const int pc = h.iperm[h.colidx[k]];
+                const int pc = h.iperm[h.col(k)];
                 if (pc >= h.ni) {
                     sib_col[qi] = boff[s] + pc - h.ni;
                     sib_pos[qi] = h_nzoff[s] + k;
@@ -979,7 +1075,7 @@ void espreso_feti_s::load() {
         for (int j = 0; j < h.m; j++) {
             const int lr = h.bdofs[j];
             for (int k = h.rowptr[lr]; k < h.rowptr[lr + 1]; k++) {
-                const int pc = h.iperm[h.colidx[k]];
+                const int pc = h.iperm[h.col(k)];
                 if (pc < h.ni) {
                     sbi_col[qb] = near_off[s] + near_idx[pc];
                     sbi_pos[qb] = h_nzoff[s] + k;
@@ -1021,12 +1117,25 @@ void espreso_feti_s::load() {
         launch_widen_u16(colidx16, P.colidx, nz_off[nsub], st);
         release(colidx16);
     }
+    if (pin_img) {
+        std::vector<int64_t> pos(nsub);
+        int64_t max_nnz = 0;
+        for (int s = 0; s < nsub; s++) {
+            pos[s] = sub[s].pin_pos;
+            max_nnz = std::max(max_nnz, nz_off[s + 1] - nz_off[s]);
+        }
+        int64_t *d_pos = upload(pos), *d_off = upload(nz_off);
+        launch_widen_runs(pin_img, d_pos, d_off, nsub, max_nnz, P.colidx, st);
+        release(d_pos);
+        release(d_off);
+        release(pin_img);
+    }
     if (measure) {
         sync();
         std::fprintf(stderr,
-                     "espreso: load: K CSR staging buffers at %.4f s, last chunk queued at %.4f s (host copies %.4f s, "
-                     "waits for the copy engine %.4f s)\n",
-                     csr_t[0], csr_t[1], stage_copy_s, stage_wait_s);
+                     "espreso: load: K CSR staging buffers at %.4f s, row pointers queued at %.4f s, last chunk queued "
+                     "at %.4f s (host copies %.4f s, waits for the copy engine %.4f s)\n",
+                     csr_t[0], csr_t[2], csr_t[1], stage_copy_s, stage_wait_s);
     }
     note("load: K CSR upload joined", t0);
     D.nsub = nsub;
@@ -2868,14 +2977,28 @@ int espreso_feti_set_subdomain(espreso_feti *h, int32_t i, int32_t n, const int3
     s.rowptr.assign(rowptr, rowptr + n + 1);
     ESP_CHECK(s.rowptr[0] == 0, ESPRESO_ERR_ARG, "CSR must be 0-based");
     const int nnz = s.rowptr[n];
-    s.colidx.assign(colidx, colidx + nnz);
+    // subdomains of at most 65536 DOFs keep 16-bit column indices in pinned
+    // memory (read once here for the range check, then uploaded from there)
+    uint16_t *c16 = n <= 65536 ? h->pin_u16((size_t)nnz, s.pin_pos) : nullptr;
+    if (c16) {
+        std::vector<int>().swap(s.colidx);
+        for (int q = 0; q < nnz; q++) {
+            const int c = colidx[q];
+            ESP_CHECK(c >= 0 && c < n, ESPRESO_ERR_ARG, "CSR column out of range");
+            c16[q] = (uint16_t)c;
+        }
+    } else {
+        s.colidx.assign(colidx, colidx + nnz);
+        for (int q = 0; q < nnz; q++)
+            ESP_CHECK(s.colidx[q] >= 0 && s.colidx[q] < n, ESPRESO_ERR_ARG, "CSR column out of range");
+    }
+    s.col16 = c16;
     ESP_CHECK(values || h->grid_dev, ESPRESO_ERR_ARG, "null K values");
     if (values) s.vals.assign(values, values + nnz);
     else s.vals.clear();  // assembled on the device (structured grid)
-    for (int q = 0; q < nnz; q++) {
-        ESP_CHECK(s.colidx[q] >= 0 && s.colidx[q] < n, ESPRESO_ERR_ARG, "CSR column out of range");
-        ESP_CHECK(!values || std::isfinite(s.vals[q]), ESPRESO_ERR_NUMERIC, "non-finite value in K");
-    }
+    if (values)
+        for (int q = 0; q < nnz; q++)
+            ESP_CHECK(std::isfinite(s.vals[q]), ESPRESO_ERR_NUMERIC, "non-finite value in K");
     s.f.assign(f, f + n);
     for (double v : s.f) ESP_CHECK(std::isfinite(v), ESPRESO_ERR_NUMERIC, "non-finite value in f");
     s.kdim = kdim;
