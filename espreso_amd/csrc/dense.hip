@@ -854,13 +854,15 @@ __global__ void k_widen_u16(const uint16_t *s, int *d, int64_t n) {
 // d[off[s] + k] = img[pos[s] + k] for k < off[s + 1] - off[s]: the 16-bit
 // column indices of subdomain s, at pos[s] in the image of the pinned blocks,
 // widened into their place in the 32-bit CSR array
-__global__ void k_widen_runs(const uint16_t *img, const int64_t *pos, const int64_t *off, int *d) {
+// (and the same for the K values, doubles, unconverted)
+template <class S, class D>
+__global__ void k_widen_runs(const S *img, const int64_t *pos, const int64_t *off, D *d) {
     const int s = blockIdx.y;
     const int64_t n = off[s + 1] - off[s];
-    const uint16_t *src = img + pos[s];
-    int *dst = d + off[s];
+    const S *src = img + pos[s];
+    D *dst = d + off[s];
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
-        dst[k] = src[k];
+        dst[k] = (D)src[k];
 }
 
 __device__ __forceinline__ double block_max(double v) {
@@ -1343,7 +1345,14 @@ void launch_widen_runs(const uint16_t *img, const int64_t *pos, const int64_t *o
                        hipStream_t st) {
     if (nsub <= 0 || max_n <= 0) return;
     const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((max_n + 1023) / 1024, 256));
-    hipLaunchKernelGGL(k_widen_runs, dim3(g, nsub), dim3(256), 0, st, img, pos, off, d);
+    hipLaunchKernelGGL((k_widen_runs<uint16_t, int>), dim3(g, nsub), dim3(256), 0, st, img, pos, off, d);
+}
+
+void launch_copy_runs(const double *img, const int64_t *pos, const int64_t *off, int nsub, int64_t max_n, double *d,
+                      hipStream_t st) {
+    if (nsub <= 0 || max_n <= 0) return;
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((max_n + 1023) / 1024, 256));
+    hipLaunchKernelGGL((k_widen_runs<double, double>), dim3(g, nsub), dim3(256), 0, st, img, pos, off, d);
 }
 
 void launch_widen_u16(const uint16_t *s, int *d, int64_t n, hipStream_t st) {

@@ -72,6 +72,10 @@ struct SubHost {
     const uint16_t *col16 = nullptr;
     int64_t pin_pos = 0;  // position of col16 in the instance's pinned blocks laid end to end (16-bit units)
     int col(size_t k) const { return col16 ? (int)col16[k] : colidx[k]; }
+    // the K values in pinned host memory (then vals stays empty), at vpin_pos
+    // in the value blocks laid end to end (doubles)
+    double *val_pin = nullptr;
+    int64_t vpin_pos = 0;
     std::vector<double> vals, f, R;
     std::vector<int> lam, dof;
     std::vector<double> sign, scale;
@@ -354,27 +358,40 @@ struct espreso_feti_s {
     // their partial sums over row groups)
     double *d_kkt = nullptr;
     int kkt_max_n = 0;
-    // pinned blocks holding the subdomains' 16-bit column indices (bump
-    // allocation; espreso_feti_set_subdomain may run in parallel)
+    // pinned blocks holding the subdomains' 16-bit column indices and K values
+    // (bump allocation; espreso_feti_set_subdomain may run in parallel)
+    struct PinArena {
+        std::vector<void *> blocks;
+        size_t used = PinnedPool::BLOCK;  // bytes taken in the last block
+        // bytes in the image of the blocks laid end to end
+        size_t image_bytes() const { return blocks.size() * PinnedPool::BLOCK; }
+    };
     std::mutex pin_mu;
-    std::vector<void *> pin_blocks;
-    size_t pin_used = PinnedPool::BLOCK;
-    // n 16-bit values in pinned memory; nullptr when none is left or the
-    // request exceeds a block (the caller keeps 32-bit indices then)
-    uint16_t *pin_u16(size_t n, int64_t &pos) {
-        const size_t bytes = (n * sizeof(uint16_t) + 63) & ~(size_t)63;
+    PinArena pin_idx, pin_val;
+    // n values of type T in pinned memory of arena a, at pos (in T units) of
+    // its image; nullptr when none is left or the request exceeds a block
+    // (the caller keeps an ordinary host copy then)
+    template <class T>
+    T *pin_alloc(PinArena &a, size_t n, int64_t &pos) {
+        const size_t bytes = (n * sizeof(T) + 63) & ~(size_t)63;
         if (bytes > PinnedPool::BLOCK) return nullptr;
         std::lock_guard<std::mutex> lock(pin_mu);
-        if (pin_used + bytes > PinnedPool::BLOCK) {
+        if (a.used + bytes > PinnedPool::BLOCK) {
             void *b = pinned_pool().acquire();
             if (!b) return nullptr;
-            pin_blocks.push_back(b);
-            pin_used = 0;
+            a.blocks.push_back(b);
+            a.used = 0;
         }
-        uint16_t *p = (uint16_t *)((char *)pin_blocks.back() + pin_used);
-        pos = (int64_t)((pin_blocks.size() - 1) * PinnedPool::BLOCK + pin_used) / (int64_t)sizeof(uint16_t);
-        pin_used += bytes;
+        T *p = (T *)((char *)a.blocks.back() + a.used);
+        pos = (int64_t)((a.blocks.size() - 1) * PinnedPool::BLOCK + a.used) / (int64_t)sizeof(T);
+        a.used += bytes;
         return p;
+    }
+    // the blocks of arena a into dst (an image of a.image_bytes()) on stream s
+    void upload_arena(const PinArena &a, void *dst, hipStream_t s) {
+        for (size_t b = 0; b < a.blocks.size(); b++)
+            ESP_HIP(hipMemcpyAsync((char *)dst + b * PinnedPool::BLOCK, a.blocks[b],
+                                   b + 1 < a.blocks.size() ? PinnedPool::BLOCK : a.used, hipMemcpyHostToDevice, s));
     }
     int measure = 0;                                // ESPRESO_MEASURE: phase times on stderr
     void note(const char *what, double t0) const {
@@ -411,10 +428,12 @@ struct espreso_feti_s {
         if (ggt_ev) (void)hipEventDestroy(ggt_ev);
         if (ggt_st) (void)hipStreamDestroy(ggt_st);
         staging_release();
-        if (!pin_blocks.empty()) {
+        if (!pin_idx.blocks.empty() || !pin_val.blocks.empty()) {
             if (st) (void)hipStreamSynchronize(st);  // the uploads from the blocks have finished
-            for (void *b : pin_blocks) pinned_pool().give_back(b);
-            pin_blocks.clear();
+            for (PinArena *a : {&pin_idx, &pin_val}) {
+                for (void *b : a->blocks) pinned_pool().give_back(b);
+                a->blocks.clear();
+            }
         }
         if (asm_pool || !cached_allocs.empty()) {
             if (st) (void)hipStreamSynchronize(st);
@@ -750,11 +769,12 @@ void espreso_feti_s::load() {
     ESP_CHECK(opt.preconditioner >= 0 && opt.preconditioner <= 4, ESPRESO_ERR_ARG,
               "preconditioner must be NONE, LUMPED, WEIGHT_FUNCTION, DIRICHLET or SUPER_DIRICHLET (0..4)");
 
-    // The K_i CSR arrays need no preprocessing: their upload (the 16-bit
-    // column indices straight from the pinned blocks they were handed over
-    // into, the rest staged by eight copy threads) runs on a host thread beside
-    // the host work below, which puts nothing on the stream before the thread
-    // has been joined
+    // The K_i CSR arrays need no preprocessing: their upload (16-bit column
+    // indices and K values straight from the pinned blocks they were handed
+    // over into, the rest staged by eight copy threads) and their placement on
+    // the device run on a stream of their own, queued by a host thread beside
+    // the host work below; the main stream waits for it only before the first
+    // kernel that reads them, so the other uploads overlap the transfer
     std::vector<int64_t> nz_off(nsub + 1, 0), rp_off(nsub + 1, 0);
     for (int s = 0; s < nsub; s++) {
         nz_off[s + 1] = nz_off[s] + sub[s].rowptr[sub[s].n];
@@ -778,8 +798,36 @@ void espreso_feti_s::load() {
     if (measure)
         std::fprintf(stderr, "espreso: load: 16-bit column indices pinned at the handoff: %d of %d subdomains\n",
                      n_pinned, nsub);
-    uint16_t *pin_img = all_pinned ? alloc<uint16_t>(pin_blocks.size() * (PinnedPool::BLOCK / sizeof(uint16_t))) : nullptr;
+    uint16_t *pin_img = all_pinned ? (uint16_t *)alloc_bytes(pin_idx.image_bytes(), true) : nullptr;
+    int n_vpinned = 0;
+    for (int s = 0; s < nsub; s++) n_vpinned += sub[s].val_pin != nullptr;
+    const bool vals_pinned = !grid_dev && n_vpinned == nsub && nsub > 0;
+    double *vpin_img = vals_pinned ? (double *)alloc_bytes(pin_val.image_bytes(), true) : nullptr;
     uint16_t *colidx16 = max_n_in <= 65536 && !all_pinned ? alloc<uint16_t>(nz_off[nsub]) : nullptr;
+    // where each subdomain's run sits in the images, and the largest run
+    std::vector<int64_t> pin_pos_h(nsub), vpin_pos_h(nsub);
+    int64_t max_nnz = 0;
+    for (int s = 0; s < nsub; s++) {
+        pin_pos_h[s] = sub[s].pin_pos;
+        vpin_pos_h[s] = sub[s].vpin_pos;
+        max_nnz = std::max(max_nnz, nz_off[s + 1] - nz_off[s]);
+    }
+    int64_t *d_pos = (pin_img || vpin_img) ? alloc<int64_t>(3 * (size_t)nsub + 1) : nullptr;
+    // the upload stream starts after the allocations above (stream-ordered on st)
+    struct UpStream {
+        hipStream_t s = nullptr;
+        hipEvent_t ready = nullptr, done = nullptr;
+        ~UpStream() {
+            if (s) (void)hipStreamSynchronize(s);
+            if (ready) (void)hipEventDestroy(ready);
+            if (done) (void)hipEventDestroy(done);
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } up;
+    ESP_HIP(hipStreamCreate(&up.s));
+    ESP_HIP(hipEventCreateWithFlags(&up.ready, hipEventDisableTiming));
+    ESP_HIP(hipEventCreateWithFlags(&up.done, hipEventDisableTiming));
+    ESP_HIP(hipEventRecord(up.ready, st));
     std::exception_ptr csr_err;
     double csr_t[3] = {0, 0, 0};  // staging buffers ready, row pointers queued, last chunk queued (measure mode)
     struct Joined {
@@ -792,15 +840,21 @@ void espreso_feti_s::load() {
     csr_up.t = std::thread([&] {
         try {
             ESP_HIP(hipSetDevice(opt.device));
+            ESP_HIP(hipStreamWaitEvent(up.s, up.ready, 0));
             staging_ready();
             csr_t[0] = wall() - t0;
-            stage_pieces(P.rowptr, rp_off, [&](int s) { return sub[s].rowptr.data(); }, st, 8);
+            stage_pieces(P.rowptr, rp_off, [&](int s) { return sub[s].rowptr.data(); }, up.s, 8);
             csr_t[2] = wall() - t0;
+            if (d_pos) {
+                ESP_HIP(hipMemcpyAsync(d_pos, pin_pos_h.data(), nsub * sizeof(int64_t), hipMemcpyHostToDevice, up.s));
+                ESP_HIP(hipMemcpyAsync(d_pos + nsub, vpin_pos_h.data(), nsub * sizeof(int64_t), hipMemcpyHostToDevice,
+                                       up.s));
+                ESP_HIP(hipMemcpyAsync(d_pos + 2 * nsub, nz_off.data(), (nsub + 1) * sizeof(int64_t),
+                                       hipMemcpyHostToDevice, up.s));
+            }
             if (all_pinned) {
-                for (size_t b = 0; b < pin_blocks.size(); b++)
-                    ESP_HIP(hipMemcpyAsync(pin_img + b * (PinnedPool::BLOCK / sizeof(uint16_t)), pin_blocks[b],
-                                           b + 1 < pin_blocks.size() ? PinnedPool::BLOCK : pin_used,
-                                           hipMemcpyHostToDevice, st));
+                upload_arena(pin_idx, pin_img, up.s);
+                launch_widen_runs(pin_img, d_pos, d_pos + 2 * nsub, nsub, max_nnz, P.colidx, up.s);
             } else {
                 // mixed: the staged path reads 32-bit indices of every subdomain
 #pragma omp parallel for schedule(dynamic) num_threads(8)
@@ -809,11 +863,26 @@ void espreso_feti_s::load() {
                     if (h.col16 && h.colidx.empty())
                         h.colidx.assign(h.col16, h.col16 + h.rowptr[h.n]);
                 }
-                if (colidx16) stage_pieces(colidx16, nz_off, [&](int s) { return sub[s].colidx.data(); }, st, 8);
-                else stage_pieces(P.colidx, nz_off, [&](int s) { return sub[s].colidx.data(); }, st, 8);
+                if (colidx16) {
+                    stage_pieces(colidx16, nz_off, [&](int s) { return sub[s].colidx.data(); }, up.s, 8);
+                    launch_widen_u16(colidx16, P.colidx, nz_off[nsub], up.s);
+                } else {
+                    stage_pieces(P.colidx, nz_off, [&](int s) { return sub[s].colidx.data(); }, up.s, 8);
+                }
             }
             // the structured grid assembles the values on the device
-            if (!grid_dev) stage_pieces(P.vals, nz_off, [&](int s) { return sub[s].vals.data(); }, st, 8);
+            if (vals_pinned) {
+                upload_arena(pin_val, vpin_img, up.s);
+                launch_copy_runs(vpin_img, d_pos + nsub, d_pos + 2 * nsub, nsub, max_nnz, P.vals, up.s);
+            } else if (!grid_dev) {
+#pragma omp parallel for schedule(dynamic) num_threads(8)
+                for (int s = 0; s < nsub; s++) {
+                    SubHost &h = sub[s];
+                    if (h.val_pin && h.vals.empty()) h.vals.assign(h.val_pin, h.val_pin + h.rowptr[h.n]);
+                }
+                stage_pieces(P.vals, nz_off, [&](int s) { return sub[s].vals.data(); }, up.s, 8);
+            }
+            ESP_HIP(hipEventRecord(up.done, up.s));
             csr_t[1] = wall() - t0;
         } catch (...) {
             csr_err = std::current_exception();
@@ -1113,25 +1182,8 @@ void espreso_feti_s::load() {
     note("load: host preprocessing", t0);
     csr_up.t.join();
     if (csr_err) std::rethrow_exception(csr_err);
-    if (colidx16) {
-        launch_widen_u16(colidx16, P.colidx, nz_off[nsub], st);
-        release(colidx16);
-    }
-    if (pin_img) {
-        std::vector<int64_t> pos(nsub);
-        int64_t max_nnz = 0;
-        for (int s = 0; s < nsub; s++) {
-            pos[s] = sub[s].pin_pos;
-            max_nnz = std::max(max_nnz, nz_off[s + 1] - nz_off[s]);
-        }
-        int64_t *d_pos = upload(pos), *d_off = upload(nz_off);
-        launch_widen_runs(pin_img, d_pos, d_off, nsub, max_nnz, P.colidx, st);
-        release(d_pos);
-        release(d_off);
-        release(pin_img);
-    }
     if (measure) {
-        sync();
+        ESP_HIP(hipStreamSynchronize(up.s));
         std::fprintf(stderr,
                      "espreso: load: K CSR staging buffers at %.4f s, row pointers queued at %.4f s, last chunk queued "
                      "at %.4f s (host copies %.4f s, waits for the copy engine %.4f s)\n",
@@ -1249,6 +1301,8 @@ void espreso_feti_s::load() {
         note("load: small uploads", t0);
         P.nz_off = upload(nz_off);
         P.rp_off = upload(rp_off);
+        // the first readers of the K_i CSR arrays follow
+        ESP_HIP(hipStreamWaitEvent(st, up.done, 0));
         // structured grid: K_i values from the unit element matrix and the element materials
         if (grid_dev) grid_stiffness();
         gather_prec_values();
@@ -1336,6 +1390,12 @@ void espreso_feti_s::load() {
     sync();
     loaded = true;
     last.time_load = wall() - t0;
+    // after st has waited for the upload stream: the scratch of the upload
+    // (block-cache images, whose release waits for st) goes back
+    if (colidx16) release(colidx16);
+    if (d_pos) release(d_pos);
+    if (pin_img) release(pin_img);
+    if (vpin_img) release(vpin_img);
     note("load: total", t0);
 }
 
@@ -2979,7 +3039,7 @@ int espreso_feti_set_subdomain(espreso_feti *h, int32_t i, int32_t n, const int3
     const int nnz = s.rowptr[n];
     // subdomains of at most 65536 DOFs keep 16-bit column indices in pinned
     // memory (read once here for the range check, then uploaded from there)
-    uint16_t *c16 = n <= 65536 ? h->pin_u16((size_t)nnz, s.pin_pos) : nullptr;
+    uint16_t *c16 = n <= 65536 ? h->pin_alloc<uint16_t>(h->pin_idx, (size_t)nnz, s.pin_pos) : nullptr;
     if (c16) {
         std::vector<int>().swap(s.colidx);
         for (int q = 0; q < nnz; q++) {
@@ -2994,11 +3054,22 @@ int espreso_feti_set_subdomain(espreso_feti *h, int32_t i, int32_t n, const int3
     }
     s.col16 = c16;
     ESP_CHECK(values || h->grid_dev, ESPRESO_ERR_ARG, "null K values");
-    if (values) s.vals.assign(values, values + nnz);
-    else s.vals.clear();  // assembled on the device (structured grid)
-    if (values)
+    // the values go into pinned memory too (uploaded from there at the set-up)
+    double *vp = values ? h->pin_alloc<double>(h->pin_val, (size_t)nnz, s.vpin_pos) : nullptr;
+    s.val_pin = vp;
+    if (vp) {
+        std::vector<double>().swap(s.vals);
+        for (int q = 0; q < nnz; q++) {
+            ESP_CHECK(std::isfinite(values[q]), ESPRESO_ERR_NUMERIC, "non-finite value in K");
+            vp[q] = values[q];
+        }
+    } else if (values) {
+        s.vals.assign(values, values + nnz);
         for (int q = 0; q < nnz; q++)
             ESP_CHECK(std::isfinite(s.vals[q]), ESPRESO_ERR_NUMERIC, "non-finite value in K");
+    } else {
+        s.vals.clear();  // assembled on the device (structured grid)
+    }
     s.f.assign(f, f + n);
     for (double v : s.f) ESP_CHECK(std::isfinite(v), ESPRESO_ERR_NUMERIC, "non-finite value in f");
     s.kdim = kdim;
@@ -3063,13 +3134,15 @@ int espreso_feti_update_values(espreso_feti *h, int32_t i, const double *values)
     const int nnz = s.rowptr[s.n];
     for (int q = 0; q < nnz; q++) ESP_CHECK(std::isfinite(values[q]), ESPRESO_ERR_NUMERIC, "non-finite value in K");
     if (!h->loaded) {
-        s.vals.assign(values, values + nnz);
+        if (s.val_pin) std::copy(values, values + nnz, s.val_pin);
+        else s.vals.assign(values, values + nnz);
         return ESPRESO_OK;
     }
     ESP_HIP(hipSetDevice(h->opt.device));
     ESP_HIP(hipMemcpyAsync(h->P.vals + h->h_nzoff[i], values, nnz * sizeof(double), hipMemcpyHostToDevice, h->st));
     h->sync();
-    if (!s.vals.empty()) s.vals.assign(values, values + nnz);
+    if (s.val_pin) std::copy(values, values + nnz, s.val_pin);
+    else if (!s.vals.empty()) s.vals.assign(values, values + nnz);
     h->values_dirty = true;
     h->ready = false;   // F_i, S_D and the coarse data follow at the next espreso_feti_setup
     h->solved = false;
@@ -3675,12 +3748,18 @@ int espreso_grid_update_material(espreso_feti *h, const double *material_elem) {
     if (!h->loaded) {
         // the values are computed on the device at the first set-up
         h->grid_dev = true;
-        for (SubHost &s : h->sub) std::vector<double>().swap(s.vals);
+        for (SubHost &s : h->sub) {
+            std::vector<double>().swap(s.vals);
+            s.val_pin = nullptr;
+        }
         return ESPRESO_OK;
     }
     ESP_HIP(hipSetDevice(h->opt.device));
     h->grid_stiffness();
-    for (SubHost &s : h->sub) std::vector<double>().swap(s.vals);
+    for (SubHost &s : h->sub) {
+            std::vector<double>().swap(s.vals);
+            s.val_pin = nullptr;
+        }
     h->values_dirty = true;
     h->ready = false;
     h->solved = false;

@@ -94,6 +94,8 @@ void launch_widen_u16(const uint16_t *s, int *d, int64_t n, hipStream_t st);
 // d[off[s] + k] = img[pos[s] + k], k < off[s + 1] - off[s] (max_n: the largest run)
 void launch_widen_runs(const uint16_t *img, const int64_t *pos, const int64_t *off, int nsub, int64_t max_n, int *d,
                        hipStream_t st);
+void launch_copy_runs(const double *img, const int64_t *pos, const int64_t *off, int nsub, int64_t max_n, double *d,
+                      hipStream_t st);
 void launch_potrf_step(const DenseDesc *desc, int nb, int which, int step, int max_band, hipStream_t st);
 // the two halves of a step: factor and invert the diagonal tile, then the panel below it
 void launch_potrf_diag(const DenseDesc *desc, int nb, int which, int step, hipStream_t st);
