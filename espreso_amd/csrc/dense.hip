@@ -865,6 +865,92 @@ __global__ void k_widen_runs(const S *img, const int64_t *pos, const int64_t *of
         dst[k] = (D)src[k];
 }
 
+// K values handed over as the lower triangle (half[s] != 0: entries with
+// column <= row, rows with sorted columns, so a prefix of each row).
+// low[rp_off[s] + r] = offset of row r's lower entries in the subdomain's
+// run: per row an upper bound of r among its columns, then a scan over the
+// rows (one workgroup per subdomain, 256 rows per pass).
+__global__ void __launch_bounds__(256) k_lower_offsets(const int *rowptr, const int64_t *rp_off, const int *colidx,
+                                                       const int64_t *nz_off, const int *half, int *low) {
+    const int s = blockIdx.x, tid = threadIdx.x;
+    if (!half[s]) return;
+    const int n = (int)(rp_off[s + 1] - rp_off[s]) - 1;
+    const int *rp = rowptr + rp_off[s];
+    const int *ci = colidx + nz_off[s];
+    __shared__ int sh[256];
+    int carry = 0;
+    for (int base = 0; base < n; base += 256) {
+        const int r = base + tid;
+        int cnt = 0;
+        if (r < n) {
+            int lo = rp[r], hi = rp[r + 1];  // first column > r
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (ci[mid] <= r) lo = mid + 1;
+                else hi = mid;
+            }
+            cnt = lo - rp[r];
+        }
+        sh[tid] = cnt;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const int t = tid >= o ? sh[tid - o] : 0;
+            __syncthreads();
+            sh[tid] += t;
+            __syncthreads();
+        }
+        if (r < n) low[rp_off[s] + r] = carry + sh[tid] - cnt;
+        carry += sh[255];
+        __syncthreads();
+    }
+}
+
+// vals[nz_off[s] + q] for every CSR entry q of subdomain s from its run at
+// pos[s] of the value image: the run itself (half[s] == 0), or the lower
+// entry (q, or the mirror of q found in row col(q)) through low (half[s]).
+// Workgroup (g, s): rows 64g..64g+63, 16 lanes per row.  A mirror missing
+// from the structure sets *err.
+__global__ void __launch_bounds__(256) k_expand_sym(const double *img, const int64_t *pos, const int64_t *nz_off,
+                                                    const int *rowptr, const int64_t *rp_off, const int *colidx,
+                                                    const int *half, const int *low, double *vals, int *err) {
+    const int s = blockIdx.y, g = blockIdx.x;
+    const int n = (int)(rp_off[s + 1] - rp_off[s]) - 1;
+    const int *rp = rowptr + rp_off[s];
+    const int *ci = colidx + nz_off[s];
+    const int *lw = low + rp_off[s];
+    const double *src = img + pos[s];
+    double *dst = vals + nz_off[s];
+    const bool h = half[s] != 0;
+    const int grp = threadIdx.x >> 4, l = threadIdx.x & 15;
+    for (int k = 0; k < 4; k++) {
+        const int r = g * 64 + 16 * k + grp;
+        if (r >= n) break;
+        for (int q = rp[r] + l; q < rp[r + 1]; q += 16) {
+            if (!h) {
+                dst[q] = src[q];
+                continue;
+            }
+            const int c = ci[q];
+            if (c <= r) {
+                dst[q] = src[lw[r] + (q - rp[r])];
+                continue;
+            }
+            int lo = rp[c], hi = rp[c + 1];  // r among row c's columns
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (ci[mid] < r) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo < rp[c + 1] && ci[lo] == r) {
+                dst[q] = src[lw[c] + (lo - rp[c])];
+            } else {
+                dst[q] = 0.0;
+                atomicOr(err, 1);
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ double block_max(double v) {
     __shared__ double red[256];
     red[threadIdx.x] = v;
@@ -1353,6 +1439,15 @@ void launch_copy_runs(const double *img, const int64_t *pos, const int64_t *off,
     if (nsub <= 0 || max_n <= 0) return;
     const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((max_n + 1023) / 1024, 256));
     hipLaunchKernelGGL((k_widen_runs<double, double>), dim3(g, nsub), dim3(256), 0, st, img, pos, off, d);
+}
+
+void launch_expand_sym(const double *img, const int64_t *pos, const int64_t *nz_off, const int *rowptr,
+                       const int64_t *rp_off, const int *colidx, const int *half, int *low, int nsub, int max_n,
+                       double *vals, int *err, hipStream_t st) {
+    if (nsub <= 0 || max_n <= 0) return;
+    hipLaunchKernelGGL(k_lower_offsets, dim3(nsub), dim3(256), 0, st, rowptr, rp_off, colidx, nz_off, half, low);
+    hipLaunchKernelGGL(k_expand_sym, dim3((max_n + 63) / 64, nsub), dim3(256), 0, st, img, pos, nz_off, rowptr, rp_off,
+                       colidx, half, low, vals, err);
 }
 
 void launch_widen_u16(const uint16_t *s, int *d, int64_t n, hipStream_t st) {

@@ -73,9 +73,46 @@ struct SubHost {
     int64_t pin_pos = 0;  // position of col16 in the instance's pinned blocks laid end to end (16-bit units)
     int col(size_t k) const { return col16 ? (int)col16[k] : colidx[k]; }
     // the K values in pinned host memory (then vals stays empty), at vpin_pos
-    // in the value blocks laid end to end (doubles)
+    // in the value blocks laid end to end (doubles); val_half: only the lower
+    // triangle (the entries with column <= row, a prefix of each row, whose
+    // columns are sorted), the device mirrors the rest (K is symmetric)
     double *val_pin = nullptr;
     int64_t vpin_pos = 0;
+    bool val_half = false;
+    // full values from the lower ones of this subdomain (rows sorted); false
+    // when an entry's mirror is missing from the structure
+    bool expand_half(const double *half, std::vector<double> &full) const {
+        std::vector<int64_t> low(n + 1, 0);
+        for (int r = 0; r < n; r++) {
+            int c = 0;
+            for (int q = rowptr[r]; q < rowptr[r + 1] && col(q) <= r; q++) c++;
+            low[r + 1] = low[r] + c;
+        }
+        full.resize(rowptr[n]);
+        for (int r = 0; r < n; r++)
+            for (int q = rowptr[r]; q < rowptr[r + 1]; q++) {
+                const int c = col(q);
+                if (c <= r) {
+                    full[q] = half[low[r] + (q - rowptr[r])];
+                    continue;
+                }
+                int lo = rowptr[c], hi = rowptr[c + 1];
+                while (lo < hi) {
+                    const int mid = (lo + hi) / 2;
+                    if (col(mid) < r) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo == rowptr[c + 1] || col(lo) != r) return false;
+                full[q] = half[low[c] + (lo - rowptr[c])];
+            }
+        return true;
+    }
+    // the lower values (column <= row) of full into half, in CSR order
+    void lower_values(const double *full, double *half) const {
+        for (int r = 0; r < n; r++)
+            for (int q = rowptr[r]; q < rowptr[r + 1]; q++)
+                if (col(q) <= r) *half++ = full[q];
+    }
     std::vector<double> vals, f, R;
     std::vector<int> lam, dof;
     std::vector<double> sign, scale;
@@ -795,9 +832,15 @@ void espreso_feti_s::load() {
     // than in bytes) into an image of the blocks, and a gather widens each
     // subdomain's run into place on the device
     const bool all_pinned = max_n_in <= 65536 && n_pinned == nsub && nsub > 0;
-    if (measure)
-        std::fprintf(stderr, "espreso: load: 16-bit column indices pinned at the handoff: %d of %d subdomains\n",
-                     n_pinned, nsub);
+    if (measure) {
+        int n_half = 0;
+        for (int s = 0; s < nsub; s++) n_half += sub[s].val_half;
+        std::fprintf(stderr,
+                     "espreso: load: pinned at the handoff: 16-bit column indices of %d, K values of %d (%d as lower "
+                     "triangles) of %d subdomains\n",
+                     n_pinned, (int)std::count_if(sub.begin(), sub.end(), [](const SubHost &h) { return h.val_pin; }),
+                     n_half, nsub);
+    }
     uint16_t *pin_img = all_pinned ? (uint16_t *)alloc_bytes(pin_idx.image_bytes(), true) : nullptr;
     int n_vpinned = 0;
     for (int s = 0; s < nsub; s++) n_vpinned += sub[s].val_pin != nullptr;
@@ -813,6 +856,18 @@ void espreso_feti_s::load() {
         max_nnz = std::max(max_nnz, nz_off[s + 1] - nz_off[s]);
     }
     int64_t *d_pos = (pin_img || vpin_img) ? alloc<int64_t>(3 * (size_t)nsub + 1) : nullptr;
+    // values handed over as lower triangles: mirrored on the device
+    std::vector<int> half_h(nsub);
+    bool any_half = false;
+    for (int s = 0; s < nsub; s++) {
+        half_h[s] = sub[s].val_half;
+        any_half = any_half || sub[s].val_half;
+    }
+    any_half = any_half && vals_pinned;
+    int *d_half = any_half ? alloc<int>(nsub + 1) : nullptr;  // [nsub]: the missing-mirror flag
+    int *d_low = any_half ? alloc<int>(rp_off[nsub]) : nullptr;
+    int64_t *d_rpo = any_half ? alloc<int64_t>(nsub + 1) : nullptr;
+    std::atomic<int> sym_err{0};
     // the upload stream starts after the allocations above (stream-ordered on st)
     struct UpStream {
         hipStream_t s = nullptr;
@@ -873,12 +928,25 @@ void espreso_feti_s::load() {
             // the structured grid assembles the values on the device
             if (vals_pinned) {
                 upload_arena(pin_val, vpin_img, up.s);
-                launch_copy_runs(vpin_img, d_pos + nsub, d_pos + 2 * nsub, nsub, max_nnz, P.vals, up.s);
+                if (any_half) {
+                    half_h.push_back(0);
+                    ESP_HIP(hipMemcpyAsync(d_half, half_h.data(), (nsub + 1) * sizeof(int), hipMemcpyHostToDevice,
+                                           up.s));
+                    ESP_HIP(hipMemcpyAsync(d_rpo, rp_off.data(), (nsub + 1) * sizeof(int64_t), hipMemcpyHostToDevice,
+                                           up.s));
+                    launch_expand_sym(vpin_img, d_pos + nsub, d_pos + 2 * nsub, P.rowptr, d_rpo, P.colidx, d_half,
+                                      d_low, nsub, max_n_in, P.vals, d_half + nsub, up.s);
+                } else {
+                    launch_copy_runs(vpin_img, d_pos + nsub, d_pos + 2 * nsub, nsub, max_nnz, P.vals, up.s);
+                }
             } else if (!grid_dev) {
 #pragma omp parallel for schedule(dynamic) num_threads(8)
                 for (int s = 0; s < nsub; s++) {
                     SubHost &h = sub[s];
-                    if (h.val_pin && h.vals.empty()) h.vals.assign(h.val_pin, h.val_pin + h.rowptr[h.n]);
+                    if (h.val_pin && h.vals.empty()) {
+                        if (!h.val_half) h.vals.assign(h.val_pin, h.val_pin + h.rowptr[h.n]);
+                        else if (!h.expand_half(h.val_pin, h.vals)) sym_err = 1;
+                    }
                 }
                 stage_pieces(P.vals, nz_off, [&](int s) { return sub[s].vals.data(); }, up.s, 8);
             }
@@ -1392,6 +1460,16 @@ void espreso_feti_s::load() {
     last.time_load = wall() - t0;
     // after st has waited for the upload stream: the scratch of the upload
     // (block-cache images, whose release waits for st) goes back
+    ESP_CHECK(!sym_err, ESPRESO_ERR_ARG, "K_i is not structurally symmetric (an entry's mirror is missing)");
+    if (d_half) {
+        int e = 0;
+        ESP_HIP(hipMemcpyAsync(&e, d_half + nsub, sizeof(int), hipMemcpyDeviceToHost, st));
+        sync();
+        ESP_CHECK(!e, ESPRESO_ERR_ARG, "K_i is not structurally symmetric (an entry's mirror is missing)");
+        release(d_half);
+        release(d_low);
+        release(d_rpo);
+    }
     if (colidx16) release(colidx16);
     if (d_pos) release(d_pos);
     if (pin_img) release(pin_img);
@@ -3040,12 +3118,20 @@ int espreso_feti_set_subdomain(espreso_feti *h, int32_t i, int32_t n, const int3
     // subdomains of at most 65536 DOFs keep 16-bit column indices in pinned
     // memory (read once here for the range check, then uploaded from there)
     uint16_t *c16 = n <= 65536 ? h->pin_alloc<uint16_t>(h->pin_idx, (size_t)nnz, s.pin_pos) : nullptr;
+    bool sorted = true;  // strictly increasing columns in every row
+    int64_t nlow = 0;    // entries with column <= row
     if (c16) {
         std::vector<int>().swap(s.colidx);
-        for (int q = 0; q < nnz; q++) {
-            const int c = colidx[q];
-            ESP_CHECK(c >= 0 && c < n, ESPRESO_ERR_ARG, "CSR column out of range");
-            c16[q] = (uint16_t)c;
+        for (int r = 0; r < n; r++) {
+            int prev = -1;
+            for (int q = s.rowptr[r]; q < s.rowptr[r + 1]; q++) {
+                const int c = colidx[q];
+                ESP_CHECK(c >= 0 && c < n, ESPRESO_ERR_ARG, "CSR column out of range");
+                c16[q] = (uint16_t)c;
+                sorted = sorted && c > prev;
+                prev = c;
+                nlow += c <= r;
+            }
         }
     } else {
         s.colidx.assign(colidx, colidx + nnz);
@@ -3054,15 +3140,20 @@ int espreso_feti_set_subdomain(espreso_feti *h, int32_t i, int32_t n, const int3
     }
     s.col16 = c16;
     ESP_CHECK(values || h->grid_dev, ESPRESO_ERR_ARG, "null K values");
-    // the values go into pinned memory too (uploaded from there at the set-up)
-    double *vp = values ? h->pin_alloc<double>(h->pin_val, (size_t)nnz, s.vpin_pos) : nullptr;
+    // the values go into pinned memory too (uploaded from there at the set-up):
+    // with sorted rows only the lower triangle, since K is symmetric (the
+    // device mirrors the rest; half the bytes over the link)
+    s.val_half = values && c16 && sorted;
+    double *vp = values ? h->pin_alloc<double>(h->pin_val, s.val_half ? (size_t)nlow : (size_t)nnz, s.vpin_pos)
+                        : nullptr;
+    if (!vp) s.val_half = false;
     s.val_pin = vp;
     if (vp) {
         std::vector<double>().swap(s.vals);
-        for (int q = 0; q < nnz; q++) {
+        for (int q = 0; q < nnz; q++)
             ESP_CHECK(std::isfinite(values[q]), ESPRESO_ERR_NUMERIC, "non-finite value in K");
-            vp[q] = values[q];
-        }
+        if (s.val_half) s.lower_values(values, vp);
+        else std::copy(values, values + nnz, vp);
     } else if (values) {
         s.vals.assign(values, values + nnz);
         for (int q = 0; q < nnz; q++)
@@ -3134,14 +3225,16 @@ int espreso_feti_update_values(espreso_feti *h, int32_t i, const double *values)
     const int nnz = s.rowptr[s.n];
     for (int q = 0; q < nnz; q++) ESP_CHECK(std::isfinite(values[q]), ESPRESO_ERR_NUMERIC, "non-finite value in K");
     if (!h->loaded) {
-        if (s.val_pin) std::copy(values, values + nnz, s.val_pin);
+        if (s.val_pin && s.val_half) s.lower_values(values, s.val_pin);
+        else if (s.val_pin) std::copy(values, values + nnz, s.val_pin);
         else s.vals.assign(values, values + nnz);
         return ESPRESO_OK;
     }
     ESP_HIP(hipSetDevice(h->opt.device));
     ESP_HIP(hipMemcpyAsync(h->P.vals + h->h_nzoff[i], values, nnz * sizeof(double), hipMemcpyHostToDevice, h->st));
     h->sync();
-    if (s.val_pin) std::copy(values, values + nnz, s.val_pin);
+    if (s.val_pin && s.val_half) s.lower_values(values, s.val_pin);
+    else if (s.val_pin) std::copy(values, values + nnz, s.val_pin);
     else if (!s.vals.empty()) s.vals.assign(values, values + nnz);
     h->values_dirty = true;
     h->ready = false;   // F_i, S_D and the coarse data follow at the next espreso_feti_setup

@@ -94,6 +94,11 @@ void launch_widen_u16(const uint16_t *s, int *d, int64_t n, hipStream_t st);
 // d[off[s] + k] = img[pos[s] + k], k < off[s + 1] - off[s] (max_n: the largest run)
 void launch_widen_runs(const uint16_t *img, const int64_t *pos, const int64_t *off, int nsub, int64_t max_n, int *d,
                        hipStream_t st);
+// K values from lower-triangle runs (half[s]) or full runs: low is scratch of
+// rp_off[nsub] ints (per-row offsets), *err set when a mirror is missing
+void launch_expand_sym(const double *img, const int64_t *pos, const int64_t *nz_off, const int *rowptr,
+                       const int64_t *rp_off, const int *colidx, const int *half, int *low, int nsub, int max_n,
+                       double *vals, int *err, hipStream_t st);
 void launch_copy_runs(const double *img, const int64_t *pos, const int64_t *off, int nsub, int64_t max_n, double *d,
                       hipStream_t st);
 void launch_potrf_step(const DenseDesc *desc, int nb, int which, int step, int max_band, hipStream_t st);

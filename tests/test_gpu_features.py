@@ -13,6 +13,8 @@
     of the same rows re-solve to the direct solution of the changed problem
   * status: an unconverged solve returns ESPRESO_NOT_CONVERGED with the stats
   * the SYMV's LDS guard refuses an interface that does not fit (no launch)
+  * K values handed over as lower triangles (sorted rows) solve like the full
+    values (unsorted rows), and a structurally asymmetric K is refused
   * low-precision block re-solves leave the instance's own solution in place,
     and the mixed-precision stats report the outer ||b||
 """
@@ -210,6 +212,59 @@ def test_lds_guard_refuses_wide_interface():
                     np.ones(n), np.ones(n))
     s.set_dual(np.zeros(n))
     with pytest.raises(RuntimeError, match="LDS"):
+        s.setup()
+    s.close()
+
+
+def _chain(n, drop=None, reverse=False):
+    """CSR of the chain Laplacian (2 on the diagonal, -1 beside it), 1 at
+    both ends; drop = (i, j) leaves that entry out, reverse orders every row's
+    columns downward."""
+    rp, cols, vals = [0], [], []
+    for i in range(n):
+        row = [(j, v) for j, v in ((i - 1, -1.0), (i, 2.0 if 0 < i < n - 1 else 1.0), (i + 1, -1.0))
+               if 0 <= j < n and (i, j) != drop]
+        if reverse:
+            row = row[::-1]
+        cols += [j for j, _ in row]
+        vals += [v for _, v in row]
+        rp.append(len(cols))
+    return np.array(rp, np.int32), np.array(cols, np.int32), np.array(vals)
+
+
+def _chain_solver(K, n, c):
+    s = core.FETISolver(1, preconditioner="dirichlet", precision=1e-12)
+    R = np.full((n, 1), 1.0 / np.sqrt(n))
+    dof = np.arange(0, n, 2, dtype=np.int32)  # every other DOF prescribed
+    lam = np.arange(len(dof), dtype=np.int32)
+    f = np.sin(np.arange(n, dtype=np.float64))
+    s.set_subdomain(0, K, f, R, lam, dof, np.ones(len(dof)), np.ones(len(dof)))
+    s.set_dual(c[:len(dof)])
+    return s
+
+
+def test_lower_triangle_values_and_asymmetric_structure():
+    """Sorted rows: the K values cross as the lower triangle and the device
+    mirrors the rest; reversed rows: the full values cross.  Both solve the
+    same chain problem to the same primal (1e-12 relative).  Dropping one
+    lower entry (its mirror, an upper entry of another row, stays) makes the
+    structure asymmetric: set-up refuses it."""
+    say("case: lower-triangle K values, asymmetric structure")
+    n = 60
+    c = np.cos(np.arange(n, dtype=np.float64))
+    us = []
+    for reverse in (False, True):
+        s = _chain_solver(_chain(n, reverse=reverse), n, c)
+        s.setup()
+        st = s.solve()
+        assert st["converged"] == 1
+        us.append(s.primal(0, n))
+        s.close()
+    err = np.abs(us[0] - us[1]).max() / np.abs(us[1]).max()
+    say(f"  sorted vs reversed rows: {err:.2e}")
+    assert err <= 1e-12
+    s = _chain_solver(_chain(n, drop=(11, 10)), n, c)
+    with pytest.raises(RuntimeError, match="structurally symmetric"):
         s.setup()
     s.close()
 
