@@ -218,7 +218,12 @@ int espreso_feti_sizes(espreso_feti *h, int32_t *sizes);
 
 /* K: full symmetric CSR (0-based) of n x n; f: n; R: n x kdim column-major,
  * orthonormal columns spanning ker K; lambda rows touching the subdomain
- * (ascending lambda id) with local DOF, sign (+1/-1) and scaling weight.     */
+ * (ascending lambda id) with local DOF, sign (+1/-1) and scaling weight.
+ * K is read as symmetric: when every row's columns are sorted (n <= 65536),
+ * only its lower triangle (column <= row) is kept and the upper entries take
+ * their mirrors' values; espreso_feti_setup then returns ESPRESO_ERR_ARG if
+ * an upper entry's mirror is missing from the structure.  The arrays are
+ * copied (into pinned host memory where possible); the caller may free them. */
 int espreso_feti_set_subdomain(espreso_feti *h, int32_t i, int32_t n,
                                const int32_t *rowptr, const int32_t *colidx, const double *values,
                                const double *f, int32_t kdim, const double *R,
