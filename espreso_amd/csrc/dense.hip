@@ -1097,19 +1097,28 @@ __global__ void k_pack(const DenseDesc *D, T *pool, const int64_t *off, const fl
     const double inv = scale ? 1.0 / (double)scale[blockIdx.z] : 1.0;
     constexpr int VEC = 16 / (int)sizeof(T);
     T *base = pool + off[blockIdx.z];
-#pragma unroll 4
-    for (int r = 0; r < R; r++) {
-        T *dst = base + strip_index(m, R, VEC, t, r, c);
-        const int row = r0 + r;
-        double v = 0.0;
-        if (row < m && row <= c) {
-            v = d.S[c + (size_t)row * m];
-            if (row == c) v *= 0.5;
+    // the VEC rows of one 16-byte group of the column, written at once: the
+    // lanes of a wave write consecutive groups (1 KB contiguous)
+#pragma unroll 2
+    for (int g = 0; g < R / VEC; g++) {
+        union {
+            T v[VEC];
+            uint4 q;
+        } grp;
+#pragma unroll
+        for (int u = 0; u < VEC; u++) {
+            const int row = r0 + g * VEC + u;
+            double v = 0.0;
+            if (row < m && row <= c) {
+                v = d.S[c + (size_t)row * m];
+                if (row == c) v *= 0.5;
+            }
+            v *= inv;
+            if constexpr (sizeof(T) == 8) grp.v[u] = (T)v;
+            else if constexpr (sizeof(T) == 4) grp.v[u] = (T)(float)v;
+            else grp.v[u] = to_bf16((float)v);
         }
-        v *= inv;
-        if constexpr (sizeof(T) == 8) *dst = (T)v;
-        else if constexpr (sizeof(T) == 4) *dst = (T)(float)v;
-        else *dst = to_bf16((float)v);
+        *reinterpret_cast<uint4 *>(base + strip_index(m, R, VEC, t, g * VEC, c)) = grp.q;
     }
 }
 
